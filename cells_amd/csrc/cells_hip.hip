@@ -1,0 +1,1651 @@
+// MI355X (gfx950) kernels and C-ABI of the batched tissue step.
+//
+// One step of every world (models/surface_pump_v3.py:3989 checked_step of the reference, for
+// lattices of cells on one held bath) runs as a fixed sequence of launches over all worlds packed
+// back to back:
+//   k_edges        junction routes: GHK flux, heat, gate rate, slopes (thread per edge)
+//   k_cells        membrane routes, Na/K- and Ca-ATPase, owner sums, stage-matrix diagonal
+//                  (thread per cell; a wavefront reads one species of 64 consecutive cells)
+//   k_factor_solve level-scheduled sparse Cholesky of C + gamma dt G and its solves
+//                  (workgroup per world)
+//   k_*_update     stage combinations and domain checks (thread per cell and per edge)
+//   k_world_*      per-world ledgers, fixed-order reductions (workgroup per world)
+//   k_water        Heun water half-steps (workgroup per world)
+//   k_commit       per-world acceptance: the state buffer pair flips only where valid
+// The state is field-major SoA (cells, then edges, then worlds). Two state buffers alternate per
+// world (sel[w]), so a refused world keeps its old state without a copy.
+//
+// FP64 throughout; compiled with -ffp-contract=off so every product and sum rounds as written
+// (the factor's sums follow the reference's pairwise order, numerics/sparse_cholesky.py:32-40).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/cells_hip.h"
+#include "symbolic.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t err_ = (expr);                                                          \
+    if (err_ != hipSuccess)                                                            \
+      return fail(CELLS_EHIP, std::string(#expr) + ": " + hipGetErrorString(err_));    \
+  } while (0)
+
+constexpr double kAvogadro = 6.02214076e23;
+constexpr double kF = kAvogadro * 1.602176634e-19;
+constexpr double kR = kAvogadro * 1.380649e-23;
+constexpr double kMolPerFmol = 1e-15;
+constexpr double kChargePerFmol = kF * kMolPerFmol;
+constexpr double kMaxAbsX = 50.0;
+constexpr double kNakMaxAffinity = 80.0;
+constexpr double kNakMaxMu = 1600.0;
+constexpr double kCaMaxAffinity = 80.0;
+constexpr double kCaMaxVoltage = 0.15;
+constexpr double kGateMaxVoltage = 0.120;
+constexpr double kWaterMaxRelaxation = 0.05;
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------------------------------
+// Device-side view of the packed worlds (kernel argument, by value).
+
+struct Params {
+  int S, W, water, calcium;
+  int ncell, nedge;
+  double z[4], cb[4], log_cb[4], mu_b[4];
+  double T, RT, Vb;
+  double nak_drive, log_km_na, log_km_k, nak_atp_sat;  // nak_atp_sat = logaddexp(0, ln Km_atp - ln ATP)
+  double ca_dk, ca_drive, log_ca_km, ca_atp_sat;
+  double dg0_over_rt_nak_terms;  // delta_g0/RT + ln ADP + ln Pi - ln ATP - ln 1000 (Na/K domain)
+  double ca_aff_base;            // delta_g0/RT + ln ADP + ln Pi - ln ATP - ln 1000 (Ca domain)
+  double gate_thr, gate_min, log_0065;
+  double wK, wP0, vmin, vmax, psib;
+  const int* cell_off;
+  const int* edge_off;
+  const int* cell_world;
+  const int* edge_world;
+  const double* vol0;
+  const double* cap;
+  const double* pam;
+  const double* nak_cap;
+  const double* ca_cap;
+  const double* n0;
+  const double* q0;
+  const double* lpa;
+  const double* imperm;
+  const int* snd;
+  const int* rcv;
+  const double* paj;
+  const int* inc_ptr;
+  const int* inc_edge;
+  const int* inc_out;
+  // state layout
+  int cf, ef, wf;
+  int f_dn, f_mtr, f_mdiss, f_pcyc, f_pdiss, f_pwork, f_ccyc, f_cdiss, f_cwork, f_dvol, f_wtr, f_wdiss;
+  int f_jtr, f_jdiss, f_gate;
+  int f_sup, f_echg, f_rwork, f_ework, f_wsup, f_wwork;
+  long edge_base, world_base, total;
+};
+
+struct Ctl {
+  double membrane, junction, pump, calcium_pump, water;
+};
+
+struct ShapeDev {
+  int n, E, levels;
+  int nnz;
+  long o_cs, o_entry, o_rowptr, o_rowent, o_lev, o_levcol, o_levent, o_node;
+};
+
+__device__ __forceinline__ double& CF(const Params& P, double* y, int f, int i) {
+  return y[(long)f * P.ncell + i];
+}
+__device__ __forceinline__ double CFc(const Params& P, const double* y, int f, int i) {
+  return y[(long)f * P.ncell + i];
+}
+__device__ __forceinline__ double& EF(const Params& P, double* y, int f, int e) {
+  return y[P.edge_base + (long)f * P.nedge + e];
+}
+__device__ __forceinline__ double EFc(const Params& P, const double* y, int f, int e) {
+  return y[P.edge_base + (long)f * P.nedge + e];
+}
+__device__ __forceinline__ double& WF(const Params& P, double* y, int f, int w) {
+  return y[P.world_base + (long)f * P.W + w];
+}
+__device__ __forceinline__ double WFc(const Params& P, const double* y, int f, int w) {
+  return y[P.world_base + (long)f * P.W + w];
+}
+
+// ------------------------------------------------------------------------------------------
+// Point laws (laws/electrochemistry.py:25-48, laws/junctions/selective.py:28-50,
+// laws/routes/patch_electrodiffusion.py:43-54, laws/pumps/*, junction_voltage_relaxation.py).
+
+__device__ __forceinline__ double bernoulli(double x) {
+  if (fabs(x) < 1e-4) {
+    double x2 = x * x;
+    return 1.0 - x / 2.0 + x2 / 12.0 - (x2 * x2) / 720.0;
+  }
+  return x / expm1(x);
+}
+
+__device__ __forceinline__ double bernoulli_slope(double x) {
+  if (fabs(x) < 1e-3) return -0.5 + x / 6.0 - (x * x * x) / 180.0;
+  double em = exp(-fabs(x));
+  if (x > 0) {
+    double d = 1.0 - em;
+    return (em * (1.0 - x) - em * em) / (d * d);
+  }
+  double d = em - 1.0;
+  return (em - 1.0 - x * em) / (d * d);
+}
+
+__device__ __forceinline__ double logaddexp0(double t) {
+  double m = t > 0.0 ? t : 0.0;
+  return m + log1p(exp(-fabs(0.0 - t)));
+}
+
+// GHK route flux (sender -> receiver) and its slope dJ/d(dv).
+__device__ __forceinline__ void route(double cf, double ct, double dv, double z, double T,
+                                      double pa, double avail, bool slope, double& J, double& g) {
+  double x = z * kF * dv / (kR * T);
+  J = pa * avail / kMolPerFmol * (cf * bernoulli(-x) - ct * bernoulli(x));
+  if (slope) {
+    double k = z * kF / (kR * T);
+    double xs = k * dv;
+    double pre = pa * avail / kMolPerFmol;
+    g = pre * k * (-cf * bernoulli_slope(-xs) - ct * bernoulli_slope(xs));
+  }
+}
+
+__device__ __forceinline__ void cell_concentrations(const Params& P, const double* y,
+                                                    const double* yv, int i, double* c,
+                                                    double& vol) {
+  vol = P.water ? P.vol0[i] + CFc(P, yv, P.f_dvol, i) : P.vol0[i];
+  for (int s = 0; s < P.S; ++s) c[s] = (P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i)) / vol;
+}
+
+__device__ __forceinline__ double charge_increment(const Params& P, const double* y, int i) {
+  double t = P.z[0] * CFc(P, y, P.f_dn, i);
+  for (int s = 1; s < P.S; ++s) t = t + P.z[s] * CFc(P, y, P.f_dn + s, i);
+  return kChargePerFmol * t;
+}
+
+__device__ __forceinline__ double potential(const Params& P, const double* y, int i) {
+  double q = P.q0[i] + charge_increment(P, y, i);
+  return P.Vb + q / P.cap[i];
+}
+
+__device__ __forceinline__ const double* current(const double* A, const double* B, int s) {
+  return s ? B : A;
+}
+__device__ __forceinline__ double* next(double* A, double* B, int s) { return s ? A : B; }
+
+// ------------------------------------------------------------------------------------------
+// Junction routes (numerics/surface_transport.py:310-367, junction branch; gate rhs
+// junction_voltage_relaxation.py:41-59; slopes models/surface_pump_v3.py:2747-2755).
+
+__global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, const double* B,
+                                                  const int* sel, const double* src_override,
+                                                  int vol_next, int e0, int e1, Ctl c, double* R,
+                                                  double* gj, double* Gj, int* valid) {
+  int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e1) return;
+  int w = P.edge_world[e];
+  int sw = sel[w];
+  const double* y = src_override ? src_override : current(A, B, sw);
+  const double* yv = vol_next ? next(const_cast<double*>(A), const_cast<double*>(B), sw) : y;
+  int a = P.snd[e], b = P.rcv[e];
+  double ca[4], cb[4], va, vb;
+  cell_concentrations(P, y, yv, a, ca, va);
+  cell_concentrations(P, y, yv, b, cb, vb);
+  double dv = potential(P, y, a) - potential(P, y, b);
+  double g = EFc(P, y, P.f_gate, e);
+  double avail = c.junction * g;
+  bool ok = true;
+  double diss = 0.0, G = 0.0;
+  bool slopes = gj != nullptr;
+  for (int s = 0; s < P.S; ++s) {
+    double z = P.z[s];
+    double pa = P.paj[(long)s * P.nedge + e];
+    ok = ok && isfinite(ca[s]) && ca[s] > 0 && isfinite(cb[s]) && cb[s] > 0;
+    double J, sl = 0.0;
+    route(ca[s], cb[s], dv, z, P.T, pa, avail, slopes, J, sl);
+    double mu = P.RT * (log(ca[s]) - log(cb[s])) + z * kF * dv;
+    diss = s == 0 ? J * mu : diss + J * mu;
+    ok = ok && fabs(z * kF * dv / P.RT) <= kMaxAbsX && isfinite(J);
+    EF(P, R, P.f_jtr + s, e) = J;
+    if (slopes) {
+      gj[(long)s * P.nedge + e] = sl;
+      G = s == 0 ? z * sl : G + z * sl;
+    }
+  }
+  if (slopes) Gj[e] = kChargePerFmol * G;
+  EF(P, R, P.f_jdiss, e) = 1e-3 * diss;
+  ok = ok && isfinite(1e-3 * diss);
+  double shift = fabs(dv) - P.gate_thr;
+  double alpha = 1.3 * exp(-77.0 * shift);
+  double log_beta = P.log_0065 + 140.0 * shift;
+  double beta = 20.0 * exp(-logaddexp0(-log_beta));
+  EF(P, R, P.f_gate, e) = alpha * (1.0 - g) - beta * (g - P.gate_min);
+  ok = ok && isfinite(dv) && fabs(dv) <= kGateMaxVoltage && isfinite(g) && g >= P.gate_min &&
+       g <= 1.0;
+  if (!ok) valid[w] = 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Membrane routes, pumps and owners (surface_transport.py:310-375, surface_pump_v2.py:370-450).
+// With coef > 0 also the stage matrix's diagonal and its Jacobi scale
+// (numerics/surface_imex.py:228-259 on C + coef G).
+
+__global__ __launch_bounds__(kBlock) void k_cells(Params P, const double* A, const double* B,
+                                                  const int* sel, const double* src_override,
+                                                  int vol_next, int c0, int c1, Ctl c, double* R,
+                                                  double* contrib, double* gm, const double* Gj,
+                                                  double coef, double* mdiag, double* msc,
+                                                  int* valid) {
+  int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c1) return;
+  int w = P.cell_world[i];
+  int sw = sel[w];
+  const double* y = src_override ? src_override : current(A, B, sw);
+  const double* yv = vol_next ? next(const_cast<double*>(A), const_cast<double*>(B), sw) : y;
+  double ci[4], vol;
+  cell_concentrations(P, y, yv, i, ci, vol);
+  double dv = potential(P, y, i) - P.Vb;
+  bool ok = true;
+  bool slopes = gm != nullptr;
+  double Jm[4], mu[4];
+  double diss = 0.0, Gm = 0.0;
+  for (int s = 0; s < P.S; ++s) {
+    double z = P.z[s];
+    ok = ok && isfinite(ci[s]) && ci[s] > 0;
+    double pa = P.pam[(long)s * P.ncell + i];
+    double sl = 0.0;
+    route(ci[s], P.cb[s], dv, z, P.T, pa, c.membrane, slopes, Jm[s], sl);
+    mu[s] = P.RT * (log(ci[s]) - P.log_cb[s]) + z * kF * dv;
+    diss = s == 0 ? Jm[s] * mu[s] : diss + Jm[s] * mu[s];
+    ok = ok && fabs(z * kF * dv / P.RT) <= kMaxAbsX && isfinite(Jm[s]);
+    if (slopes) {
+      gm[(long)s * P.ncell + i] = sl;
+      Gm = s == 0 ? z * sl : Gm + z * sl;
+    }
+  }
+  // Na/K-ATPase (nak_atp_mass_action_positive.py:63-149)
+  double cap = P.nak_cap[i];
+  double energy = P.nak_drive + (-3.0) * mu[0] + 2.0 * mu[1];
+  double a = energy / P.RT;
+  double ls = -logaddexp0(3.0 * (P.log_km_na - log(ci[0]))) -
+              logaddexp0(2.0 * (P.log_km_k - P.log_cb[1])) - P.nak_atp_sat;
+  bool enabled = cap > 0 && c.pump > 0;
+  double mag = exp(log(enabled ? cap : 1.0) + log(enabled ? c.pump : 1.0) + ls + (a > 0.0 ? a : 0.0));
+  double dep = a >= 0 ? expm1(-a) : -expm1(a);
+  double cyc = enabled ? mag * dep : 0.0;
+  {
+    double sn = mu[0] / P.RT, sk = mu[1] / P.RT;
+    double aff = P.dg0_over_rt_nak_terms + (-3.0) * sn + 2.0 * sk;
+    ok = ok && isfinite(ci[0]) && ci[0] > 0 && isfinite(cap) && cap >= 0 && isfinite(sn) &&
+         fabs(sn) <= kNakMaxMu && isfinite(sk) && fabs(sk) <= kNakMaxMu && isfinite(aff) &&
+         fabs(aff) <= kNakMaxAffinity;
+  }
+  double extra[4] = {3.0 * cyc, -2.0 * cyc, -0.0 * cyc, -0.0 * cyc};
+  double ccyc = 0.0, cenergy = 0.0;
+  if (P.calcium) {
+    // ca_atp_mass_action_positive.py:70-150 via er_calcium.pump_rates(ci, co, -v)
+    double ccap = P.ca_cap[i];
+    double ce = P.ca_dk + P.RT * (P.log_cb[3] - log(ci[3])) + 2.0 * kF * (-dv);
+    double aff = ce / P.RT;
+    double cls = -logaddexp0(P.log_ca_km - log(ci[3]));
+    cls -= P.ca_atp_sat;
+    bool cen = ccap > 0 && c.calcium_pump > 0;
+    double cmag = exp(log(cen ? ccap : 1.0) + log(cen ? c.calcium_pump : 1.0) + cls +
+                      (aff > 0.0 ? aff : 0.0));
+    double cdep = aff >= 0 ? expm1(-aff) : -expm1(aff);
+    ccyc = cen ? cmag * cdep : 0.0;
+    cenergy = ce;
+    double daff = P.ca_aff_base + P.log_cb[3] - log(ci[3]) - 2.0 * kF * dv / P.RT;
+    ok = ok && isfinite(ccap) && ccap >= 0 && isfinite(dv) && fabs(dv) <= kCaMaxVoltage &&
+         isfinite(daff) && fabs(daff) <= kCaMaxAffinity;
+    extra[3] = extra[3] - (-1.0) * ccyc;
+  }
+  int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
+  for (int s = 0; s < P.S; ++s) {
+    double out = 0.0, in = 0.0;
+    for (int p = p0; p < pm; ++p) out += -EFc(P, R, P.f_jtr + s, P.inc_edge[p]);
+    for (int p = pm; p < p1; ++p) in += EFc(P, R, P.f_jtr + s, P.inc_edge[p]);
+    double cell = -Jm[s];
+    cell += out;
+    cell += in;
+    double r = cell - extra[s];
+    CF(P, R, P.f_dn + s, i) = r;
+    CF(P, R, P.f_mtr + s, i) = Jm[s];
+    contrib[(long)s * P.ncell + i] = -(Jm[s] + extra[s]);
+    ok = ok && isfinite(r);
+  }
+  double mdiss = 1e-3 * diss;
+  CF(P, R, P.f_mdiss, i) = mdiss;
+  CF(P, R, P.f_pcyc, i) = cyc;
+  CF(P, R, P.f_pdiss, i) = -1e-3 * cyc * energy;
+  CF(P, R, P.f_pwork, i) = -1e-3 * cyc * P.nak_drive;
+  ok = ok && isfinite(mdiss) && isfinite(cyc) && isfinite(-1e-3 * cyc * energy);
+  if (P.calcium) {
+    CF(P, R, P.f_ccyc, i) = ccyc;
+    CF(P, R, P.f_cdiss, i) = -1e-3 * ccyc * cenergy;
+    CF(P, R, P.f_cwork, i) = -1e-3 * ccyc * P.ca_drive;
+    ok = ok && isfinite(ccyc) && isfinite(-1e-3 * ccyc * cenergy);
+  }
+  if (P.water) {
+    CF(P, R, P.f_dvol, i) = 0.0;
+    CF(P, R, P.f_wtr, i) = 0.0;
+    CF(P, R, P.f_wdiss, i) = 0.0;
+  }
+  if (slopes) {
+    double g = kChargePerFmol * Gm;
+    for (int p = p0; p < p1; ++p) g += Gj[P.inc_edge[p]];
+    double d = P.cap[i] + coef * g;
+    ok = ok && d > 0;
+    double sc = 1.0 / sqrt(d > 0 ? d : 1.0);
+    mdiag[i] = (d * sc * sc + d * sc * sc) / 2;
+    msc[i] = sc;
+  }
+  if (!ok) valid[w] = 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Deterministic block reductions: each thread folds its strided items in order, then a fixed
+// LDS tree.
+
+__device__ double block_sum(double v, double* sh) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (t < o) sh[t] = sh[t] + sh[t + o];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_max(double v, double* sh) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (t < o) sh[t] = fmax(sh[t], sh[t + o]);
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// Domain checks of a new state (models/surface_tissue.py:391-420, surface_kinetic.py:422-451).
+
+__device__ bool cell_domain(const Params& P, const double* y, const double* yv, int i) {
+  bool ok = true;
+  for (int f = 0; f < P.cf; ++f) ok = ok && isfinite(CFc(P, y, f, i));
+  for (int s = 0; s < P.S; ++s) {
+    double n = P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i);
+    ok = ok && isfinite(n) && n > 0;
+  }
+  if (P.water) {
+    double v = P.vol0[i] + CFc(P, yv, P.f_dvol, i);
+    ok = ok && v > P.vmin * P.vol0[i] && v < P.vmax * P.vol0[i];
+  }
+  return ok;
+}
+
+__device__ bool edge_domain(const Params& P, const double* y, int e) {
+  bool ok = true;
+  for (int f = 0; f < P.ef; ++f) ok = ok && isfinite(EFc(P, y, f, e));
+  double g = EFc(P, y, P.f_gate, e);
+  return ok && g >= P.gate_min && g <= 1.0;
+}
+
+__device__ __forceinline__ bool is_water_field(const Params& P, int f) {
+  return P.water && (f == P.f_dvol || f == P.f_wtr || f == P.f_wdiss);
+}
+
+// ------------------------------------------------------------------------------------------
+// IMEX Euler (surface_pump.py:292-362 with no faces): y + dt r, the stage potentials' residual
+// check, the domain. Threads [0, nc) take cells c0.., [nc, nc+ne) edges e0...
+
+__global__ __launch_bounds__(kBlock) void k_euler_update(Params P, double* A, double* B,
+                                                         const int* sel, int c0, int nc, int e0,
+                                                         int ne, double dt, const double* R,
+                                                         int* valid) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double h = 1.0 * dt;
+  if (t < nc) {
+    int i = c0 + t;
+    int w = P.cell_world[i];
+    const double* y = current(A, B, sel[w]);
+    double* o = next(A, B, sel[w]);
+    for (int f = 0; f < P.cf; ++f) {
+      if (is_water_field(P, f)) continue;
+      CF(P, o, f, i) = CFc(P, y, f, i) + h * CFc(P, R, f, i);
+    }
+    double q = P.q0[i] + charge_increment(P, o, i);
+    double phi = P.Vb + q / P.cap[i];
+    double cphi = P.cap[i] * (phi - P.Vb);
+    double scale = fabs(q) > fabs(cphi) ? fabs(q) : fabs(cphi);
+    bool ok = isfinite(phi) && fabs(cphi - q) <= 3e-25 + 2e-12 * scale;
+    ok = ok && cell_domain(P, o, o, i);
+    if (!ok) valid[w] = 0;
+  } else if (t < nc + ne) {
+    int e = e0 + (t - nc);
+    int w = P.edge_world[e];
+    const double* y = current(A, B, sel[w]);
+    double* o = next(A, B, sel[w]);
+    for (int f = 0; f < P.ef; ++f) EF(P, o, f, e) = EFc(P, y, f, e) + h * EFc(P, R, f, e);
+    if (!edge_domain(P, o, e)) valid[w] = 0;
+  }
+}
+
+// Per-world ledgers of an ionic step (surface_pump.py:346-359): bath supplies, reservoir work,
+// electrode charge and work. sup_h1/sup_h2 weigh the two stage supply sums (Euler: dt, 0).
+__global__ __launch_bounds__(kBlock) void k_world_ionic(Params P, double* A, double* B,
+                                                        const int* sel, int w0,
+                                                        const double* contrib1, double h1,
+                                                        const double* contrib2, double h2,
+                                                        int* valid) {
+  __shared__ double sh[kBlock];
+  int w = w0 + blockIdx.x;
+  int sw = sel[w];
+  const double* y = current(A, B, sw);
+  double* o = next(A, B, sw);
+  int c0 = P.cell_off[w], c1 = P.cell_off[w + 1];
+  double applied[4];
+  bool ok = true;
+  for (int s = 0; s < P.S; ++s) {
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+      a1 += contrib1[(long)s * P.ncell + i];
+      if (contrib2) a2 += contrib2[(long)s * P.ncell + i];
+    }
+    double s1 = block_sum(a1, sh);
+    double s2 = contrib2 ? block_sum(a2, sh) : 0.0;
+    double sup = WFc(P, y, P.f_sup + s, w);
+    double nv = contrib2 ? (sup + h1 * s1) + h2 * s2 : sup + h1 * s1;
+    applied[s] = contrib2 ? h1 * s1 + h2 * s2 : h1 * s1;
+    if (threadIdx.x == 0) WF(P, o, P.f_sup + s, w) = nv;
+    ok = ok && isfinite(nv);
+  }
+  double qs = 0.0, qe = 0.0;
+  for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    qs += charge_increment(P, y, i);
+    qe += charge_increment(P, o, i);
+  }
+  qs = block_sum(qs, sh);
+  qe = block_sum(qe, sh);
+  if (threadIdx.x == 0) {
+    double electrode = -qe - -qs;
+    double work = 0.0;
+    for (int s = 0; s < P.S; ++s) work = s == 0 ? P.mu_b[s] * applied[s] : work + P.mu_b[s] * applied[s];
+    double ec = WFc(P, y, P.f_echg, w) + electrode;
+    double rw = WFc(P, y, P.f_rwork, w) + 1e-3 * work;
+    double ew = WFc(P, y, P.f_ework, w) + 1e12 * P.Vb * electrode;
+    WF(P, o, P.f_echg, w) = ec;
+    WF(P, o, P.f_rwork, w) = rw;
+    WF(P, o, P.f_ework, w) = ew;
+    ok = ok && isfinite(ec) && isfinite(rw) && isfinite(ew);
+    if (!ok) valid[w] = 0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Water half-step (operators/osmotic_water.py:127-221): Heun over h at frozen amounts, each cell's
+// water route to its held bath. from_next = 0 reads the step's start state, 1 the new state.
+// Writes the new state's water fields. Workgroup per world.
+
+__device__ __forceinline__ double water_psi(const Params& P, const double* y, int i, double vol,
+                                            double& total) {
+  double t = 0.0;
+  for (int s = 0; s < P.S; ++s) t += P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i);
+  t = t + P.imperm[i];
+  total = t;
+  double osm = P.RT * t / vol;
+  double hydro = P.wK * (vol - P.vol0[i]) / P.vol0[i] + P.wP0;
+  return hydro - osm;
+}
+
+__global__ __launch_bounds__(kBlock) void k_water(Params P, double* A, double* B, const int* sel,
+                                                  int w0, int from_next, double h, Ctl c,
+                                                  int* valid) {
+  __shared__ double sh[kBlock];
+  int w = w0 + blockIdx.x;
+  int sw = sel[w];
+  const double* y = from_next ? next(A, B, sw) : current(A, B, sw);
+  double* o = next(A, B, sw);
+  int c0 = P.cell_off[w], c1 = P.cell_off[w + 1];
+  bool ok = isfinite(c.water) && c.water >= 0 && c.water <= 1;
+  double maxrow = 0.0, sumrow = 0.0;
+  for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    double vol = P.vol0[i] + CFc(P, y, P.f_dvol, i);
+    double total;
+    (void)water_psi(P, y, i, vol, total);
+    double slope = P.wK / P.vol0[i] + P.RT * total / (vol * vol);
+    double edge = P.lpa[i] * c.water * (slope + 0.0);
+    maxrow = fmax(maxrow, edge);
+    sumrow += edge;
+    ok = ok && vol > P.vmin * P.vol0[i] && vol < P.vmax * P.vol0[i];
+    ok = ok && isfinite(CFc(P, y, P.f_wtr, i)) && isfinite(CFc(P, y, P.f_wdiss, i));
+  }
+  maxrow = block_max(maxrow, sh);
+  sumrow = block_sum(sumrow, sh);
+  double row = fmax(maxrow, sumrow);
+  ok = ok && 2 * h * row <= kWaterMaxRelaxation;
+  double f0sum = 0.0, f1sum = 0.0;
+  for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    double dvol = CFc(P, y, P.f_dvol, i), wtr = CFc(P, y, P.f_wtr, i), wd = CFc(P, y, P.f_wdiss, i);
+    double total;
+    double vol = P.vol0[i] + dvol;
+    double drive0 = water_psi(P, y, i, vol, total) - P.psib;
+    double flow0 = P.lpa[i] * c.water * drive0;
+    double r0v = -flow0, r0t = flow0, r0h = 1e-3 * flow0 * drive0;
+    double pv = dvol + h * r0v;
+    double pvol = P.vol0[i] + pv;
+    ok = ok && pvol > P.vmin * P.vol0[i] && pvol < P.vmax * P.vol0[i] && isfinite(wtr + h * r0t) &&
+         isfinite(wd + h * r0h);
+    double drive1 = water_psi(P, y, i, pvol, total) - P.psib;
+    double flow1 = P.lpa[i] * c.water * drive1;
+    double heat1 = 1e-3 * flow1 * drive1;
+    double mv = 0.5 * (r0v + -flow1), mt = 0.5 * (r0t + flow1), mh = 0.5 * (r0h + heat1);
+    double nv = dvol + h * mv, nt = wtr + h * mt, nh = wd + h * mh;
+    CF(P, o, P.f_dvol, i) = nv;
+    CF(P, o, P.f_wtr, i) = nt;
+    CF(P, o, P.f_wdiss, i) = nh;
+    double nvol = P.vol0[i] + nv;
+    ok = ok && nvol > P.vmin * P.vol0[i] && nvol < P.vmax * P.vol0[i] && isfinite(nt) &&
+         isfinite(nh);
+    f0sum += flow0;
+    f1sum += flow1;
+  }
+  f0sum = block_sum(f0sum, sh);
+  f1sum = block_sum(f1sum, sh);
+  if (threadIdx.x == 0) {
+    double sup0 = -f0sum, sup1 = -f1sum;
+    double supm = 0.5 * (sup0 + sup1);
+    double supplied = h * supm;
+    double ws = WFc(P, y, P.f_wsup, w), ww = WFc(P, y, P.f_wwork, w);
+    WF(P, o, P.f_wsup, w) = ws + supplied;
+    WF(P, o, P.f_wwork, w) = ww + 1e-3 * P.psib * supplied;
+    ok = ok && isfinite(ws + supplied) && isfinite(ww + 1e-3 * P.psib * supplied);
+  }
+  if (!ok) valid[w] = 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Sparse Cholesky on the stage pattern, workgroup per world (numerics/sparse_cholesky.py:474-514).
+// Entry (r, k) = assembled - pairwise fold over m in rows(k) & rows(r) of L[r,m] L[k,m]; a pivot
+// is its square root, a below entry times the reciprocal of its column's pivot. The fold is the
+// reference's adjacent-pairs tree, evaluated as a binary-counter stack (the same additions).
+
+struct PlanView {
+  const int* cs;      // [n+1] column starts
+  const int* rrank;   // [nnz] entry row ranks
+  const int* crank;   // [nnz] entry column ranks
+  const int* rowptr;  // [n+1]
+  const int* rowcol;  // [nnz-n] columns of each row's strictly lower entries
+  const int* rowpos;  // [nnz-n] their positions
+  const int* levptr;  // [levels+1]
+  const int* levcol;  // [n]
+  const int* leventptr;  // [levels+1]
+  const int* levent;     // [nnz]
+  const int* order;      // [n] rank -> node
+  const int* diagpos;    // [n] node -> diagonal position
+  const int* edgepos;    // [E] world edge -> position
+};
+
+struct Pairwise {
+  double v[24];
+  int c[24];
+  int top;
+  __device__ void reset() { top = 0; }
+  __device__ void push(double t) {
+    v[top] = t;
+    c[top] = 1;
+    ++top;
+    while (top >= 2 && c[top - 1] == c[top - 2]) {
+      v[top - 2] = v[top - 2] + v[top - 1];
+      c[top - 2] *= 2;
+      --top;
+    }
+  }
+  __device__ double sum() const {
+    if (top == 0) return 0.0;
+    double s = v[top - 1];
+    for (int i = top - 2; i >= 0; --i) s = v[i] + s;
+    return s;
+  }
+};
+
+__device__ PlanView plan_view(const ShapeDev& s, const int* pool) {
+  PlanView v;
+  v.cs = pool + s.o_cs;
+  v.rrank = pool + s.o_entry;
+  v.crank = pool + s.o_entry + s.nnz;
+  v.rowptr = pool + s.o_rowptr;
+  v.rowcol = pool + s.o_rowent;
+  v.rowpos = pool + s.o_rowent + (s.nnz - s.n);
+  v.levptr = pool + s.o_lev;
+  v.leventptr = pool + s.o_lev + (s.levels + 1);
+  v.levcol = pool + s.o_levcol;
+  v.levent = pool + s.o_levent;
+  v.order = pool + s.o_node;
+  v.diagpos = pool + s.o_node + s.n;
+  v.edgepos = pool + s.o_node + 2 * s.n;
+  return v;
+}
+
+__device__ void factor_world(const ShapeDev& s, const PlanView& v, double* fv) {
+  for (int lev = 0; lev < s.levels; ++lev) {
+    int a = v.leventptr[lev], b = v.leventptr[lev + 1];
+    for (int q = a + threadIdx.x; q < b; q += blockDim.x) {
+      int p = v.levent[q];
+      int r = v.rrank[p], k = v.crank[p];
+      Pairwise pw;
+      pw.reset();
+      int x = v.rowptr[k], xe = v.rowptr[k + 1];
+      if (r == k) {
+        for (; x < xe; ++x) {
+          double t = fv[v.rowpos[x]];
+          pw.push(t * t);
+        }
+      } else {
+        int y = v.rowptr[r], ye = v.rowptr[r + 1];
+        while (x < xe && y < ye) {
+          int mk = v.rowcol[x], mr = v.rowcol[y];
+          if (mk == mr) {
+            pw.push(fv[v.rowpos[y]] * fv[v.rowpos[x]]);
+            ++x;
+            ++y;
+          } else if (mk < mr) {
+            ++x;
+          } else {
+            ++y;
+          }
+        }
+      }
+      double acc = fv[p] - pw.sum();
+      fv[p] = r == k ? sqrt(acc) : acc;
+    }
+    __syncthreads();
+    for (int q = a + threadIdx.x; q < b; q += blockDim.x) {
+      int p = v.levent[q];
+      int r = v.rrank[p], k = v.crank[p];
+      if (r != k) fv[p] = fv[p] * (1.0 / fv[v.cs[k]]);
+    }
+    __syncthreads();
+  }
+}
+
+// y (LDS, by rank) <- L^-T L^-1 y, each pivot applied as a multiplication by its reciprocal.
+__device__ void solve_world(const ShapeDev& s, const PlanView& v, const double* fv, double* y) {
+  for (int lev = 0; lev < s.levels; ++lev) {
+    for (int q = v.levptr[lev] + threadIdx.x; q < v.levptr[lev + 1]; q += blockDim.x) {
+      int k = v.levcol[q];
+      Pairwise pw;
+      pw.reset();
+      for (int x = v.rowptr[k]; x < v.rowptr[k + 1]; ++x) pw.push(fv[v.rowpos[x]] * y[v.rowcol[x]]);
+      double known = y[k] - pw.sum();
+      y[k] = known * (1.0 / fv[v.cs[k]]);
+    }
+    __syncthreads();
+  }
+  for (int lev = s.levels - 1; lev >= 0; --lev) {
+    for (int q = v.levptr[lev] + threadIdx.x; q < v.levptr[lev + 1]; q += blockDim.x) {
+      int k = v.levcol[q];
+      Pairwise pw;
+      pw.reset();
+      for (int p = v.cs[k] + 1; p < v.cs[k + 1]; ++p) pw.push(fv[p] * y[v.rrank[p]]);
+      double known = y[k] - pw.sum();
+      y[k] = known * (1.0 / fv[v.cs[k]]);
+    }
+    __syncthreads();
+  }
+}
+
+// Assemble (Jacobi-scaled, symmetrised; numerics/surface_imex.py:228-259), factor, and solve
+// (C + coef G) dphi = q(K) for each world: rhs' = q s, dphi = x s.
+__global__ __launch_bounds__(kBlock) void k_factor_solve(Params P, const ShapeDev* shapes,
+                                                         const int* shape_of, const int* pool,
+                                                         const long* foff, int w0, int assemble,
+                                                         double coef, const double* mdiag,
+                                                         const double* msc, const double* Gj,
+                                                         double* fvals, const double* K,
+                                                         double* dphi, int* valid) {
+  extern __shared__ double lds[];
+  int w = w0 + blockIdx.x;
+  const ShapeDev s = shapes[shape_of[w]];
+  const PlanView v = plan_view(s, pool);
+  double* fv = fvals + foff[w];
+  int c0 = P.cell_off[w], e0 = P.edge_off[w];
+  if (assemble) {
+    for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < s.n; i += blockDim.x) fv[v.diagpos[i]] = mdiag[c0 + i];
+    for (int e = threadIdx.x; e < s.E; e += blockDim.x) {
+      double val = -(coef * Gj[e0 + e]);
+      double sa = msc[P.snd[e0 + e]], sb = msc[P.rcv[e0 + e]];
+      fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
+    }
+    __syncthreads();
+    factor_world(s, v, fv);
+    bool ok = true;
+    for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) ok = ok && isfinite(fv[p]);
+    if (!ok) valid[w] = 0;
+  }
+  double* y = lds;
+  for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
+    int i = c0 + v.order[k];
+    y[k] = charge_increment(P, K, i) * msc[i];
+  }
+  __syncthreads();
+  solve_world(s, v, fv, y);
+  for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
+    int i = c0 + v.order[k];
+    dphi[i] = y[k] * msc[i];
+  }
+}
+
+// Test entry: factor given assembled values in place, per world.
+__global__ __launch_bounds__(kBlock) void k_factor_only(const ShapeDev* shapes, const int* shape_of,
+                                                        const int* pool, const long* foff,
+                                                        double* fvals) {
+  int w = blockIdx.x;
+  const ShapeDev s = shapes[shape_of[w]];
+  const PlanView v = plan_view(s, pool);
+  factor_world(s, v, fvals + foff[w]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_solve_only(const ShapeDev* shapes, const int* shape_of,
+                                                       const int* pool, const long* foff,
+                                                       const int* cell_off, const double* fvals,
+                                                       const double* rhs, double* x) {
+  extern __shared__ double lds[];
+  int w = blockIdx.x;
+  const ShapeDev s = shapes[shape_of[w]];
+  const PlanView v = plan_view(s, pool);
+  int c0 = cell_off[w];
+  for (int k = threadIdx.x; k < s.n; k += blockDim.x) lds[k] = rhs[c0 + v.order[k]];
+  __syncthreads();
+  solve_world(s, v, fvals + foff[w], lds);
+  for (int k = threadIdx.x; k < s.n; k += blockDim.x) x[c0 + v.order[k]] = lds[k];
+}
+
+// ------------------------------------------------------------------------------------------
+// ROS2 stage combinations (the W-method's k = F + coef (df/dphi) dphi on the route fields).
+
+// Adds coef (df/dphi) dphi to K's transfers and amounts (and the cell's supply contribution).
+__device__ void correct_cell(const Params& P, int i, double coef, const double* gm, const double* gj,
+                             const double* dphi, double* K, double* contrib) {
+  int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
+  for (int s = 0; s < P.S; ++s) {
+    double dm = coef * (gm[(long)s * P.ncell + i] * dphi[i]);
+    double out = 0.0, in = 0.0;
+    for (int p = p0; p < pm; ++p) {
+      int e = P.inc_edge[p];
+      out += -(coef * (gj[(long)s * P.nedge + e] * (dphi[P.snd[e]] - dphi[P.rcv[e]])));
+    }
+    for (int p = pm; p < p1; ++p) {
+      int e = P.inc_edge[p];
+      in += coef * (gj[(long)s * P.nedge + e] * (dphi[P.snd[e]] - dphi[P.rcv[e]]));
+    }
+    double cell = -dm;
+    cell += out;
+    cell += in;
+    CF(P, K, P.f_dn + s, i) += cell;
+    CF(P, K, P.f_mtr + s, i) += dm;
+    contrib[(long)s * P.ncell + i] = contrib[(long)s * P.ncell + i] - dm;
+  }
+}
+
+__device__ void correct_edge(const Params& P, int e, double coef, const double* gj, const double* dphi,
+                             double* K) {
+  double d = dphi[P.snd[e]] - dphi[P.rcv[e]];
+  for (int s = 0; s < P.S; ++s) EF(P, K, P.f_jtr + s, e) += coef * (gj[(long)s * P.nedge + e] * d);
+}
+
+// Stage 1: K1 += correction(dphi1); Y1 = y + dt K1.
+__global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* A, const double* B,
+                                                        const int* sel, int c0, int nc, int e0,
+                                                        int ne, double dt, double coef,
+                                                        const double* gm, const double* gj,
+                                                        const double* dphi, double* K1,
+                                                        double* contrib1, double* Y1) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nc) {
+    int i = c0 + t;
+    int sw = sel[P.cell_world[i]];
+    const double* y = current(A, B, sw);
+    const double* yv = P.water ? next(const_cast<double*>(A), const_cast<double*>(B), sw) : y;
+    correct_cell(P, i, coef, gm, gj, dphi, K1, contrib1);
+    for (int f = 0; f < P.cf; ++f) {
+      if (is_water_field(P, f)) {
+        CF(P, Y1, f, i) = CFc(P, yv, f, i);
+        continue;
+      }
+      CF(P, Y1, f, i) = CFc(P, y, f, i) + dt * CFc(P, K1, f, i);
+    }
+  } else if (t < nc + ne) {
+    int e = e0 + (t - nc);
+    const double* y = current(A, B, sel[P.edge_world[e]]);
+    correct_edge(P, e, coef, gj, dphi, K1);
+    for (int f = 0; f < P.ef; ++f) EF(P, Y1, f, e) = EFc(P, y, f, e) + dt * EFc(P, K1, f, e);
+  }
+}
+
+// K2 <- K2 - 2 K1 (the rates at Y1 less twice the first stage), contributions likewise.
+__global__ __launch_bounds__(kBlock) void k_ros2_stage2(Params P, int c0, int nc, int e0, int ne,
+                                                        const double* K1, const double* contrib1,
+                                                        double* K2, double* contrib2) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nc) {
+    int i = c0 + t;
+    for (int f = 0; f < P.cf; ++f) CF(P, K2, f, i) = CFc(P, K2, f, i) + -2.0 * CFc(P, K1, f, i);
+    for (int s = 0; s < P.S; ++s)
+      contrib2[(long)s * P.ncell + i] =
+          contrib2[(long)s * P.ncell + i] + -2.0 * contrib1[(long)s * P.ncell + i];
+  } else if (t < nc + ne) {
+    int e = e0 + (t - nc);
+    for (int f = 0; f < P.ef; ++f) EF(P, K2, f, e) = EFc(P, K2, f, e) + -2.0 * EFc(P, K1, f, e);
+  }
+}
+
+// Final: K2 += correction(dphi2); y_new = (y + 1.5 dt K1) + 0.5 dt K2; domain.
+__global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, double* B,
+                                                       const int* sel, int c0, int nc, int e0,
+                                                       int ne, double dt, double coef,
+                                                       const double* gm, const double* gj,
+                                                       const double* dphi, const double* K1,
+                                                       double* K2, double* contrib2, int* valid) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double h1 = 1.5 * dt, h2 = 0.5 * dt;
+  if (t < nc) {
+    int i = c0 + t;
+    int w = P.cell_world[i];
+    const double* y = current(A, B, sel[w]);
+    double* o = next(A, B, sel[w]);
+    correct_cell(P, i, coef, gm, gj, dphi, K2, contrib2);
+    for (int f = 0; f < P.cf; ++f) {
+      if (is_water_field(P, f)) continue;
+      CF(P, o, f, i) = (CFc(P, y, f, i) + h1 * CFc(P, K1, f, i)) + h2 * CFc(P, K2, f, i);
+    }
+    if (!cell_domain(P, o, o, i)) valid[w] = 0;
+  } else if (t < nc + ne) {
+    int e = e0 + (t - nc);
+    int w = P.edge_world[e];
+    const double* y = current(A, B, sel[w]);
+    double* o = next(A, B, sel[w]);
+    correct_edge(P, e, coef, gj, dphi, K2);
+    for (int f = 0; f < P.ef; ++f)
+      EF(P, o, f, e) = (EFc(P, y, f, e) + h1 * EFc(P, K1, f, e)) + h2 * EFc(P, K2, f, e);
+    if (!edge_domain(P, o, e)) valid[w] = 0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Acceptance bookkeeping.
+
+__global__ void k_init(int W, const int* active, int* sel, int* alive, int* valid) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  sel[w] = 0;
+  alive[w] = active ? (active[w] != 0) : 1;
+  valid[w] = 1;
+}
+
+__global__ void k_commit(int w0, int nw, int* sel, int* alive, int* valid) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nw) return;
+  int w = w0 + t;
+  if (alive[w]) {
+    if (valid[w]) sel[w] ^= 1;
+    else alive[w] = 0;
+  }
+  valid[w] = 1;
+}
+
+// out[field, item] = the selected buffer's value; threads over cells, edges, worlds.
+__global__ void k_gather(Params P, const double* A, const double* B, const int* sel, double* out,
+                         const int* alive, int* ok_out) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < P.ncell) {
+    int i = (int)t;
+    const double* y = current(A, B, sel[P.cell_world[i]]);
+    for (int f = 0; f < P.cf; ++f) CF(P, out, f, i) = CFc(P, y, f, i);
+  } else if (t < (long)P.ncell + P.nedge) {
+    int e = (int)(t - P.ncell);
+    const double* y = current(A, B, sel[P.edge_world[e]]);
+    for (int f = 0; f < P.ef; ++f) EF(P, out, f, e) = EFc(P, y, f, e);
+  } else if (t < (long)P.ncell + P.nedge + P.W) {
+    int w = (int)(t - P.ncell - P.nedge);
+    const double* y = current(A, B, sel[w]);
+    for (int f = 0; f < P.wf; ++f) WF(P, out, f, w) = WFc(P, y, f, w);
+    if (ok_out) ok_out[w] = alive[w];
+  }
+}
+
+// World supply rates for cells_rates.
+__global__ __launch_bounds__(kBlock) void k_world_rates(Params P, const double* contrib, double* R) {
+  __shared__ double sh[kBlock];
+  int w = blockIdx.x;
+  int c0 = P.cell_off[w], c1 = P.cell_off[w + 1];
+  for (int s = 0; s < P.S; ++s) {
+    double a = 0.0;
+    for (int i = c0 + threadIdx.x; i < c1; i += kBlock) a += contrib[(long)s * P.ncell + i];
+    a = block_sum(a, sh);
+    if (threadIdx.x == 0) WF(P, R, P.f_sup + s, w) = a;
+  }
+  if (threadIdx.x == 0)
+    for (int f = P.S; f < P.wf; ++f) WF(P, R, f, w) = 0.0;
+}
+
+int blocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ==========================================================================================
+// Host side.
+
+struct cells_symbolic {
+  cells::Symbolic s;
+};
+
+struct cells_ensemble {
+  int device = 0;
+  int method = 0;
+  Params P{};
+  int S = 0, W = 0, ncell = 0, nedge = 0, max_n = 0;
+  std::vector<int64_t> cell_off, edge_off;
+  std::vector<int> shape_of_host;
+  std::vector<cells::Symbolic> shapes_host;
+  std::vector<long> foff_host;
+  std::vector<ShapeDev> sd_host;
+  std::vector<int> pool_host;
+  long factor_total = 0;
+  // device allocations
+  std::vector<void*> allocations;
+  ShapeDev* shapes = nullptr;
+  int* shape_of = nullptr;
+  int* pool = nullptr;
+  long* foff = nullptr;
+  double *A = nullptr, *B = nullptr, *Y1 = nullptr, *K1 = nullptr, *K2 = nullptr;
+  double *contrib1 = nullptr, *contrib2 = nullptr, *gm = nullptr, *gj = nullptr, *Gj = nullptr;
+  double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *fvals = nullptr;
+  int *sel = nullptr, *alive = nullptr, *valid = nullptr;
+  hipStream_t capture_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  double graph_dt = 0.0;
+  Ctl graph_ctl{};
+  ~cells_ensemble() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+    for (void* p : allocations) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+template <typename T>
+int upload(cells_ensemble* e, T** dst, const T* src, size_t count) {
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return fail(CELLS_ENOMEM, "hipMalloc failed at set-up");
+  e->allocations.push_back(p);
+  if (src && count) HIP_TRY(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  *dst = static_cast<T*>(p);
+  return CELLS_OK;
+}
+
+template <typename T>
+int alloc(cells_ensemble* e, T** dst, size_t count) {
+  return upload<T>(e, dst, nullptr, count);
+}
+
+std::vector<int> narrow(const int64_t* a, size_t n) {
+  std::vector<int> out(n);
+  for (size_t i = 0; i < n; ++i) out[i] = static_cast<int>(a[i]);
+  return out;
+}
+
+double host_logaddexp0(double t) {
+  double m = t > 0.0 ? t : 0.0;
+  return m + std::log1p(std::exp(-std::fabs(0.0 - t)));
+}
+
+int check_world(const cells_world* w) {
+  if (!w) return fail(CELLS_EINVAL, "world is NULL");
+  if (w->S < 3 || w->S > 4) return fail(CELLS_EINVAL, "species count must be 3 or 4");
+  if (w->calcium && w->S != 4) return fail(CELLS_EINVAL, "the Ca-ATPase needs species 3 (Ca)");
+  if (w->W < 1) return fail(CELLS_EINVAL, "need at least one world");
+  if (w->ncell < 1 || w->ncell > (int64_t)1 << 30 || w->nedge < 0 || w->nedge > (int64_t)1 << 30)
+    return fail(CELLS_EINVAL, "cell or edge count out of range");
+  if (w->cell_off[0] != 0 || w->cell_off[w->W] != w->ncell || w->edge_off[0] != 0 ||
+      w->edge_off[w->W] != w->nedge)
+    return fail(CELLS_EINVAL, "world offsets disagree with the counts");
+  for (int k = 0; k < w->W; ++k) {
+    if (w->cell_off[k + 1] <= w->cell_off[k]) return fail(CELLS_EINVAL, "a world has no cells");
+    if (w->edge_off[k + 1] < w->edge_off[k]) return fail(CELLS_EINVAL, "edge offsets decrease");
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+      int64_t a = w->snd[e], b = w->rcv[e];
+      if (a < w->cell_off[k] || a >= w->cell_off[k + 1] || b < w->cell_off[k] ||
+          b >= w->cell_off[k + 1] || a == b)
+        return fail(CELLS_EINVAL, "an edge leaves its world or joins a cell to itself");
+    }
+  }
+  if (w->inc_ptr[0] != 0 || w->inc_ptr[w->ncell] != 2 * w->nedge)
+    return fail(CELLS_EINVAL, "incidence lists disagree with the edges");
+  for (int64_t i = 0; i < w->ncell; ++i) {
+    int64_t p0 = w->inc_ptr[i], p1 = w->inc_ptr[i + 1];
+    if (p1 < p0 || w->inc_out[i] < 0 || w->inc_out[i] > p1 - p0)
+      return fail(CELLS_EINVAL, "malformed incidence list");
+    for (int64_t p = p0; p < p1; ++p) {
+      int64_t e = w->inc_edge[p];
+      if (e < 0 || e >= w->nedge) return fail(CELLS_EINVAL, "incidence names an unknown edge");
+      bool out = p < p0 + w->inc_out[i];
+      if ((out && w->snd[e] != i) || (!out && w->rcv[e] != i))
+        return fail(CELLS_EINVAL, "incidence list disagrees with the edge's ends");
+    }
+  }
+  return CELLS_OK;
+}
+
+void fill_layout(Params& P, const cells_world* w) {
+  int S = w->S, f = 0;
+  P.f_dn = f; f += S;
+  P.f_mtr = f; f += S;
+  P.f_mdiss = f++;
+  P.f_pcyc = f++;
+  P.f_pdiss = f++;
+  P.f_pwork = f++;
+  P.f_ccyc = P.f_cdiss = P.f_cwork = -1;
+  if (w->calcium) { P.f_ccyc = f++; P.f_cdiss = f++; P.f_cwork = f++; }
+  P.f_dvol = P.f_wtr = P.f_wdiss = -1;
+  if (w->water) { P.f_dvol = f++; P.f_wtr = f++; P.f_wdiss = f++; }
+  P.cf = f;
+  P.f_jtr = 0;
+  P.f_jdiss = S;
+  P.f_gate = S + 1;
+  P.ef = S + 2;
+  f = 0;
+  P.f_sup = f; f += S;
+  P.f_echg = f++;
+  P.f_rwork = f++;
+  P.f_ework = f++;
+  P.f_wsup = P.f_wwork = -1;
+  if (w->water) { P.f_wsup = f++; P.f_wwork = f++; }
+  P.wf = f;
+  P.edge_base = (long)P.cf * w->ncell;
+  P.world_base = P.edge_base + (long)P.ef * w->nedge;
+  P.total = P.world_base + (long)P.wf * w->W;
+}
+
+// The lattice pattern of world k in local indices.
+void world_pattern(const cells_world* w, int k, std::vector<int32_t>& rows, std::vector<int32_t>& cols) {
+  int64_t c0 = w->cell_off[k];
+  rows.clear();
+  cols.clear();
+  for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+    rows.push_back(static_cast<int32_t>(w->snd[e] - c0));
+    cols.push_back(static_cast<int32_t>(w->rcv[e] - c0));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cells_last_error(void) { return g_error.c_str(); }
+
+int cells_minimum_degree(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols,
+                         int32_t* order_out) {
+  try {
+    if (n < 1) return fail(CELLS_EINVAL, "a sparse pattern needs a positive size");
+    auto order = cells::minimum_degree(n, m, rows, cols);
+    std::copy(order.begin(), order.end(), order_out);
+    return CELLS_OK;
+  } catch (const std::exception& ex) {
+    return fail(CELLS_EINVAL, ex.what());
+  }
+}
+
+int cells_bind(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols,
+               const int32_t* order, cells_symbolic** out) {
+  try {
+    auto* s = new cells_symbolic;
+    s->s = cells::bind(n, m, rows, cols, order);
+    *out = s;
+    return CELLS_OK;
+  } catch (const std::exception& ex) {
+    return fail(CELLS_EINVAL, ex.what());
+  }
+}
+
+int cells_symbolic_sizes(const cells_symbolic* s, int64_t* nnz, int32_t* levels) {
+  if (!s) return fail(CELLS_EINVAL, "symbolic is NULL");
+  if (nnz) *nnz = s->s.nnz();
+  if (levels) *levels = s->s.levels();
+  return CELLS_OK;
+}
+
+int cells_symbolic_export(const cells_symbolic* sym, int32_t* order, int32_t* rank,
+                          int64_t* column_start, int32_t* row_node, int32_t* column_node,
+                          int32_t* parent, int32_t* level) {
+  if (!sym) return fail(CELLS_EINVAL, "symbolic is NULL");
+  const cells::Symbolic& s = sym->s;
+  if (order) std::copy(s.order.begin(), s.order.end(), order);
+  if (rank) std::copy(s.rank.begin(), s.rank.end(), rank);
+  if (column_start) std::copy(s.column_start.begin(), s.column_start.end(), column_start);
+  for (int64_t p = 0; p < s.nnz(); ++p) {
+    if (row_node) row_node[p] = s.order[s.row_rank[p]];
+    if (column_node) column_node[p] = s.order[s.col_rank[p]];
+  }
+  if (parent) std::copy(s.parent.begin(), s.parent.end(), parent);
+  if (level) std::copy(s.level.begin(), s.level.end(), level);
+  return CELLS_OK;
+}
+
+void cells_symbolic_free(cells_symbolic* s) { delete s; }
+
+int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
+                          cells_ensemble** out) {
+  int rc = check_world(w);
+  if (rc) return rc;
+  if (method != CELLS_METHOD_EULER && method != CELLS_METHOD_ROS2)
+    return fail(CELLS_EINVAL, "unknown method");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(CELLS_ENODEV, "no GPU visible: the HIP step has no CPU fallback");
+  if (device < 0 || device >= count) return fail(CELLS_EINVAL, "device out of range");
+  HIP_TRY(hipSetDevice(device));
+  std::unique_ptr<cells_ensemble> e(new cells_ensemble);
+  e->device = device;
+  e->method = method;
+  e->S = w->S;
+  e->W = w->W;
+  e->ncell = (int)w->ncell;
+  e->nedge = (int)w->nedge;
+  e->cell_off.assign(w->cell_off, w->cell_off + w->W + 1);
+  e->edge_off.assign(w->edge_off, w->edge_off + w->W + 1);
+  Params& P = e->P;
+  P.S = w->S;
+  P.W = w->W;
+  P.water = w->water;
+  P.calcium = w->calcium;
+  P.ncell = e->ncell;
+  P.nedge = e->nedge;
+  const double T = w->temperature, RT = kR * T;
+  for (int s = 0; s < 4; ++s) {
+    P.z[s] = s < w->S ? w->valence[s] : 0.0;
+    P.cb[s] = s < w->S ? w->bath_c[s] : 1.0;
+    P.log_cb[s] = std::log(P.cb[s]);
+    P.mu_b[s] = RT * std::log(P.cb[s]) + P.z[s] * kF * w->bath_V;
+  }
+  P.T = T;
+  P.RT = RT;
+  P.Vb = w->bath_V;
+  P.nak_drive = w->dg0 + kR * T * (std::log(w->adp) + std::log(w->pi) - std::log(w->atp) - std::log(1000.0));
+  P.log_km_na = std::log(w->km_na);
+  P.log_km_k = std::log(w->km_k);
+  P.nak_atp_sat = host_logaddexp0(std::log(w->km_atp) - std::log(w->atp));
+  P.dg0_over_rt_nak_terms = w->dg0 / RT + std::log(w->adp) + std::log(w->pi) - std::log(w->atp) - std::log(1000.0);
+  double dg0k = w->ca_dg0 - kR * T * std::log(1000.0);
+  P.ca_dk = dg0k + kR * T * (std::log(w->adp) + std::log(w->pi) - std::log(w->atp));
+  P.ca_drive = w->ca_dg0 + kR * T * (std::log(w->adp) + std::log(w->pi) - std::log(w->atp) - std::log(1000.0));
+  P.log_ca_km = std::log(w->ca_km_ca);
+  P.ca_atp_sat = host_logaddexp0(std::log(w->ca_km_atp) - std::log(w->atp));
+  P.ca_aff_base = w->ca_dg0 / RT + std::log(w->adp) + std::log(w->pi) - std::log(w->atp) - std::log(1000.0);
+  P.gate_thr = w->gate_threshold;
+  P.gate_min = w->gate_minimum;
+  P.log_0065 = std::log(0.065);
+  P.wK = w->water_K;
+  P.wP0 = w->water_P0;
+  P.vmin = w->water_vmin;
+  P.vmax = w->water_vmax;
+  {
+    double total = 0.0;
+    for (int s = 0; s < w->S; ++s) total += w->bath_c[s];
+    P.psib = w->bath_pressure - kR * T * total;
+  }
+  fill_layout(P, w);
+  if (!(T >= 200.0 && T <= 400.0)) return fail(CELLS_EINVAL, "temperature outside 200-400 K");
+  if (!(w->gate_threshold >= 0 && w->gate_threshold <= kGateMaxVoltage && w->gate_minimum >= 0 &&
+        w->gate_minimum <= 1))
+    return fail(CELLS_EINVAL, "junction gate parameters outside their domain");
+  // per-cell and per-edge arrays
+  std::vector<int> cell_world(e->ncell), edge_world(e->nedge);
+  for (int k = 0; k < e->W; ++k) {
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) cell_world[i] = k;
+    for (int64_t q = w->edge_off[k]; q < w->edge_off[k + 1]; ++q) edge_world[q] = k;
+    e->max_n = std::max<int>(e->max_n, (int)(w->cell_off[k + 1] - w->cell_off[k]));
+  }
+  auto co = narrow(w->cell_off, w->W + 1), eo = narrow(w->edge_off, w->W + 1);
+  auto snd = narrow(w->snd, e->nedge), rcv = narrow(w->rcv, e->nedge);
+  auto iptr = narrow(w->inc_ptr, e->ncell + 1), iedge = narrow(w->inc_edge, 2 * (size_t)e->nedge);
+  int* d_i = nullptr;
+  double* d_d = nullptr;
+  size_t nc = e->ncell, ne = e->nedge, S = w->S;
+#define UP_I(field, vec) if ((rc = upload(e.get(), &d_i, vec.data(), vec.size()))) return rc; P.field = d_i;
+#define UP_D(field, src, count) if ((rc = upload(e.get(), &d_d, src, count))) return rc; P.field = d_d;
+  UP_I(cell_off, co);
+  UP_I(edge_off, eo);
+  UP_I(cell_world, cell_world);
+  UP_I(edge_world, edge_world);
+  UP_I(snd, snd);
+  UP_I(rcv, rcv);
+  UP_I(inc_ptr, iptr);
+  UP_I(inc_edge, iedge);
+  if ((rc = upload(e.get(), &d_i, w->inc_out, nc))) return rc;
+  P.inc_out = d_i;
+  UP_D(vol0, w->vol0, nc);
+  UP_D(cap, w->cap, nc);
+  UP_D(pam, w->pam, S * nc);
+  UP_D(nak_cap, w->nak_cap, nc);
+  UP_D(ca_cap, w->ca_cap, nc);
+  UP_D(n0, w->n0, S * nc);
+  UP_D(q0, w->q0, nc);
+  UP_D(lpa, w->lpa, nc);
+  UP_D(imperm, w->imperm, nc);
+  UP_D(paj, w->paj, S * ne);
+#undef UP_I
+#undef UP_D
+  // workspace
+  long total = P.total;
+  if ((rc = alloc(e.get(), &e->A, total)) || (rc = alloc(e.get(), &e->B, total)) ||
+      (rc = alloc(e.get(), &e->K1, total)) || (rc = alloc(e.get(), &e->contrib1, S * nc)) ||
+      (rc = alloc(e.get(), &e->sel, e->W)) || (rc = alloc(e.get(), &e->alive, e->W)) ||
+      (rc = alloc(e.get(), &e->valid, e->W)))
+    return rc;
+  if (method == CELLS_METHOD_ROS2) {
+    if (e->max_n > 8192) return fail(CELLS_EINVAL, "a world over 8192 cells exceeds the solve's LDS vector");
+    if ((rc = alloc(e.get(), &e->Y1, total)) || (rc = alloc(e.get(), &e->K2, total)) ||
+        (rc = alloc(e.get(), &e->contrib2, S * nc)) || (rc = alloc(e.get(), &e->gm, S * nc)) ||
+        (rc = alloc(e.get(), &e->gj, std::max<size_t>(S * ne, 1))) ||
+        (rc = alloc(e.get(), &e->Gj, std::max<size_t>(ne, 1))) || (rc = alloc(e.get(), &e->mdiag, nc)) ||
+        (rc = alloc(e.get(), &e->msc, nc)) || (rc = alloc(e.get(), &e->dphi, nc)))
+      return rc;
+  }
+  // symbolic analysis once per lattice shape (worlds with the same cell count share it)
+  {
+    std::map<int, int> by_n;
+    std::vector<int> rep;  // a representative world per shape
+    e->shape_of_host.resize(e->W);
+    for (int k = 0; k < e->W; ++k) {
+      int n = (int)(w->cell_off[k + 1] - w->cell_off[k]);
+      auto it = by_n.find(n);
+      if (it == by_n.end()) {
+        it = by_n.emplace(n, (int)rep.size()).first;
+        rep.push_back(k);
+      }
+      e->shape_of_host[k] = it->second;
+    }
+    // the shape is keyed by n: verify every world's pattern equals its representative's
+    for (int k = 0; k < e->W; ++k) {
+      int r = rep[e->shape_of_host[k]];
+      int64_t ek = w->edge_off[k + 1] - w->edge_off[k], er = w->edge_off[r + 1] - w->edge_off[r];
+      if (ek != er) return fail(CELLS_EINVAL, "worlds of one cell count with different edges");
+      for (int64_t q = 0; q < ek; ++q)
+        if (w->snd[w->edge_off[k] + q] - w->cell_off[k] != w->snd[w->edge_off[r] + q] - w->cell_off[r] ||
+            w->rcv[w->edge_off[k] + q] - w->cell_off[k] != w->rcv[w->edge_off[r] + q] - w->cell_off[r])
+          return fail(CELLS_EINVAL, "worlds of one cell count with different lattices");
+    }
+    e->shapes_host.resize(rep.size());
+    std::atomic<int> next_shape{0};
+    std::atomic<int> bad{0};
+    std::string bad_msg;
+    unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool_threads;
+    for (unsigned t = 0; t < hw; ++t)
+      pool_threads.emplace_back([&]() {
+        std::vector<int32_t> rows, cols;
+        for (int q; (q = next_shape.fetch_add(1)) < (int)rep.size();) {
+          try {
+            world_pattern(w, rep[q], rows, cols);
+            int n = (int)(w->cell_off[rep[q] + 1] - w->cell_off[rep[q]]);
+            e->shapes_host[q] = cells::bind(n, (int64_t)rows.size(), rows.data(), cols.data(), nullptr);
+          } catch (const std::exception& ex) {
+            bad = 1;
+            bad_msg = ex.what();
+          }
+        }
+      });
+    for (auto& t : pool_threads) t.join();
+    if (bad) return fail(CELLS_EINVAL, bad_msg);
+    // device plans, concatenated
+    std::vector<ShapeDev> sd(rep.size());
+    std::vector<int> pool;
+    for (size_t q = 0; q < rep.size(); ++q) {
+      const cells::Symbolic& s = e->shapes_host[q];
+      ShapeDev d{};
+      d.n = s.n;
+      d.nnz = (int)s.nnz();
+      d.levels = s.levels();
+      int k = rep[q];
+      d.E = (int)(w->edge_off[k + 1] - w->edge_off[k]);
+      d.o_cs = (long)pool.size();
+      for (int64_t v : s.column_start) pool.push_back((int)v);
+      d.o_entry = (long)pool.size();
+      pool.insert(pool.end(), s.row_rank.begin(), s.row_rank.end());
+      pool.insert(pool.end(), s.col_rank.begin(), s.col_rank.end());
+      d.o_rowptr = (long)pool.size();
+      for (int64_t v : s.row_ptr) pool.push_back((int)v);
+      d.o_rowent = (long)pool.size();
+      pool.insert(pool.end(), s.row_col.begin(), s.row_col.end());
+      pool.insert(pool.end(), s.row_pos.begin(), s.row_pos.end());
+      d.o_lev = (long)pool.size();
+      pool.insert(pool.end(), s.level_ptr.begin(), s.level_ptr.end());
+      // entries grouped by level: each level's columns in order, each column's entries in order
+      std::vector<int> lev_ent_ptr(d.levels + 1, 0), lev_ent;
+      lev_ent.reserve(d.nnz);
+      for (int v = 0; v < d.levels; ++v) {
+        for (int x = s.level_ptr[v]; x < s.level_ptr[v + 1]; ++x) {
+          int col = s.level_col[x];
+          for (int64_t p = s.column_start[col]; p < s.column_start[col + 1]; ++p) lev_ent.push_back((int)p);
+        }
+        lev_ent_ptr[v + 1] = (int)lev_ent.size();
+      }
+      pool.insert(pool.end(), lev_ent_ptr.begin(), lev_ent_ptr.end());
+      d.o_levcol = (long)pool.size();
+      pool.insert(pool.end(), s.level_col.begin(), s.level_col.end());
+      d.o_levent = (long)pool.size();
+      pool.insert(pool.end(), lev_ent.begin(), lev_ent.end());
+      d.o_node = (long)pool.size();
+      pool.insert(pool.end(), s.order.begin(), s.order.end());
+      for (int i = 0; i < s.n; ++i) pool.push_back((int)s.column_start[s.rank[i]]);
+      std::vector<int32_t> rows, cols;
+      world_pattern(w, k, rows, cols);
+      for (size_t q2 = 0; q2 < rows.size(); ++q2) pool.push_back((int)cells::position(s, rows[q2], cols[q2]));
+      sd[q] = d;
+    }
+    if (pool.size() > (size_t)0x7fffffff) return fail(CELLS_EINVAL, "plans exceed 2^31 entries");
+    e->sd_host = sd;
+    e->pool_host = pool;
+    e->foff_host.resize(e->W + 1);
+    e->foff_host[0] = 0;
+    for (int k = 0; k < e->W; ++k) e->foff_host[k + 1] = e->foff_host[k] + sd[e->shape_of_host[k]].nnz;
+    e->factor_total = e->foff_host[e->W];
+    if ((rc = upload(e.get(), &e->shapes, sd.data(), sd.size())) ||
+        (rc = upload(e.get(), &e->shape_of, e->shape_of_host.data(), e->shape_of_host.size())) ||
+        (rc = upload(e.get(), &e->pool, pool.data(), pool.size())) ||
+        (rc = upload(e.get(), &e->foff, e->foff_host.data(), e->foff_host.size())) ||
+        (rc = alloc(e.get(), &e->fvals, std::max<long>(e->factor_total, 1))))
+      return rc;
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
+  *out = e.release();
+  return CELLS_OK;
+}
+
+void cells_ensemble_destroy(cells_ensemble* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  (void)hipDeviceSynchronize();
+  delete e;
+}
+
+int64_t cells_state_size(const cells_ensemble* e) { return e ? e->P.total : -1; }
+int64_t cells_factor_size(const cells_ensemble* e) { return e ? e->factor_total : -1; }
+
+int cells_world_positions(const cells_ensemble* e, int32_t w, int64_t* diag_pos, int64_t* edge_pos,
+                          int64_t* factor_offset) {
+  if (!e || w < 0 || w >= e->W) return fail(CELLS_EINVAL, "world out of range");
+  const ShapeDev& d = e->sd_host[e->shape_of_host[w]];
+  const int* node = e->pool_host.data() + d.o_node;
+  if (diag_pos)
+    for (int i = 0; i < d.n; ++i) diag_pos[i] = node[d.n + i];
+  if (edge_pos)
+    for (int q = 0; q < d.E; ++q) edge_pos[q] = node[2 * d.n + q];
+  if (factor_offset) *factor_offset = e->foff_host[w];
+  return CELLS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One step's launches. world = -1: every world in each launch (batched); otherwise world k only.
+// Records an event after each launch when timing (never under graph capture).
+struct KernelTimer {
+  std::vector<const char*> names;
+  std::vector<hipEvent_t> events;
+  hipStream_t stream = nullptr;
+  void mark(const char* name) {
+    hipEvent_t ev;
+    (void)hipEventCreate(&ev);
+    (void)hipEventRecord(ev, stream);
+    names.push_back(name);
+    events.push_back(ev);
+  }
+  ~KernelTimer() {
+    for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
+  }
+};
+
+int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int world,
+                KernelTimer* tm = nullptr) {
+  auto mark = [&](const char* name) {
+    if (tm) tm->mark(name);
+  };
+  const Params& P = e->P;
+  int w0 = world < 0 ? 0 : world;
+  int nw = world < 0 ? e->W : 1;
+  int c0 = (int)e->cell_off[w0], c1 = (int)e->cell_off[w0 + nw];
+  int ed0 = (int)e->edge_off[w0], ed1 = (int)e->edge_off[w0 + nw];
+  int nc = c1 - c0, ne = ed1 - ed0;
+  if (P.water) {
+    hipLaunchKernelGGL(k_water, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, 0, 0.5 * dt, c, e->valid);
+    mark("k_water");
+  }
+  int vol_next = P.water ? 1 : 0;
+  if (e->method == CELLS_METHOD_EULER) {
+    if (ne) {
+      hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+                         ed0, ed1, c, e->K1, nullptr, nullptr, e->valid);
+      mark("k_edges");
+    }
+    hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+                       c0, c1, c, e->K1, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+    mark("k_cells");
+    hipLaunchKernelGGL(k_euler_update, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
+                       ed0, ne, dt, e->K1, e->valid);
+    mark("k_euler_update");
+    hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, e->contrib1,
+                       1.0 * dt, nullptr, 0.0, e->valid);
+    mark("k_world_ionic");
+  } else {
+    const double gamma = 1.0 + 1.0 / std::sqrt(2.0);
+    const double coef = gamma * dt;
+    size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
+    if (ne) {
+      hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+                         ed0, ed1, c, e->K1, e->gj, e->Gj, e->valid);
+      mark("k_edges");
+    }
+    hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+                       c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
+    mark("k_cells");
+    hipLaunchKernelGGL(k_factor_solve, dim3(nw), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool,
+                       e->foff, w0, 1, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->K1, e->dphi, e->valid);
+    mark("k_factor_solve");
+    hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
+                       ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->contrib1, e->Y1);
+    mark("k_ros2_stage1");
+    if (ne) {
+      hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, ed0,
+                         ed1, c, e->K2, nullptr, nullptr, e->valid);
+      mark("k_edges");
+    }
+    hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, c0, c1,
+                       c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+    mark("k_cells");
+    hipLaunchKernelGGL(k_ros2_stage2, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, c0, nc, ed0, ne, e->K1,
+                       e->contrib1, e->K2, e->contrib2);
+    mark("k_ros2_stage2");
+    hipLaunchKernelGGL(k_factor_solve, dim3(nw), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool,
+                       e->foff, w0, 0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->K2, e->dphi, e->valid);
+    mark("k_factor_solve");
+    hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
+                       ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->K2, e->contrib2, e->valid);
+    mark("k_ros2_final");
+    hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, e->contrib1,
+                       1.5 * dt, e->contrib2, 0.5 * dt, e->valid);
+    mark("k_world_ionic");
+  }
+  if (P.water) {
+    hipLaunchKernelGGL(k_water, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, 1, 0.5 * dt, c, e->valid);
+    mark("k_water");
+  }
+  hipLaunchKernelGGL(k_commit, dim3(blocks(nw)), dim3(kBlock), 0, st, w0, nw, e->sel, e->alive, e->valid);
+  mark("k_commit");
+  return CELLS_OK;
+}
+
+}  // namespace
+
+extern "C" int cells_step(cells_ensemble* e, void* stream, const double* y_in, double* y_out,
+                          double dt, int32_t nsteps, const cells_control* control,
+                          const int32_t* active, int32_t* ok_out, int32_t batched) {
+  if (!e || !y_in || !y_out || !control) return fail(CELLS_EINVAL, "NULL argument");
+  if (nsteps < 0) return fail(CELLS_EINVAL, "negative step count");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(CELLS_EINVAL, "dt must be positive and finite");
+  HIP_TRY(hipSetDevice(e->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
+  const Params& P = e->P;
+  HIP_TRY(hipMemcpyAsync(e->A, y_in, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
+  // The second buffer starts as a copy too: fields a step leaves alone (water fields outside a
+  // water world, world fields) must hold the current values in both.
+  HIP_TRY(hipMemcpyAsync(e->B, y_in, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_init, dim3(blocks(e->W)), dim3(kBlock), 0, st, e->W, active, e->sel, e->alive, e->valid);
+  HIP_TRY(hipGetLastError());
+  if (batched) {
+    bool same = e->graph && e->graph_dt == dt && std::memcmp(&e->graph_ctl, &c, sizeof(Ctl)) == 0;
+    if (!same && nsteps > 0) {
+      if (e->graph) {
+        HIP_TRY(hipGraphExecDestroy(e->graph));
+        e->graph = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal));
+      launch_step(e, e->capture_stream, dt, c, -1);
+      hipError_t cap = hipStreamEndCapture(e->capture_stream, &g);
+      if (cap != hipSuccess) return fail(CELLS_EHIP, std::string("step capture: ") + hipGetErrorString(cap));
+      hipError_t inst = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (inst != hipSuccess) return fail(CELLS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
+      e->graph_dt = dt;
+      e->graph_ctl = c;
+    }
+    for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(e->graph, st));
+  } else {
+    for (int k = 0; k < nsteps; ++k)
+      for (int w = 0; w < e->W; ++w) launch_step(e, st, dt, c, w);
+  }
+  HIP_TRY(hipGetLastError());
+  long items = (long)P.ncell + P.nedge + P.W;
+  hipLaunchKernelGGL(k_gather, dim3(blocks(items)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, y_out, e->alive,
+                     ok_out);
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_kernel_times(cells_ensemble* e, void* stream, const double* y, double dt,
+                                  const cells_control* control, int32_t reps, int32_t capacity,
+                                  double* ms_out, char* names_out, int32_t* count_out) {
+  if (!e || !y || !control || !ms_out || !names_out || !count_out || reps < 1)
+    return fail(CELLS_EINVAL, "bad argument");
+  HIP_TRY(hipSetDevice(e->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
+  const Params& P = e->P;
+  HIP_TRY(hipMemcpyAsync(e->A, y, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(e->B, y, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_init, dim3(blocks(e->W)), dim3(kBlock), 0, st, e->W, nullptr, e->sel, e->alive, e->valid);
+  std::vector<double> total;
+  std::vector<const char*> names;
+  for (int r = 0; r < reps; ++r) {
+    KernelTimer tm;
+    tm.stream = st;
+    tm.mark("start");
+    launch_step(e, st, dt, c, -1, &tm);
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    if (r == 0) {
+      names.assign(tm.names.begin() + 1, tm.names.end());
+      total.assign(names.size(), 0.0);
+    }
+    for (size_t k = 1; k < tm.events.size(); ++k) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, tm.events[k - 1], tm.events[k]));
+      total[k - 1] += ms;
+    }
+  }
+  int n = (int)std::min<size_t>(names.size(), (size_t)capacity);
+  for (int k = 0; k < n; ++k) {
+    ms_out[k] = total[k] / reps;
+    std::snprintf(names_out + 32 * k, 32, "%s", names[k]);
+  }
+  *count_out = n;
+  return CELLS_OK;
+}
+
+extern "C" int cells_rates(cells_ensemble* e, void* stream, const double* y,
+                           const cells_control* control, double* rates_out, int32_t* valid_out) {
+  if (!e || !y || !rates_out || !control) return fail(CELLS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(e->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
+  const Params& P = e->P;
+  HIP_TRY(hipMemcpyAsync(e->A, y, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_init, dim3(blocks(e->W)), dim3(kBlock), 0, st, e->W, nullptr, e->sel, e->alive, e->valid);
+  if (e->nedge)
+    hipLaunchKernelGGL(k_edges, dim3(blocks(e->nedge)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
+                       e->nedge, c, rates_out, nullptr, nullptr, e->valid);
+  hipLaunchKernelGGL(k_cells, dim3(blocks(e->ncell)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
+                     e->ncell, c, rates_out, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+  hipLaunchKernelGGL(k_world_rates, dim3(e->W), dim3(kBlock), 0, st, P, e->contrib1, rates_out);
+  if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * e->W, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_factor(cells_ensemble* e, void* stream, double* values) {
+  if (!e || !values) return fail(CELLS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(e->device));
+  hipLaunchKernelGGL(k_factor_only, dim3(e->W), dim3(kBlock), 0, static_cast<hipStream_t>(stream), e->shapes,
+                     e->shape_of, e->pool, e->foff, values);
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_solve(cells_ensemble* e, void* stream, const double* values, const double* rhs, double* x) {
+  if (!e || !values || !rhs || !x) return fail(CELLS_EINVAL, "NULL argument");
+  if (e->max_n > 8192) return fail(CELLS_EINVAL, "a world over 8192 cells exceeds the solve's LDS vector");
+  HIP_TRY(hipSetDevice(e->device));
+  size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
+  hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kBlock), lds, static_cast<hipStream_t>(stream), e->shapes,
+                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x);
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}

@@ -1,0 +1,182 @@
+// Host symbolic analysis of the stage matrix's pattern, once per lattice shape.
+//
+// Restates the reference's src/cells/numerics/sparse_cholesky.py host phase:
+//   minimum_degree (:137-162)  heap of (current degree, original degree, index), stale entries
+//                              skipped, the eliminated node's neighbours made a clique;
+//   bind (:165-215)            rank = inverse permutation; below-sets by the elimination tree's
+//                              union rule; parent = min(below); column-major entry positions,
+//                              diagonal first then rows ascending by rank; levels.
+// Their integer outputs are bit-identical to the reference's (order, rank, column_start, entry
+// rows and columns, parent, level). From them this file also builds the device plan the batched
+// factor and solves read (cells_hip.hip): the row structures (CSR of the strictly lower factor by
+// rows, with factor positions), each entry's column, the level schedule, and each lattice entry's
+// factor position.
+#include "symbolic.h"
+
+#include <algorithm>
+#include <cstring>
+#include <queue>
+#include <stdexcept>
+#include <tuple>
+#include <vector>
+
+namespace cells {
+
+namespace {
+
+using Adjacency = std::vector<std::vector<int32_t>>;
+
+Adjacency neighbours(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols) {
+  Adjacency adj(n);
+  for (int64_t q = 0; q < m; ++q) {
+    int32_t i = rows[q], j = cols[q];
+    if (i < 0 || j < 0 || i >= n || j >= n) throw std::invalid_argument("pattern outside its size");
+    if (i == j) continue;
+    adj[i].push_back(j);
+    adj[j].push_back(i);
+  }
+  for (auto& a : adj) {
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+  }
+  return adj;
+}
+
+}  // namespace
+
+std::vector<int32_t> minimum_degree(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols) {
+  Adjacency adj = neighbours(n, m, rows, cols);
+  std::vector<int32_t> original(n);
+  for (int32_t i = 0; i < n; ++i) original[i] = static_cast<int32_t>(adj[i].size());
+  using Key = std::tuple<int32_t, int32_t, int32_t>;
+  std::priority_queue<Key, std::vector<Key>, std::greater<Key>> heap;
+  for (int32_t i = 0; i < n; ++i) heap.emplace(original[i], original[i], i);
+  std::vector<char> alive(n, 1);
+  std::vector<int32_t> order;
+  order.reserve(n);
+  std::vector<int32_t> merged;
+  while (!heap.empty()) {
+    auto [degree, orig, node] = heap.top();
+    (void)orig;
+    heap.pop();
+    if (!alive[node] || degree != static_cast<int32_t>(adj[node].size())) continue;
+    alive[node] = 0;
+    order.push_back(node);
+    const std::vector<int32_t> clique = adj[node];
+    for (int32_t a : clique) {
+      // adjacency[a] = (adjacency[a] - {node}) | (clique - {a})
+      std::vector<int32_t>& own = adj[a];
+      merged.clear();
+      merged.reserve(own.size() + clique.size());
+      size_t x = 0, y = 0;
+      while (x < own.size() || y < clique.size()) {
+        int32_t v;
+        if (y == clique.size() || (x < own.size() && own[x] < clique[y])) {
+          v = own[x++];
+        } else if (x == own.size() || clique[y] < own[x]) {
+          v = clique[y++];
+        } else {
+          v = own[x++];
+          ++y;
+        }
+        if (v != node && v != a) merged.push_back(v);
+      }
+      own.swap(merged);
+      heap.emplace(static_cast<int32_t>(own.size()), original[a], a);
+    }
+    adj[node].clear();
+  }
+  return order;
+}
+
+Symbolic bind(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, const int32_t* given) {
+  if (n < 1) throw std::invalid_argument("a sparse pattern needs a positive size");
+  Adjacency adj = neighbours(n, m, rows, cols);
+  Symbolic s;
+  s.n = n;
+  if (given) {
+    s.order.assign(given, given + n);
+  } else {
+    s.order = minimum_degree(n, m, rows, cols);
+  }
+  std::vector<char> seen(n, 0);
+  if (static_cast<int32_t>(s.order.size()) != n) throw std::invalid_argument("order is not a permutation");
+  for (int32_t v : s.order) {
+    if (v < 0 || v >= n || seen[v]) throw std::invalid_argument("order is not a permutation");
+    seen[v] = 1;
+  }
+  s.rank.assign(n, 0);
+  for (int32_t k = 0; k < n; ++k) s.rank[s.order[k]] = k;
+  // below sets by the union rule
+  std::vector<std::vector<int32_t>> below(n);
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t r = s.rank[i];
+    for (int32_t a : adj[i])
+      if (s.rank[a] > r) below[r].push_back(s.rank[a]);
+  }
+  for (auto& b : below) std::sort(b.begin(), b.end());
+  s.parent.assign(n, -1);
+  std::vector<int32_t> merged;
+  for (int32_t k = 0; k < n; ++k) {
+    if (below[k].empty()) continue;
+    int32_t p = below[k].front();
+    s.parent[k] = p;
+    merged.clear();
+    std::set_union(below[p].begin(), below[p].end(), below[k].begin() + 1, below[k].end(),
+                   std::back_inserter(merged));
+    below[p].swap(merged);
+  }
+  s.column_start.assign(n + 1, 0);
+  for (int32_t k = 0; k < n; ++k) s.column_start[k + 1] = s.column_start[k] + 1 + static_cast<int64_t>(below[k].size());
+  const int64_t nnz = s.column_start[n];
+  s.row_rank.resize(nnz);
+  s.col_rank.resize(nnz);
+  for (int32_t k = 0; k < n; ++k) {
+    int64_t p = s.column_start[k];
+    s.row_rank[p] = k;
+    s.col_rank[p] = k;
+    for (size_t q = 0; q < below[k].size(); ++q) {
+      s.row_rank[p + 1 + q] = below[k][q];
+      s.col_rank[p + 1 + q] = k;
+    }
+  }
+  s.level.assign(n, 0);
+  for (int32_t k = 0; k < n; ++k)
+    if (s.parent[k] >= 0) s.level[s.parent[k]] = std::max(s.level[s.parent[k]], s.level[k] + 1);
+  // row structures: for row r, the columns m < r holding it, ascending, with positions
+  s.row_ptr.assign(n + 1, 0);
+  for (int64_t p = 0; p < nnz; ++p)
+    if (s.row_rank[p] != s.col_rank[p]) s.row_ptr[s.row_rank[p] + 1]++;
+  for (int32_t r = 0; r < n; ++r) s.row_ptr[r + 1] += s.row_ptr[r];
+  s.row_col.resize(s.row_ptr[n]);
+  s.row_pos.resize(s.row_ptr[n]);
+  std::vector<int64_t> fill(s.row_ptr.begin(), s.row_ptr.end() - 1);
+  for (int32_t k = 0; k < n; ++k)
+    for (int64_t p = s.column_start[k] + 1; p < s.column_start[k + 1]; ++p) {
+      int32_t r = s.row_rank[p];
+      s.row_col[fill[r]] = k;
+      s.row_pos[fill[r]] = static_cast<int32_t>(p);
+      fill[r]++;
+    }
+  // level schedule
+  int32_t levels = 0;
+  for (int32_t k = 0; k < n; ++k) levels = std::max(levels, s.level[k] + 1);
+  s.level_ptr.assign(levels + 1, 0);
+  for (int32_t k = 0; k < n; ++k) s.level_ptr[s.level[k] + 1]++;
+  for (int32_t v = 0; v < levels; ++v) s.level_ptr[v + 1] += s.level_ptr[v];
+  s.level_col.resize(n);
+  std::vector<int32_t> lf(s.level_ptr.begin(), s.level_ptr.end() - 1);
+  for (int32_t k = 0; k < n; ++k) s.level_col[lf[s.level[k]]++] = k;
+  return s;
+}
+
+int64_t position(const Symbolic& s, int32_t i, int32_t j) {
+  int32_t a = s.rank[i], b = s.rank[j];
+  int32_t hi = std::max(a, b), lo = std::min(a, b);
+  int64_t p0 = s.column_start[lo], p1 = s.column_start[lo + 1];
+  auto it = std::lower_bound(s.row_rank.begin() + p0, s.row_rank.begin() + p1, hi);
+  if (it == s.row_rank.begin() + p1 || *it != hi) throw std::out_of_range("entry not in the bound pattern");
+  return it - s.row_rank.begin();
+}
+
+}  // namespace cells

@@ -1,0 +1,155 @@
+"""Device ensembles of worlds: the host side of the batched step, over the C-ABI.
+
+PyTorch provides device memory and streams only; every step runs in libcells_hip.so.
+
+Counterparts in the reference (src/cells/ of Eidosoma/Cells):
+- `Ensemble.step(..., batched=True)`  execution/vector_windows.py:348-413, `vmap(checked_step)`
+  inside one scan, one launch sequence for every world (here a captured HIP graph per step);
+- `Ensemble.step(..., batched=False)` execution/batch.py:321-357, `lax.map` over worlds, one
+  world per launch;
+- `Ensemble.rates`                    models/surface_pump_v2.py:370-450, explicit_rates at the
+  state's own potentials;
+- `Ensemble.factor` / `.solve`        numerics/sparse_cholesky.py:474-514 on the stage pattern.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from cells_amd import lib as _lib
+from cells_amd.world import Packed, StateLayout
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU: the HIP step has no CPU fallback")
+    return torch
+
+
+def control(membrane=1.0, junction=1.0, pump=1.0, calcium_pump=1.0, water=1.0):
+    """Held availabilities in [0, 1] (definitions/surface_pump_v2_spec.py:118-123)."""
+    return _lib.ControlStruct(membrane, junction, pump, calcium_pump, water)
+
+
+class Ensemble:
+    """Packed worlds bound on one GPU: parameters, per-shape plans and the stepping workspace."""
+
+    def __init__(self, packed: Packed, method: str = "euler", device: int = 0):
+        torch = _torch()
+        self.torch = torch
+        self.packed = packed
+        self.layout = StateLayout.of(packed)
+        self.method = {"euler": _lib.METHOD_EULER, "ros2": _lib.METHOD_ROS2}[method]
+        self.device = torch.device("cuda", device)
+        L = _lib.load()
+        self._world = _lib.world_struct(packed)
+        handle = ctypes.c_void_p()
+        torch.cuda.init()
+        _lib.check(L.cells_ensemble_create(ctypes.byref(self._world), self.method, device,
+                                           ctypes.byref(handle)))
+        self._h = handle
+        self.state_size = int(L.cells_state_size(handle))
+        if self.state_size != self.layout.total:
+            raise RuntimeError("state layout disagrees with the library's")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().cells_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def to_device(self, y):
+        t = self.torch
+        if isinstance(y, np.ndarray):
+            y = t.from_numpy(np.ascontiguousarray(y, dtype=np.float64))
+        return y.to(self.device, dtype=t.float64).contiguous()
+
+    def step(self, y, dt, nsteps=1, ctl=None, active=None, batched=True, out=None, ok=None):
+        """nsteps checked steps of every world; returns (new state, ok per world) on the device."""
+        t = self.torch
+        y = self.to_device(y)
+        if y.numel() != self.state_size:
+            raise ValueError("state size disagrees with the worlds")
+        out = t.empty_like(y) if out is None else out
+        ok = t.empty(self.packed.W, dtype=t.int32, device=self.device) if ok is None else ok
+        act = None
+        if active is not None:
+            act = t.as_tensor(np.asarray(active, dtype=np.int32)).to(self.device)
+        ctl = ctl or control()
+        _lib.check(_lib.load().cells_step(
+            self._h, self._stream(), ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+            float(dt), int(nsteps), ctypes.byref(ctl),
+            None if act is None else ctypes.c_void_p(act.data_ptr()),
+            ctypes.c_void_p(ok.data_ptr()), 1 if batched else 0))
+        return out, ok
+
+    def rates(self, y, ctl=None):
+        t = self.torch
+        y = self.to_device(y)
+        r = t.zeros_like(y)
+        valid = t.empty(self.packed.W, dtype=t.int32, device=self.device)
+        ctl = ctl or control()
+        _lib.check(_lib.load().cells_rates(self._h, self._stream(), ctypes.c_void_p(y.data_ptr()),
+                                           ctypes.byref(ctl), ctypes.c_void_p(r.data_ptr()),
+                                           ctypes.c_void_p(valid.data_ptr())))
+        return r, valid
+
+    @property
+    def factor_size(self):
+        return int(_lib.load().cells_factor_size(self._h))
+
+    def positions(self, w):
+        """(diag_pos [n] by node, edge_pos [E] by world edge, factor offset) of world w."""
+        p = self.packed
+        n = int(p.cell_off[w + 1] - p.cell_off[w])
+        E = int(p.edge_off[w + 1] - p.edge_off[w])
+        d = np.empty(n, np.int64)
+        e = np.empty(max(E, 1), np.int64)
+        off = ctypes.c_int64()
+        _lib.check(_lib.load().cells_world_positions(self._h, w, d.ctypes.data_as(_i64p),
+                                                     e.ctypes.data_as(_i64p), ctypes.byref(off)))
+        return d, e[:E], off.value
+
+    def factor(self, values):
+        v = self.to_device(values).clone()
+        _lib.check(_lib.load().cells_factor(self._h, self._stream(), ctypes.c_void_p(v.data_ptr())))
+        return v
+
+    def solve(self, factor_values, rhs):
+        f = self.to_device(factor_values)
+        b = self.to_device(rhs)
+        x = self.torch.empty_like(b)
+        _lib.check(_lib.load().cells_solve(self._h, self._stream(), ctypes.c_void_p(f.data_ptr()),
+                                           ctypes.c_void_p(b.data_ptr()),
+                                           ctypes.c_void_p(x.data_ptr())))
+        return x
+
+    def kernel_times(self, y, dt, reps=3, ctl=None):
+        """[(kernel name, mean ms)] of one step's launches, in launch order (HIP events)."""
+        y = self.to_device(y)
+        cap = 64
+        ms = np.zeros(cap)
+        names = ctypes.create_string_buffer(32 * cap)
+        count = ctypes.c_int32()
+        ctl = ctl or control()
+        _lib.check(_lib.load().cells_kernel_times(
+            self._h, self._stream(), ctypes.c_void_p(y.data_ptr()), float(dt), ctypes.byref(ctl),
+            int(reps), cap, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), names,
+            ctypes.byref(count)))
+        raw = names.raw
+        return [(raw[32 * k:32 * k + 32].split(b"\0")[0].decode(), float(ms[k]))
+                for k in range(count.value)]

@@ -1,0 +1,152 @@
+"""ctypes binding of libcells_hip.so (include/cells_hip.h).
+
+The library is built in-tree by cells_amd.build (hipcc for gfx950). Loading fails loudly when it
+is missing: the HIP step has no CPU fallback.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libcells_hip.so")
+
+CELLS_OK = 0
+CELLS_EINVAL = 1
+CELLS_EHIP = 2
+CELLS_ENOMEM = 3
+CELLS_ENODEV = 4
+METHOD_EULER = 0
+METHOD_ROS2 = 1
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+EXPORTS = (
+    "cells_last_error", "cells_minimum_degree", "cells_bind", "cells_symbolic_sizes",
+    "cells_symbolic_export", "cells_symbolic_free", "cells_ensemble_create",
+    "cells_ensemble_destroy", "cells_state_size", "cells_step", "cells_rates",
+    "cells_factor_size", "cells_factor", "cells_solve", "cells_world_positions",
+    "cells_kernel_times",
+)
+
+
+class CellsError(RuntimeError):
+    """A status other than CELLS_OK; .code is the status."""
+
+    def __init__(self, code, message):
+        super().__init__(f"cells_hip error {code}: {message}")
+        self.code = code
+
+
+class WorldStruct(ctypes.Structure):
+    _fields_ = [
+        ("S", ctypes.c_int32), ("W", ctypes.c_int32), ("water", ctypes.c_int32),
+        ("calcium", ctypes.c_int32), ("ncell", ctypes.c_int64), ("nedge", ctypes.c_int64),
+        ("cell_off", _i64p), ("edge_off", _i64p),
+        ("valence", ctypes.c_double * 4), ("bath_c", ctypes.c_double * 4),
+        ("temperature", ctypes.c_double), ("bath_V", ctypes.c_double),
+        ("atp", ctypes.c_double), ("adp", ctypes.c_double), ("pi", ctypes.c_double),
+        ("km_na", ctypes.c_double), ("km_k", ctypes.c_double), ("km_atp", ctypes.c_double),
+        ("dg0", ctypes.c_double), ("ca_km_ca", ctypes.c_double), ("ca_km_atp", ctypes.c_double),
+        ("ca_dg0", ctypes.c_double), ("gate_threshold", ctypes.c_double),
+        ("gate_minimum", ctypes.c_double), ("water_K", ctypes.c_double),
+        ("water_P0", ctypes.c_double), ("water_vmin", ctypes.c_double),
+        ("water_vmax", ctypes.c_double), ("bath_pressure", ctypes.c_double),
+        ("vol0", _f64p), ("cap", _f64p), ("pam", _f64p), ("nak_cap", _f64p), ("ca_cap", _f64p),
+        ("n0", _f64p), ("q0", _f64p), ("lpa", _f64p), ("imperm", _f64p),
+        ("snd", _i64p), ("rcv", _i64p), ("paj", _f64p),
+        ("inc_ptr", _i64p), ("inc_edge", _i64p), ("inc_out", _i32p),
+    ]
+
+
+class ControlStruct(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("membrane", "junction", "pump", "calcium_pump", "water")]
+
+
+_lib = None
+
+
+def load():
+    """The loaded library (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with cells_amd.build.build() (hipcc, gfx950); "
+            "the HIP step has no CPU fallback")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.cells_last_error.restype = ctypes.c_char_p
+    lib.cells_state_size.restype = ctypes.c_int64
+    lib.cells_factor_size.restype = ctypes.c_int64
+    lib.cells_ensemble_create.argtypes = [ctypes.POINTER(WorldStruct), ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    lib.cells_ensemble_destroy.argtypes = [ctypes.c_void_p]
+    lib.cells_state_size.argtypes = [ctypes.c_void_p]
+    lib.cells_factor_size.argtypes = [ctypes.c_void_p]
+    lib.cells_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ControlStruct),
+                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]
+    lib.cells_rates.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.POINTER(ControlStruct), ctypes.c_void_p, ctypes.c_void_p]
+    lib.cells_factor.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.cells_solve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_void_p]
+    lib.cells_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_double, ctypes.POINTER(ControlStruct),
+                                       ctypes.c_int32, ctypes.c_int32, _f64p, ctypes.c_char_p,
+                                       _i32p]
+    lib.cells_world_positions.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i64p, _i64p, _i64p]
+    lib.cells_minimum_degree.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p]
+    lib.cells_bind.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p,
+                               ctypes.POINTER(ctypes.c_void_p)]
+    lib.cells_symbolic_sizes.argtypes = [ctypes.c_void_p, _i64p, _i32p]
+    lib.cells_symbolic_export.argtypes = [ctypes.c_void_p, _i32p, _i32p, _i64p, _i32p, _i32p,
+                                          _i32p, _i32p]
+    lib.cells_symbolic_free.argtypes = [ctypes.c_void_p]
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != CELLS_OK:
+        raise CellsError(rc, load().cells_last_error().decode())
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def world_struct(p):
+    """The cells_world view of a cells_amd.world.Packed (arrays kept alive on the struct)."""
+    c = p.config
+    keep = {}
+
+    def arr(name, dtype):
+        a = np.ascontiguousarray(getattr(p, name), dtype=dtype)
+        keep[name] = a
+        return a
+
+    w = WorldStruct()
+    w.S, w.W, w.water, w.calcium = p.S, p.W, int(p.water), int(p.calcium)
+    w.ncell, w.nedge = p.ncell, p.nedge
+    w.cell_off = ptr(arr("cell_off", np.int64), _i64p)
+    w.edge_off = ptr(arr("edge_off", np.int64), _i64p)
+    for s in range(p.S):
+        w.valence[s] = p.valence[s]
+        w.bath_c[s] = p.bath_c[s]
+    w.temperature, w.bath_V = c.temperature, c.bath_V
+    for k in ("atp", "adp", "pi", "km_na", "km_k", "km_atp", "dg0", "ca_km_ca", "ca_km_atp",
+              "ca_dg0", "gate_threshold", "gate_minimum", "water_K", "water_P0", "water_vmin",
+              "water_vmax", "bath_pressure"):
+        setattr(w, k, getattr(c, k))
+    for k in ("vol0", "cap", "pam", "nak_cap", "ca_cap", "n0", "q0", "lpa", "imperm", "paj"):
+        setattr(w, k, ptr(arr(k, np.float64), _f64p))
+    for k in ("snd", "rcv", "inc_ptr", "inc_edge"):
+        setattr(w, k, ptr(arr(k, np.int64), _i64p))
+    w.inc_out = ptr(arr("inc_out", np.int32), _i32p)
+    w._keep = keep
+    return w

@@ -1,0 +1,135 @@
+/* cells_hip: the C-ABI of the MI355X batched tissue step (libcells_hip.so).
+ *
+ * Plain C types only: pointers, sizes, int status codes. Device buffers are the caller's
+ * (stream-ordered; no allocation on the stepping path). A status other than CELLS_OK leaves a
+ * message for cells_last_error().
+ *
+ * Each entry point names the reference interface it replaces (paths under src/cells/ of the
+ * reference, Eidosoma/Cells).
+ */
+#ifndef CELLS_HIP_H
+#define CELLS_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CELLS_OK 0
+#define CELLS_EINVAL 1   /* a malformed argument or pattern (the reference's ValueError/ConfigError) */
+#define CELLS_EHIP 2     /* a HIP runtime error */
+#define CELLS_ENOMEM 3   /* device allocation failed at set-up */
+#define CELLS_ENODEV 4   /* no GPU: the library never falls back to the CPU */
+
+#define CELLS_METHOD_EULER 0 /* numerics/surface_imex.py:36 imex_euler_face_electromigration@1 */
+#define CELLS_METHOD_ROS2 1  /* ROS2 (gamma = 1 + 1/sqrt 2), stage matrix C + gamma dt G */
+
+/* The packed worlds: lattices of cells on one held bath each, back to back. The reference's
+ * counterpart is the layout pytree of models/surface_pump_v3.py:140-173 (LAYOUT_TYPE) built by
+ * initial_values (models/surface_tissue.py:106) for these worlds. Host pointers. */
+typedef struct cells_world {
+    int32_t S, W, water, calcium;
+    int64_t ncell, nedge;
+    const int64_t* cell_off; /* [W+1] */
+    const int64_t* edge_off; /* [W+1] */
+    double valence[4];
+    double bath_c[4];
+    double temperature, bath_V;
+    double atp, adp, pi, km_na, km_k, km_atp, dg0;
+    double ca_km_ca, ca_km_atp, ca_dg0;
+    double gate_threshold, gate_minimum;
+    double water_K, water_P0, water_vmin, water_vmax, bath_pressure;
+    const double* vol0;
+    const double* cap;
+    const double* pam;     /* [S][ncell] */
+    const double* nak_cap;
+    const double* ca_cap;
+    const double* n0;      /* [S][ncell] */
+    const double* q0;
+    const double* lpa;
+    const double* imperm;
+    const int64_t* snd;
+    const int64_t* rcv;
+    const double* paj;     /* [S][nedge] */
+    const int64_t* inc_ptr;
+    const int64_t* inc_edge;
+    const int32_t* inc_out;
+} cells_world;
+
+/* Held availabilities (definitions/surface_pump_v2_spec.py:118-123, Control), one per kind. */
+typedef struct cells_control {
+    double membrane, junction, pump, calcium_pump, water;
+} cells_control;
+
+typedef struct cells_symbolic cells_symbolic;
+typedef struct cells_ensemble cells_ensemble;
+
+const char* cells_last_error(void);
+
+/* ---- host symbolic analysis: numerics/sparse_cholesky.py ---- */
+
+/* minimum_degree (sparse_cholesky.py:137-162): order_out[n] */
+int cells_minimum_degree(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols,
+                         int32_t* order_out);
+/* bind (sparse_cholesky.py:165-215); order may be NULL (minimum degree). */
+int cells_bind(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols,
+               const int32_t* order, cells_symbolic** out);
+int cells_symbolic_sizes(const cells_symbolic* s, int64_t* nnz, int32_t* levels);
+/* The Symbolic's integer arrays (sparse_cholesky.py:69-85): order, rank [n]; column_start [n+1];
+ * entry rows and columns as nodes [nnz]; parent, level [n] by rank. Any pointer may be NULL. */
+int cells_symbolic_export(const cells_symbolic* s, int32_t* order, int32_t* rank,
+                          int64_t* column_start, int32_t* row_node, int32_t* column_node,
+                          int32_t* parent, int32_t* level);
+void cells_symbolic_free(cells_symbolic* s);
+
+/* ---- device ensemble: checked_step under the execution layer's loops ---- */
+
+/* Binds the worlds once (models/surface_pump_v3.py:6851 model_for; numerics/surface_field.py:371
+ * _bind_sparse): uploads parameters, runs the symbolic analysis once per lattice shape, and
+ * allocates the stepping workspace on `device`. */
+int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
+                          cells_ensemble** out);
+void cells_ensemble_destroy(cells_ensemble* e);
+/* Doubles in one state vector (cell, edge and world fields; models/surface_pump_v3.py:122-126). */
+int64_t cells_state_size(const cells_ensemble* e);
+
+/* nsteps checked steps of every world (models/surface_pump_v3.py:3989 checked_step, run as
+ * execution/vector_windows.py:348-413 runs it under vmap, or as execution/batch.py:321-357's
+ * lax.map when batched = 0: one world per launch). y_in, y_out: device [state size].
+ * active, ok_out: device int32 [W]. A world stops at its first refused step and keeps its last
+ * accepted state (refusals="world"); ok_out[w] = 0 then. batched = 1 replays the captured
+ * step graph. Stream-ordered on `stream` (a hipStream_t, NULL for the null stream). */
+int cells_step(cells_ensemble* e, void* stream, const double* y_in, double* y_out, double dt,
+               int32_t nsteps, const cells_control* control, const int32_t* active,
+               int32_t* ok_out, int32_t batched);
+
+/* Per-kernel timing of one step's launches (no counterpart in the reference, whose profile is
+ * scripts/profile_step.py's per-piece timing): runs `reps` steps from y with a HIP event after
+ * every launch on `stream`, and returns each launch's mean duration (ms) and kernel name (32
+ * bytes each) in launch order. */
+int cells_kernel_times(cells_ensemble* e, void* stream, const double* y, double dt,
+                       const cells_control* control, int32_t reps, int32_t capacity,
+                       double* ms_out, char* names_out, int32_t* count_out);
+
+/* Explicit rates of every field at y (models/surface_pump_v2.py:370-450 explicit_rates at the
+ * state's own potentials): rates_out has the state's layout (world fields: bath supply rates);
+ * valid_out: device int32 [W]. */
+int cells_rates(cells_ensemble* e, void* stream, const double* y, const cells_control* control,
+                double* rates_out, int32_t* valid_out);
+
+/* The batched numeric factor and solve on the stage pattern (numerics/sparse_cholesky.py:474-514):
+ * values: device [factor size] (each world's assembled entries at its factor positions), factored
+ * in place; rhs/x: device [ncell] in node order, one system per world. */
+int64_t cells_factor_size(const cells_ensemble* e);
+int cells_factor(cells_ensemble* e, void* stream, double* values);
+int cells_solve(cells_ensemble* e, void* stream, const double* values, const double* rhs,
+                double* x);
+/* Host copy of world w's plan positions: diag_pos [n] by node, edge_pos [E] by edge, and the
+ * offset of its factor slice. */
+int cells_world_positions(const cells_ensemble* e, int32_t w, int64_t* diag_pos,
+                          int64_t* edge_pos, int64_t* factor_offset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,205 @@
+"""Oracle (test infrastructure only): ctypes access to oracle/_build/libcells_ref.so.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg import this package. It is the
+checker, never the thing measured or shipped: the product (cells_amd) must not import it.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libcells_ref.so")
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class World(ctypes.Structure):
+    _fields_ = [
+        ("S", ctypes.c_int32), ("W", ctypes.c_int32), ("water", ctypes.c_int32),
+        ("calcium", ctypes.c_int32), ("ncell", ctypes.c_int64), ("nedge", ctypes.c_int64),
+        ("cell_off", _i64p), ("edge_off", _i64p),
+        ("valence", ctypes.c_double * 4), ("bath_c", ctypes.c_double * 4),
+        ("temperature", ctypes.c_double), ("bath_V", ctypes.c_double),
+        ("atp", ctypes.c_double), ("adp", ctypes.c_double), ("pi", ctypes.c_double),
+        ("km_na", ctypes.c_double), ("km_k", ctypes.c_double), ("km_atp", ctypes.c_double),
+        ("dg0", ctypes.c_double), ("ca_km_ca", ctypes.c_double), ("ca_km_atp", ctypes.c_double),
+        ("ca_dg0", ctypes.c_double), ("gate_threshold", ctypes.c_double),
+        ("gate_minimum", ctypes.c_double), ("water_K", ctypes.c_double),
+        ("water_P0", ctypes.c_double), ("water_vmin", ctypes.c_double),
+        ("water_vmax", ctypes.c_double), ("bath_pressure", ctypes.c_double),
+        ("vol0", _f64p), ("cap", _f64p), ("pam", _f64p), ("nak_cap", _f64p), ("ca_cap", _f64p),
+        ("n0", _f64p), ("q0", _f64p), ("lpa", _f64p), ("imperm", _f64p),
+        ("snd", _i64p), ("rcv", _i64p), ("paj", _f64p),
+        ("inc_ptr", _i64p), ("inc_edge", _i64p), ("inc_out", _i32p),
+    ]
+
+
+class Control(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("membrane", "junction", "pump", "calcium_pump", "water")]
+
+
+class Symbolic(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("nnz", ctypes.c_int64), ("column_start", _i64p),
+                ("row_rank", _i64p), ("order", _i64p), ("diag_pos", _i64p), ("edge_pos", _i64p)]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        for name in ("ref_bernoulli", "ref_bernoulli_slope", "ref_logaddexp"):
+            getattr(_lib, name).restype = ctypes.c_double
+        for name in ("ref_selective_flux", "ref_route_slope", "ref_gate_rhs"):
+            getattr(_lib, name).restype = ctypes.c_double
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def world_struct(p):
+    """A World over the arrays of a cells_amd.world.Packed (kept alive on the struct)."""
+    c = p.config
+    keep = {}
+
+    def arr(name, dtype):
+        a = np.ascontiguousarray(getattr(p, name), dtype=dtype)
+        keep[name] = a
+        return a
+
+    w = World()
+    w.S, w.W, w.water, w.calcium = p.S, p.W, int(p.water), int(p.calcium)
+    w.ncell, w.nedge = p.ncell, p.nedge
+    w.cell_off = _p(arr("cell_off", np.int64), _i64p)
+    w.edge_off = _p(arr("edge_off", np.int64), _i64p)
+    for s in range(p.S):
+        w.valence[s] = p.valence[s]
+        w.bath_c[s] = p.bath_c[s]
+    w.temperature, w.bath_V = c.temperature, c.bath_V
+    for k in ("atp", "adp", "pi", "km_na", "km_k", "km_atp", "dg0", "ca_km_ca", "ca_km_atp",
+              "ca_dg0", "gate_threshold", "gate_minimum", "water_K", "water_P0", "water_vmin",
+              "water_vmax", "bath_pressure"):
+        setattr(w, k, getattr(c, k))
+    for k in ("vol0", "cap", "pam", "nak_cap", "ca_cap", "n0", "q0", "lpa", "imperm", "paj"):
+        setattr(w, k, _p(arr(k, np.float64), _f64p))
+    for k in ("snd", "rcv", "inc_ptr", "inc_edge"):
+        setattr(w, k, _p(arr(k, np.int64), _i64p))
+    w.inc_out = _p(arr("inc_out", np.int32), _i32p)
+    w._keep = keep
+    return w
+
+
+def control(membrane=1.0, junction=1.0, pump=1.0, calcium_pump=1.0, water=1.0):
+    return Control(membrane, junction, pump, calcium_pump, water)
+
+
+def step_euler(p, y, dt, ctl=None, world=None):
+    w = world if world is not None else world_struct(p)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(y)
+    valid = np.zeros(p.W, dtype=np.int32)
+    lib().ref_step_euler(ctypes.byref(w), _p(y, _f64p), _p(out, _f64p), ctypes.c_double(dt),
+                         ctypes.byref(ctl), _p(valid, _i32p))
+    return out, valid.astype(bool)
+
+
+def symbolic_structs(p, symbolics):
+    """Symbolic structs per world from oracle.symbolic.bind results (one per world, in order),
+    with each lattice entry's factor position."""
+    out = (Symbolic * p.W)()
+    keep = []
+    for k, s in enumerate(symbolics):
+        c0, c1 = int(p.cell_off[k]), int(p.cell_off[k + 1])
+        e0, e1 = int(p.edge_off[k]), int(p.edge_off[k + 1])
+        n = c1 - c0
+        row_rank = s.rank[s.row]
+        diag_pos = s.column_start[s.rank[np.arange(n)]]
+        a = s.rank[p.snd[e0:e1] - c0]
+        b = s.rank[p.rcv[e0:e1] - c0]
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        pos = {}
+        for kk in range(n):
+            for q in range(int(s.column_start[kk]), int(s.column_start[kk + 1])):
+                pos[(int(row_rank[q]), kk)] = q
+        edge_pos = np.array([pos[(int(h), int(l))] for h, l in zip(hi, lo)], dtype=np.int64)
+        arrays = [np.ascontiguousarray(x, dtype=np.int64) for x in
+                  (s.column_start, row_rank, s.order, diag_pos, edge_pos)]
+        keep.append(arrays)
+        out[k].n, out[k].nnz = n, int(s.column_start[-1])
+        (out[k].column_start, out[k].row_rank, out[k].order, out[k].diag_pos,
+         out[k].edge_pos) = [_p(x, _i64p) for x in arrays]
+    out._keep = keep
+    return out
+
+
+def step_ros2(p, y, dt, sym, ctl=None, world=None):
+    w = world if world is not None else world_struct(p)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(y)
+    valid = np.zeros(p.W, dtype=np.int32)
+    lib().ref_step_ros2(ctypes.byref(w), sym, _p(y, _f64p), _p(out, _f64p), ctypes.c_double(dt),
+                        ctypes.byref(ctl), _p(valid, _i32p))
+    return out, valid.astype(bool)
+
+
+def factor(sym_one, values):
+    v = np.ascontiguousarray(values, dtype=np.float64).copy()
+    lib().ref_factor(ctypes.byref(sym_one), _p(v, _f64p))
+    return v
+
+
+def solve(sym_one, factor_values, rhs):
+    f = np.ascontiguousarray(factor_values, dtype=np.float64)
+    b = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.empty_like(b)
+    lib().ref_solve(ctypes.byref(sym_one), _p(f, _f64p), _p(b, _f64p), _p(x, _f64p))
+    return x
+
+
+def rates(p, y, ctl=None, world=None):
+    """Explicit rates of every field at y (the state's own potentials), world supplies included."""
+    w = world if world is not None else world_struct(p)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    r = np.zeros_like(y)
+    phi = np.zeros(p.ncell)
+    valid = np.zeros(p.W, dtype=bool)
+    for k in range(p.W):
+        lib().ref_potentials(ctypes.byref(w), k, _p(y, _f64p), _p(phi, _f64p))
+        valid[k] = bool(lib().ref_rates(ctypes.byref(w), k, _p(y, _f64p), _p(phi, _f64p),
+                                        ctypes.byref(ctl), _p(r, _f64p), None, None))
+    return r, valid
+
+
+def world_symbolics(p):
+    """oracle.symbolic.bind of every world's lattice pattern, and their structs."""
+    from oracle.symbolic import bind
+
+    syms = []
+    cache = {}
+    for k in range(p.W):
+        c0, c1 = int(p.cell_off[k]), int(p.cell_off[k + 1])
+        e0, e1 = int(p.edge_off[k]), int(p.edge_off[k + 1])
+        n = c1 - c0
+        if n not in cache:
+            cache[n] = bind(n, p.snd[e0:e1] - c0, p.rcv[e0:e1] - c0)
+        syms.append(cache[n])
+    return syms, symbolic_structs(p, syms)

@@ -1,0 +1,830 @@
+/* Oracle (test infrastructure only): FP64 CPU restatement of the reference's tissue step for
+ * lattices of cells on one held bath. Only tests, the smoke check and the benchmark's CPU-baseline
+ * leg load it; the product never does.
+ *
+ * Reference files followed (paths under src/cells/ of the reference):
+ *   laws/electrochemistry.py:11-48            F, R, bernoulli, electrochemical potential
+ *   laws/junctions/selective.py:28-50         selective_flux (GHK route flux)
+ *   laws/routes/patch_electrodiffusion.py:43-54  bernoulli_slope
+ *   models/surface_pump_v3.py:2747-2755       route_slope (dJ/d dv)
+ *   models/surface_pump_v3.py:2816-2837       conductance_laplacian (G = F' sum_s z_s slope_s)
+ *   laws/pumps/nak_atp_mass_action_positive.py:55-149  Na/K-ATPase kinetics and domain
+ *   laws/pumps/ca_atp_mass_action_positive.py:70-150, laws/reactions/er_calcium.py:30-50
+ *                                             Ca-ATPase kinetics and domain
+ *   laws/junctions/junction_voltage_relaxation.py:41-97  junction gate kinetics, rhs, domain
+ *   numerics/surface_transport.py:298-386     transport: route fluxes, mu, dissipation, owners
+ *   models/surface_pump_v2.py:370-450         explicit_rates with both pumps
+ *   models/surface_pump.py:259-362            _combine, node_charge, electrode_increment, imex_step
+ *   numerics/surface_imex.py:47-60,207-278    Euler tableau; Jacobi-scaled factor; solve_operator
+ *   numerics/sparse_cholesky.py:474-514       factor / solve (pairwise folds, reciprocal pivots)
+ *   operators/osmotic_water.py:127-221, numerics/osmotic_water.py:193-215  water half-steps
+ *   models/pump_composite.py:1769-1815        Strang split: water half, ionic step, water half
+ *   models/surface_tissue.py:269-420          charge_increment, capacitor_change, domain_valid
+ * The ROS2 stage driver (Verwer, Spee, Blom, Hundsdorfer 1999, SIAM J. Sci. Comput. 20:1456,
+ * gamma = 1 + 1/sqrt 2) is this build's own method: the reference has none. Its stage matrix is
+ * C + gamma dt G with G the routes' slope-conductance Laplacian, as N2's operator
+ * (surface_pump_v3.py:2758-2837).
+ */
+#include "cells_ref.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define AVOGADRO 6.02214076e23
+#define F_CONST (AVOGADRO * 1.602176634e-19)
+#define R_CONST (AVOGADRO * 1.380649e-23)
+#define MOL_PER_FMOL 1e-15
+#define CHARGE_PER_FMOL (F_CONST * MOL_PER_FMOL)
+#define MAX_ABS_X 50.0
+#define NAK_MAX_AFFINITY 80.0
+#define NAK_MAX_MU 1600.0
+#define CA_MAX_AFFINITY 80.0
+#define CA_MAX_VOLTAGE 0.15
+#define GATE_MAX_VOLTAGE 0.120
+#define STANDARD_STATE_mM 1000.0
+#define WATER_MAX_RELAXATION 0.05
+#define ROS2_GAMMA (1.0 + 1.0 / 1.4142135623730951)
+
+/* ---------------------------------------------------------------- point laws */
+
+double ref_bernoulli(double x) {
+    /* laws/electrochemistry.py:25-32 */
+    if (fabs(x) < 1e-4) {
+        double x2 = x * x;
+        return 1.0 - x / 2.0 + x2 / 12.0 - (x2 * x2) / 720.0;
+    }
+    return x / expm1(x);
+}
+
+double ref_bernoulli_slope(double x) {
+    /* laws/routes/patch_electrodiffusion.py:43-54 */
+    if (fabs(x) < 1e-3) return -0.5 + x / 6.0 - (x * x * x) / 180.0;
+    double em = exp(-fabs(x));
+    if (x > 0) {
+        double d = 1.0 - em;
+        return (em * (1.0 - x) - em * em) / (d * d);
+    }
+    double d = em - 1.0;
+    return (em - 1.0 - x * em) / (d * d);
+}
+
+double ref_selective_flux(double c_from, double c_to, double dv, double z, double T, double pa,
+                          double avail) {
+    /* laws/junctions/selective.py:28-50 with residual 0, gate 1, multiplier avail. */
+    double x = z * F_CONST * dv / (R_CONST * T);
+    return pa * avail / MOL_PER_FMOL * (c_from * ref_bernoulli(-x) - c_to * ref_bernoulli(x));
+}
+
+double ref_route_slope(double c_from, double c_to, double dv, double z, double T, double pa,
+                       double avail) {
+    /* models/surface_pump_v3.py:2747-2755 */
+    double k = z * F_CONST / (R_CONST * T);
+    double x = k * dv;
+    double pre = pa * avail / MOL_PER_FMOL;
+    return pre * k * (-c_from * ref_bernoulli_slope(-x) - c_to * ref_bernoulli_slope(x));
+}
+
+double ref_logaddexp(double a, double b) {
+    double m = a > b ? a : b;
+    return m + log1p(exp(-fabs(a - b)));
+}
+
+static double nak_drive(const cells_world* w) {
+    /* nak_atp_mass_action_positive.py:55-60 */
+    return w->dg0 + R_CONST * w->temperature *
+                        (log(w->adp) + log(w->pi) - log(w->atp) - log(STANDARD_STATE_mM));
+}
+
+void ref_nak_kinetics(const cells_world* w, double na_in, double k_out, double mu_na, double mu_k,
+                      double cap, double avail, double* cycles, double* energy, double* drive) {
+    /* nak_atp_mass_action_positive.py:63-98 */
+    double T = w->temperature;
+    double d = nak_drive(w);
+    double e = d + (-3.0) * mu_na + 2.0 * mu_k;
+    double a = e / (R_CONST * T);
+    double ls = -ref_logaddexp(0.0, 3.0 * (log(w->km_na) - log(na_in))) -
+                ref_logaddexp(0.0, 2.0 * (log(w->km_k) - log(k_out))) -
+                ref_logaddexp(0.0, log(w->km_atp) - log(w->atp));
+    int enabled = cap > 0 && avail > 0;
+    double sc = enabled ? cap : 1.0, sa = enabled ? avail : 1.0;
+    double mag = exp(log(sc) + log(sa) + ls + (a > 0.0 ? a : 0.0));
+    double dep = a >= 0 ? expm1(-a) : -expm1(a);
+    *cycles = enabled ? mag * dep : 0.0;
+    *energy = e;
+    *drive = d;
+}
+
+int ref_nak_domain(const cells_world* w, double na_in, double k_out, double mu_na, double mu_k,
+                   double cap, double avail) {
+    /* nak_atp_mass_action_positive.py:101-149 */
+    double T = w->temperature;
+    int ok = isfinite(na_in) && na_in > 0 && isfinite(k_out) && k_out > 0;
+    ok = ok && T >= 200.0 && T <= 400.0;
+    ok = ok && isfinite(cap) && cap >= 0 && isfinite(avail) && avail >= 0 && avail <= 1;
+    double rt = R_CONST * T;
+    double sn = mu_na / rt, sk = mu_k / rt;
+    ok = ok && isfinite(sn) && fabs(sn) <= NAK_MAX_MU && isfinite(sk) && fabs(sk) <= NAK_MAX_MU;
+    double aff = w->dg0 / rt + log(w->adp) + log(w->pi) - log(w->atp) - log(STANDARD_STATE_mM) +
+                 (-3.0) * sn + 2.0 * sk;
+    return ok && isfinite(aff) && fabs(aff) <= NAK_MAX_AFFINITY;
+}
+
+void ref_ca_kinetics(const cells_world* w, double ca_in, double ca_out, double v, double cap,
+                     double avail, double* cycles, double* energy, double* drive) {
+    /* ca_atp_mass_action_positive.py:70-99 -> plasma_calcium.pump_rates(v) ->
+     * er_calcium.pump_rates(ci, co, -v) with delta_g0 - RT ln 1000 (er_calcium.py:30-50). */
+    double T = w->temperature, rt = R_CONST * T;
+    double dg0k = w->ca_dg0 - R_CONST * T * log(STANDARD_STATE_mM);
+    double dk = dg0k + R_CONST * T * (log(w->adp) + log(w->pi) - log(w->atp));
+    double e = dk + R_CONST * T * (log(ca_out) - log(ca_in)) + 2.0 * F_CONST * (-v);
+    double aff = e / rt;
+    double ls = -ref_logaddexp(0.0, log(w->ca_km_ca) - log(ca_in));
+    ls -= ref_logaddexp(0.0, log(w->ca_km_atp) - log(w->atp));
+    int enabled = cap > 0 && avail > 0;
+    double mag = exp(log(enabled ? cap : 1.0) + log(enabled ? avail : 1.0) + ls +
+                     (aff > 0.0 ? aff : 0.0));
+    double dep = aff >= 0 ? expm1(-aff) : -expm1(aff);
+    *cycles = enabled ? mag * dep : 0.0;
+    *energy = e;
+    *drive = w->ca_dg0 + R_CONST * T *
+                             (log(w->adp) + log(w->pi) - log(w->atp) - log(STANDARD_STATE_mM));
+}
+
+static int ca_domain(const cells_world* w, double ca_in, double ca_out, double v, double cap,
+                     double avail) {
+    /* ca_atp_mass_action_positive.py:102-140 */
+    double T = w->temperature, rt = R_CONST * T;
+    int ok = isfinite(ca_in) && ca_in > 0 && isfinite(ca_out) && ca_out > 0;
+    ok = ok && isfinite(cap) && cap >= 0 && isfinite(avail) && avail >= 0 && avail <= 1;
+    ok = ok && isfinite(v) && fabs(v) <= CA_MAX_VOLTAGE;
+    double aff = w->ca_dg0 / rt + log(w->adp) + log(w->pi) - log(w->atp) -
+                 log(STANDARD_STATE_mM) + log(ca_out) - log(ca_in) - 2.0 * F_CONST * v / rt;
+    return ok && isfinite(aff) && fabs(aff) <= CA_MAX_AFFINITY;
+}
+
+double ref_gate_rhs(double gate, double v, double threshold, double minimum) {
+    /* junction_voltage_relaxation.py:41-59 */
+    double shift = fabs(v) - threshold;
+    double alpha = 1.3 * exp(-77.0 * shift);
+    double log_beta = log(0.065) + 140.0 * shift;
+    double beta = 20.0 * exp(-ref_logaddexp(0.0, -log_beta));
+    return alpha * (1.0 - gate) - beta * (gate - minimum);
+}
+
+static int gate_domain(double gate, double v, double thr, double mn, double avail) {
+    /* junction_voltage_relaxation.py:78-97 */
+    return isfinite(thr) && thr >= 0 && thr <= GATE_MAX_VOLTAGE && isfinite(mn) && mn >= 0 &&
+           mn <= 1 && isfinite(v) && fabs(v) <= GATE_MAX_VOLTAGE && isfinite(gate) &&
+           gate >= mn && gate <= 1 && isfinite(avail) && avail >= 0 && avail <= 1;
+}
+
+/* ---------------------------------------------------------------- layout */
+
+void ref_state_layout(const cells_world* w, cells_state_layout* L) {
+    int32_t S = w->S, f = 0;
+    memset(L, 0, sizeof(*L));
+    L->dn = f; f += S;
+    L->mtr = f; f += S;
+    L->mdiss = f++;
+    L->pcyc = f++;
+    L->pdiss = f++;
+    L->pwork = f++;
+    L->ccyc = L->cdiss = L->cwork = -1;
+    if (w->calcium) { L->ccyc = f++; L->cdiss = f++; L->cwork = f++; }
+    L->dvol = L->wtr = L->wdiss = -1;
+    if (w->water) { L->dvol = f++; L->wtr = f++; L->wdiss = f++; }
+    L->cell_fields = f;
+    f = 0;
+    L->jtr = f; f += S;
+    L->jdiss = f++;
+    L->gate = f++;
+    L->edge_fields = f;
+    f = 0;
+    L->sup = f; f += S;
+    L->echg = f++;
+    L->rwork = f++;
+    L->ework = f++;
+    L->wsup = L->wwork = -1;
+    if (w->water) { L->wsup = f++; L->wwork = f++; }
+    L->world_fields = f;
+    L->cell_base = 0;
+    L->edge_base = (int64_t)L->cell_fields * w->ncell;
+    L->world_base = L->edge_base + (int64_t)L->edge_fields * w->nedge;
+    L->total = L->world_base + (int64_t)L->world_fields * w->W;
+}
+
+#define CF(L, w, y, f, i) (y)[(L)->cell_base + (int64_t)(f) * (w)->ncell + (i)]
+#define EF(L, w, y, f, e) (y)[(L)->edge_base + (int64_t)(f) * (w)->nedge + (e)]
+#define WF(L, w, y, f, k) (y)[(L)->world_base + (int64_t)(f) * (w)->W + (k)]
+
+static double cell_volume(const cells_world* w, const cells_state_layout* L, const double* y,
+                          int64_t i) {
+    return w->water ? w->vol0[i] + CF(L, w, y, L->dvol, i) : w->vol0[i];
+}
+
+static double cell_charge_increment(const cells_world* w, const cells_state_layout* L,
+                                    const double* y, int64_t i) {
+    /* surface_tissue._species_sum order: z0 n0 + z1 n1 + ... */
+    double t = w->valence[0] * CF(L, w, y, L->dn, i);
+    for (int s = 1; s < w->S; ++s) t = t + w->valence[s] * CF(L, w, y, L->dn + s, i);
+    return CHARGE_PER_FMOL * t;
+}
+
+void ref_potentials(const cells_world* w, int32_t k, const double* y, double* phi) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double q = w->q0[i] + cell_charge_increment(w, &L, y, i);
+        phi[i] = w->bath_V + q / w->cap[i];
+    }
+}
+
+/* ---------------------------------------------------------------- explicit rates */
+
+int ref_rates(const cells_world* w, int32_t k, const double* y, const double* phi,
+              const cells_control* c, double* rt_out, double* gm, double* gj) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const int S = w->S;
+    const double T = w->temperature, RT = R_CONST * T;
+    int valid = 1;
+    for (int s = 0; s < S; ++s)
+        valid = valid && isfinite(w->bath_c[s]) && w->bath_c[s] > 0;
+    const double ctl[5] = {c->membrane, c->junction, c->pump, c->calcium_pump, c->water};
+    for (int q = 0; q < 5; ++q) valid = valid && isfinite(ctl[q]) && ctl[q] >= 0 && ctl[q] <= 1;
+
+    /* junction routes (surface_transport.py:310-367, junction branch) */
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        int64_t a = w->snd[e], b = w->rcv[e];
+        double va = cell_volume(w, &L, y, a), vb = cell_volume(w, &L, y, b);
+        double dv = phi[a] - phi[b];
+        double g = EF(&L, w, y, L.gate, e);
+        double avail = c->junction * g;
+        double diss = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double ca = (w->n0[(int64_t)s * w->ncell + a] + CF(&L, w, y, L.dn + s, a)) / va;
+            double cb = (w->n0[(int64_t)s * w->ncell + b] + CF(&L, w, y, L.dn + s, b)) / vb;
+            double z = w->valence[s];
+            double pa = w->paj[(int64_t)s * w->nedge + e];
+            valid = valid && isfinite(ca) && ca > 0 && isfinite(cb) && cb > 0;
+            double J = ref_selective_flux(ca, cb, dv, z, T, pa, avail);
+            double mu = RT * (log(ca) - log(cb)) + z * F_CONST * dv;
+            diss = s == 0 ? J * mu : diss + J * mu;
+            valid = valid && fabs(z * F_CONST * dv / RT) <= MAX_ABS_X && isfinite(J);
+            EF(&L, w, rt_out, L.jtr + s, e) = J;
+            if (gj) gj[(int64_t)s * w->nedge + e] = ref_route_slope(ca, cb, dv, z, T, pa, avail);
+        }
+        EF(&L, w, rt_out, L.jdiss, e) = 1e-3 * diss;
+        valid = valid && isfinite(1e-3 * diss);
+        EF(&L, w, rt_out, L.gate, e) = ref_gate_rhs(g, dv, w->gate_threshold, w->gate_minimum);
+        valid = valid && gate_domain(g, dv, w->gate_threshold, w->gate_minimum, c->junction);
+    }
+
+    /* membrane routes, pumps, owners (surface_transport.py:310-375; surface_pump_v2.py:370-450) */
+    double sup[4] = {0, 0, 0, 0};
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double vol = cell_volume(w, &L, y, i);
+        double dv = phi[i] - w->bath_V;
+        double ci[4], Jm[4], mu[4];
+        double diss = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double z = w->valence[s], co = w->bath_c[s];
+            ci[s] = (w->n0[(int64_t)s * w->ncell + i] + CF(&L, w, y, L.dn + s, i)) / vol;
+            valid = valid && isfinite(ci[s]) && ci[s] > 0;
+            double pa = w->pam[(int64_t)s * w->ncell + i];
+            Jm[s] = ref_selective_flux(ci[s], co, dv, z, T, pa, c->membrane);
+            mu[s] = RT * (log(ci[s]) - log(co)) + z * F_CONST * dv;
+            diss = s == 0 ? Jm[s] * mu[s] : diss + Jm[s] * mu[s];
+            valid = valid && fabs(z * F_CONST * dv / RT) <= MAX_ABS_X && isfinite(Jm[s]);
+            if (gm) gm[(int64_t)s * w->ncell + i] = ref_route_slope(ci[s], co, dv, z, T, pa, c->membrane);
+        }
+        double cyc, energy, drive;
+        ref_nak_kinetics(w, ci[0], w->bath_c[1], mu[0], mu[1], w->nak_cap[i], c->pump, &cyc,
+                         &energy, &drive);
+        valid = valid && ref_nak_domain(w, ci[0], w->bath_c[1], mu[0], mu[1], w->nak_cap[i], c->pump);
+        double extra[4] = {3.0 * cyc, -2.0 * cyc, 0.0, 0.0}; /* -stoichiometry * cycles */
+        if (S > 2) extra[2] = -0.0 * cyc;
+        if (S > 3) extra[3] = -0.0 * cyc;
+        double ccyc = 0, cenergy = 0, cdrive = 0;
+        if (w->calcium) {
+            ref_ca_kinetics(w, ci[3], w->bath_c[3], dv, w->ca_cap[i], c->calcium_pump, &ccyc,
+                            &cenergy, &cdrive);
+            valid = valid && ca_domain(w, ci[3], w->bath_c[3], dv, w->ca_cap[i], c->calcium_pump);
+            /* extra = -stoichiometry*cycles - calcium_stoichiometry*ca_cycles, calcium -1 */
+            for (int s = 0; s < S; ++s) extra[s] = extra[s] - (s == 3 ? -1.0 : 0.0) * ccyc;
+        }
+        /* owners: cell = (-membrane + sum(-junction at sender)) + sum(junction at receiver) */
+        int64_t p0 = w->inc_ptr[i], p1 = w->inc_ptr[i + 1], pm = p0 + w->inc_out[i];
+        for (int s = 0; s < S; ++s) {
+            double out = 0.0, in = 0.0;
+            for (int64_t p = p0; p < pm; ++p) out += -EF(&L, w, rt_out, L.jtr + s, w->inc_edge[p]);
+            for (int64_t p = pm; p < p1; ++p) in += EF(&L, w, rt_out, L.jtr + s, w->inc_edge[p]);
+            double cell = -Jm[s];
+            cell += out;
+            cell += in;
+            CF(&L, w, rt_out, L.dn + s, i) = cell - extra[s];
+            CF(&L, w, rt_out, L.mtr + s, i) = Jm[s];
+            sup[s] -= Jm[s] + extra[s]; /* supply = -inflow at the held bath */
+        }
+        CF(&L, w, rt_out, L.mdiss, i) = 1e-3 * diss;
+        CF(&L, w, rt_out, L.pcyc, i) = cyc;
+        CF(&L, w, rt_out, L.pdiss, i) = -1e-3 * cyc * energy;
+        CF(&L, w, rt_out, L.pwork, i) = -1e-3 * cyc * drive;
+        if (w->calcium) {
+            CF(&L, w, rt_out, L.ccyc, i) = ccyc;
+            CF(&L, w, rt_out, L.cdiss, i) = -1e-3 * ccyc * cenergy;
+            CF(&L, w, rt_out, L.cwork, i) = -1e-3 * ccyc * cdrive;
+        }
+        if (w->water) {
+            CF(&L, w, rt_out, L.dvol, i) = 0.0;
+            CF(&L, w, rt_out, L.wtr, i) = 0.0;
+            CF(&L, w, rt_out, L.wdiss, i) = 0.0;
+        }
+        for (int f = 0; f < L.cell_fields; ++f) valid = valid && isfinite(CF(&L, w, rt_out, f, i));
+    }
+    for (int s = 0; s < S; ++s) WF(&L, w, rt_out, L.sup + s, k) = sup[s];
+    WF(&L, w, rt_out, L.echg, k) = 0.0;
+    WF(&L, w, rt_out, L.rwork, k) = 0.0;
+    WF(&L, w, rt_out, L.ework, k) = 0.0;
+    if (w->water) { WF(&L, w, rt_out, L.wsup, k) = 0.0; WF(&L, w, rt_out, L.wwork, k) = 0.0; }
+    return valid;
+}
+
+/* ---------------------------------------------------------------- domain */
+
+static int domain_valid(const cells_world* w, const cells_state_layout* L, int32_t k,
+                        const double* y) {
+    /* surface_tissue.py:391-420, surface_kinetic.py:422-451: finite fields, positive amounts,
+     * gates in [minimum, 1]; water volumes strictly inside their bounds. */
+    int ok = 1;
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        for (int f = 0; f < L->cell_fields; ++f) ok = ok && isfinite(CF(L, w, y, f, i));
+        for (int s = 0; s < w->S; ++s) {
+            double n = w->n0[(int64_t)s * w->ncell + i] + CF(L, w, y, L->dn + s, i);
+            ok = ok && isfinite(n) && n > 0;
+        }
+        if (w->water) {
+            double v = cell_volume(w, L, y, i);
+            ok = ok && v > w->water_vmin * w->vol0[i] && v < w->water_vmax * w->vol0[i];
+        }
+    }
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        for (int f = 0; f < L->edge_fields; ++f) ok = ok && isfinite(EF(L, w, y, f, e));
+        double g = EF(L, w, y, L->gate, e);
+        ok = ok && g >= w->gate_minimum && g <= 1.0;
+    }
+    for (int f = 0; f < L->world_fields; ++f) ok = ok && isfinite(WF(L, w, y, f, k));
+    return ok;
+}
+
+static void copy_world(const cells_world* w, const cells_state_layout* L, int32_t k,
+                       const double* src, double* dst) {
+    for (int f = 0; f < L->cell_fields; ++f)
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+            CF(L, w, dst, f, i) = CF(L, w, src, f, i);
+    for (int f = 0; f < L->edge_fields; ++f)
+        for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e)
+            EF(L, w, dst, f, e) = EF(L, w, src, f, e);
+    for (int f = 0; f < L->world_fields; ++f) WF(L, w, dst, f, k) = WF(L, w, src, f, k);
+}
+
+/* y_out = y + (wt*dt) * r on every cell and edge field of world k, and on the world fields
+ * listed by with_world (the stage-integrated ones: bath supplies). */
+static void combine(const cells_world* w, const cells_state_layout* L, int32_t k, const double* y,
+                    double h, const double* r, double* out) {
+    for (int f = 0; f < L->cell_fields; ++f)
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+            CF(L, w, out, f, i) = CF(L, w, y, f, i) + h * CF(L, w, r, f, i);
+    for (int f = 0; f < L->edge_fields; ++f)
+        for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e)
+            EF(L, w, out, f, e) = EF(L, w, y, f, e) + h * EF(L, w, r, f, e);
+    for (int s = 0; s < w->S; ++s)
+        WF(L, w, out, L->sup + s, k) = WF(L, w, y, L->sup + s, k) + h * WF(L, w, r, L->sup + s, k);
+}
+
+/* Electrode charge delivered over a step: the held bath's plate-charge change,
+ * -(sum_i q_i(end) - sum_i q_i(start)) (surface_pump.py:273-289 with one held bath). */
+static double electrode_increment(const cells_world* w, const cells_state_layout* L, int32_t k,
+                                  const double* start, const double* end) {
+    double qs = 0.0, qe = 0.0;
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        qs += cell_charge_increment(w, L, start, i);
+        qe += cell_charge_increment(w, L, end, i);
+    }
+    return -qe - -qs;
+}
+
+static void ionic_tail(const cells_world* w, const cells_state_layout* L, int32_t k,
+                       const double* start, double* stage, const double* applied_supply) {
+    /* surface_pump.py:346-359: electrode charge and work, reservoir work of the applied supply. */
+    double electrode = electrode_increment(w, L, k, start, stage);
+    double RT = R_CONST * w->temperature;
+    double work = 0.0;
+    for (int s = 0; s < w->S; ++s) {
+        double mu = RT * log(w->bath_c[s]) + w->valence[s] * F_CONST * w->bath_V;
+        work = s == 0 ? mu * applied_supply[s] : work + mu * applied_supply[s];
+    }
+    WF(L, w, stage, L->echg, k) = WF(L, w, start, L->echg, k) + electrode;
+    WF(L, w, stage, L->rwork, k) = WF(L, w, start, L->rwork, k) + 1e-3 * work;
+    WF(L, w, stage, L->ework, k) = WF(L, w, start, L->ework, k) + 1e12 * w->bath_V * electrode;
+}
+
+/* ---------------------------------------------------------------- water (Heun, h) */
+
+static double water_psi_bath(const cells_world* w) {
+    double total = 0.0;
+    for (int s = 0; s < w->S; ++s) total += w->bath_c[s];
+    return w->bath_pressure - R_CONST * w->temperature * total;
+}
+
+static double water_psi_cell(const cells_world* w, const cells_state_layout* L, const double* y,
+                             int64_t i, double* osm_total) {
+    /* numerics/osmotic_water.py:193-215: psi = K (V - Vref)/Vref + P0 - RT (sum N + X)/V */
+    double v = cell_volume(w, L, y, i);
+    double total = 0.0;
+    for (int s = 0; s < w->S; ++s)
+        total += w->n0[(int64_t)s * w->ncell + i] + CF(L, w, y, L->dn + s, i);
+    total = total + w->imperm[i];
+    if (osm_total) *osm_total = total;
+    double rt = R_CONST * w->temperature;
+    double osmotic = rt * total / v;
+    double hydro = w->water_K * (v - w->vol0[i]) / w->vol0[i] + w->water_P0;
+    return hydro - osmotic;
+}
+
+static int water_half(const cells_world* w, const cells_state_layout* L, int32_t k, double h,
+                      const cells_control* c, double* y /* in/out */) {
+    /* operators/osmotic_water.py:127-221: Heun over h at frozen amounts, cell -> bath flow. */
+    double psib = water_psi_bath(w), rt = R_CONST * w->temperature;
+    int valid = domain_valid(w, L, k, y);
+    valid = valid && isfinite(c->water) && c->water >= 0 && c->water <= 1;
+    /* Gershgorin rate: each cell row G (slope_cell + 0), the bath row the sum of them. */
+    double maxrow = 0.0, bath_row = 0.0;
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double total;
+        (void)water_psi_cell(w, L, y, i, &total);
+        double v = cell_volume(w, L, y, i);
+        double slope = w->water_K / w->vol0[i] + rt * total / (v * v);
+        double edge = w->lpa[i] * c->water * (slope + 0.0);
+        if (edge > maxrow) maxrow = edge;
+        bath_row += edge;
+    }
+    if (bath_row > maxrow) maxrow = bath_row;
+    valid = valid && 2 * h * maxrow <= WATER_MAX_RELAXATION;
+    int64_t c0 = w->cell_off[k], c1 = w->cell_off[k + 1];
+    int64_t n = c1 - c0;
+    double* r0 = (double*)malloc(sizeof(double) * 3 * n);
+    double sup0 = 0.0, sup1 = 0.0;
+    for (int64_t i = c0; i < c1; ++i) {
+        double drive = water_psi_cell(w, L, y, i, NULL) - psib;
+        double flow = w->lpa[i] * c->water * drive;
+        r0[3 * (i - c0)] = -flow;
+        r0[3 * (i - c0) + 1] = flow;
+        r0[3 * (i - c0) + 2] = 1e-3 * flow * drive;
+        sup0 += flow; /* bath supply = -(inflow into the bath) */
+    }
+    sup0 = -sup0;
+    /* predicted state: only the water fields move */
+    double* saved = (double*)malloc(sizeof(double) * 3 * n);
+    for (int64_t i = c0; i < c1; ++i) {
+        saved[3 * (i - c0)] = CF(L, w, y, L->dvol, i);
+        saved[3 * (i - c0) + 1] = CF(L, w, y, L->wtr, i);
+        saved[3 * (i - c0) + 2] = CF(L, w, y, L->wdiss, i);
+        CF(L, w, y, L->dvol, i) = saved[3 * (i - c0)] + h * r0[3 * (i - c0)];
+        CF(L, w, y, L->wtr, i) = saved[3 * (i - c0) + 1] + h * r0[3 * (i - c0) + 1];
+        CF(L, w, y, L->wdiss, i) = saved[3 * (i - c0) + 2] + h * r0[3 * (i - c0) + 2];
+    }
+    double ws = WF(L, w, y, L->wsup, k), ww = WF(L, w, y, L->wwork, k);
+    WF(L, w, y, L->wsup, k) = ws + h * sup0;
+    WF(L, w, y, L->wwork, k) = ww + 1e-3 * psib * (h * sup0);
+    valid = valid && domain_valid(w, L, k, y);
+    for (int64_t i = c0; i < c1; ++i) {
+        double drive = water_psi_cell(w, L, y, i, NULL) - psib;
+        double flow = w->lpa[i] * c->water * drive;
+        double heat = 1e-3 * flow * drive;
+        sup1 += flow;
+        double* a = &r0[3 * (i - c0)];
+        a[0] = 0.5 * (a[0] + -flow);
+        a[1] = 0.5 * (a[1] + flow);
+        a[2] = 0.5 * (a[2] + heat);
+    }
+    sup1 = -sup1;
+    double supm = 0.5 * (sup0 + sup1);
+    for (int64_t i = c0; i < c1; ++i) {
+        CF(L, w, y, L->dvol, i) = saved[3 * (i - c0)] + h * r0[3 * (i - c0)];
+        CF(L, w, y, L->wtr, i) = saved[3 * (i - c0) + 1] + h * r0[3 * (i - c0) + 1];
+        CF(L, w, y, L->wdiss, i) = saved[3 * (i - c0) + 2] + h * r0[3 * (i - c0) + 2];
+    }
+    double supplied = h * supm;
+    WF(L, w, y, L->wsup, k) = ws + supplied;
+    WF(L, w, y, L->wwork, k) = ww + 1e-3 * psib * supplied;
+    valid = valid && domain_valid(w, L, k, y);
+    free(r0);
+    free(saved);
+    return valid;
+}
+
+/* ---------------------------------------------------------------- IMEX Euler */
+
+int ref_step_euler(const cells_world* w, const double* y, double* out, double dt,
+                   const cells_control* c, int32_t* valid_out) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    double* r = (double*)calloc((size_t)L.total, sizeof(double));
+    double* phi = (double*)malloc(sizeof(double) * (size_t)(w->ncell ? w->ncell : 1));
+    double* start = (double*)malloc(sizeof(double) * (size_t)L.total);
+    int all = 1;
+    for (int32_t k = 0; k < w->W; ++k) {
+        int valid = 1;
+        copy_world(w, &L, k, y, start);
+        if (w->water) valid = water_half(w, &L, k, 0.5 * dt, c, start) && valid;
+        /* surface_pump.py:292-362 with no faces: the stage is y + dt r(start potentials) */
+        ref_potentials(w, k, start, phi);
+        valid = ref_rates(w, k, start, phi, c, r, NULL, NULL) && valid;
+        copy_world(w, &L, k, start, out);
+        combine(w, &L, k, start, 1.0 * dt, r, out);
+        double applied[4];
+        for (int s = 0; s < w->S; ++s) applied[s] = dt * WF(&L, w, r, L.sup + s, k);
+        ionic_tail(w, &L, k, start, out, applied);
+        /* stage potentials (the diagonal operator solve) and their residual */
+        ref_potentials(w, k, out, phi);
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+            double q = w->q0[i] + cell_charge_increment(w, &L, out, i);
+            double cphi = w->cap[i] * (phi[i] - w->bath_V);
+            double scale = fabs(q) > fabs(cphi) ? fabs(q) : fabs(cphi);
+            valid = valid && isfinite(phi[i]) && fabs(cphi - q) <= 3e-25 + 2e-12 * scale;
+        }
+        valid = valid && domain_valid(w, &L, k, out);
+        if (w->water) valid = water_half(w, &L, k, 0.5 * dt, c, out) && valid;
+        if (!valid) copy_world(w, &L, k, y, out);
+        valid_out[k] = valid;
+        all = all && valid;
+    }
+    free(r);
+    free(phi);
+    free(start);
+    return all;
+}
+
+/* ---------------------------------------------------------------- sparse factor and solve */
+
+typedef struct {
+    int64_t* ptr; /* [n+1] row structures: columns m < r holding row r, ascending */
+    int64_t* col;
+    int64_t* pos;
+} rowstruct;
+
+static void build_rows(const cells_symbolic* s, rowstruct* rs) {
+    int64_t n = s->n;
+    rs->ptr = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int64_t k = 0; k < n; ++k)
+        for (int64_t p = s->column_start[k] + 1; p < s->column_start[k + 1]; ++p)
+            rs->ptr[s->row_rank[p] + 1]++;
+    for (int64_t r = 0; r < n; ++r) rs->ptr[r + 1] += rs->ptr[r];
+    int64_t m = rs->ptr[n] ? rs->ptr[n] : 1;
+    rs->col = (int64_t*)malloc(sizeof(int64_t) * (size_t)m);
+    rs->pos = (int64_t*)malloc(sizeof(int64_t) * (size_t)m);
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    for (int64_t r = 0; r < n; ++r) fill[r] = rs->ptr[r];
+    for (int64_t k = 0; k < n; ++k) /* ascending k, so each row's columns come out ascending */
+        for (int64_t p = s->column_start[k] + 1; p < s->column_start[k + 1]; ++p) {
+            int64_t r = s->row_rank[p];
+            rs->col[fill[r]] = k;
+            rs->pos[fill[r]] = p;
+            fill[r]++;
+        }
+    free(fill);
+}
+
+static void free_rows(rowstruct* rs) {
+    free(rs->ptr);
+    free(rs->col);
+    free(rs->pos);
+}
+
+static double pairwise(double* t, int64_t m) {
+    /* sparse_cholesky._pairwise_steps / fold: adjacent pairs step by step, +0.0 when empty */
+    if (m == 0) return 0.0;
+    while (m > 1) {
+        int64_t o = 0;
+        for (int64_t a = 0; a < m; a += 2) t[o++] = a + 1 < m ? t[a] + t[a + 1] : t[a];
+        m = o;
+    }
+    return t[0];
+}
+
+void ref_factor(const cells_symbolic* s, double* v) {
+    /* sparse_cholesky.py:474-490 in column order (each level's columns depend only on earlier
+     * ones, so column order gives the level order's bits). */
+    rowstruct rs;
+    build_rows(s, &rs);
+    int64_t n = s->n;
+    int64_t maxc = 1;
+    for (int64_t r = 0; r < n; ++r)
+        if (rs.ptr[r + 1] - rs.ptr[r] > maxc) maxc = rs.ptr[r + 1] - rs.ptr[r];
+    double* terms = (double*)malloc(sizeof(double) * (size_t)maxc);
+    double* acc = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t c0 = s->column_start[k], c1 = s->column_start[k + 1];
+        for (int64_t p = c0; p < c1; ++p) {
+            int64_t r = s->row_rank[p];
+            int64_t m = 0;
+            int64_t a = rs.ptr[k], ae = rs.ptr[k + 1], b = rs.ptr[r], be = rs.ptr[r + 1];
+            while (a < ae && b < be) {
+                if (rs.col[a] == rs.col[b]) {
+                    terms[m++] = v[rs.pos[b]] * v[rs.pos[a]];
+                    ++a;
+                    ++b;
+                } else if (rs.col[a] < rs.col[b]) {
+                    ++a;
+                } else {
+                    ++b;
+                }
+            }
+            acc[p - c0] = v[p] - pairwise(terms, m);
+        }
+        double root = sqrt(acc[0]);
+        v[c0] = root;
+        for (int64_t p = c0 + 1; p < c1; ++p) v[p] = acc[p - c0] * (1.0 / root);
+    }
+    free(terms);
+    free(acc);
+    free_rows(&rs);
+}
+
+void ref_solve(const cells_symbolic* s, const double* f, const double* rhs, double* x) {
+    /* sparse_cholesky.py:493-514, the device form (times each pivot's reciprocal). */
+    rowstruct rs;
+    build_rows(s, &rs);
+    int64_t n = s->n;
+    double* y = (double*)malloc(sizeof(double) * (size_t)n); /* by rank */
+    int64_t maxc = 1;
+    for (int64_t r = 0; r < n; ++r) {
+        if (rs.ptr[r + 1] - rs.ptr[r] > maxc) maxc = rs.ptr[r + 1] - rs.ptr[r];
+        if (s->column_start[r + 1] - s->column_start[r] > maxc)
+            maxc = s->column_start[r + 1] - s->column_start[r];
+    }
+    double* t = (double*)malloc(sizeof(double) * (size_t)maxc);
+    for (int64_t k = 0; k < n; ++k) y[k] = rhs[s->order[k]];
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t m = 0;
+        for (int64_t q = rs.ptr[k]; q < rs.ptr[k + 1]; ++q) t[m++] = f[rs.pos[q]] * y[rs.col[q]];
+        double known = y[k] - pairwise(t, m);
+        y[k] = known * (1.0 / f[s->column_start[k]]);
+    }
+    for (int64_t k = n - 1; k >= 0; --k) {
+        int64_t m = 0;
+        for (int64_t p = s->column_start[k] + 1; p < s->column_start[k + 1]; ++p)
+            t[m++] = f[p] * y[s->row_rank[p]];
+        double known = y[k] - pairwise(t, m);
+        y[k] = known * (1.0 / f[s->column_start[k]]);
+    }
+    for (int64_t k = 0; k < n; ++k) x[s->order[k]] = y[k];
+    free(y);
+    free(t);
+    free_rows(&rs);
+}
+
+/* ---------------------------------------------------------------- ROS2 */
+
+/* The W-method's correction (d f / d phi) dphi on every field the route slopes reach: membrane
+ * and junction transfers, cell amounts and the bath supplies. */
+static void correct(const cells_world* w, const cells_state_layout* L, int32_t k, double coef,
+                    const double* gm, const double* gj, const double* dphi, double* kv) {
+    const int S = w->S;
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        double d = dphi[w->snd[e]] - dphi[w->rcv[e]];
+        for (int s = 0; s < S; ++s)
+            EF(L, w, kv, L->jtr + s, e) += coef * (gj[(int64_t)s * w->nedge + e] * d);
+    }
+    double sup[4] = {0, 0, 0, 0};
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        int64_t p0 = w->inc_ptr[i], p1 = w->inc_ptr[i + 1], pm = p0 + w->inc_out[i];
+        for (int s = 0; s < S; ++s) {
+            double dm = coef * (gm[(int64_t)s * w->ncell + i] * dphi[i]);
+            double out = 0.0, in = 0.0;
+            for (int64_t p = p0; p < pm; ++p) {
+                int64_t e = w->inc_edge[p];
+                out += -(coef * (gj[(int64_t)s * w->nedge + e] * (dphi[w->snd[e]] - dphi[w->rcv[e]])));
+            }
+            for (int64_t p = pm; p < p1; ++p) {
+                int64_t e = w->inc_edge[p];
+                in += coef * (gj[(int64_t)s * w->nedge + e] * (dphi[w->snd[e]] - dphi[w->rcv[e]]));
+            }
+            double cell = -dm;
+            cell += out;
+            cell += in;
+            CF(L, w, kv, L->dn + s, i) += cell;
+            CF(L, w, kv, L->mtr + s, i) += dm;
+            sup[s] -= dm;
+        }
+    }
+    for (int s = 0; s < S; ++s) WF(L, w, kv, L->sup + s, k) += sup[s];
+}
+
+static void charge_rate(const cells_world* w, const cells_state_layout* L, int32_t k,
+                        const double* kv, double* q) {
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+        q[i - w->cell_off[k]] = cell_charge_increment(w, L, kv, i);
+}
+
+int ref_step_ros2(const cells_world* w, const cells_symbolic* sym, const double* y, double* out,
+                  double dt, const cells_control* c, int32_t* valid_out) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const int S = w->S;
+    const double coef = ROS2_GAMMA * dt;
+    double* start = (double*)malloc(sizeof(double) * (size_t)L.total);
+    double* k1 = (double*)calloc((size_t)L.total, sizeof(double));
+    double* k2 = (double*)calloc((size_t)L.total, sizeof(double));
+    double* y1 = (double*)malloc(sizeof(double) * (size_t)L.total);
+    double* phi = (double*)malloc(sizeof(double) * (size_t)(w->ncell ? w->ncell : 1));
+    double* gm = (double*)malloc(sizeof(double) * (size_t)(S * (w->ncell ? w->ncell : 1)));
+    double* gj = (double*)malloc(sizeof(double) * (size_t)(S * (w->nedge ? w->nedge : 1)));
+    int all = 1;
+    for (int32_t k = 0; k < w->W; ++k) {
+        const cells_symbolic* s = &sym[k];
+        int64_t c0 = w->cell_off[k], n = w->cell_off[k + 1] - c0;
+        int64_t e0 = w->edge_off[k];
+        int valid = 1;
+        copy_world(w, &L, k, y, start);
+        if (w->water) valid = water_half(w, &L, k, 0.5 * dt, c, start) && valid;
+        ref_potentials(w, k, start, phi);
+        valid = ref_rates(w, k, start, phi, c, k1, gm, gj) && valid;
+        /* stage matrix M = C + gamma dt G on the lattice pattern, Jacobi-scaled and symmetrised
+         * as surface_imex._factor_sparse (:228-259) */
+        double* Gm = (double*)malloc(sizeof(double) * (size_t)n);
+        double* Gj = (double*)malloc(sizeof(double) * (size_t)(w->edge_off[k + 1] - e0 + 1));
+        for (int64_t i = 0; i < n; ++i) {
+            double t = w->valence[0] * gm[c0 + i];
+            for (int q = 1; q < S; ++q) t = t + w->valence[q] * gm[(int64_t)q * w->ncell + c0 + i];
+            Gm[i] = CHARGE_PER_FMOL * t;
+        }
+        for (int64_t e = e0; e < w->edge_off[k + 1]; ++e) {
+            double t = w->valence[0] * gj[e];
+            for (int q = 1; q < S; ++q) t = t + w->valence[q] * gj[(int64_t)q * w->nedge + e];
+            Gj[e - e0] = CHARGE_PER_FMOL * t;
+        }
+        double* diag = (double*)malloc(sizeof(double) * (size_t)n);
+        double* sc = (double*)malloc(sizeof(double) * (size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t gi = c0 + i;
+            int64_t p0 = w->inc_ptr[gi], p1 = w->inc_ptr[gi + 1];
+            double g = Gm[i];
+            for (int64_t p = p0; p < p1; ++p) g += Gj[w->inc_edge[p] - e0];
+            diag[i] = w->cap[gi] + coef * g;
+            valid = valid && diag[i] > 0;
+            sc[i] = 1.0 / sqrt(diag[i] > 0 ? diag[i] : 1.0);
+        }
+        double* fv = (double*)calloc((size_t)s->nnz, sizeof(double));
+        for (int64_t i = 0; i < n; ++i) {
+            double v = diag[i];
+            fv[s->diag_pos[i]] = (v * sc[i] * sc[i] + v * sc[i] * sc[i]) / 2;
+        }
+        for (int64_t e = e0; e < w->edge_off[k + 1]; ++e) {
+            double v = -(coef * Gj[e - e0]);
+            int64_t a = w->snd[e] - c0, b = w->rcv[e] - c0;
+            fv[s->edge_pos[e - e0]] = (v * sc[a] * sc[b] + v * sc[b] * sc[a]) / 2;
+        }
+        ref_factor(s, fv);
+        for (int64_t p = 0; p < s->nnz; ++p) valid = valid && isfinite(fv[p]);
+        double* q = (double*)malloc(sizeof(double) * (size_t)n);
+        double* x = (double*)malloc(sizeof(double) * (size_t)n);
+        double* dphi = (double*)calloc((size_t)w->ncell, sizeof(double));
+        /* stage 1 */
+        charge_rate(w, &L, k, k1, q);
+        for (int64_t i = 0; i < n; ++i) q[i] = q[i] * sc[i];
+        ref_solve(s, fv, q, x);
+        for (int64_t i = 0; i < n; ++i) dphi[c0 + i] = x[i] * sc[i];
+        correct(w, &L, k, coef, gm, gj, dphi, k1);
+        combine(w, &L, k, start, dt, k1, y1);
+        /* stage 2: f(y + dt k1) - 2 k1 */
+        ref_potentials(w, k, y1, phi);
+        valid = ref_rates(w, k, y1, phi, c, k2, NULL, NULL) && valid;
+        combine(w, &L, k, k2, -2.0, k1, k2);
+        charge_rate(w, &L, k, k2, q);
+        for (int64_t i = 0; i < n; ++i) q[i] = q[i] * sc[i];
+        ref_solve(s, fv, q, x);
+        for (int64_t i = 0; i < n; ++i) dphi[c0 + i] = x[i] * sc[i];
+        correct(w, &L, k, coef, gm, gj, dphi, k2);
+        /* y + (3/2) dt k1 + (1/2) dt k2 */
+        copy_world(w, &L, k, start, out);
+        combine(w, &L, k, start, 1.5 * dt, k1, out);
+        combine(w, &L, k, out, 0.5 * dt, k2, out);
+        for (int f = 0; f < L.world_fields; ++f)
+            if (f < L.sup || f >= L.sup + S) WF(&L, w, out, f, k) = WF(&L, w, start, f, k);
+        double applied[4];
+        for (int q2 = 0; q2 < S; ++q2)
+            applied[q2] = (1.5 * dt) * WF(&L, w, k1, L.sup + q2, k) + (0.5 * dt) * WF(&L, w, k2, L.sup + q2, k);
+        ionic_tail(w, &L, k, start, out, applied);
+        valid = valid && domain_valid(w, &L, k, out);
+        if (w->water) valid = water_half(w, &L, k, 0.5 * dt, c, out) && valid;
+        if (!valid) copy_world(w, &L, k, y, out);
+        valid_out[k] = valid;
+        all = all && valid;
+        free(Gm); free(Gj); free(diag); free(sc); free(fv); free(q); free(x); free(dphi);
+    }
+    free(start); free(k1); free(k2); free(y1); free(phi); free(gm); free(gj);
+    return all;
+}

@@ -1,0 +1,129 @@
+/* Oracle (test infrastructure only): FP64 CPU restatement of the reference's batched tissue step.
+ *
+ * Only tests, the smoke check and the CPU-baseline leg of the benchmark may load this library.
+ * The product (cells_amd/csrc, include/cells_hip.h) never links it.
+ *
+ * The world description below is the packed ensemble layout the product also takes
+ * (include/cells_hip.h, struct cells_world); the two structs carry the same fields in the same
+ * order so one Python description feeds both.
+ */
+#ifndef CELLS_REF_H
+#define CELLS_REF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int32_t S;        /* species (3: Na K Cl, or 4: Na K Cl Ca) */
+    int32_t W;        /* worlds */
+    int32_t water;    /* 1: Strang water half-steps around the ionic step */
+    int32_t calcium;  /* 1: the Ca-ATPase on species 3 */
+    int64_t ncell, nedge;
+    const int64_t* cell_off; /* [W+1] */
+    const int64_t* edge_off; /* [W+1] */
+    double valence[4];
+    double bath_c[4];        /* held bath, mM */
+    double temperature, bath_V;
+    /* Na/K-ATPase and Ca-ATPase constants (maintained pools) */
+    double atp, adp, pi, km_na, km_k, km_atp, dg0;
+    double ca_km_ca, ca_km_atp, ca_dg0;
+    /* junction gate */
+    double gate_threshold, gate_minimum;
+    /* water: modulus, offset, volume bounds as fractions of the reference volume, bath pressure */
+    double water_K, water_P0, water_vmin, water_vmax, bath_pressure;
+    /* per cell [ncell] (species-major [S][ncell] where marked) */
+    const double* vol0;   /* pL */
+    const double* cap;    /* F */
+    const double* pam;    /* [S][ncell] membrane area x permeability, m^3/s */
+    const double* nak_cap;/* fmol cycles/s */
+    const double* ca_cap; /* fmol cycles/s */
+    const double* n0;     /* [S][ncell] reference amounts, fmol */
+    const double* q0;     /* initial cell charge, C */
+    const double* lpa;    /* water conductance, pL/(Pa s) */
+    const double* imperm; /* impermeant osmolytes, fmol */
+    /* per edge [nedge] (global cell indices, sender < receiver) */
+    const int64_t* snd;
+    const int64_t* rcv;
+    const double* paj;    /* [S][nedge] junction area x permeability, m^3/s */
+    /* per cell incidence: out-edges (cell is sender) ascending, then in-edges ascending */
+    const int64_t* inc_ptr;  /* [ncell+1] */
+    const int64_t* inc_edge; /* [2*nedge] */
+    const int32_t* inc_out;  /* [ncell] number of out-edges at the head of each list */
+} cells_world;
+
+typedef struct {
+    double membrane, junction, pump, calcium_pump, water;
+} cells_control;
+
+/* State layout: field-major SoA blocks, cells then edges then worlds. */
+typedef struct {
+    int32_t cell_fields, edge_fields, world_fields;
+    /* cell field indices */
+    int32_t dn, mtr, mdiss, pcyc, pdiss, pwork, ccyc, cdiss, cwork, dvol, wtr, wdiss;
+    /* edge field indices */
+    int32_t jtr, jdiss, gate;
+    /* world field indices */
+    int32_t sup, echg, rwork, ework, wsup, wwork;
+    int64_t cell_base, edge_base, world_base, total;
+} cells_state_layout;
+
+void ref_state_layout(const cells_world* w, cells_state_layout* out);
+
+/* Explicit rates of every state field at the given cell potentials (V), one world at a time.
+ * rates has the state's layout. gm [S][ncell] and gj [S][nedge] receive the routes' slopes
+ * dJ/d(dv) (may be NULL). Returns 1 when every rate is valid. */
+int ref_rates(const cells_world* w, int32_t world, const double* y, const double* phi,
+              const cells_control* c, double* rates, double* gm, double* gj);
+
+/* Cell potentials from the state's charges: V_bath + q_i / C_i. */
+void ref_potentials(const cells_world* w, int32_t world, const double* y, double* phi);
+
+/* One checked step of every world (IMEX Euler of the reference, or the ROS2 W-method), with
+ * water half-steps when w->water. out receives the new state, or the old one where a world
+ * refuses; valid[W] receives each world's acceptance. */
+int ref_step_euler(const cells_world* w, const double* y, double* out, double dt,
+                   const cells_control* c, int32_t* valid);
+
+/* Sparse symbolic structure of one world's stage matrix (the product of oracle/symbolic.py's
+ * bind): column_start [n+1] and each entry's row rank [nnz]; order [n]; plus, per lattice entry,
+ * its factor position: diag_pos [n] (node order), edge_pos [E] (world edge order). */
+typedef struct {
+    int64_t n, nnz;
+    const int64_t* column_start;
+    const int64_t* row_rank;
+    const int64_t* order;
+    const int64_t* diag_pos;
+    const int64_t* edge_pos;
+} cells_symbolic;
+
+int ref_step_ros2(const cells_world* w, const cells_symbolic* sym /* [W] */, const double* y,
+                  double* out, double dt, const cells_control* c, int32_t* valid);
+
+/* Numeric factor in the reference's order (left-looking, pairwise folds in ascending column
+ * order, below entries times the pivot's reciprocal) and the solve (forward then back, each pivot
+ * applied as a multiplication by its reciprocal, the device path of the reference). */
+void ref_factor(const cells_symbolic* s, double* values);
+void ref_solve(const cells_symbolic* s, const double* factor, const double* rhs_node, double* x_node);
+
+/* Point laws, exposed for closed-form tests. */
+double ref_bernoulli(double x);
+double ref_bernoulli_slope(double x);
+double ref_selective_flux(double c_from, double c_to, double dv, double z, double T, double pa,
+                          double avail);
+double ref_route_slope(double c_from, double c_to, double dv, double z, double T, double pa,
+                       double avail);
+double ref_logaddexp(double a, double b);
+void ref_nak_kinetics(const cells_world* w, double na_in, double k_out, double mu_na, double mu_k,
+                      double cap, double avail, double* cycles, double* energy, double* drive);
+int ref_nak_domain(const cells_world* w, double na_in, double k_out, double mu_na, double mu_k,
+                   double cap, double avail);
+void ref_ca_kinetics(const cells_world* w, double ca_in, double ca_out, double v, double cap,
+                     double avail, double* cycles, double* energy, double* drive);
+double ref_gate_rhs(double gate, double v, double threshold, double minimum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

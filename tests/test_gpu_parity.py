@@ -1,0 +1,202 @@
+"""GPU parity of the HIP step against the oracle (FP64 CPU restatement), through the C-ABI.
+
+Bars: rates within 1e-12 relative (ulp-level differences of the FP64 transcendentals); trajectories
+of Vmem, concentrations and volumes within 1e-9 relative over the plain configuration's 1e4 steps;
+the numeric factor and the solves bit for bit (same sums in the same order, no FMA contraction);
+batched (one graph for all worlds) equal to plain (one world per launch) bit for bit; material
+ledgers closed to 1e-12 of their magnitudes.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from cells_amd.world import Config, StateLayout, build, config0, config1, initial_state
+from tests.helpers import conc, material_residual, relerr, vmem, volume
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+def ensemble(p, method="euler"):
+    from cells_amd.ensemble import Ensemble
+
+    return Ensemble(p, method)
+
+
+def test_native_library_loads_on_gpu(torch):
+    from cells_amd import lib
+
+    L = lib.load()
+    for name in lib.EXPORTS:
+        assert hasattr(L, name)
+
+
+def test_rates_match_oracle(torch):
+    for name, cfg in (("config0", config0()), ("config1-small", config1(worlds=3, cells=(40, 120)))):
+        say(f"rates parity {name}")
+        p = build(cfg)
+        y = initial_state(p)
+        # perturb the amounts so every route carries flux both ways
+        L = StateLayout.of(p)
+        rng = np.random.default_rng(3)
+        for s in range(p.S):
+            L.cell(y, L.dn + s)[:] = p.n0[s] * rng.uniform(-0.05, 0.05, p.ncell)
+        e = ensemble(p)
+        r, valid = e.rates(y)
+        r = r.cpu().numpy()
+        ro, vo = oracle.rates(p, y)
+        assert valid.cpu().numpy().all() and vo.all()
+        for f in range(L.cell_fields):
+            a, b = L.cell(r, f), L.cell(ro, f)
+            assert relerr(a, b, floor=1e-12 * np.abs(b).max() + 1e-300) <= 1e-12, (name, "cell", f)
+        for f in range(L.edge_fields):
+            a, b = L.edge(r, f), L.edge(ro, f)
+            assert relerr(a, b, floor=1e-12 * np.abs(b).max() + 1e-300) <= 1e-12, (name, "edge", f)
+        for s in range(p.S):
+            a, b = L.world(r, L.sup + s), L.world(ro, L.sup + s)
+            assert relerr(a, b, floor=1e-12 * np.abs(b).max() + 1e-300) <= 1e-11, (name, "supply", s)
+        e.close()
+
+
+def test_plain_config_trajectory_matches_oracle(torch):
+    """Config 0: one 8x8 world, IMEX Euler dt = 1 ms, water half-steps, 1e4 steps."""
+    p = build(config0())
+    e = ensemble(p)
+    y_gpu = initial_state(p)
+    y_ref = initial_state(p)
+    w = oracle.world_struct(p)
+    worst = 0.0
+    for block in range(10):
+        say(f"config0 block {block}")
+        out, ok = e.step(y_gpu, 1e-3, nsteps=1000)
+        assert ok.cpu().numpy().all()
+        y_gpu = out.cpu().numpy()
+        for _ in range(1000):
+            y_ref, okr = oracle.step_euler(p, y_ref, 1e-3, world=w)
+            assert okr.all()
+        worst = max(worst, relerr(vmem(p, y_gpu), vmem(p, y_ref)),
+                    relerr(conc(p, y_gpu), conc(p, y_ref)),
+                    relerr(volume(p, y_gpu), volume(p, y_ref)))
+    say(f"config0 worst relative error {worst:.3e}")
+    assert worst <= 1e-9
+    assert material_residual(p, y_gpu) <= 1e-12
+    e.close()
+
+
+def test_batched_equals_plain_bitwise(torch):
+    for method, cfg, dt in (("euler", config0(worlds=4, cells=(20, 70), seed=5), 1e-3),
+                            ("ros2", config1(worlds=4, cells=(30, 90), seed=6), 5e-3)):
+        say(f"batched vs plain {method}")
+        p = build(cfg)
+        e = ensemble(p, method)
+        y = initial_state(p)
+        a, oka = e.step(y, dt, nsteps=7, batched=True)
+        b, okb = e.step(y, dt, nsteps=7, batched=False)
+        assert oka.cpu().numpy().all() and okb.cpu().numpy().all()
+        assert torch.equal(a, b)
+        e.close()
+
+
+def test_ros2_trajectory_matches_oracle(torch):
+    p = build(config1(worlds=3, cells=(256, 420), seed=7))
+    syms, sym = oracle.world_symbolics(p)
+    e = ensemble(p, "ros2")
+    y_gpu = initial_state(p)
+    y_ref = initial_state(p)
+    w = oracle.world_struct(p)
+    for block in range(4):
+        say(f"ros2 block {block}")
+        out, ok = e.step(y_gpu, 5e-3, nsteps=25)
+        assert ok.cpu().numpy().all()
+        y_gpu = out.cpu().numpy()
+        for _ in range(25):
+            y_ref, okr = oracle.step_ros2(p, y_ref, 5e-3, sym, world=w)
+            assert okr.all()
+    err = max(relerr(vmem(p, y_gpu), vmem(p, y_ref)), relerr(conc(p, y_gpu), conc(p, y_ref)))
+    say(f"ros2 worst relative error {err:.3e}")
+    assert err <= 1e-9
+    assert material_residual(p, y_gpu) <= 1e-12
+    e.close()
+
+
+def test_factor_and_solve_bitwise_against_oracle(torch):
+    p = build(config1(worlds=3, cells=(64, 300), seed=8))
+    syms, sym = oracle.world_symbolics(p)
+    e = ensemble(p, "ros2")
+    rng = np.random.default_rng(9)
+    values = np.zeros(e.factor_size)
+    rhs = rng.uniform(-1, 1, p.ncell)
+    expect_f, expect_x = [], []
+    for k in range(p.W):
+        d, ep, off = e.positions(k)
+        c0, c1 = p.cell_off[k], p.cell_off[k + 1]
+        e0, e1 = p.edge_off[k], p.edge_off[k + 1]
+        n = c1 - c0
+        off_diag = rng.uniform(-1, 0, e1 - e0)
+        diag = np.full(n, 0.5) + rng.uniform(0.5, 2.0, n)
+        np.add.at(diag, p.snd[e0:e1] - c0, -off_diag)
+        np.add.at(diag, p.rcv[e0:e1] - c0, -off_diag)
+        nnz = int(syms[k].column_start[-1])
+        v = np.zeros(nnz)
+        v[d] = diag
+        v[ep] = off_diag
+        values[off:off + nnz] = v
+        # the oracle's positions are the same pattern: check, then factor and solve there
+        vo = np.zeros(nnz)
+        vo[np.asarray([sym[k].diag_pos[i] for i in range(n)])] = diag
+        vo[np.asarray([sym[k].edge_pos[q] for q in range(e1 - e0)])] = off_diag
+        assert np.array_equal(v, vo)
+        f = oracle.factor(sym[k], v)
+        expect_f.append(f)
+        expect_x.append(oracle.solve(sym[k], f, rhs[c0:c1]))
+    say("factor bitwise")
+    fg = e.factor(values).cpu().numpy()
+    xg = e.solve(fg, rhs).cpu().numpy()
+    for k in range(p.W):
+        _, _, off = e.positions(k)
+        f = expect_f[k]
+        assert fg[off:off + f.size].tobytes() == f.tobytes()
+        c0, c1 = p.cell_off[k], p.cell_off[k + 1]
+        assert xg[c0:c1].tobytes() == expect_x[k].tobytes()
+    e.close()
+
+
+def test_a_refused_world_keeps_its_state_and_the_others_step(torch):
+    p = build(config0(worlds=3, cells=(16, 16), seed=2))
+    L = StateLayout.of(p)
+    y = initial_state(p)
+    # world 1: an amount below zero is outside the domain, so its every step refuses
+    c0 = p.cell_off[1]
+    L.cell(y, L.dn)[c0] = -2.0 * p.n0[0][c0]
+    e = ensemble(p)
+    out, ok = e.step(y, 1e-3, nsteps=3)
+    out = out.cpu().numpy()
+    ok = ok.cpu().numpy()
+    assert ok.tolist() == [1, 0, 1]
+    yo = y.copy()
+    ref = y.copy()
+    w = oracle.world_struct(p)
+    for _ in range(3):
+        ref, okr = oracle.step_euler(p, ref, 1e-3, world=w)
+    assert okr.tolist() == [True, False, True]
+    c1 = p.cell_off[2]
+    for f in range(L.cell_fields):
+        assert np.array_equal(L.cell(out, f)[c0:c1], L.cell(yo, f)[c0:c1])
+    assert relerr(vmem(p, out)[:c0], vmem(p, ref)[:c0]) <= 1e-12
+    e.close()
