@@ -52,11 +52,17 @@ def test_rates_match_oracle(torch):
         say(f"rates parity {name}")
         p = build(cfg)
         y = initial_state(p)
-        # perturb the amounts so every route carries flux both ways
+        # perturb the amounts so every route carries flux both ways: Na and Cl by the same
+        # amount (no charge), K by a few mV's worth of charge, Ca by 5 %
         L = StateLayout.of(p)
         rng = np.random.default_rng(3)
-        for s in range(p.S):
-            L.cell(y, L.dn + s)[:] = p.n0[s] * rng.uniform(-0.05, 0.05, p.ncell)
+        a = p.n0[0] * rng.uniform(-0.05, 0.05, p.ncell)
+        L.cell(y, L.dn + 0)[:] = a
+        L.cell(y, L.dn + 2)[:] = a
+        L.cell(y, L.dn + 1)[:] = rng.uniform(-5e-4, 5e-4, p.ncell)
+        if p.S > 3:
+            L.cell(y, L.dn + 3)[:] = p.n0[3] * rng.uniform(-0.05, 0.05, p.ncell)
+        L.edge(y, L.gate)[:] = rng.uniform(0.5, 1.0, p.nedge)
         e = ensemble(p)
         r, valid = e.rates(y)
         r = r.cpu().numpy()

@@ -1,0 +1,48 @@
+"""The C-ABI library on the CPU: it loads, exports every entry point include/*.h declares, and
+refuses (never falls back) without a GPU. No compute call touches a device here."""
+
+import glob
+import os
+import re
+
+import pytest
+
+from cells_amd import lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(cells_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_every_declared_entry_point_is_exported(hip_lib):
+    names = declared()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(hip_lib, name), name
+    assert set(names) == set(lib.EXPORTS)
+
+
+def test_the_product_never_imports_the_oracle():
+    for path in glob.glob(os.path.join(ROOT, "cells_amd", "**", "*.*"), recursive=True):
+        if path.endswith((".py", ".hip", ".cpp", ".h")):
+            text = open(path).read()
+            assert "import oracle" not in text and "from oracle" not in text, path
+            assert "cells_ref" not in text, path
+
+
+def test_no_gpu_means_an_error_not_a_fallback():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from cells_amd.ensemble import Ensemble
+    from cells_amd.world import build, config0
+
+    with pytest.raises(RuntimeError):
+        Ensemble(build(config0()))

@@ -1,0 +1,99 @@
+"""The oracle's whole step on the CPU: ledgers, observed order, refusals, restart, water.
+
+The reference's own whole-step checks restated here (tests/unit/test_surface_pump.py of the
+reference): material ledgers closed, observed order (Euler 1, second-order methods 2) against a
+fine reference, restart bitwise (:339), refusals keep the state (:339, :359), and water ledgers
+(tests/unit/test_osmotic_water.py). ROS2 is this build's own method; its order 2 holds for any
+stage Jacobian (a W-method), which these tests confirm on the coupled tissue.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from cells_amd.world import StateLayout, build, config0, config1, initial_state
+from tests.helpers import conc, material_residual, relerr, vmem, volume
+
+
+def run(p, y, dt, n, method="euler", sym=None, ctl=None):
+    w = oracle.world_struct(p)
+    for _ in range(n):
+        if method == "euler":
+            y, ok = oracle.step_euler(p, y, dt, ctl=ctl, world=w)
+        else:
+            y, ok = oracle.step_ros2(p, y, dt, sym, ctl=ctl, world=w)
+        assert ok.all()
+    return y
+
+
+def test_euler_ledgers_close_with_water():
+    p = build(config0())
+    y = run(p, initial_state(p), 1e-3, 300)
+    assert material_residual(p, y) <= 1e-13
+    L = StateLayout.of(p)
+    dv = L.cell(y, L.dvol)
+    # water: each cell's transfer is its volume loss, the bath supplies the net
+    assert np.allclose(L.cell(y, L.wtr), -dv, rtol=0, atol=1e-15 * np.abs(dv).max())
+    assert abs(dv.sum() - L.world(y, L.wsup)[0]) <= 1e-13 * np.abs(dv).sum()
+    # heat ledgers only grow
+    for f in (L.mdiss, L.wdiss):
+        assert np.all(L.cell(y, f) >= 0)
+    assert np.all(L.edge(y, L.jdiss) >= 0)
+
+
+def test_ros2_ledgers_close():
+    p = build(config1(worlds=2, cells=(50, 120), seed=3))
+    _, sym = oracle.world_symbolics(p)
+    y = run(p, initial_state(p), 5e-3, 100, "ros2", sym)
+    assert material_residual(p, y) <= 1e-13
+
+
+def test_restart_is_bitwise():
+    p = build(config0())
+    a = run(p, initial_state(p), 1e-3, 20)
+    b = run(p, run(p, initial_state(p), 1e-3, 10), 1e-3, 10)
+    assert a.tobytes() == b.tobytes()
+
+
+def _observed_order(p, method, dts, t_end, sym=None, fine=None):
+    ref = run(p, initial_state(p), t_end / fine, fine, "ros2", sym)
+    errs = []
+    for dt in dts:
+        n = int(round(t_end / dt))
+        y = run(p, initial_state(p), dt, n, method, sym)
+        errs.append(max(relerr(vmem(p, y), vmem(p, ref)), relerr(conc(p, y), conc(p, ref))))
+    return [np.log2(errs[i] / errs[i + 1]) for i in range(len(errs) - 1)], errs
+
+
+def test_ros2_is_second_order():
+    p = build(config1(worlds=1, cells=(36, 36), seed=4))
+    _, sym = oracle.world_symbolics(p)
+    orders, errs = _observed_order(p, "ros2", [0.02, 0.01, 0.005], 0.2, sym, fine=1600)
+    assert all(1.8 <= o <= 2.3 for o in orders), (orders, errs)
+
+
+def test_euler_is_first_order():
+    p = build(config0(water=False, cells=(36, 36), seed=4))
+    # the fine reference is ROS2 on the same world (the order test needs no symbolic for Euler)
+    _, sym = oracle.world_symbolics(p)
+    orders, errs = _observed_order(p, "euler", [1e-3, 5e-4, 2.5e-4], 0.05, sym, fine=800)
+    assert all(0.85 <= o <= 1.2 for o in orders), (orders, errs)
+
+
+def test_an_availability_above_one_refuses_and_keeps_the_state():
+    p = build(config0())
+    y = initial_state(p)
+    out, ok = oracle.step_euler(p, y, 1e-3, ctl=oracle.control(pump=1.5))
+    assert not ok.any()
+    assert out.tobytes() == y.tobytes()
+
+
+def test_water_relaxes_to_osmotic_balance():
+    """Swelling after a bath dilution: the volume moves toward the van 't Hoff balance."""
+    p = build(config0(water=True))
+    L = StateLayout.of(p)
+    y = initial_state(p)
+    p.bath_c[:] = p.bath_c * 0.95  # hypotonic bath (the oracle reads it from the world struct)
+    y = run(p, y, 1e-3, 500)
+    dv = L.cell(y, L.dvol)
+    assert np.all(dv > 0)  # cells swell in a hypotonic bath
