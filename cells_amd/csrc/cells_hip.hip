@@ -961,6 +961,119 @@ __global__ __launch_bounds__(kBlock) void k_world_rates(Params P, const double* 
     for (int f = P.S; f < P.wf; ++f) WF(P, R, f, w) = 0.0;
 }
 
+// ------------------------------------------------------------------------------------------
+// Device row checks (models/surface_pump_checks.py:112-192, bounds surface_pump.py:752-800):
+// per world, the domain, the material ledgers per cell and species and per world, the global
+// charge (electrode charge against the cells' charge increments), the water ledger, and
+// dissipation non-negative and non-decreasing since `prev`. Bits of failed[w] in that order;
+// ratios[w*4 + k] = the largest |residual| / bound of ledger k (material, total, charge, water).
+constexpr double kEps = 2.220446049250313e-16;
+constexpr double kLedgerStepRounding = 2.5;
+constexpr double kDissipationAtol = 1e-12;
+constexpr double kGlobalChargeAtol = 3e-25;
+
+__device__ __forceinline__ double ratio_of(double residual, double bound) {
+  double m = fabs(residual);
+  if (m == 0.0) return 0.0;
+  double r = m / bound;
+  return isnan(r) ? INFINITY : r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_device_checks(Params P, const double* prev, const double* y,
+                                                          double steps, double material_atol,
+                                                          int* failed, double* ratios) {
+  __shared__ double sh[kBlock];
+  int w = blockIdx.x;
+  int c0 = P.cell_off[w], c1 = P.cell_off[w + 1], e0 = P.edge_off[w], e1 = P.edge_off[w + 1];
+  double rounding = kLedgerStepRounding * kEps * steps;
+  bool domain = true, dissip = false;
+  double rmat = 0.0;
+  double tot[4] = {0, 0, 0, 0}, totmag[4] = {0, 0, 0, 0};
+  double q = 0.0, qmag = 0.0, wsum = 0.0, wmag = 0.0, rwat = 0.0;
+  for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    domain = domain && cell_domain(P, y, y, i);
+    double cyc = CFc(P, y, P.f_pcyc, i);
+    int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
+    double zq = 0.0, zm = 0.0;
+    for (int s = 0; s < P.S; ++s) {
+      double dn = CFc(P, y, P.f_dn + s, i), mtr = CFc(P, y, P.f_mtr + s, i);
+      double pump = s == 0 ? 3.0 * cyc : s == 1 ? -2.0 * cyc : 0.0;
+      if (s == 3 && P.calcium) pump += CFc(P, y, P.f_ccyc, i);
+      double net = 0.0, netm = 0.0;
+      for (int p = p0; p < pm; ++p) {
+        double j = EFc(P, y, P.f_jtr + s, P.inc_edge[p]);
+        net -= j;
+        netm += fabs(j);
+      }
+      for (int p = pm; p < p1; ++p) {
+        double j = EFc(P, y, P.f_jtr + s, P.inc_edge[p]);
+        net += j;
+        netm += fabs(j);
+      }
+      double res = dn + mtr + pump - net;
+      double mag = fabs(dn) + fabs(mtr) + fabs(pump) + netm;
+      rmat = fmax(rmat, ratio_of(res, material_atol + rounding * mag));
+      tot[s] += dn;
+      totmag[s] += fabs(dn);
+      zq += P.z[s] * dn;
+      zm += fabs(P.z[s] * dn);
+    }
+    q += kChargePerFmol * zq;
+    qmag += kChargePerFmol * zm;
+    int df[4] = {P.f_mdiss, P.f_pdiss, P.calcium ? P.f_cdiss : -1, P.water ? P.f_wdiss : -1};
+    for (int k = 0; k < 4; ++k) {
+      if (df[k] < 0) continue;
+      double v = CFc(P, y, df[k], i), u = CFc(P, prev, df[k], i);
+      dissip = dissip || v < -kDissipationAtol || v - u < -kDissipationAtol;
+    }
+    if (P.water) {
+      double dv = CFc(P, y, P.f_dvol, i), tr = CFc(P, y, P.f_wtr, i);
+      rwat = fmax(rwat, ratio_of(dv + tr, material_atol + rounding * (fabs(dv) + fabs(tr))));
+      wsum += dv;
+      wmag += fabs(dv);
+    }
+  }
+  for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
+    domain = domain && edge_domain(P, y, e);
+    double v = EFc(P, y, P.f_jdiss, e), u = EFc(P, prev, P.f_jdiss, e);
+    dissip = dissip || v < -kDissipationAtol || v - u < -kDissipationAtol;
+  }
+  double d = block_max(domain ? 0.0 : 1.0, sh);
+  double di = block_max(dissip ? 1.0 : 0.0, sh);
+  rmat = block_max(rmat, sh);
+  rwat = block_max(rwat, sh);
+  double rtot = 0.0;
+  for (int s = 0; s < P.S; ++s) {
+    double t = block_sum(tot[s], sh), m = block_sum(totmag[s], sh);
+    double sup = WFc(P, y, P.f_sup + s, w);
+    rtot = fmax(rtot, ratio_of(t - sup, material_atol + rounding * (m + fabs(sup))));
+  }
+  q = block_sum(q, sh);
+  qmag = block_sum(qmag, sh);
+  wsum = block_sum(wsum, sh);
+  wmag = block_sum(wmag, sh);
+  if (threadIdx.x == 0) {
+    double echg = WFc(P, y, P.f_echg, w);
+    double rq = ratio_of(echg + q, kGlobalChargeAtol + (256.0 * kEps + rounding) * (qmag + fabs(echg)));
+    if (P.water) {
+      double ws = WFc(P, y, P.f_wsup, w);
+      rwat = fmax(rwat, ratio_of(wsum - ws, material_atol + rounding * (wmag + fabs(ws))));
+    }
+    int bits = 0;
+    if (d > 0) bits |= 1;
+    if (!(rmat <= 1.0)) bits |= 2;
+    if (!(rtot <= 1.0)) bits |= 4;
+    if (!(rq <= 1.0)) bits |= 8;
+    if (di > 0) bits |= 16;
+    if (!(rwat <= 1.0)) bits |= 32;
+    failed[w] = bits;
+    ratios[4 * w + 0] = rmat;
+    ratios[4 * w + 1] = rtot;
+    ratios[4 * w + 2] = rq;
+    ratios[4 * w + 3] = rwat;
+  }
+}
+
 int blocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -1646,6 +1759,18 @@ extern "C" int cells_solve(cells_ensemble* e, void* stream, const double* values
   size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
   hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kBlock), lds, static_cast<hipStream_t>(stream), e->shapes,
                      e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x);
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_device_checks(cells_ensemble* e, void* stream, const double* previous,
+                                   const double* y, double steps, double material_atol,
+                                   int32_t* failed_out, double* ratios_out) {
+  if (!e || !previous || !y || !failed_out || !ratios_out) return fail(CELLS_EINVAL, "NULL argument");
+  if (!(steps >= 0) || !(material_atol >= 0)) return fail(CELLS_EINVAL, "steps and tolerance must be >= 0");
+  HIP_TRY(hipSetDevice(e->device));
+  hipLaunchKernelGGL(k_device_checks, dim3(e->W), dim3(kBlock), 0, static_cast<hipStream_t>(stream), e->P,
+                     previous, y, steps, material_atol, failed_out, ratios_out);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }

@@ -138,6 +138,18 @@ class Ensemble:
                                            ctypes.c_void_p(x.data_ptr())))
         return x
 
+    def device_checks(self, previous, y, steps, material_atol=1e-12):
+        """(failed bits [W], ratios [W, 4]) of a state against its predecessor."""
+        t = self.torch
+        a, b = self.to_device(previous), self.to_device(y)
+        failed = t.empty(self.packed.W, dtype=t.int32, device=self.device)
+        ratios = t.empty((self.packed.W, 4), dtype=t.float64, device=self.device)
+        _lib.check(_lib.load().cells_device_checks(
+            self._h, self._stream(), ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+            float(steps), float(material_atol), ctypes.c_void_p(failed.data_ptr()),
+            ctypes.c_void_p(ratios.data_ptr())))
+        return failed, ratios
+
     def kernel_times(self, y, dt, reps=3, ctl=None):
         """[(kernel name, mean ms)] of one step's launches, in launch order (HIP events)."""
         y = self.to_device(y)

@@ -29,7 +29,7 @@ EXPORTS = (
     "cells_symbolic_export", "cells_symbolic_free", "cells_ensemble_create",
     "cells_ensemble_destroy", "cells_state_size", "cells_step", "cells_rates",
     "cells_factor_size", "cells_factor", "cells_solve", "cells_world_positions",
-    "cells_kernel_times",
+    "cells_kernel_times", "cells_device_checks",
 )
 
 
@@ -99,6 +99,9 @@ def load():
                                        ctypes.c_double, ctypes.POINTER(ControlStruct),
                                        ctypes.c_int32, ctypes.c_int32, _f64p, ctypes.c_char_p,
                                        _i32p]
+    lib.cells_device_checks.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_void_p, ctypes.c_void_p]
     lib.cells_world_positions.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i64p, _i64p, _i64p]
     lib.cells_minimum_degree.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p]
     lib.cells_bind.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p,

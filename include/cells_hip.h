@@ -117,6 +117,16 @@ int cells_kernel_times(cells_ensemble* e, void* stream, const double* y, double 
 int cells_rates(cells_ensemble* e, void* stream, const double* y, const cells_control* control,
                 double* rates_out, int32_t* valid_out);
 
+/* Device row checks of a state against its predecessor (models/surface_pump_checks.py:112
+ * row_checks; bounds models/surface_pump.py:752-800): failed_out[w] bits 1 domain, 2 material
+ * ledger per cell and species, 4 material ledger per world (cells vs bath supply), 8 global charge
+ * (electrode vs cells), 16 dissipation negative or decreasing, 32 water ledger;
+ * ratios_out[4 w + k] = largest |residual| / bound of ledger k (material, total, charge, water).
+ * steps: accepted steps behind y (the rounding allowance 2.5 eps steps x magnitudes). */
+int cells_device_checks(cells_ensemble* e, void* stream, const double* previous, const double* y,
+                        double steps, double material_atol, int32_t* failed_out,
+                        double* ratios_out);
+
 /* The batched numeric factor and solve on the stage pattern (numerics/sparse_cholesky.py:474-514):
  * values: device [factor size] (each world's assembled entries at its factor positions), factored
  * in place; rhs/x: device [ncell] in node order, one system per world. */

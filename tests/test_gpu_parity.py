@@ -206,3 +206,30 @@ def test_a_refused_world_keeps_its_state_and_the_others_step(torch):
         assert np.array_equal(L.cell(out, f)[c0:c1], L.cell(yo, f)[c0:c1])
     assert relerr(vmem(p, out)[:c0], vmem(p, ref)[:c0]) <= 1e-12
     e.close()
+
+
+def test_device_checks_pass_on_accepted_rows_and_catch_a_planted_ledger_error(torch):
+    p = build(config1(worlds=3, cells=(40, 90), seed=12))
+    e = ensemble(p, "ros2")
+    y0 = initial_state(p)
+    y1, ok = e.step(y0, 5e-3, nsteps=20)
+    assert ok.cpu().numpy().all()
+    y2, ok = e.step(y1, 5e-3, nsteps=20)
+    failed, ratios = e.device_checks(y1, y2, steps=40)
+    assert failed.cpu().numpy().tolist() == [0, 0, 0], ratios.cpu().numpy()
+    r = ratios.cpu().numpy()
+    assert np.all(r[:, :3] <= 1.0)
+    # a planted booking error: one cell's amount moved without its transfer (material ledgers)
+    bad = y2.cpu().numpy().copy()
+    L = StateLayout.of(p)
+    c = p.cell_off[1] + 3
+    L.cell(bad, L.dn + 1)[c] += 1e-9 * abs(p.n0[1][c])
+    failed, _ = e.device_checks(y1, bad, steps=40)
+    f = failed.cpu().numpy().tolist()
+    assert f[0] == 0 and f[2] == 0 and f[1] & 2 and f[1] & 4
+    # heat that decreased since the previous row
+    bad = y2.cpu().numpy().copy()
+    L.cell(bad, L.mdiss)[p.cell_off[2]] = L.cell(y1.cpu().numpy(), L.mdiss)[p.cell_off[2]] - 1.0
+    failed, _ = e.device_checks(y1, bad, steps=40)
+    assert failed.cpu().numpy().tolist()[2] & 16
+    e.close()
