@@ -203,3 +203,22 @@ def world_symbolics(p):
             cache[n] = bind(n, p.snd[e0:e1] - c0, p.rcv[e0:e1] - c0)
         syms.append(cache[n])
     return syms, symbolic_structs(p, syms)
+
+
+def ros2_vjp(p, y, dt, sym, ybar, ctl=None, world=None):
+    """Reverse of one ROS2 step at y: (cotangent on y's amounts and gates, pam_bar [S, n],
+    nak_cap_bar [n], paj_bar [S, E]) for a cotangent ybar on the step's output."""
+    w = world if world is not None else world_struct(p)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    yb = np.ascontiguousarray(ybar, dtype=np.float64)
+    out = np.zeros_like(y)
+    pam_bar = np.zeros((p.S, p.ncell))
+    cap_bar = np.zeros(p.ncell)
+    paj_bar = np.zeros((p.S, p.nedge))
+    ok = lib().ref_ros2_vjp(ctypes.byref(w), sym, _p(y, _f64p), ctypes.c_double(dt), ctypes.byref(ctl),
+                            _p(yb, _f64p), _p(out, _f64p), _p(pam_bar, _f64p), _p(cap_bar, _f64p),
+                            _p(paj_bar, _f64p))
+    if not ok:
+        raise ValueError("the step at y is refused, or the world has water or calcium")
+    return out, pam_bar, cap_bar, paj_bar

@@ -828,3 +828,361 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym, const double*
     free(start); free(k1); free(k2); free(y1); free(phi); free(gm); free(gj);
     return all;
 }
+
+/* ================================================================ adjoint of the ROS2 step
+ *
+ * The reference has no adjoint (it differentiates only forward, analysis/stationary.py:505-517);
+ * this is the exact reverse of ref_step_ros2's arithmetic restricted to the fields the dynamics
+ * read: the amounts dN [S][n] and the junction gates [E] (3-ion worlds without water; the
+ * ledgers never feed back). Parameter cotangents are taken with respect to the membrane
+ * area x permeability pam [S][n], the Na/K capacity [n] and the junction area x permeability
+ * paj [S][E]. It is pinned by central finite differences of ref_step_ros2 (tests).
+ */
+
+static double bernoulli_second(double x) {
+    /* B''(x) of B(x) = x / (e^x - 1): the Bernoulli-number series below |x| = 0.1, else
+     * e^x (x (e^x + 1) - 2 (e^x - 1)) / (e^x - 1)^3 in the form that stays finite for large |x|. */
+    if (fabs(x) < 0.1) {
+        double x2 = x * x;
+        return 1.0 / 6.0 - x2 / 60.0 + x2 * x2 / 1008.0 - x2 * x2 * x2 / 21600.0 +
+               x2 * x2 * x2 * x2 / 532224.0;
+    }
+    if (x > 0) {
+        double em = exp(-x), d = 1.0 - em;
+        return em * (x * (1.0 + em) - 2.0 * d) / (d * d * d);
+    }
+    double e = exp(x), d = e - 1.0;
+    return e * (x * (e + 1.0) - 2.0 * d) / (d * d * d);
+}
+
+double ref_bernoulli_second(double x) { return bernoulli_second(x); }
+
+typedef struct {
+    double *cbar, *vbar; /* [S][n], [n] cotangents of concentrations and potentials */
+} local_bars;
+
+/* Cotangents of f_N (amount rates) and f_g (gate rates) at y pulled back to concentrations,
+ * potentials, gates and parameters; adds. */
+static void rates_vjp(const cells_world* w, const cells_state_layout* L, int32_t k, const double* y,
+                      const double* phi, const cells_control* c, const double* fN, const double* fg,
+                      const double* gmbar, const double* gjbar, double* cbar, double* vbar,
+                      double* gatebar, double* pam_bar, double* cap_bar, double* paj_bar) {
+    const int S = w->S;
+    const double T = w->temperature, RT = R_CONST * T;
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        int64_t a = w->snd[e], b = w->rcv[e];
+        double va = cell_volume(w, L, y, a), vb = cell_volume(w, L, y, b);
+        double dv = phi[a] - phi[b];
+        double g = EF(L, w, y, L->gate, e);
+        double vsum = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double z = w->valence[s], kk = z * F_CONST / RT, x = kk * dv;
+            double ca = (w->n0[(int64_t)s * w->ncell + a] + CF(L, w, y, L->dn + s, a)) / va;
+            double cb = (w->n0[(int64_t)s * w->ncell + b] + CF(L, w, y, L->dn + s, b)) / vb;
+            double pa = w->paj[(int64_t)s * w->nedge + e];
+            double unit = c->junction / MOL_PER_FMOL; /* J = pa g unit (...) */
+            double Bm = ref_bernoulli(-x), Bp = ref_bernoulli(x);
+            double Sm = ref_bernoulli_slope(-x), Sp = ref_bernoulli_slope(x);
+            double Qm = bernoulli_second(-x), Qp = bernoulli_second(x);
+            double A = pa * g * unit;
+            double core = ca * Bm - cb * Bp;
+            double gj = A * kk * (-ca * Sm - cb * Sp);
+            /* J_e leaves a (sender) and enters b */
+            double Jbar = (fN ? fN[(int64_t)s * w->ncell + b] - fN[(int64_t)s * w->ncell + a] : 0.0);
+            double Gbar = gjbar ? gjbar[(int64_t)s * w->nedge + e] : 0.0;
+            cbar[(int64_t)s * w->ncell + a] += Jbar * A * Bm + Gbar * (-A * kk * Sm);
+            cbar[(int64_t)s * w->ncell + b] += Jbar * (-A * Bp) + Gbar * (-A * kk * Sp);
+            double dv_bar = Jbar * gj + Gbar * (A * kk * kk * (ca * Qm - cb * Qp));
+            vsum += dv_bar;
+            gatebar[e] += Jbar * (pa * unit * core) + Gbar * (pa * unit * kk * (-ca * Sm - cb * Sp));
+            if (paj_bar)
+                paj_bar[(int64_t)s * w->nedge + e] += Jbar * (g * unit * core) +
+                                                      Gbar * (g * unit * kk * (-ca * Sm - cb * Sp));
+        }
+        if (fg) {
+            double shift = fabs(dv) - w->gate_threshold;
+            double alpha = 1.3 * exp(-77.0 * shift);
+            double log_beta = log(0.065) + 140.0 * shift;
+            double beta = 20.0 * exp(-ref_logaddexp(0.0, -log_beta));
+            double gb = fg[e];
+            gatebar[e] += gb * -(alpha + beta);
+            double sgn = dv > 0 ? 1.0 : (dv < 0 ? -1.0 : 0.0);
+            double dal = -77.0 * alpha, dbe = 140.0 * beta * (1.0 - beta / 20.0);
+            vsum += gb * (dal * (1.0 - g) - dbe * (g - w->gate_minimum)) * sgn;
+        }
+        vbar[a] += vsum;
+        vbar[b] -= vsum;
+    }
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double vol = cell_volume(w, L, y, i);
+        double dv = phi[i] - w->bath_V;
+        double ci[4];
+        for (int s = 0; s < S; ++s)
+            ci[s] = (w->n0[(int64_t)s * w->ncell + i] + CF(L, w, y, L->dn + s, i)) / vol;
+        double unit = c->membrane / MOL_PER_FMOL;
+        for (int s = 0; s < S; ++s) {
+            double z = w->valence[s], kk = z * F_CONST / RT, x = kk * dv, co = w->bath_c[s];
+            double pa = w->pam[(int64_t)s * w->ncell + i];
+            double A = pa * unit;
+            double Bm = ref_bernoulli(-x), Bp = ref_bernoulli(x);
+            double Sm = ref_bernoulli_slope(-x), Sp = ref_bernoulli_slope(x);
+            double Qm = bernoulli_second(-x), Qp = bernoulli_second(x);
+            double gm = A * kk * (-ci[s] * Sm - co * Sp);
+            double Jbar = fN ? -fN[(int64_t)s * w->ncell + i] : 0.0; /* f_N -= J_m */
+            double Gbar = gmbar ? gmbar[(int64_t)s * w->ncell + i] : 0.0;
+            cbar[(int64_t)s * w->ncell + i] += Jbar * A * Bm + Gbar * (-A * kk * Sm);
+            vbar[i] += Jbar * gm + Gbar * (A * kk * kk * (ci[s] * Qm - co * Qp));
+            if (pam_bar)
+                pam_bar[(int64_t)s * w->ncell + i] += Jbar * (unit * (ci[s] * Bm - co * Bp)) +
+                                                      Gbar * (unit * kk * (-ci[s] * Sm - co * Sp));
+        }
+        if (fN) {
+            /* Na/K-ATPase: f_N,Na -= 3 cyc, f_N,K += 2 cyc */
+            double cyc_bar = -3.0 * fN[i] + 2.0 * fN[w->ncell + i];
+            double cap = w->nak_cap[i], av = c->pump;
+            if (cap > 0 && av > 0 && cyc_bar != 0.0) {
+                double mu_na = RT * (log(ci[0]) - log(w->bath_c[0])) + w->valence[0] * F_CONST * dv;
+                double mu_k = RT * (log(ci[1]) - log(w->bath_c[1])) + w->valence[1] * F_CONST * dv;
+                double e = nak_drive(w) + (-3.0) * mu_na + 2.0 * mu_k, a = e / RT;
+                double r = pow(w->km_na / ci[0], 3.0);
+                double s1 = 1.0 / (1.0 + r);
+                double s2 = 1.0 / (1.0 + pow(w->km_k / w->bath_c[1], 2.0));
+                double s3 = 1.0 / (1.0 + w->km_atp / w->atp);
+                double sat = s1 * s2 * s3;
+                double ea = exp(a);
+                double abar = cyc_bar * (-cap * av * sat * ea);
+                double mu_na_bar = abar * (-3.0 / RT), mu_k_bar = abar * (2.0 / RT);
+                double ds1 = (3.0 * r / ci[0]) * s1 * s1;
+                cbar[i] += cyc_bar * cap * av * ds1 * s2 * s3 * (1.0 - ea) + mu_na_bar * RT / ci[0];
+                cbar[w->ncell + i] += mu_k_bar * RT / ci[1];
+                vbar[i] += (mu_na_bar * w->valence[0] + mu_k_bar * w->valence[1]) * F_CONST;
+                if (cap_bar) cap_bar[i] += cyc_bar * av * sat * (1.0 - ea);
+            }
+        }
+    }
+}
+
+/* dN cotangents from concentration and potential cotangents (amounts over volume; potentials
+ * q / C with q = q0 + F' sum_s z_s dN). */
+static void pull_to_amounts(const cells_world* w, const cells_state_layout* L, int32_t k,
+                            const double* y, const double* cbar, const double* vbar, double* Nbar) {
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double vol = cell_volume(w, L, y, i);
+        for (int s = 0; s < w->S; ++s)
+            Nbar[(int64_t)s * w->ncell + i] +=
+                cbar[(int64_t)s * w->ncell + i] / vol + vbar[i] * CHARGE_PER_FMOL * w->valence[s] / w->cap[i];
+    }
+}
+
+/* u -> D^T u: cotangent on the potential correction from a cotangent u [S][n] on the amounts. */
+static void dtrans(const cells_world* w, int32_t k, const double* gm, const double* gj,
+                   const double* u, double* out /* [n] global index */) {
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double t = 0.0;
+        for (int s = 0; s < w->S; ++s) t += -gm[(int64_t)s * w->ncell + i] * u[(int64_t)s * w->ncell + i];
+        out[i] = t;
+    }
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        int64_t a = w->snd[e], b = w->rcv[e];
+        double t = 0.0;
+        for (int s = 0; s < w->S; ++s)
+            t += gj[(int64_t)s * w->nedge + e] * (u[(int64_t)s * w->ncell + b] - u[(int64_t)s * w->ncell + a]);
+        out[a] += t;
+        out[b] -= t;
+    }
+}
+
+/* slope cotangents of d(D(g) delta)/dg applied to u. */
+static void dslopes(const cells_world* w, int32_t k, const double* delta, const double* u,
+                    double coef, double* gmbar, double* gjbar) {
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+        for (int s = 0; s < w->S; ++s)
+            gmbar[(int64_t)s * w->ncell + i] += coef * (u[(int64_t)s * w->ncell + i] * -delta[i]);
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        int64_t a = w->snd[e], b = w->rcv[e];
+        for (int s = 0; s < w->S; ++s)
+            gjbar[(int64_t)s * w->nedge + e] +=
+                coef * ((delta[a] - delta[b]) * (u[(int64_t)s * w->ncell + b] - u[(int64_t)s * w->ncell + a]));
+    }
+}
+
+/* The solve's cotangent: w = M^-1 dbar (M symmetric), then rhs cotangent q^T w onto the amounts'
+ * rate cotangent, and M's cotangent -w delta^T onto the slopes (through G = F' sum z slope). */
+static void solve_vjp(const cells_world* w, const cells_symbolic* s, int32_t k, const double* fv,
+                      const double* sc, double coef, const double* dbar, const double* delta,
+                      double* rate_bar, double* gmbar, double* gjbar) {
+    int64_t c0 = w->cell_off[k], n = w->cell_off[k + 1] - c0;
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)n);
+    double* x = (double*)malloc(sizeof(double) * (size_t)n);
+    double* wv = (double*)calloc((size_t)w->ncell, sizeof(double));
+    for (int64_t i = 0; i < n; ++i) rhs[i] = dbar[c0 + i] * sc[i];
+    ref_solve(s, fv, rhs, x);
+    for (int64_t i = 0; i < n; ++i) wv[c0 + i] = x[i] * sc[i];
+    for (int64_t i = c0; i < c0 + n; ++i)
+        for (int q = 0; q < w->S; ++q) {
+            rate_bar[(int64_t)q * w->ncell + i] += CHARGE_PER_FMOL * w->valence[q] * wv[i];
+            gmbar[(int64_t)q * w->ncell + i] += CHARGE_PER_FMOL * w->valence[q] * (-coef * wv[i] * delta[i]);
+        }
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        int64_t a = w->snd[e], b = w->rcv[e];
+        double t = -coef * (wv[a] - wv[b]) * (delta[a] - delta[b]);
+        for (int q = 0; q < w->S; ++q) gjbar[(int64_t)q * w->nedge + e] += CHARGE_PER_FMOL * w->valence[q] * t;
+    }
+    free(rhs);
+    free(x);
+    free(wv);
+}
+
+int ref_ros2_vjp(const cells_world* w, const cells_symbolic* sym, const double* y, double dt,
+                 const cells_control* c, const double* ybar_out, double* ybar_in, double* pam_bar,
+                 double* cap_bar, double* paj_bar) {
+    if (w->water || w->calcium) return 0;
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const int S = w->S;
+    const double coef = ROS2_GAMMA * dt;
+    const int64_t nc = w->ncell, ne = w->nedge;
+    double* k1 = (double*)calloc((size_t)L.total, sizeof(double));
+    double* f1 = (double*)calloc((size_t)L.total, sizeof(double));
+    double* y1 = (double*)malloc(sizeof(double) * (size_t)L.total);
+    double* phi = (double*)malloc(sizeof(double) * (size_t)nc);
+    double* phi1 = (double*)malloc(sizeof(double) * (size_t)nc);
+    double* gm = (double*)malloc(sizeof(double) * (size_t)(S * nc));
+    double* gj = (double*)malloc(sizeof(double) * (size_t)(S * (ne ? ne : 1)));
+    double* d1 = (double*)calloc((size_t)nc, sizeof(double));
+    double* d2 = (double*)calloc((size_t)nc, sizeof(double));
+    double* gmbar = (double*)calloc((size_t)(S * nc), sizeof(double));
+    double* gjbar = (double*)calloc((size_t)(S * (ne ? ne : 1)), sizeof(double));
+    double* k1bar = (double*)calloc((size_t)(S * nc), sizeof(double));
+    double* k1gbar = (double*)calloc((size_t)(ne ? ne : 1), sizeof(double));
+    double* r2bar = (double*)calloc((size_t)(S * nc), sizeof(double));
+    double* cbar = (double*)calloc((size_t)(S * nc), sizeof(double));
+    double* vbar = (double*)calloc((size_t)nc, sizeof(double));
+    double* dbar = (double*)calloc((size_t)nc, sizeof(double));
+    double* x1bar = (double*)calloc((size_t)(S * nc), sizeof(double));
+    double* g1bar = (double*)calloc((size_t)(ne ? ne : 1), sizeof(double));
+    int ok = 1;
+    for (int32_t k = 0; k < w->W; ++k) {
+        const cells_symbolic* s = &sym[k];
+        int64_t c0 = w->cell_off[k], n = w->cell_off[k + 1] - c0, e0 = w->edge_off[k];
+        int64_t e1 = w->edge_off[k + 1];
+        /* ---- forward recomputation (ref_step_ros2's arithmetic) */
+        ref_potentials(w, k, y, phi);
+        ok = ref_rates(w, k, y, phi, c, k1, gm, gj) && ok;
+        double* Gj = (double*)malloc(sizeof(double) * (size_t)(e1 - e0 + 1));
+        double* sc = (double*)malloc(sizeof(double) * (size_t)n);
+        double* fv = (double*)calloc((size_t)s->nnz, sizeof(double));
+        for (int64_t e = e0; e < e1; ++e) {
+            double t = w->valence[0] * gj[e];
+            for (int q = 1; q < S; ++q) t = t + w->valence[q] * gj[(int64_t)q * ne + e];
+            Gj[e - e0] = CHARGE_PER_FMOL * t;
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t gi = c0 + i;
+            double t = w->valence[0] * gm[gi];
+            for (int q = 1; q < S; ++q) t = t + w->valence[q] * gm[(int64_t)q * nc + gi];
+            double g = CHARGE_PER_FMOL * t;
+            for (int64_t p = w->inc_ptr[gi]; p < w->inc_ptr[gi + 1]; ++p) g += Gj[w->inc_edge[p] - e0];
+            double d = w->cap[gi] + coef * g;
+            sc[i] = 1.0 / sqrt(d > 0 ? d : 1.0);
+            fv[s->diag_pos[i]] = (d * sc[i] * sc[i] + d * sc[i] * sc[i]) / 2;
+        }
+        for (int64_t e = e0; e < e1; ++e) {
+            double v = -(coef * Gj[e - e0]);
+            int64_t a = w->snd[e] - c0, b = w->rcv[e] - c0;
+            fv[s->edge_pos[e - e0]] = (v * sc[a] * sc[b] + v * sc[b] * sc[a]) / 2;
+        }
+        ref_factor(s, fv);
+        double* q = (double*)malloc(sizeof(double) * (size_t)n);
+        double* x = (double*)malloc(sizeof(double) * (size_t)n);
+        charge_rate(w, &L, k, k1, q);
+        for (int64_t i = 0; i < n; ++i) q[i] = q[i] * sc[i];
+        ref_solve(s, fv, q, x);
+        for (int64_t i = 0; i < n; ++i) d1[c0 + i] = x[i] * sc[i];
+        correct(w, &L, k, coef, gm, gj, d1, k1);
+        combine(w, &L, k, y, dt, k1, y1);
+        ref_potentials(w, k, y1, phi1);
+        ok = ref_rates(w, k, y1, phi1, c, f1, NULL, NULL) && ok;
+        /* r2 = f1 - 2 k1 (amounts) and its solve */
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t gi = c0 + i;
+            double t = w->valence[0] * (CF(&L, w, f1, L.dn, gi) + -2.0 * CF(&L, w, k1, L.dn, gi));
+            for (int qq = 1; qq < S; ++qq)
+                t = t + w->valence[qq] * (CF(&L, w, f1, L.dn + qq, gi) + -2.0 * CF(&L, w, k1, L.dn + qq, gi));
+            q[i] = CHARGE_PER_FMOL * t * sc[i];
+        }
+        ref_solve(s, fv, q, x);
+        for (int64_t i = 0; i < n; ++i) d2[c0 + i] = x[i] * sc[i];
+        /* ---- reverse */
+        for (int64_t i = c0; i < c0 + n; ++i)
+            for (int qq = 0; qq < S; ++qq) {
+                double xb = CF(&L, w, ybar_out, L.dn + qq, i);
+                CF(&L, w, ybar_in, L.dn + qq, i) = xb;
+                k1bar[(int64_t)qq * nc + i] = 1.5 * dt * xb;
+                r2bar[(int64_t)qq * nc + i] = 0.5 * dt * xb; /* k2bar, then r2bar */
+            }
+        for (int64_t e = e0; e < e1; ++e) {
+            double gb = EF(&L, w, ybar_out, L.gate, e);
+            EF(&L, w, ybar_in, L.gate, e) = gb;
+            k1gbar[e] = 1.5 * dt * gb;
+            g1bar[e] = 0.5 * dt * gb; /* k2's gate cotangent = r2's = f1's */
+        }
+        /* k2 = r2 + coef D delta2: delta2bar = coef D^T k2bar; slopes */
+        dtrans(w, k, gm, gj, r2bar, dbar);
+        for (int64_t i = c0; i < c0 + n; ++i) dbar[i] *= coef;
+        dslopes(w, k, d2, r2bar, coef, gmbar, gjbar);
+        /* delta2 = M^-1 q(r2): r2bar += q^T w2; slopes from M */
+        solve_vjp(w, s, k, fv, sc, coef, dbar, d2, r2bar, gmbar, gjbar);
+        /* r2 = f1 - 2 k1 */
+        for (int64_t i = c0; i < c0 + n; ++i)
+            for (int qq = 0; qq < S; ++qq) k1bar[(int64_t)qq * nc + i] += -2.0 * r2bar[(int64_t)qq * nc + i];
+        for (int64_t e = e0; e < e1; ++e) k1gbar[e] += -2.0 * g1bar[e];
+        /* f1 = f(y1): pull r2bar (amount rates) and g1bar (gate rates) back to y1 */
+        for (int64_t i = c0; i < c0 + n; ++i) {
+            vbar[i] = 0.0;
+            for (int qq = 0; qq < S; ++qq) { cbar[(int64_t)qq * nc + i] = 0.0; x1bar[(int64_t)qq * nc + i] = 0.0; }
+        }
+        double* gate1bar = (double*)calloc((size_t)(ne ? ne : 1), sizeof(double));
+        rates_vjp(w, &L, k, y1, phi1, c, r2bar, g1bar, NULL, NULL, cbar, vbar, gate1bar, pam_bar,
+                  cap_bar, paj_bar);
+        pull_to_amounts(w, &L, k, y1, cbar, vbar, x1bar);
+        /* y1 = y + dt k1 */
+        for (int64_t i = c0; i < c0 + n; ++i)
+            for (int qq = 0; qq < S; ++qq) {
+                CF(&L, w, ybar_in, L.dn + qq, i) += x1bar[(int64_t)qq * nc + i];
+                k1bar[(int64_t)qq * nc + i] += dt * x1bar[(int64_t)qq * nc + i];
+            }
+        for (int64_t e = e0; e < e1; ++e) {
+            EF(&L, w, ybar_in, L.gate, e) += gate1bar[e];
+            k1gbar[e] += dt * gate1bar[e];
+        }
+        free(gate1bar);
+        /* k1 = f(y) + coef D delta1 */
+        dtrans(w, k, gm, gj, k1bar, dbar);
+        for (int64_t i = c0; i < c0 + n; ++i) dbar[i] *= coef;
+        dslopes(w, k, d1, k1bar, coef, gmbar, gjbar);
+        solve_vjp(w, s, k, fv, sc, coef, dbar, d1, k1bar, gmbar, gjbar);
+        /* f(y) and the slopes at y */
+        for (int64_t i = c0; i < c0 + n; ++i) {
+            vbar[i] = 0.0;
+            for (int qq = 0; qq < S; ++qq) { cbar[(int64_t)qq * nc + i] = 0.0; x1bar[(int64_t)qq * nc + i] = 0.0; }
+        }
+        double* gate0bar = (double*)calloc((size_t)(ne ? ne : 1), sizeof(double));
+        rates_vjp(w, &L, k, y, phi, c, k1bar, k1gbar, gmbar, gjbar, cbar, vbar, gate0bar, pam_bar,
+                  cap_bar, paj_bar);
+        pull_to_amounts(w, &L, k, y, cbar, vbar, x1bar);
+        for (int64_t i = c0; i < c0 + n; ++i)
+            for (int qq = 0; qq < S; ++qq) CF(&L, w, ybar_in, L.dn + qq, i) += x1bar[(int64_t)qq * nc + i];
+        for (int64_t e = e0; e < e1; ++e) EF(&L, w, ybar_in, L.gate, e) += gate0bar[e];
+        free(gate0bar);
+        for (int64_t i = c0; i < c0 + n; ++i)
+            for (int qq = 0; qq < S; ++qq) gmbar[(int64_t)qq * nc + i] = 0.0;
+        for (int64_t e = e0; e < e1; ++e)
+            for (int qq = 0; qq < S; ++qq) gjbar[(int64_t)qq * ne + e] = 0.0;
+        free(Gj); free(sc); free(fv); free(q); free(x);
+    }
+    free(k1); free(f1); free(y1); free(phi); free(phi1); free(gm); free(gj); free(d1); free(d2);
+    free(gmbar); free(gjbar); free(k1bar); free(k1gbar); free(r2bar); free(cbar); free(vbar);
+    free(dbar); free(x1bar); free(g1bar);
+    return ok;
+}
