@@ -1,13 +1,14 @@
 """Benchmark: cell-steps/s of the batched HIP tissue step on one MI355X.
 
 Workload: the batched ensemble configuration (4096 worlds of U{256..4096} cells, Na/K/Cl/Ca,
-ROS2 with the sparse-Cholesky stage solves, dt = 5 ms), synthetic seeded parameters. A step is
-one ROS2 step of every world. The timed region is exactly --steps steps of the packed ensemble
-(one captured HIP graph per step), with the state resident in HBM before it starts.
+ROS2 with the sparse-Cholesky stage solves, dt = 5 ms), synthetic seeded parameters. A step is one
+ROS2 step of every world. The timed region is exactly --steps steps of the packed ensemble (one
+captured HIP graph per step), with the state resident in HBM before it starts.
 
-Printed beside the headline: the same step one world per launch ("plain"), the batched/plain
-ratio, the dominant kernel's roofline figures from HIP events, a parity check of the first worlds
-against the FP64 oracle, and the oracle's own throughput on this host (CPU baseline).
+Beside the headline: the same step one world per launch ("plain", timed on the ensemble's first
+--plain-worlds worlds), the batched/plain ratio, the dominant kernel's roofline figures from HIP
+events, a parity check of the first worlds against the FP64 oracle, and the oracle's own
+throughput on this host (CPU baseline).
 """
 
 import argparse
@@ -24,33 +25,14 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+BYTES_PER_CELL_STEP = 539.0  # one read + write of the state, one read of the parameters (4 ions)
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def factor_flops(p, e):
-    """Sum over worlds of sum_j c_j^2 (c_j the entries of factor column j), the reference's cost
-    of a sparse factorisation, and the factor's stored entries."""
-    from cells_amd import symbolic
-
-    flops, nnz = 0, 0
-    cache = {}
-    for k in range(p.W):
-        n = int(p.cell_off[k + 1] - p.cell_off[k])
-        if n not in cache:
-            e0, e1 = int(p.edge_off[k]), int(p.edge_off[k + 1])
-            c0 = int(p.cell_off[k])
-            s = symbolic.bind(n, p.snd[e0:e1] - c0, p.rcv[e0:e1] - c0)
-            cnt = np.diff(s.column_start)
-            cache[n] = (int((cnt.astype(np.int64) ** 2).sum()), int(s.column_start[-1]))
-        flops += cache[n][0]
-        nnz += cache[n][1]
-    return flops, nnz
-
-
-def cpu_baseline(p, seconds):
+def cpu_baseline(p, seconds, dt):
     """The oracle (FP64 C restatement, one core) on the first worlds of the same ensemble."""
     import oracle
     from cells_amd.world import head, initial_state
@@ -63,15 +45,15 @@ def cpu_baseline(p, seconds):
     t0 = time.perf_counter()
     steps = 0
     while True:
-        y, ok = oracle.step_ros2(sub, y, 5e-3, sym, world=w)
+        y, ok = oracle.step_ros2(sub, y, dt, sym, world=w)
         steps += 1
         if time.perf_counter() - t0 >= seconds:
             break
-    dt = time.perf_counter() - t0
+    el = time.perf_counter() - t0
     return {
-        "value": sub.ncell * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+        "value": sub.ncell * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
         "sample": f"oracle ROS2 steps of the ensemble's first {k} worlds ({sub.ncell} cells), "
-                  f"{steps} steps in {dt:.1f} s on one core",
+                  f"{steps} steps in {el:.1f} s on one core",
     }
 
 
@@ -81,14 +63,17 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--plain-worlds", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--parity-worlds", type=int, default=3)
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: plain,parity,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: independent worlds shard across "
                           "GPUs with no exchange, and no multi-GPU figure is measured"}))
         sys.exit(2)
+    skip = set(filter(None, args.skip.split(",")))
 
     import torch
 
@@ -103,12 +88,12 @@ def main():
     log(f"{p.W} worlds, {p.ncell} cells, {p.nedge} edges; binding")
     t0 = time.perf_counter()
     e = Ensemble(p, "ros2")
-    log(f"bound in {time.perf_counter() - t0:.1f} s")
+    stats = e.plan_stats()
+    log(f"bound in {time.perf_counter() - t0:.1f} s; plans {stats}")
     y0 = e.to_device(initial_state(p))
     out = torch.empty_like(y0)
     ok = torch.empty(p.W, dtype=torch.int32, device=y0.device)
-    # warmup (captures the step graph)
-    y, _ = e.step(y0, dt, nsteps=args.warmup, out=out, ok=ok)
+    y, _ = e.step(y0, dt, nsteps=args.warmup, out=out, ok=ok)  # also captures the step graph
     yw = y.clone()
     torch.cuda.synchronize()
     log("timing batched")
@@ -120,58 +105,69 @@ def main():
     cell_steps = p.ncell * args.steps / elapsed
     all_ok = bool(ok.cpu().numpy().all())
     final = y.cpu().numpy()
-
-    log("timing plain (one world per launch)")
-    yp, _ = e.step(yw, dt, nsteps=1, batched=False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    yp, okp = e.step(yw, dt, nsteps=args.plain_steps, batched=False)
-    torch.cuda.synchronize()
-    plain_elapsed = time.perf_counter() - t0
-    plain = p.ncell * args.plain_steps / plain_elapsed
+    log(f"batched: {ms_per_step:.3f} ms/step, {cell_steps:.4g} cell-steps/s, all accepted {all_ok}")
 
     log("kernel times")
     times = e.kernel_times(yw, dt, reps=3)
     per_kernel = {}
     for name, ms in times:
         per_kernel[name] = per_kernel.get(name, 0.0) + ms
+    log(f"kernels (ms per step): {json.dumps({k: round(v, 4) for k, v in per_kernel.items()})}")
     dominant = max(per_kernel, key=per_kernel.get)
     launches = [ms for name, ms in times if name == dominant]
-    avg_ms = float(np.mean(launches))
-    log(f"kernels: {json.dumps({k: round(v, 4) for k, v in per_kernel.items()})}")
-
-    roofline = None
-    flops, nnz = factor_flops(p, e)
     if dominant == "k_factor_solve":
-        # the launch that factors (the first) does sum c_j^2 flops; report that launch
-        fact_ms = launches[0]
-        achieved = flops / (fact_ms * 1e-3) / 1e12
+        fact_ms = launches[0]  # the launch that factors (and does the first solve)
+        achieved = stats["flops"] / (fact_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "k_factor_solve (factor + first solve)",
-                    "algorithmic": f"sum_j c_j^2 = {flops} flops per launch"}
+                    "kernel": "k_factor_solve (factor + first solve)", "launch_ms": fact_ms,
+                    "algorithmic": f"sum over worlds of sum_j c_j^2 = {stats['flops']:.4g} flops"}
     else:
-        # bytes: one read of the state and rates it touches; use the whole-step figure
-        bytes_step = 539.0 * p.ncell
-        achieved = bytes_step / (avg_ms * 1e-3) / 1e9
+        avg_ms = float(np.mean(launches))
+        achieved = BYTES_PER_CELL_STEP * p.ncell / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant}
-    step_bytes = 539.0 * p.ncell
-    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                    "launch_ms": avg_ms}
+    step_gbs = BYTES_PER_CELL_STEP * p.ncell / (ms_per_step * 1e-3) / 1e9
 
-    log("parity against the oracle")
-    k = args.parity_worlds
-    sub = head(p, k)
-    _, sym = oracle.world_symbolics(sub)
-    ws = oracle.world_struct(sub)
-    ref = initial_state(sub)
-    for _ in range(args.warmup + args.steps):
-        ref, okr = oracle.step_ros2(sub, ref, dt, sym, world=ws)
-    got = take_state(p, final, k)
-    parity = max(relerr(vmem(sub, got), vmem(sub, ref)), relerr(conc(sub, got), conc(sub, ref)))
+    plain = None
+    if "plain" not in skip:
+        k = min(args.plain_worlds, p.W)
+        sub = head(p, k)
+        log(f"timing plain (one world per launch) on the first {k} worlds ({sub.ncell} cells)")
+        es = Ensemble(sub, "ros2")
+        ys = es.to_device(initial_state(sub))
+        ysw, _ = es.step(ys, dt, nsteps=1, batched=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        es.step(ysw, dt, nsteps=args.plain_steps, batched=False)
+        torch.cuda.synchronize()
+        plain = sub.ncell * args.plain_steps / (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        es.step(ysw, dt, nsteps=args.plain_steps, batched=True)
+        torch.cuda.synchronize()
+        sub_batched = sub.ncell * args.plain_steps / (time.perf_counter() - t0)
+        log(f"plain: {plain:.4g} cell-steps/s; batched on the same worlds {sub_batched:.4g}")
+        es.close()
 
-    log("cpu baseline")
-    cpu = cpu_baseline(p, args.cpu_seconds)
+    parity = None
+    if "parity" not in skip:
+        log("parity against the oracle")
+        k = args.parity_worlds
+        sub = head(p, k)
+        _, sym = oracle.world_symbolics(sub)
+        ws = oracle.world_struct(sub)
+        ref = initial_state(sub)
+        for _ in range(args.warmup + args.steps):
+            ref, okr = oracle.step_ros2(sub, ref, dt, sym, world=ws)
+        got = take_state(p, final, k)
+        parity = max(relerr(vmem(sub, got), vmem(sub, ref)), relerr(conc(sub, got), conc(sub, ref)))
+        log(f"parity max relative error {parity:.3e}")
+
+    cpu = None
+    if "cpu" not in skip:
+        log("cpu baseline")
+        cpu = cpu_baseline(p, args.cpu_seconds, dt)
 
     line = {
         "metric": "cell-steps/s batched vs plain HIP at <=1e-9 rel err (FP64 oracle); adjoint grads/s",
@@ -191,17 +187,18 @@ def main():
                    "worlds": p.W, "cells": p.ncell, "edges": p.nedge, "dt": dt,
                    "method": "ros2"},
         "plain_value": plain,
-        "batched_over_plain": cell_steps / plain,
+        "plain_worlds": min(args.plain_worlds, p.W) if plain else None,
+        "batched_over_plain": (cell_steps / plain) if plain else None,
         "all_worlds_accepted": all_ok,
         "parity_max_rel_err": parity,
-        "parity_worlds": k,
+        "parity_worlds": args.parity_worlds if parity is not None else None,
         "step_effective_GBs": step_gbs,
         "kernel_ms": {kk: round(v, 4) for kk, v in per_kernel.items()},
-        "factor_nnz": nnz,
+        "plans": stats,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
     e.close()
 
 

@@ -1074,6 +1074,288 @@ __global__ __launch_bounds__(kBlock) void k_device_checks(Params P, const double
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// Adjoint of the ROS2 step (3-ion worlds, no water): reverse of the forward arithmetic restricted
+// to the fields the dynamics read (amounts, gates). The reference differentiates only forward
+// (analysis/stationary.py:505-517); this is the step's exact discrete adjoint, pinned by central
+// differences of the forward step.
+
+__device__ __forceinline__ double bernoulli_second(double x) {
+  if (fabs(x) < 0.1) {
+    double x2 = x * x;
+    return 1.0 / 6.0 - x2 / 60.0 + x2 * x2 / 1008.0 - x2 * x2 * x2 / 21600.0 +
+           x2 * x2 * x2 * x2 / 532224.0;
+  }
+  if (x > 0) {
+    double em = exp(-x), d = 1.0 - em;
+    return em * (x * (1.0 + em) - 2.0 * d) / (d * d * d);
+  }
+  double ex = exp(x), d = ex - 1.0;
+  return ex * (x * (ex + 1.0) - 2.0 * d) / (d * d * d);
+}
+
+// Per edge: pull the amount-rate cotangent fN [S][n] (J leaves the sender, enters the receiver),
+// the gate-rate cotangent fg [E] and the slope cotangent gjbar [S][E] back to the edge's two
+// concentrations (ca_bar, cb_bar [S][E]), its voltage (ev_bar [E]), its gate (added to gate_out
+// [E] and, times h2, to gate_out2 [E]) and its parameters (paj_bar, added).
+__global__ __launch_bounds__(kBlock) void k_edges_vjp(Params P, const double* y, Ctl c,
+                                                      const double* fN, const double* fg,
+                                                      const double* gjbar, double* ca_bar,
+                                                      double* cb_bar, double* ev_bar,
+                                                      double* gate_out, double* gate_out2, double h2,
+                                                      double* paj_bar) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= P.nedge) return;
+  int a = P.snd[e], b = P.rcv[e];
+  double ca[4], cb[4], va, vb;
+  cell_concentrations(P, y, y, a, ca, va);
+  cell_concentrations(P, y, y, b, cb, vb);
+  double dv = potential(P, y, a) - potential(P, y, b);
+  double g = EFc(P, y, P.f_gate, e);
+  double unit = c.junction / kMolPerFmol;
+  double vsum = 0.0, gbar = 0.0;
+  for (int s = 0; s < P.S; ++s) {
+    double z = P.z[s], kk = z * kF / P.RT, x = kk * dv;
+    double pa = P.paj[(long)s * P.nedge + e];
+    double Bm = bernoulli(-x), Bp = bernoulli(x);
+    double Sm = bernoulli_slope(-x), Sp = bernoulli_slope(x);
+    double A = pa * g * unit;
+    double core = ca[s] * Bm - cb[s] * Bp;
+    double score = -ca[s] * Sm - cb[s] * Sp;
+    double gjv = A * kk * score;
+    double Jbar = fN ? fN[(long)s * P.ncell + b] - fN[(long)s * P.ncell + a] : 0.0;
+    double Gbar = gjbar ? gjbar[(long)s * P.nedge + e] : 0.0;
+    double cab = Jbar * A * Bm, cbb = Jbar * (-A * Bp), dvb = Jbar * gjv;
+    if (gjbar) {
+      double Qm = bernoulli_second(-x), Qp = bernoulli_second(x);
+      cab += Gbar * (-A * kk * Sm);
+      cbb += Gbar * (-A * kk * Sp);
+      dvb += Gbar * (A * kk * kk * (ca[s] * Qm - cb[s] * Qp));
+    }
+    ca_bar[(long)s * P.nedge + e] = cab;
+    cb_bar[(long)s * P.nedge + e] = cbb;
+    vsum += dvb;
+    gbar += Jbar * (pa * unit * core) + Gbar * (pa * unit * kk * score);
+    paj_bar[(long)s * P.nedge + e] += Jbar * (g * unit * core) + Gbar * (g * unit * kk * score);
+  }
+  if (fg) {
+    double shift = fabs(dv) - P.gate_thr;
+    double alpha = 1.3 * exp(-77.0 * shift);
+    double log_beta = P.log_0065 + 140.0 * shift;
+    double beta = 20.0 * exp(-logaddexp0(-log_beta));
+    double fb = fg[e];
+    gbar += fb * -(alpha + beta);
+    double sgn = dv > 0 ? 1.0 : (dv < 0 ? -1.0 : 0.0);
+    double dal = -77.0 * alpha, dbe = 140.0 * beta * (1.0 - beta / 20.0);
+    vsum += fb * (dal * (1.0 - g) - dbe * (g - P.gate_min)) * sgn;
+  }
+  ev_bar[e] = vsum;
+  gate_out[e] += gbar;
+  if (gate_out2) gate_out2[e] += h2 * gbar;
+}
+
+// Per cell: the membrane routes' and the Na/K-ATPase's pull-back plus the edges' partials, then
+// to the amounts (c = N / V, phi = V_bath + q / C): adds to out (and h2 times to out2).
+__global__ __launch_bounds__(kBlock) void k_cells_vjp(Params P, const double* y, Ctl c,
+                                                      const double* fN, const double* gmbar,
+                                                      const double* ca_bar, const double* cb_bar,
+                                                      const double* ev_bar, double* out,
+                                                      double* out2, double h2, double* pam_bar,
+                                                      double* cap_bar) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.ncell) return;
+  double ci[4], vol;
+  cell_concentrations(P, y, y, i, ci, vol);
+  double dv = potential(P, y, i) - P.Vb;
+  double unit = c.membrane / kMolPerFmol;
+  double cbar[4] = {0, 0, 0, 0};
+  double vbar = 0.0;
+  for (int s = 0; s < P.S; ++s) {
+    double z = P.z[s], kk = z * kF / P.RT, x = kk * dv, co = P.cb[s];
+    double pa = P.pam[(long)s * P.ncell + i];
+    double A = pa * unit;
+    double Bm = bernoulli(-x), Bp = bernoulli(x);
+    double Sm = bernoulli_slope(-x), Sp = bernoulli_slope(x);
+    double score = -ci[s] * Sm - co * Sp;
+    double gmv = A * kk * score;
+    double Jbar = fN ? -fN[(long)s * P.ncell + i] : 0.0;
+    double Gbar = gmbar ? gmbar[(long)s * P.ncell + i] : 0.0;
+    cbar[s] += Jbar * A * Bm;
+    vbar += Jbar * gmv;
+    if (gmbar) {
+      double Qm = bernoulli_second(-x), Qp = bernoulli_second(x);
+      cbar[s] += Gbar * (-A * kk * Sm);
+      vbar += Gbar * (A * kk * kk * (ci[s] * Qm - co * Qp));
+    }
+    pam_bar[(long)s * P.ncell + i] += Jbar * (unit * (ci[s] * Bm - co * Bp)) + Gbar * (unit * kk * score);
+  }
+  if (fN) {
+    double cyc_bar = -3.0 * fN[i] + 2.0 * fN[P.ncell + i];
+    double cap = P.nak_cap[i], av = c.pump;
+    if (cap > 0 && av > 0 && cyc_bar != 0.0) {
+      double mu_na = P.RT * (log(ci[0]) - P.log_cb[0]) + P.z[0] * kF * dv;
+      double mu_k = P.RT * (log(ci[1]) - P.log_cb[1]) + P.z[1] * kF * dv;
+      double a = (P.nak_drive + (-3.0) * mu_na + 2.0 * mu_k) / P.RT;
+      double r = exp(3.0 * (P.log_km_na - log(ci[0])));
+      double s1 = 1.0 / (1.0 + r);
+      double s23 = exp(-logaddexp0(2.0 * (P.log_km_k - P.log_cb[1])) - P.nak_atp_sat);
+      double sat = s1 * s23;
+      double ea = exp(a);
+      double abar = cyc_bar * (-cap * av * sat * ea);
+      double mu_na_bar = abar * (-3.0 / P.RT), mu_k_bar = abar * (2.0 / P.RT);
+      double ds1 = (3.0 * r / ci[0]) * s1 * s1;
+      cbar[0] += cyc_bar * cap * av * ds1 * s23 * (1.0 - ea) + mu_na_bar * P.RT / ci[0];
+      cbar[1] += mu_k_bar * P.RT / ci[1];
+      vbar += (mu_na_bar * P.z[0] + mu_k_bar * P.z[1]) * kF;
+      cap_bar[i] += cyc_bar * av * sat * (1.0 - ea);
+    }
+  }
+  int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
+  for (int p = p0; p < pm; ++p) {
+    int e = P.inc_edge[p];
+    for (int s = 0; s < P.S; ++s) cbar[s] += ca_bar[(long)s * P.nedge + e];
+    vbar += ev_bar[e];
+  }
+  for (int p = pm; p < p1; ++p) {
+    int e = P.inc_edge[p];
+    for (int s = 0; s < P.S; ++s) cbar[s] += cb_bar[(long)s * P.nedge + e];
+    vbar -= ev_bar[e];
+  }
+  for (int s = 0; s < P.S; ++s) {
+    double nb = cbar[s] / vol + vbar * kChargePerFmol * P.z[s] / P.cap[i];
+    CF(P, out, P.f_dn + s, i) += nb;
+    if (out2) out2[(long)s * P.ncell + i] += h2 * nb;
+  }
+}
+
+// Seeds the reverse sweep from the output cotangent: ybar_in = ybar_out on amounts and gates;
+// k1bar = 1.5 dt ybar_out, k2bar = 0.5 dt ybar_out (amounts [S][n] and gates [E]).
+__global__ __launch_bounds__(kBlock) void k_adj_seed(Params P, const double* ybar, double dt,
+                                                     double* ybar_in, double* k1bar, double* k2bar,
+                                                     double* k1gbar, double* k2gbar) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < P.ncell) {
+    int i = (int)t;
+    for (int s = 0; s < P.S; ++s) {
+      double v = CFc(P, ybar, P.f_dn + s, i);
+      CF(P, ybar_in, P.f_dn + s, i) = v;
+      k1bar[(long)s * P.ncell + i] = 1.5 * dt * v;
+      k2bar[(long)s * P.ncell + i] = 0.5 * dt * v;
+    }
+  } else if (t < (long)P.ncell + P.nedge) {
+    int e = (int)(t - P.ncell);
+    double v = EFc(P, ybar, P.f_gate, e);
+    EF(P, ybar_in, P.f_gate, e) = v;
+    k1gbar[e] = 1.5 * dt * v;
+    k2gbar[e] = 0.5 * dt * v;
+  }
+}
+
+// dbar = coef D^T u and the slopes' cotangent of D(g) delta against u (cells, then edges).
+__global__ __launch_bounds__(kBlock) void k_adj_dtrans(Params P, double coef, const double* gm,
+                                                       const double* gj, const double* u,
+                                                       const double* delta, double* dbar,
+                                                       double* gmbar, double* gjbar) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < P.ncell) {
+    int i = (int)t;
+    double acc = 0.0;
+    for (int s = 0; s < P.S; ++s) {
+      double us = u[(long)s * P.ncell + i];
+      acc += -gm[(long)s * P.ncell + i] * us;
+      gmbar[(long)s * P.ncell + i] += coef * (us * -delta[i]);
+    }
+    int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
+    for (int p = p0; p < p1; ++p) {
+      int e = P.inc_edge[p];
+      int a = P.snd[e], b = P.rcv[e];
+      double tt = 0.0;
+      for (int s = 0; s < P.S; ++s)
+        tt += gj[(long)s * P.nedge + e] * (u[(long)s * P.ncell + b] - u[(long)s * P.ncell + a]);
+      acc += p < pm ? tt : -tt;
+    }
+    dbar[i] = coef * acc;
+  } else if (t < (long)P.ncell + P.nedge) {
+    int e = (int)(t - P.ncell);
+    int a = P.snd[e], b = P.rcv[e];
+    for (int s = 0; s < P.S; ++s)
+      gjbar[(long)s * P.nedge + e] +=
+          coef * ((delta[a] - delta[b]) * (u[(long)s * P.ncell + b] - u[(long)s * P.ncell + a]));
+  }
+}
+
+// w = M^-1 dbar on the step's factor (workgroup per world); then rate_bar += q^T w and the
+// slopes' cotangent of M: -coef w delta^T through G = F' sum_s z_s slope_s.
+__global__ __launch_bounds__(kBlock) void k_adj_solve(Params P, const ShapeDev* shapes,
+                                                      const int* shape_of, const int* pool,
+                                                      const long* foff, double coef,
+                                                      const double* msc, const double* fvals,
+                                                      const double* dbar, const double* delta,
+                                                      double* wv, double* rate_bar, double* gmbar,
+                                                      double* gjbar) {
+  extern __shared__ double lds[];
+  int w = blockIdx.x;
+  const ShapeDev s = shapes[shape_of[w]];
+  const PlanView v = plan_view(s, pool);
+  const double* fv = fvals + foff[w];
+  int c0 = P.cell_off[w], e0 = P.edge_off[w];
+  for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
+    int i = c0 + v.order[k];
+    lds[k] = dbar[i] * msc[i];
+  }
+  __syncthreads();
+  solve_world(s, v, fv, lds);
+  for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
+    int i = c0 + v.order[k];
+    wv[i] = lds[k] * msc[i];
+  }
+  __syncthreads();
+  for (int i = c0 + threadIdx.x; i < c0 + s.n; i += blockDim.x)
+    for (int q = 0; q < P.S; ++q) {
+      rate_bar[(long)q * P.ncell + i] += kChargePerFmol * P.z[q] * wv[i];
+      gmbar[(long)q * P.ncell + i] += kChargePerFmol * P.z[q] * (-coef * wv[i] * delta[i]);
+    }
+  for (int e = e0 + threadIdx.x; e < e0 + s.E; e += blockDim.x) {
+    int a = P.snd[e], b = P.rcv[e];
+    double t = -coef * (wv[a] - wv[b]) * (delta[a] - delta[b]);
+    for (int q = 0; q < P.S; ++q) gjbar[(long)q * P.nedge + e] += kChargePerFmol * P.z[q] * t;
+  }
+}
+
+// k1bar += -2 r2bar (amounts and gates).
+__global__ __launch_bounds__(kBlock) void k_adj_axpy(long n1, double* k1bar, const double* r2bar,
+                                                     long n2, double* k1gbar, const double* g2bar) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n1) k1bar[t] += -2.0 * r2bar[t];
+  else if (t < n1 + n2) k1gbar[t - n1] += -2.0 * g2bar[t - n1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(long n, double* x, double v) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = v;
+}
+
+// The Vmem loss of a sample row: sum_i wgt (V_i - target_i)^2 per world into loss[w] (added),
+// and its cotangent on the amounts added to ybar (dV_i/dN_si = F' z_s / C_i).
+__global__ __launch_bounds__(kBlock) void k_vmem_loss(Params P, const double* y, const double* target,
+                                                      double wgt, double* loss, double* ybar) {
+  __shared__ double sh[kBlock];
+  int w = blockIdx.x;
+  int c0 = P.cell_off[w], c1 = P.cell_off[w + 1];
+  double acc = 0.0;
+  for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    double v = potential(P, y, i) - P.Vb;
+    double d = v - target[i];
+    acc += wgt * d * d;
+    if (ybar) {
+      double g = 2.0 * wgt * d;
+      for (int s = 0; s < P.S; ++s) CF(P, ybar, P.f_dn + s, i) += g * kChargePerFmol * P.z[s] / P.cap[i];
+    }
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) loss[w] += acc;
+}
+
 int blocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -1107,6 +1389,10 @@ struct cells_ensemble {
   double *contrib1 = nullptr, *contrib2 = nullptr, *gm = nullptr, *gj = nullptr, *Gj = nullptr;
   double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *fvals = nullptr;
   int *sel = nullptr, *alive = nullptr, *valid = nullptr;
+  // adjoint workspace (allocated at the first cells_step_vjp)
+  double *d1 = nullptr, *d2 = nullptr, *k1bar = nullptr, *k2bar = nullptr, *k1gbar = nullptr;
+  double *g2bar = nullptr, *gmbar = nullptr, *gjbar = nullptr, *dbar = nullptr, *wv = nullptr;
+  double *ca_bar = nullptr, *cb_bar = nullptr, *ev_bar = nullptr, *loss_w = nullptr;
   hipStream_t capture_stream = nullptr;
   hipGraphExec_t graph = nullptr;
   double graph_dt = 0.0;
@@ -1772,5 +2058,147 @@ extern "C" int cells_device_checks(cells_ensemble* e, void* stream, const double
   hipLaunchKernelGGL(k_device_checks, dim3(e->W), dim3(kBlock), 0, static_cast<hipStream_t>(stream), e->P,
                      previous, y, steps, material_atol, failed_out, ratios_out);
   HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+namespace {
+
+int adjoint_workspace(cells_ensemble* e) {
+  if (e->d1) return CELLS_OK;
+  size_t nc = e->ncell, ne = std::max(e->nedge, 1), S = e->S;
+  int rc;
+  if ((rc = alloc(e, &e->d1, nc)) || (rc = alloc(e, &e->d2, nc)) || (rc = alloc(e, &e->k1bar, S * nc)) ||
+      (rc = alloc(e, &e->k2bar, S * nc)) || (rc = alloc(e, &e->k1gbar, ne)) || (rc = alloc(e, &e->g2bar, ne)) ||
+      (rc = alloc(e, &e->gmbar, S * nc)) || (rc = alloc(e, &e->gjbar, S * ne)) || (rc = alloc(e, &e->dbar, nc)) ||
+      (rc = alloc(e, &e->wv, nc)) || (rc = alloc(e, &e->ca_bar, S * ne)) || (rc = alloc(e, &e->cb_bar, S * ne)) ||
+      (rc = alloc(e, &e->ev_bar, ne)) || (rc = alloc(e, &e->loss_w, e->W)))
+    return rc;
+  return CELLS_OK;
+}
+
+}  // namespace
+
+extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const double* ybar_out,
+                              double dt, const cells_control* control, double* ybar_in,
+                              double* pam_bar, double* cap_bar, double* paj_bar, int32_t* valid_out) {
+  if (!e || !y || !ybar_out || !control || !ybar_in || !pam_bar || !cap_bar || !paj_bar)
+    return fail(CELLS_EINVAL, "NULL argument");
+  if (e->method != CELLS_METHOD_ROS2 || e->P.water || e->P.calcium || e->S != 3)
+    return fail(CELLS_EINVAL, "the adjoint covers ROS2 on 3-ion worlds without water");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(CELLS_EINVAL, "dt must be positive and finite");
+  HIP_TRY(hipSetDevice(e->device));
+  int rc = adjoint_workspace(e);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
+  const Params& P = e->P;
+  const double coef = (1.0 + 1.0 / std::sqrt(2.0)) * dt;
+  const int nc = e->ncell, ne = e->nedge, W = e->W;
+  size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
+  long S = e->S;
+  HIP_TRY(hipMemcpyAsync(e->A, y, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_init, dim3(blocks(W)), dim3(kBlock), 0, st, W, nullptr, e->sel, e->alive, e->valid);
+  // forward recomputation of the step's internals (the forward kernels, in the same order)
+  if (ne)
+    hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, ne,
+                       c, e->K1, e->gj, e->Gj, e->valid);
+  hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
+                     e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
+  hipLaunchKernelGGL(k_factor_solve, dim3(W), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+                     0, 1, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->K1, e->d1, e->valid);
+  hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
+                     ne, dt, coef, e->gm, e->gj, e->d1, e->K1, e->contrib1, e->Y1);
+  if (ne)
+    hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, ne, c,
+                       e->K2, nullptr, nullptr, e->valid);
+  hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
+                     e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+  hipLaunchKernelGGL(k_ros2_stage2, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, 0, nc, 0, ne, e->K1,
+                     e->contrib1, e->K2, e->contrib2);
+  hipLaunchKernelGGL(k_factor_solve, dim3(W), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+                     0, 0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->K2, e->d2, e->valid);
+  // reverse sweep
+  hipLaunchKernelGGL(k_adj_seed, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, ybar_out, dt, ybar_in, e->k1bar,
+                     e->k2bar, e->k1gbar, e->g2bar);
+  hipLaunchKernelGGL(k_fill, dim3(blocks(S * nc)), dim3(kBlock), 0, st, S * nc, e->gmbar, 0.0);
+  if (ne) hipLaunchKernelGGL(k_fill, dim3(blocks(S * ne)), dim3(kBlock), 0, st, S * ne, e->gjbar, 0.0);
+  // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
+  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k2bar,
+                     e->d2, e->dbar, e->gmbar, e->gjbar);
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+                     coef, e->msc, e->fvals, e->dbar, e->d2, e->wv, e->k2bar, e->gmbar, e->gjbar);
+  // r2 = f(Y1) - 2 k1
+  hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
+                     (long)ne, e->k1gbar, e->g2bar);
+  double* gate_in = ybar_in + P.edge_base + (long)P.f_gate * P.nedge;
+  if (ne)
+    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->Y1, c, e->k2bar, e->g2bar,
+                       nullptr, e->ca_bar, e->cb_bar, e->ev_bar, gate_in, e->k1gbar, dt, paj_bar);
+  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->Y1, c, e->k2bar, nullptr,
+                     e->ca_bar, e->cb_bar, e->ev_bar, ybar_in, e->k1bar, dt, pam_bar, cap_bar);
+  // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
+  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k1bar,
+                     e->d1, e->dbar, e->gmbar, e->gjbar);
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+                     coef, e->msc, e->fvals, e->dbar, e->d1, e->wv, e->k1bar, e->gmbar, e->gjbar);
+  // f(y) and the slopes at y
+  if (ne)
+    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, c, e->k1bar, e->k1gbar,
+                       e->gjbar, e->ca_bar, e->cb_bar, e->ev_bar, gate_in, nullptr, 0.0, paj_bar);
+  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, c, e->k1bar, e->gmbar,
+                     e->ca_bar, e->cb_bar, e->ev_bar, ybar_in, nullptr, 0.0, pam_bar, cap_bar);
+  if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * W, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_vmem_loss(cells_ensemble* e, void* stream, const double* y, const double* target,
+                               double weight, double* loss_out, double* ybar) {
+  if (!e || !y || !target || !loss_out) return fail(CELLS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(e->device));
+  hipLaunchKernelGGL(k_vmem_loss, dim3(e->W), dim3(kBlock), 0, static_cast<hipStream_t>(stream), e->P, y, target,
+                     weight, loss_out, ybar);
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_set_parameters(cells_ensemble* e, void* stream, const double* pam, const double* nak_cap,
+                                    const double* paj) {
+  if (!e) return fail(CELLS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(e->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  size_t nc = e->ncell, ne = e->nedge, S = e->S;
+  if (pam) HIP_TRY(hipMemcpyAsync(const_cast<double*>(e->P.pam), pam, sizeof(double) * S * nc, hipMemcpyDefault, st));
+  if (nak_cap) HIP_TRY(hipMemcpyAsync(const_cast<double*>(e->P.nak_cap), nak_cap, sizeof(double) * nc, hipMemcpyDefault, st));
+  if (paj && ne) HIP_TRY(hipMemcpyAsync(const_cast<double*>(e->P.paj), paj, sizeof(double) * S * ne, hipMemcpyDefault, st));
+  return CELLS_OK;
+}
+
+extern "C" int cells_plan_stats(const cells_ensemble* e, int64_t* nnz_total, double* flops_total,
+                                int32_t* max_levels, int32_t* shapes) {
+  if (!e) return fail(CELLS_EINVAL, "NULL argument");
+  int64_t nnz = 0;
+  double flops = 0.0;
+  int32_t lev = 0;
+  std::vector<double> shape_flops(e->shapes_host.size(), 0.0);
+  for (size_t q = 0; q < e->shapes_host.size(); ++q) {
+    const cells::Symbolic& s = e->shapes_host[q];
+    double f = 0.0;
+    for (int32_t k = 0; k < s.n; ++k) {
+      double c = (double)(s.column_start[k + 1] - s.column_start[k]);
+      f += c * c;
+    }
+    shape_flops[q] = f;
+    lev = std::max(lev, s.levels());
+  }
+  for (int w = 0; w < e->W; ++w) {
+    int q = e->shape_of_host[w];
+    nnz += e->shapes_host[q].nnz();
+    flops += shape_flops[q];
+  }
+  if (nnz_total) *nnz_total = nnz;
+  if (flops_total) *flops_total = flops;
+  if (max_levels) *max_levels = lev;
+  if (shapes) *shapes = (int32_t)e->shapes_host.size();
   return CELLS_OK;
 }

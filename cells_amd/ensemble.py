@@ -150,6 +150,48 @@ class Ensemble:
             ctypes.c_void_p(ratios.data_ptr())))
         return failed, ratios
 
+    def step_vjp(self, y, ybar, dt, ctl=None, pam_bar=None, cap_bar=None, paj_bar=None):
+        """Reverse of one ROS2 step at y: (cotangent on y, pam_bar, cap_bar, paj_bar, valid).
+        Parameter cotangents are added to the given tensors (zeros when None)."""
+        t = self.torch
+        p = self.packed
+        y, ybar = self.to_device(y), self.to_device(ybar)
+        yin = t.zeros_like(y)
+        dev, f64 = self.device, t.float64
+        pam_bar = t.zeros((p.S, p.ncell), dtype=f64, device=dev) if pam_bar is None else pam_bar
+        cap_bar = t.zeros(p.ncell, dtype=f64, device=dev) if cap_bar is None else cap_bar
+        paj_bar = t.zeros((p.S, max(p.nedge, 1)), dtype=f64, device=dev) if paj_bar is None else paj_bar
+        valid = t.empty(p.W, dtype=t.int32, device=dev)
+        ctl = ctl or control()
+        v = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+        _lib.check(_lib.load().cells_step_vjp(self._h, self._stream(), v(y), v(ybar), float(dt),
+                                              ctypes.byref(ctl), v(yin), v(pam_bar), v(cap_bar),
+                                              v(paj_bar), v(valid)))
+        return yin, pam_bar, cap_bar, paj_bar, valid
+
+    def vmem_loss(self, y, target, weight, loss, ybar=None):
+        """loss[w] += weight sum_i (V_i - target_i)^2; ybar (if given) += its cotangent."""
+        v = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+        _lib.check(_lib.load().cells_vmem_loss(self._h, self._stream(), v(self.to_device(y)),
+                                               v(target), float(weight), v(loss),
+                                               None if ybar is None else v(ybar)))
+
+    def set_parameters(self, pam=None, nak_cap=None, paj=None):
+        """Replace the learnable parameters on the device (tensors on this device)."""
+        v = lambda x: None if x is None else ctypes.c_void_p(self.to_device(x).data_ptr())  # noqa: E731
+        keep = [self.to_device(x) if x is not None else None for x in (pam, nak_cap, paj)]
+        _lib.check(_lib.load().cells_set_parameters(self._h, self._stream(), *(
+            None if x is None else ctypes.c_void_p(x.data_ptr()) for x in keep)))
+        self.torch.cuda.current_stream(self.device).synchronize()
+
+    def plan_stats(self):
+        """dict(nnz, flops, max_levels, shapes) of the bound factor plans."""
+        nnz, lev, shapes = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        flops = ctypes.c_double()
+        _lib.check(_lib.load().cells_plan_stats(self._h, ctypes.byref(nnz), ctypes.byref(flops),
+                                                ctypes.byref(lev), ctypes.byref(shapes)))
+        return dict(nnz=nnz.value, flops=flops.value, max_levels=lev.value, shapes=shapes.value)
+
     def kernel_times(self, y, dt, reps=3, ctl=None):
         """[(kernel name, mean ms)] of one step's launches, in launch order (HIP events)."""
         y = self.to_device(y)

@@ -29,7 +29,8 @@ EXPORTS = (
     "cells_symbolic_export", "cells_symbolic_free", "cells_ensemble_create",
     "cells_ensemble_destroy", "cells_state_size", "cells_step", "cells_rates",
     "cells_factor_size", "cells_factor", "cells_solve", "cells_world_positions",
-    "cells_kernel_times", "cells_device_checks",
+    "cells_kernel_times", "cells_device_checks", "cells_step_vjp", "cells_vmem_loss",
+    "cells_set_parameters", "cells_plan_stats",
 )
 
 
@@ -102,6 +103,12 @@ def load():
     lib.cells_device_checks.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_void_p, ctypes.c_void_p]
+    vp = ctypes.c_void_p
+    lib.cells_step_vjp.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
+                                   vp, vp, vp, vp, vp]
+    lib.cells_vmem_loss.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, vp]
+    lib.cells_set_parameters.argtypes = [vp, vp, vp, vp, vp]
+    lib.cells_plan_stats.argtypes = [vp, _i64p, _f64p, _i32p, _i32p]
     lib.cells_world_positions.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i64p, _i64p, _i64p]
     lib.cells_minimum_degree.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p]
     lib.cells_bind.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p,

@@ -127,6 +127,26 @@ int cells_device_checks(cells_ensemble* e, void* stream, const double* previous,
                         double steps, double material_atol, int32_t* failed_out,
                         double* ratios_out);
 
+/* Reverse of one ROS2 step at y for every world (no counterpart in the reference, which
+ * differentiates only forward, analysis/stationary.py:505-517): from the cotangent ybar_out on the
+ * step's output amounts and gates, writes ybar_in's amounts and gates (other fields untouched) and
+ * adds the parameter cotangents pam_bar [S][ncell] (membrane area x permeability), cap_bar [ncell]
+ * (Na/K capacity), paj_bar [S][nedge] (junction area x permeability). The stage solves reuse the
+ * step's Cholesky factor (the stage matrix is symmetric). 3-ion worlds without water; ROS2 only. */
+int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const double* ybar_out,
+                   double dt, const cells_control* control, double* ybar_in, double* pam_bar,
+                   double* cap_bar, double* paj_bar, int32_t* valid_out);
+
+/* Vmem misfit of one sample row: loss_out[w] += weight sum_i (V_i - target_i)^2 over world w's
+ * cells, and (when ybar is not NULL) its cotangent on the amounts added to ybar. */
+int cells_vmem_loss(cells_ensemble* e, void* stream, const double* y, const double* target,
+                    double weight, double* loss_out, double* ybar);
+
+/* Replaces the per-cell and per-edge parameters the calibration learns (device or host
+ * pointers; NULL keeps one): pam [S][ncell], nak_cap [ncell], paj [S][nedge]. */
+int cells_set_parameters(cells_ensemble* e, void* stream, const double* pam, const double* nak_cap,
+                         const double* paj);
+
 /* The batched numeric factor and solve on the stage pattern (numerics/sparse_cholesky.py:474-514):
  * values: device [factor size] (each world's assembled entries at its factor positions), factored
  * in place; rhs/x: device [ncell] in node order, one system per world. */
@@ -134,6 +154,11 @@ int64_t cells_factor_size(const cells_ensemble* e);
 int cells_factor(cells_ensemble* e, void* stream, double* values);
 int cells_solve(cells_ensemble* e, void* stream, const double* values, const double* rhs,
                 double* x);
+/* Totals of the bound plans: factor entries over all worlds, sum over worlds of sum_j c_j^2 (the
+ * reference's factorisation cost, c_j the entries of column j), the most levels, and the number of
+ * distinct lattice shapes. */
+int cells_plan_stats(const cells_ensemble* e, int64_t* nnz_total, double* flops_total,
+                     int32_t* max_levels, int32_t* shapes);
 /* Host copy of world w's plan positions: diag_pos [n] by node, edge_pos [E] by edge, and the
  * offset of its factor slice. */
 int cells_world_positions(const cells_ensemble* e, int32_t w, int64_t* diag_pos,

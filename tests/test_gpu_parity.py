@@ -233,3 +233,70 @@ def test_device_checks_pass_on_accepted_rows_and_catch_a_planted_ledger_error(to
     failed, _ = e.device_checks(y1, bad, steps=40)
     assert failed.cpu().numpy().tolist()[2] & 16
     e.close()
+
+
+def _three_ion(worlds=2, cells=(12, 30), seed=5):
+    return build(config1(worlds=worlds, cells=cells, species=("Na", "K", "Cl"),
+                         inside_mM=(12.0, 140.0, 10.0), bath_mM=(145.0, 5.0, 110.0),
+                         permeability=(1e-10, 5e-9, 1e-9), seed=seed))
+
+
+def test_step_adjoint_matches_the_oracle_adjoint(torch):
+    p = _three_ion()
+    L = StateLayout.of(p)
+    _, sym = oracle.world_symbolics(p)
+    w = oracle.world_struct(p)
+    y = initial_state(p)
+    for _ in range(10):
+        y, _ = oracle.step_ros2(p, y, 5e-3, sym, world=w)
+    rng = np.random.default_rng(1)
+    ybar = np.zeros_like(y)
+    for s in range(p.S):
+        L.cell(ybar, L.dn + s)[:] = rng.normal(size=p.ncell)
+    L.edge(ybar, L.gate)[:] = rng.normal(size=p.nedge)
+    yin_o, pam_o, cap_o, paj_o = oracle.ros2_vjp(p, y, 5e-3, sym, ybar, world=w)
+    e = ensemble(p, "ros2")
+    say("adjoint parity")
+    yin, pam, cap, paj, valid = e.step_vjp(y, ybar, 5e-3)
+    assert valid.cpu().numpy().all()
+    yin = yin.cpu().numpy()
+    for s in range(p.S):
+        a, b = L.cell(yin, L.dn + s), L.cell(yin_o, L.dn + s)
+        assert relerr(a, b, floor=1e-9 * np.abs(b).max()) <= 1e-9, s
+    a, b = L.edge(yin, L.gate), L.edge(yin_o, L.gate)
+    assert relerr(a, b, floor=1e-9 * np.abs(b).max()) <= 1e-9
+    for got, want in ((pam, pam_o), (cap, cap_o), (paj[:, :p.nedge], paj_o)):
+        g = got.cpu().numpy()
+        assert relerr(g, want, floor=1e-9 * np.abs(want).max()) <= 1e-9
+    e.close()
+
+
+def test_step_adjoint_meets_central_differences_of_the_hip_step(torch):
+    p = _three_ion(seed=9)
+    L = StateLayout.of(p)
+    e = ensemble(p, "ros2")
+    y, _ = e.step(initial_state(p), 5e-3, nsteps=10)
+    y = y.cpu().numpy()
+    rng = np.random.default_rng(2)
+    ybar = np.zeros_like(y)
+    for s in range(p.S):
+        L.cell(ybar, L.dn + s)[:] = rng.normal(size=p.ncell)
+    L.edge(ybar, L.gate)[:] = rng.normal(size=p.nedge)
+    yin, *_ = e.step_vjp(y, ybar, 5e-3)
+    yin = yin.cpu().numpy()
+
+    def dot(a, b):
+        t = sum(np.dot(L.cell(a, L.dn + s), L.cell(b, L.dn + s)) for s in range(p.S))
+        return t + np.dot(L.edge(a, L.gate), L.edge(b, L.gate))
+
+    for trial in range(3):
+        v = np.zeros_like(y)
+        for s in range(p.S):
+            L.cell(v, L.dn + s)[:] = rng.normal(size=p.ncell) * 1e-6
+        L.edge(v, L.gate)[:] = rng.normal(size=p.nedge) * 1e-3
+        h = 1e-3
+        plus = e.step(y + h * v, 5e-3)[0].cpu().numpy()
+        minus = e.step(y - h * v, 5e-3)[0].cpu().numpy()
+        fd = (dot(ybar, plus) - dot(ybar, minus)) / (2 * h)
+        assert abs(fd - dot(yin, v)) <= 1e-6 * abs(fd), (trial, fd, dot(yin, v))
+    e.close()
