@@ -1,0 +1,166 @@
+"""Calibration gradients through the ROS2 time loop: the adjoint step under binomial checkpointing.
+
+The reference has no adjoint and no calibration loop (it differentiates only forward,
+analysis/stationary.py:505-517); this is the build's own, for the adjoint-calibration
+configuration: per-cell log-parameters theta [6, ncell] =
+  (log P_Na, log P_K, log P_Cl, log Na/K pump capacity, log membrane leak scale,
+   log gap-junction scale),
+acting as
+  pam[s, i]  = pam0[s, i] exp(theta[s, i] + theta[4, i])          (s = Na, K, Cl)
+  nak[i]     = nak0[i] exp(theta[3, i])
+  paj[s, e]  = paj0[s, e] exp((theta[5, a(e)] + theta[5, b(e)]) / 2)
+The "leak scale" multiplies every membrane permeability of a cell (a redundant direction of the
+three log permeabilities, declared so). The loss is the mean squared Vmem misfit over sample
+rows: L = 1/(K n) sum_k sum_i (V_i(t_k) - target_k,i)^2.
+
+Checkpointing: Griewank's binomial schedule (Griewank and Walther 2000, ACM TOMS 26:19, "revolve"):
+with c free snapshot slots and l steps to reverse, the first split keeps the right part within
+beta(c-1, r) = C(c-1+r, r) steps, r the least repetition count with beta(c, r) >= l. Snapshots
+are device copies of the state. With c >= steps every state is kept and nothing is recomputed.
+
+Every step, reverse step, loss and parameter update runs in libcells_hip.so; this module only
+orders the calls and keeps the snapshots (device tensors).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from cells_amd.ensemble import control
+from cells_amd.world import Packed
+
+
+def beta(c: int, r: int) -> int:
+    return math.comb(c + r, r)
+
+
+def split(length: int, free: int) -> int:
+    """Length of the right part of [a, a + length) kept after the first advance (binomial)."""
+    r = 0
+    while beta(free, r) < length:
+        r += 1
+    return max(1, min(length - 1, beta(free - 1, r)))
+
+
+def theta_parameters(p: Packed, base, theta):
+    """(pam [S, n], nak_cap [n], paj [S, E]) of theta [6, n] on base = (pam0, nak0, paj0)."""
+    pam0, nak0, paj0 = base
+    pam = pam0 * np.exp(theta[:3] + theta[4][None, :])
+    nak = nak0 * np.exp(theta[3])
+    paj = paj0 * np.exp(0.5 * (theta[5][p.snd] + theta[5][p.rcv]))[None, :]
+    return pam, nak, paj
+
+
+def theta_cotangent(p: Packed, params, bars):
+    """theta_bar [6, n] from the parameters' cotangents (the chain rule of theta_parameters)."""
+    pam, nak, paj = params
+    pam_bar, cap_bar, paj_bar = bars
+    tb = np.zeros((6, p.ncell))
+    g = pam_bar * pam
+    tb[:3] = g
+    tb[4] = g.sum(axis=0)
+    tb[3] = cap_bar * nak
+    h = 0.5 * (paj_bar[:, :p.nedge] * paj).sum(axis=0)
+    np.add.at(tb[5], p.snd, h)
+    np.add.at(tb[5], p.rcv, h)
+    return tb
+
+
+class Calibration:
+    """Loss and gradient of the Vmem misfit over `steps` ROS2 steps of an ensemble."""
+
+    def __init__(self, ensemble, y0, dt, steps, sample_every, targets, ctl=None):
+        self.e = ensemble
+        self.t = ensemble.torch
+        self.p = ensemble.packed
+        if self.p.S != 3 or self.p.water or self.p.calcium:
+            raise ValueError("calibration covers 3-ion worlds without water")
+        self.y0 = ensemble.to_device(y0)
+        self.dt = float(dt)
+        self.steps = int(steps)
+        self.every = int(sample_every)
+        self.samples = list(range(self.every, self.steps + 1, self.every))
+        self.targets = {k: self.t.as_tensor(np.asarray(targets[i]), dtype=self.t.float64,
+                                            device=ensemble.device)
+                        for i, k in enumerate(self.samples)}
+        self.weight = 1.0 / (len(self.samples) * self.p.ncell)
+        self.ctl = ctl or control()
+        self.base = (self.p.pam.copy(), self.p.nak_cap.copy(), self.p.paj.copy())
+        self.forward_steps = 0
+        self.reverse_steps = 0
+
+    def set_theta(self, theta):
+        params = theta_parameters(self.p, self.base, theta)
+        self.e.set_parameters(*params)
+        return params
+
+    def _advance(self, y, n):
+        if n == 0:
+            return y.clone()
+        out, ok = self.e.step(y, self.dt, nsteps=n, ctl=self.ctl)
+        if not bool(ok.cpu().numpy().all()):
+            raise FloatingPointError("a world refused a step of the calibration run")
+        self.forward_steps += n
+        return out
+
+    def loss(self, theta):
+        """The loss alone (one forward sweep)."""
+        self.set_theta(theta)
+        t = self.t
+        acc = t.zeros(self.p.W, dtype=t.float64, device=self.e.device)
+        y, last = self.y0, 0
+        for k in self.samples:
+            y = self._advance(y, k - last)
+            last = k
+            self.e.vmem_loss(y, self.targets[k], self.weight, acc)
+        return float(acc.sum().cpu())
+
+    def loss_and_gradient(self, theta, snapshots=45):
+        """(loss, theta_bar [6, n]) with the binomial schedule over `snapshots` free slots."""
+        t = self.t
+        params = self.set_theta(theta)
+        loss = self.loss(theta)
+        W = self.p.W
+        S, n, E = self.p.S, self.p.ncell, max(self.p.nedge, 1)
+        dev, f64 = self.e.device, t.float64
+        self.pam_bar = t.zeros((S, n), dtype=f64, device=dev)
+        self.cap_bar = t.zeros(n, dtype=f64, device=dev)
+        self.paj_bar = t.zeros((S, E), dtype=f64, device=dev)
+        self.ybar = t.zeros_like(self.y0)
+        self.scratch = t.zeros(W, dtype=f64, device=dev)
+        store = {0: self.y0}
+        self._reverse(store, 0, self.steps, snapshots)
+        bars = (self.pam_bar.cpu().numpy(), self.cap_bar.cpu().numpy(), self.paj_bar.cpu().numpy())
+        return loss, theta_cotangent(self.p, params, bars)
+
+    def _adjoint_step(self, y_a, a):
+        """ybar <- reverse of step a -> a+1 at y_a, after adding the loss cotangent at a+1."""
+        b = a + 1
+        if b in self.targets:
+            y_b = self._advance(y_a, 1)
+            self.e.vmem_loss(y_b, self.targets[b], self.weight, self.scratch, self.ybar)
+        yin, _, _, _, valid = self.e.step_vjp(y_a, self.ybar, self.dt, ctl=self.ctl,
+                                             pam_bar=self.pam_bar, cap_bar=self.cap_bar,
+                                             paj_bar=self.paj_bar)
+        if not bool(valid.cpu().numpy().all()):
+            raise FloatingPointError("a world refused a step of the reverse sweep")
+        self.ybar = yin
+        self.reverse_steps += 1
+
+    def _reverse(self, store, a, b, free):
+        length = b - a
+        if length == 1:
+            self._adjoint_step(store[a], a)
+            return
+        if free <= 0:
+            for j in range(b - 1, a - 1, -1):
+                self._adjoint_step(self._advance(store[a], j - a), j)
+            return
+        right = split(length, free)
+        m = b - right
+        store[m] = self._advance(store[a], m - a)
+        self._reverse(store, m, b, free - 1)
+        del store[m]
+        self._reverse(store, a, m, free)
