@@ -6,7 +6,8 @@
 //   k_edges        junction routes: GHK flux, heat, gate rate, slopes (thread per edge)
 //   k_cells        membrane routes, Na/K- and Ca-ATPase, owner sums, stage-matrix diagonal
 //                  (thread per cell; a wavefront reads one species of 64 consecutive cells)
-//   k_factor_solve level-scheduled sparse Cholesky of C + gamma dt G and its solves
+//   k_factor       level-scheduled sparse Cholesky of C + gamma dt G (workgroup per world)
+//   k_solve_stage  its two triangular solves per stage (workgroup per world, vector in LDS)
 //                  (workgroup per world)
 //   k_*_update     stage combinations and domain checks (thread per cell and per edge)
 //   k_world_*      per-world ledgers, fixed-order reductions (workgroup per world)
@@ -112,6 +113,7 @@ struct ShapeDev {
   int n, E, levels;
   int nnz;
   long o_cs, o_entry, o_rowptr, o_rowent, o_lev, o_levcol, o_levent, o_node;
+  long o_mask;  // offset in the mask pool (unsigned long long words)
 };
 
 __device__ __forceinline__ double& CF(const Params& P, double* y, int f, int i) {
@@ -614,33 +616,50 @@ struct PlanView {
   const int* order;      // [n] rank -> node
   const int* diagpos;    // [n] node -> diagonal position
   const int* edgepos;    // [E] world edge -> position
+  const unsigned long long* mask;  // [2 (nnz-n)]: per (k, m) row-structure pair, column k's slots
+                                   // that column m's tail reaches (bit j: the j-th entry of k)
 };
 
-struct Pairwise {
-  double v[24];
-  int c[24];
-  int top;
-  __device__ void reset() { top = 0; }
-  __device__ void push(double t) {
-    v[top] = t;
-    c[top] = 1;
-    ++top;
-    while (top >= 2 && c[top - 1] == c[top - 2]) {
-      v[top - 2] = v[top - 2] + v[top - 1];
-      c[top - 2] *= 2;
-      --top;
+// Pairwise sums in the reference's adjacent-pairs order (numerics/sparse_cholesky.py:244-322),
+// held as a binary counter of complete blocks: s[j] is the sum of a block of 2^j consecutive terms
+// when bit j of n is set. Adding a term merges it, oldest block on the left, into every block the
+// increment of n carries through; the total folds the blocks from the newest up. The loops are
+// unrolled over a fixed depth so every partial stays in registers.
+struct PairSum {
+  static constexpr int kDepth = 13;  // up to 8191 terms
+  double s[kDepth];
+  unsigned n;
+  __device__ __forceinline__ void reset() { n = 0u; }
+  __device__ __forceinline__ void push(double t) {
+    double carry = t;
+    bool open = true;
+#pragma unroll
+    for (int j = 0; j < kDepth; ++j) {
+      bool bit = (n >> j) & 1u;
+      if (open && bit) carry = s[j] + carry;
+      if (open && !bit) s[j] = carry;
+      open = open && bit;
     }
+    ++n;
   }
-  __device__ double sum() const {
-    if (top == 0) return 0.0;
-    double s = v[top - 1];
-    for (int i = top - 2; i >= 0; --i) s = v[i] + s;
-    return s;
+  __device__ __forceinline__ double sum() const {
+    double r = 0.0;
+    bool have = false;
+#pragma unroll
+    for (int j = 0; j < kDepth; ++j) {
+      if ((n >> j) & 1u) {
+        r = have ? s[j] + r : s[j];
+        have = true;
+      }
+    }
+    return r;
   }
 };
 
-__device__ PlanView plan_view(const ShapeDev& s, const int* pool) {
+__device__ PlanView plan_view(const ShapeDev& s, const int* pool,
+                              const unsigned long long* masks = nullptr) {
   PlanView v;
+  v.mask = masks ? masks + s.o_mask : nullptr;
   v.cs = pool + s.o_cs;
   v.rrank = pool + s.o_entry;
   v.crank = pool + s.o_entry + s.nnz;
@@ -662,38 +681,32 @@ __device__ void factor_world(const ShapeDev& s, const PlanView& v, double* fv) {
     int a = v.leventptr[lev], b = v.leventptr[lev + 1];
     for (int q = a + threadIdx.x; q < b; q += blockDim.x) {
       int p = v.levent[q];
-      int r = v.rrank[p], k = v.crank[p];
-      Pairwise pw;
+      int k = v.crank[p];
+      int slot = p - v.cs[k];
+      unsigned long long lo_mask = slot >= 64 ? ~0ull : ((1ull << slot) - 1ull);
+      unsigned long long hi_mask = slot >= 64 ? ((1ull << (slot - 64)) - 1ull) : 0ull;
+      int word = slot >> 6;
+      unsigned long long bit = 1ull << (slot & 63);
+      PairSum pw;
       pw.reset();
       int x = v.rowptr[k], xe = v.rowptr[k + 1];
-      if (r == k) {
-        for (; x < xe; ++x) {
-          double t = fv[v.rowpos[x]];
-          pw.push(t * t);
-        }
-      } else {
-        int y = v.rowptr[r], ye = v.rowptr[r + 1];
-        while (x < xe && y < ye) {
-          int mk = v.rowcol[x], mr = v.rowcol[y];
-          if (mk == mr) {
-            pw.push(fv[v.rowpos[y]] * fv[v.rowpos[x]]);
-            ++x;
-            ++y;
-          } else if (mk < mr) {
-            ++x;
-          } else {
-            ++y;
-          }
+      for (; x < xe; ++x) {
+        unsigned long long m0 = v.mask[2 * (long)x], m1 = v.mask[2 * (long)x + 1];
+        unsigned long long mine = word ? m1 : m0;
+        if (mine & bit) {
+          int base = v.rowpos[x];
+          int idx = __popcll(m0 & lo_mask) + __popcll(m1 & hi_mask);
+          pw.push(fv[base + idx] * fv[base]);
         }
       }
       double acc = fv[p] - pw.sum();
-      fv[p] = r == k ? sqrt(acc) : acc;
+      fv[p] = slot == 0 ? sqrt(acc) : acc;
     }
     __syncthreads();
     for (int q = a + threadIdx.x; q < b; q += blockDim.x) {
       int p = v.levent[q];
-      int r = v.rrank[p], k = v.crank[p];
-      if (r != k) fv[p] = fv[p] * (1.0 / fv[v.cs[k]]);
+      int k = v.crank[p];
+      if (p != v.cs[k]) fv[p] = fv[p] * (1.0 / fv[v.cs[k]]);
     }
     __syncthreads();
   }
@@ -704,7 +717,7 @@ __device__ void solve_world(const ShapeDev& s, const PlanView& v, const double* 
   for (int lev = 0; lev < s.levels; ++lev) {
     for (int q = v.levptr[lev] + threadIdx.x; q < v.levptr[lev + 1]; q += blockDim.x) {
       int k = v.levcol[q];
-      Pairwise pw;
+      PairSum pw;
       pw.reset();
       for (int x = v.rowptr[k]; x < v.rowptr[k + 1]; ++x) pw.push(fv[v.rowpos[x]] * y[v.rowcol[x]]);
       double known = y[k] - pw.sum();
@@ -715,7 +728,7 @@ __device__ void solve_world(const ShapeDev& s, const PlanView& v, const double* 
   for (int lev = s.levels - 1; lev >= 0; --lev) {
     for (int q = v.levptr[lev] + threadIdx.x; q < v.levptr[lev + 1]; q += blockDim.x) {
       int k = v.levcol[q];
-      Pairwise pw;
+      PairSum pw;
       pw.reset();
       for (int p = v.cs[k] + 1; p < v.cs[k + 1]; ++p) pw.push(fv[p] * y[v.rrank[p]]);
       double known = y[k] - pw.sum();
@@ -725,36 +738,50 @@ __device__ void solve_world(const ShapeDev& s, const PlanView& v, const double* 
   }
 }
 
-// Assemble (Jacobi-scaled, symmetrised; numerics/surface_imex.py:228-259), factor, and solve
-// (C + coef G) dphi = q(K) for each world: rhs' = q s, dphi = x s.
-__global__ __launch_bounds__(kBlock) void k_factor_solve(Params P, const ShapeDev* shapes,
+// Assemble (Jacobi-scaled, symmetrised; numerics/surface_imex.py:228-259) and factor
+// C + coef G for each world (workgroup per world, no LDS: the factor lives in HBM/L2).
+constexpr int kFactorBlock = 128;
+
+__global__ __launch_bounds__(kFactorBlock) void k_factor(Params P, const ShapeDev* shapes,
                                                          const int* shape_of, const int* pool,
-                                                         const long* foff, int w0, int assemble,
-                                                         double coef, const double* mdiag,
-                                                         const double* msc, const double* Gj,
-                                                         double* fvals, const double* K,
-                                                         double* dphi, int* valid) {
+                                                         const unsigned long long* masks,
+                                                         const long* foff, int w0, double coef,
+                                                         const double* mdiag, const double* msc,
+                                                         const double* Gj, double* fvals,
+                                                         int* valid) {
+  int w = w0 + blockIdx.x;
+  const ShapeDev s = shapes[shape_of[w]];
+  const PlanView v = plan_view(s, pool, masks);
+  double* fv = fvals + foff[w];
+  int c0 = P.cell_off[w], e0 = P.edge_off[w];
+  for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = 0.0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < s.n; i += blockDim.x) fv[v.diagpos[i]] = mdiag[c0 + i];
+  for (int e = threadIdx.x; e < s.E; e += blockDim.x) {
+    double val = -(coef * Gj[e0 + e]);
+    double sa = msc[P.snd[e0 + e]], sb = msc[P.rcv[e0 + e]];
+    fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
+  }
+  __syncthreads();
+  factor_world(s, v, fv);
+  bool ok = true;
+  for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) ok = ok && isfinite(fv[p]);
+  if (!ok) valid[w] = 0;
+}
+
+// Solve (C + coef G) dphi = q(K) on each world's factor: rhs' = q s, dphi = x s.
+__global__ __launch_bounds__(kFactorBlock) void k_solve_stage(Params P, const ShapeDev* shapes,
+                                                              const int* shape_of, const int* pool,
+                                                              const long* foff, int w0,
+                                                              const double* msc,
+                                                              const double* fvals, const double* K,
+                                                              double* dphi) {
   extern __shared__ double lds[];
   int w = w0 + blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
-  double* fv = fvals + foff[w];
-  int c0 = P.cell_off[w], e0 = P.edge_off[w];
-  if (assemble) {
-    for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = 0.0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < s.n; i += blockDim.x) fv[v.diagpos[i]] = mdiag[c0 + i];
-    for (int e = threadIdx.x; e < s.E; e += blockDim.x) {
-      double val = -(coef * Gj[e0 + e]);
-      double sa = msc[P.snd[e0 + e]], sb = msc[P.rcv[e0 + e]];
-      fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
-    }
-    __syncthreads();
-    factor_world(s, v, fv);
-    bool ok = true;
-    for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) ok = ok && isfinite(fv[p]);
-    if (!ok) valid[w] = 0;
-  }
+  const double* fv = fvals + foff[w];
+  int c0 = P.cell_off[w];
   double* y = lds;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
     int i = c0 + v.order[k];
@@ -769,16 +796,17 @@ __global__ __launch_bounds__(kBlock) void k_factor_solve(Params P, const ShapeDe
 }
 
 // Test entry: factor given assembled values in place, per world.
-__global__ __launch_bounds__(kBlock) void k_factor_only(const ShapeDev* shapes, const int* shape_of,
-                                                        const int* pool, const long* foff,
-                                                        double* fvals) {
+__global__ __launch_bounds__(kFactorBlock) void k_factor_only(const ShapeDev* shapes,
+                                                              const int* shape_of, const int* pool,
+                                                              const unsigned long long* masks,
+                                                              const long* foff, double* fvals) {
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
-  const PlanView v = plan_view(s, pool);
+  const PlanView v = plan_view(s, pool, masks);
   factor_world(s, v, fvals + foff[w]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_solve_only(const ShapeDev* shapes, const int* shape_of,
+__global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* shapes, const int* shape_of,
                                                        const int* pool, const long* foff,
                                                        const int* cell_off, const double* fvals,
                                                        const double* rhs, double* x) {
@@ -1286,7 +1314,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_dtrans(Params P, double coef, co
 
 // w = M^-1 dbar on the step's factor (workgroup per world); then rate_bar += q^T w and the
 // slopes' cotangent of M: -coef w delta^T through G = F' sum_s z_s slope_s.
-__global__ __launch_bounds__(kBlock) void k_adj_solve(Params P, const ShapeDev* shapes,
+__global__ __launch_bounds__(kFactorBlock) void k_adj_solve(Params P, const ShapeDev* shapes,
                                                       const int* shape_of, const int* pool,
                                                       const long* foff, double coef,
                                                       const double* msc, const double* fvals,
@@ -1384,6 +1412,7 @@ struct cells_ensemble {
   ShapeDev* shapes = nullptr;
   int* shape_of = nullptr;
   int* pool = nullptr;
+  unsigned long long* masks = nullptr;
   long* foff = nullptr;
   double *A = nullptr, *B = nullptr, *Y1 = nullptr, *K1 = nullptr, *K2 = nullptr;
   double *contrib1 = nullptr, *contrib2 = nullptr, *gm = nullptr, *gj = nullptr, *Gj = nullptr;
@@ -1734,6 +1763,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
     // device plans, concatenated
     std::vector<ShapeDev> sd(rep.size());
     std::vector<int> pool;
+    std::vector<unsigned long long> mask_pool;
+    std::vector<int> slot_of;
     for (size_t q = 0; q < rep.size(); ++q) {
       const cells::Symbolic& s = e->shapes_host[q];
       ShapeDev d{};
@@ -1741,6 +1772,27 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       d.nnz = (int)s.nnz();
       d.levels = s.levels();
       int k = rep[q];
+      // slot masks: for each row-structure pair (column kk, source column m), the slots of
+      // column kk that column m's entries at rows >= kk reach
+      d.o_mask = (long)mask_pool.size();
+      slot_of.assign(s.n, -1);
+      for (int kk = 0; kk < s.n; ++kk) {
+        int64_t c0 = s.column_start[kk], c1 = s.column_start[kk + 1];
+        if (c1 - c0 > 128) return fail(CELLS_EINVAL, "a factor column holds more than 128 entries");
+        for (int64_t p2 = c0; p2 < c1; ++p2) slot_of[s.row_rank[p2]] = (int)(p2 - c0);
+        for (int64_t x = s.row_ptr[kk]; x < s.row_ptr[kk + 1]; ++x) {
+          int m = s.row_col[x];
+          unsigned long long w2[2] = {0ull, 0ull};
+          for (int64_t p2 = s.row_pos[x]; p2 < s.column_start[m + 1]; ++p2) {
+            int slot = slot_of[s.row_rank[p2]];
+            if (slot < 0) return fail(CELLS_EINVAL, "symbolic structure is not closed under the elimination tree");
+            w2[slot >> 6] |= 1ull << (slot & 63);
+          }
+          mask_pool.push_back(w2[0]);
+          mask_pool.push_back(w2[1]);
+        }
+        for (int64_t p2 = c0; p2 < c1; ++p2) slot_of[s.row_rank[p2]] = -1;
+      }
       d.E = (int)(w->edge_off[k + 1] - w->edge_off[k]);
       d.o_cs = (long)pool.size();
       for (int64_t v : s.column_start) pool.push_back((int)v);
@@ -1787,6 +1839,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
     if ((rc = upload(e.get(), &e->shapes, sd.data(), sd.size())) ||
         (rc = upload(e.get(), &e->shape_of, e->shape_of_host.data(), e->shape_of_host.size())) ||
         (rc = upload(e.get(), &e->pool, pool.data(), pool.size())) ||
+        (rc = upload(e.get(), &e->masks, mask_pool.data(), mask_pool.size())) ||
         (rc = upload(e.get(), &e->foff, e->foff_host.data(), e->foff_host.size())) ||
         (rc = alloc(e.get(), &e->fvals, std::max<long>(e->factor_total, 1))))
       return rc;
@@ -1884,9 +1937,12 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
                        c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
     mark("k_cells");
-    hipLaunchKernelGGL(k_factor_solve, dim3(nw), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool,
-                       e->foff, w0, 1, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->K1, e->dphi, e->valid);
-    mark("k_factor_solve");
+    hipLaunchKernelGGL(k_factor, dim3(nw), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool,
+                       e->masks, e->foff, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
+    mark("k_factor");
+    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of,
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, e->dphi);
+    mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->contrib1, e->Y1);
     mark("k_ros2_stage1");
@@ -1901,9 +1957,9 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_ros2_stage2, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, c0, nc, ed0, ne, e->K1,
                        e->contrib1, e->K2, e->contrib2);
     mark("k_ros2_stage2");
-    hipLaunchKernelGGL(k_factor_solve, dim3(nw), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool,
-                       e->foff, w0, 0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->K2, e->dphi, e->valid);
-    mark("k_factor_solve");
+    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of,
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->dphi);
+    mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->K2, e->contrib2, e->valid);
     mark("k_ros2_final");
@@ -2032,8 +2088,8 @@ extern "C" int cells_rates(cells_ensemble* e, void* stream, const double* y,
 extern "C" int cells_factor(cells_ensemble* e, void* stream, double* values) {
   if (!e || !values) return fail(CELLS_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(e->device));
-  hipLaunchKernelGGL(k_factor_only, dim3(e->W), dim3(kBlock), 0, static_cast<hipStream_t>(stream), e->shapes,
-                     e->shape_of, e->pool, e->foff, values);
+  hipLaunchKernelGGL(k_factor_only, dim3(e->W), dim3(kFactorBlock), 0, static_cast<hipStream_t>(stream),
+                     e->shapes, e->shape_of, e->pool, e->masks, e->foff, values);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }
@@ -2043,7 +2099,7 @@ extern "C" int cells_solve(cells_ensemble* e, void* stream, const double* values
   if (e->max_n > 8192) return fail(CELLS_EINVAL, "a world over 8192 cells exceeds the solve's LDS vector");
   HIP_TRY(hipSetDevice(e->device));
   size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
-  hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kBlock), lds, static_cast<hipStream_t>(stream), e->shapes,
+  hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kFactorBlock), lds, static_cast<hipStream_t>(stream), e->shapes,
                      e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
@@ -2104,8 +2160,10 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                        c, e->K1, e->gj, e->Gj, e->valid);
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
                      e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
-  hipLaunchKernelGGL(k_factor_solve, dim3(W), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
-                     0, 1, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->K1, e->d1, e->valid);
+  hipLaunchKernelGGL(k_factor, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool, e->masks,
+                     e->foff, 0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
+  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K1, e->d1);
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
                      ne, dt, coef, e->gm, e->gj, e->d1, e->K1, e->contrib1, e->Y1);
   if (ne)
@@ -2115,8 +2173,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_ros2_stage2, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, 0, nc, 0, ne, e->K1,
                      e->contrib1, e->K2, e->contrib2);
-  hipLaunchKernelGGL(k_factor_solve, dim3(W), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
-                     0, 0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->K2, e->d2, e->valid);
+  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K2, e->d2);
   // reverse sweep
   hipLaunchKernelGGL(k_adj_seed, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, ybar_out, dt, ybar_in, e->k1bar,
                      e->k2bar, e->k1gbar, e->g2bar);
@@ -2125,7 +2183,7 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k2bar,
                      e->d2, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
                      coef, e->msc, e->fvals, e->dbar, e->d2, e->wv, e->k2bar, e->gmbar, e->gjbar);
   // r2 = f(Y1) - 2 k1
   hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
@@ -2139,7 +2197,7 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k1bar,
                      e->d1, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
                      coef, e->msc, e->fvals, e->dbar, e->d1, e->wv, e->k1bar, e->gmbar, e->gjbar);
   // f(y) and the slopes at y
   if (ne)

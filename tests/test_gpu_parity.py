@@ -300,3 +300,36 @@ def test_step_adjoint_meets_central_differences_of_the_hip_step(torch):
         fd = (dot(ybar, plus) - dot(ybar, minus)) / (2 * h)
         assert abs(fd - dot(yin, v)) <= 1e-6 * abs(fd), (trial, fd, dot(yin, v))
     e.close()
+
+
+def test_calibration_gradient_meets_central_differences(torch):
+    """Loss over 40 ROS2 steps (4 sample rows), binomial checkpointing with 3 slots: the
+    gradient in theta against central differences of the HIP loss (h = 1e-6)."""
+    from cells_amd.adjoint import Calibration
+    from cells_amd.model import potentials  # noqa: F401
+    from tests.helpers import vmem as vm
+
+    p = _three_ion(worlds=2, cells=(16, 25), seed=21)
+    e = ensemble(p, "ros2")
+    y0 = initial_state(p)
+    rng = np.random.default_rng(4)
+    hidden = rng.normal(scale=0.2, size=(6, p.ncell))
+    cal = Calibration(e, y0, 5e-3, 40, 10, [np.zeros(p.ncell)] * 4)
+    cal.set_theta(hidden)
+    y = e.to_device(y0)
+    targets = []
+    for _ in range(4):
+        y, _ = e.step(y, 5e-3, nsteps=10)
+        targets.append(vm(p, y.cpu().numpy()))
+    cal = Calibration(e, y0, 5e-3, 40, 10, targets)
+    theta = np.zeros((6, p.ncell))
+    say("calibration gradient")
+    loss, grad = cal.loss_and_gradient(theta, snapshots=3)
+    assert loss > 0
+    for trial in range(4):
+        d = rng.normal(size=theta.shape)
+        h = 1e-6
+        fd = (cal.loss(theta + h * d) - cal.loss(theta - h * d)) / (2 * h)
+        an = float(np.sum(grad * d))
+        assert abs(fd - an) <= 1e-6 * abs(fd), (trial, fd, an)
+    e.close()
