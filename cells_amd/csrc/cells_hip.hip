@@ -713,26 +713,107 @@ __device__ void factor_world(const ShapeDev& s, const PlanView& v, double* fv) {
 }
 
 // y (LDS, by rank) <- L^-T L^-1 y, each pivot applied as a multiplication by its reciprocal.
+// A wavefront's sum of len terms term(i), i ascending, in the adjacent-pairs order: each chunk of
+// 64 terms is an adjacent-pairs tree across the lanes (absent lanes hold -0.0, the exact identity,
+// which gives the odd-tail pass-through), full chunks enter a binary counter of 64-term blocks, and
+// the last partial chunk is the newest partial of the fold. Uniform across the wave.
+template <typename Term>
+__device__ __forceinline__ double wave_pairwise(int len, Term term) {
+  const int lane = threadIdx.x & 63;
+  double blk[7];
+  unsigned nb = 0u;
+  double tail = 0.0;
+  bool have_tail = false;
+  for (int c0 = 0; c0 < len; c0 += 64) {
+    int idx = c0 + lane;
+    double t = idx < len ? term(idx) : -0.0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      double o = __shfl_down(t, off, 64);
+      if ((lane & (2 * off - 1)) == 0) t = t + o;
+    }
+    double chunk = __shfl(t, 0, 64);
+    if (len - c0 >= 64) {
+      double carry = chunk;
+      bool open = true;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        bool bit = (nb >> j) & 1u;
+        if (open && bit) carry = blk[j] + carry;
+        if (open && !bit) blk[j] = carry;
+        open = open && bit;
+      }
+      ++nb;
+    } else {
+      tail = chunk;
+      have_tail = true;
+    }
+  }
+  double r = tail;
+  bool have = have_tail;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    if ((nb >> j) & 1u) {
+      r = have ? blk[j] + r : blk[j];
+      have = true;
+    }
+  }
+  return have ? r : 0.0;
+}
+
+// Columns with at most this many terms are summed by one thread; longer ones by a wavefront.
+constexpr int kShortColumn = 16;
+
+// y (by rank) <- L^-T L^-1 y, each pivot applied as a multiplication by its reciprocal
+// (numerics/sparse_cholesky.py:493-514, the device form). Level by level; in each level a thread
+// takes each short column and a wavefront each long one.
 __device__ void solve_world(const ShapeDev& s, const PlanView& v, const double* fv, double* y) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
   for (int lev = 0; lev < s.levels; ++lev) {
-    for (int q = v.levptr[lev] + threadIdx.x; q < v.levptr[lev + 1]; q += blockDim.x) {
+    int q0 = v.levptr[lev], q1 = v.levptr[lev + 1];
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
       int k = v.levcol[q];
+      int x0 = v.rowptr[k], x1 = v.rowptr[k + 1];
+      if (x1 - x0 > kShortColumn) continue;
       PairSum pw;
       pw.reset();
-      for (int x = v.rowptr[k]; x < v.rowptr[k + 1]; ++x) pw.push(fv[v.rowpos[x]] * y[v.rowcol[x]]);
+      for (int x = x0; x < x1; ++x) pw.push(fv[v.rowpos[x]] * y[v.rowcol[x]]);
       double known = y[k] - pw.sum();
       y[k] = known * (1.0 / fv[v.cs[k]]);
+    }
+    for (int q = q0 + wave; q < q1; q += waves) {
+      int k = v.levcol[q];
+      int x0 = v.rowptr[k], x1 = v.rowptr[k + 1];
+      if (x1 - x0 <= kShortColumn) continue;
+      double sum = wave_pairwise(x1 - x0, [&](int i) { return fv[v.rowpos[x0 + i]] * y[v.rowcol[x0 + i]]; });
+      if (lane == 0) {
+        double known = y[k] - sum;
+        y[k] = known * (1.0 / fv[v.cs[k]]);
+      }
     }
     __syncthreads();
   }
   for (int lev = s.levels - 1; lev >= 0; --lev) {
-    for (int q = v.levptr[lev] + threadIdx.x; q < v.levptr[lev + 1]; q += blockDim.x) {
+    int q0 = v.levptr[lev], q1 = v.levptr[lev + 1];
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
       int k = v.levcol[q];
+      int p0 = v.cs[k] + 1, p1 = v.cs[k + 1];
+      if (p1 - p0 > kShortColumn) continue;
       PairSum pw;
       pw.reset();
-      for (int p = v.cs[k] + 1; p < v.cs[k + 1]; ++p) pw.push(fv[p] * y[v.rrank[p]]);
+      for (int p = p0; p < p1; ++p) pw.push(fv[p] * y[v.rrank[p]]);
       double known = y[k] - pw.sum();
       y[k] = known * (1.0 / fv[v.cs[k]]);
+    }
+    for (int q = q0 + wave; q < q1; q += waves) {
+      int k = v.levcol[q];
+      int p0 = v.cs[k] + 1, p1 = v.cs[k + 1];
+      if (p1 - p0 <= kShortColumn) continue;
+      double sum = wave_pairwise(p1 - p0, [&](int i) { return fv[p0 + i] * y[v.rrank[p0 + i]]; });
+      if (lane == 0) {
+        double known = y[k] - sum;
+        y[k] = known * (1.0 / fv[v.cs[k]]);
+      }
     }
     __syncthreads();
   }
@@ -775,14 +856,13 @@ __global__ __launch_bounds__(kFactorBlock) void k_solve_stage(Params P, const Sh
                                                               const long* foff, int w0,
                                                               const double* msc,
                                                               const double* fvals, const double* K,
-                                                              double* dphi) {
-  extern __shared__ double lds[];
+                                                              double* dphi, double* ysc) {
   int w = w0 + blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   const double* fv = fvals + foff[w];
   int c0 = P.cell_off[w];
-  double* y = lds;
+  double* y = ysc + c0;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
     int i = c0 + v.order[k];
     y[k] = charge_increment(P, K, i) * msc[i];
@@ -809,12 +889,12 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_only(const ShapeDev* sh
 __global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* shapes, const int* shape_of,
                                                        const int* pool, const long* foff,
                                                        const int* cell_off, const double* fvals,
-                                                       const double* rhs, double* x) {
-  extern __shared__ double lds[];
+                                                       const double* rhs, double* x, double* ysc) {
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   int c0 = cell_off[w];
+  double* lds = ysc + c0;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) lds[k] = rhs[c0 + v.order[k]];
   __syncthreads();
   solve_world(s, v, fvals + foff[w], lds);
@@ -1320,13 +1400,13 @@ __global__ __launch_bounds__(kFactorBlock) void k_adj_solve(Params P, const Shap
                                                       const double* msc, const double* fvals,
                                                       const double* dbar, const double* delta,
                                                       double* wv, double* rate_bar, double* gmbar,
-                                                      double* gjbar) {
-  extern __shared__ double lds[];
+                                                      double* gjbar, double* ysc) {
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   const double* fv = fvals + foff[w];
   int c0 = P.cell_off[w], e0 = P.edge_off[w];
+  double* lds = ysc + c0;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
     int i = c0 + v.order[k];
     lds[k] = dbar[i] * msc[i];
@@ -1416,7 +1496,7 @@ struct cells_ensemble {
   long* foff = nullptr;
   double *A = nullptr, *B = nullptr, *Y1 = nullptr, *K1 = nullptr, *K2 = nullptr;
   double *contrib1 = nullptr, *contrib2 = nullptr, *gm = nullptr, *gj = nullptr, *Gj = nullptr;
-  double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *fvals = nullptr;
+  double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *fvals = nullptr, *ysc = nullptr;
   int *sel = nullptr, *alive = nullptr, *valid = nullptr;
   // adjoint workspace (allocated at the first cells_step_vjp)
   double *d1 = nullptr, *d2 = nullptr, *k1bar = nullptr, *k2bar = nullptr, *k1gbar = nullptr;
@@ -1702,11 +1782,11 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
   long total = P.total;
   if ((rc = alloc(e.get(), &e->A, total)) || (rc = alloc(e.get(), &e->B, total)) ||
       (rc = alloc(e.get(), &e->K1, total)) || (rc = alloc(e.get(), &e->contrib1, S * nc)) ||
+      (rc = alloc(e.get(), &e->ysc, nc)) ||
       (rc = alloc(e.get(), &e->sel, e->W)) || (rc = alloc(e.get(), &e->alive, e->W)) ||
       (rc = alloc(e.get(), &e->valid, e->W)))
     return rc;
   if (method == CELLS_METHOD_ROS2) {
-    if (e->max_n > 8192) return fail(CELLS_EINVAL, "a world over 8192 cells exceeds the solve's LDS vector");
     if ((rc = alloc(e.get(), &e->Y1, total)) || (rc = alloc(e.get(), &e->K2, total)) ||
         (rc = alloc(e.get(), &e->contrib2, S * nc)) || (rc = alloc(e.get(), &e->gm, S * nc)) ||
         (rc = alloc(e.get(), &e->gj, std::max<size_t>(S * ne, 1))) ||
@@ -1940,8 +2020,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_factor, dim3(nw), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool,
                        e->masks, e->foff, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
     mark("k_factor");
-    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, e->dphi);
+    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of,
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, e->dphi, e->ysc);
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->contrib1, e->Y1);
@@ -1957,8 +2037,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_ros2_stage2, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, c0, nc, ed0, ne, e->K1,
                        e->contrib1, e->K2, e->contrib2);
     mark("k_ros2_stage2");
-    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->dphi);
+    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of,
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->dphi, e->ysc);
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->K2, e->contrib2, e->valid);
@@ -2096,11 +2176,10 @@ extern "C" int cells_factor(cells_ensemble* e, void* stream, double* values) {
 
 extern "C" int cells_solve(cells_ensemble* e, void* stream, const double* values, const double* rhs, double* x) {
   if (!e || !values || !rhs || !x) return fail(CELLS_EINVAL, "NULL argument");
-  if (e->max_n > 8192) return fail(CELLS_EINVAL, "a world over 8192 cells exceeds the solve's LDS vector");
   HIP_TRY(hipSetDevice(e->device));
   size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
-  hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kFactorBlock), lds, static_cast<hipStream_t>(stream), e->shapes,
-                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x);
+  hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kFactorBlock), 0, static_cast<hipStream_t>(stream), e->shapes,
+                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x, e->ysc);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }
@@ -2162,8 +2241,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                      e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
   hipLaunchKernelGGL(k_factor, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool, e->masks,
                      e->foff, 0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
-  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K1, e->d1);
+  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K1, e->d1, e->ysc);
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
                      ne, dt, coef, e->gm, e->gj, e->d1, e->K1, e->contrib1, e->Y1);
   if (ne)
@@ -2173,8 +2252,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_ros2_stage2, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, 0, nc, 0, ne, e->K1,
                      e->contrib1, e->K2, e->contrib2);
-  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K2, e->d2);
+  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K2, e->d2, e->ysc);
   // reverse sweep
   hipLaunchKernelGGL(k_adj_seed, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, ybar_out, dt, ybar_in, e->k1bar,
                      e->k2bar, e->k1gbar, e->g2bar);
@@ -2183,8 +2262,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k2bar,
                      e->d2, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
-                     coef, e->msc, e->fvals, e->dbar, e->d2, e->wv, e->k2bar, e->gmbar, e->gjbar);
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+                     coef, e->msc, e->fvals, e->dbar, e->d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc);
   // r2 = f(Y1) - 2 k1
   hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
                      (long)ne, e->k1gbar, e->g2bar);
@@ -2197,8 +2276,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k1bar,
                      e->d1, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
-                     coef, e->msc, e->fvals, e->dbar, e->d1, e->wv, e->k1bar, e->gmbar, e->gjbar);
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+                     coef, e->msc, e->fvals, e->dbar, e->d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc);
   // f(y) and the slopes at y
   if (ne)
     hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, c, e->k1bar, e->k1gbar,
