@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--parity-worlds", type=int, default=3)
     ap.add_argument("--skip", default="", help="comma list of legs to skip: plain,parity,cpu")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: independent worlds shard across "
@@ -106,6 +108,16 @@ def main():
     all_ok = bool(ok.cpu().numpy().all())
     final = y.cpu().numpy()
     log(f"batched: {ms_per_step:.3f} ms/step, {cell_steps:.4g} cell-steps/s, all accepted {all_ok}")
+
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=y0.device)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written per launch")
+        del src, dst
 
     log("kernel times")
     times = e.kernel_times(yw, dt, reps=3)

@@ -690,6 +690,32 @@ __device__ void factor_world(const ShapeDev& s, const PlanView& v, double* fv) {
       PairSum pw;
       pw.reset();
       int x = v.rowptr[k], xe = v.rowptr[k + 1];
+      // four source columns at a time: their plan words first, then their factor values, so the
+      // loads of a group are in flight together; the terms still enter the sum in column order
+      for (; x + 4 <= xe; x += 4) {
+        unsigned long long m0[4], m1[4];
+        int base[4];
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          m0[u] = v.mask[2 * (long)(x + u)];
+          m1[u] = v.mask[2 * (long)(x + u) + 1];
+          base[u] = v.rowpos[x + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          unsigned long long mine = word ? m1[u] : m0[u];
+          int idx = __popcll(m0[u] & lo_mask) + __popcll(m1[u] & hi_mask);
+          bool has = (mine & bit) != 0ull;
+          a[u] = has ? fv[base[u] + idx] : 0.0;
+          b[u] = has ? fv[base[u]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          unsigned long long mine = word ? m1[u] : m0[u];
+          if (mine & bit) pw.push(a[u] * b[u]);
+        }
+      }
       for (; x < xe; ++x) {
         unsigned long long m0 = v.mask[2 * (long)x], m1 = v.mask[2 * (long)x + 1];
         unsigned long long mine = word ? m1 : m0;
