@@ -114,6 +114,7 @@ struct ShapeDev {
   int nnz;
   long o_cs, o_entry, o_rowptr, o_rowent, o_lev, o_levcol, o_levent, o_node;
   long o_mask;  // offset in the mask pool (unsigned long long words)
+  int lds_pt;   // first factor position held in LDS during the factor (nnz: none)
 };
 
 __device__ __forceinline__ double& CF(const Params& P, double* y, int f, int i) {
@@ -676,7 +677,20 @@ __device__ PlanView plan_view(const ShapeDev& s, const int* pool,
   return v;
 }
 
-__device__ void factor_world(const ShapeDev& s, const PlanView& v, double* fv) {
+// The factor's values: the trailing columns (positions >= pt, where most update terms are read)
+// live in LDS during the factor, the rest in HBM.
+struct FactorStore {
+  double* g;
+  double* l;
+  int pt;
+  __device__ __forceinline__ double get(int p) const { return p >= pt ? l[p - pt] : g[p]; }
+  __device__ __forceinline__ void set(int p, double x) {
+    if (p >= pt) l[p - pt] = x;
+    else g[p] = x;
+  }
+};
+
+__device__ void factor_world(const ShapeDev& s, const PlanView& v, FactorStore fv) {
   for (int lev = 0; lev < s.levels; ++lev) {
     int a = v.leventptr[lev], b = v.leventptr[lev + 1];
     for (int q = a + threadIdx.x; q < b; q += blockDim.x) {
@@ -707,8 +721,8 @@ __device__ void factor_world(const ShapeDev& s, const PlanView& v, double* fv) {
           unsigned long long mine = word ? m1[u] : m0[u];
           int idx = __popcll(m0[u] & lo_mask) + __popcll(m1[u] & hi_mask);
           bool has = (mine & bit) != 0ull;
-          a[u] = has ? fv[base[u] + idx] : 0.0;
-          b[u] = has ? fv[base[u]] : 0.0;
+          a[u] = has ? fv.get(base[u] + idx) : 0.0;
+          b[u] = has ? fv.get(base[u]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -722,17 +736,17 @@ __device__ void factor_world(const ShapeDev& s, const PlanView& v, double* fv) {
         if (mine & bit) {
           int base = v.rowpos[x];
           int idx = __popcll(m0 & lo_mask) + __popcll(m1 & hi_mask);
-          pw.push(fv[base + idx] * fv[base]);
+          pw.push(fv.get(base + idx) * fv.get(base));
         }
       }
-      double acc = fv[p] - pw.sum();
-      fv[p] = slot == 0 ? sqrt(acc) : acc;
+      double acc = fv.get(p) - pw.sum();
+      fv.set(p, slot == 0 ? sqrt(acc) : acc);
     }
     __syncthreads();
     for (int q = a + threadIdx.x; q < b; q += blockDim.x) {
       int p = v.levent[q];
       int k = v.crank[p];
-      if (p != v.cs[k]) fv[p] = fv[p] * (1.0 / fv[v.cs[k]]);
+      if (p != v.cs[k]) fv.set(p, fv.get(p) * (1.0 / fv.get(v.cs[k])));
     }
     __syncthreads();
   }
@@ -848,6 +862,8 @@ __device__ void solve_world(const ShapeDev& s, const PlanView& v, const double* 
 // Assemble (Jacobi-scaled, symmetrised; numerics/surface_imex.py:228-259) and factor
 // C + coef G for each world (workgroup per world, no LDS: the factor lives in HBM/L2).
 constexpr int kFactorBlock = 128;
+constexpr int64_t kFactorLdsDoubles = 4096;     // 32 KB of trailing factor values per world
+constexpr size_t kSolveLdsBytes = 64 * 1024;     // the solve's vector in LDS up to 8192 cells
 
 __global__ __launch_bounds__(kFactorBlock) void k_factor(Params P, const ShapeDev* shapes,
                                                          const int* shape_of, const int* pool,
@@ -856,6 +872,7 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor(Params P, const ShapeDe
                                                          const double* mdiag, const double* msc,
                                                          const double* Gj, double* fvals,
                                                          int* valid) {
+  extern __shared__ double lds[];
   int w = w0 + blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool, masks);
@@ -870,7 +887,11 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor(Params P, const ShapeDe
     fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
   }
   __syncthreads();
-  factor_world(s, v, fv);
+  for (int p = s.lds_pt + threadIdx.x; p < s.nnz; p += blockDim.x) lds[p - s.lds_pt] = fv[p];
+  __syncthreads();
+  factor_world(s, v, FactorStore{fv, lds, s.lds_pt});
+  for (int p = s.lds_pt + threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = lds[p - s.lds_pt];
+  __syncthreads();
   bool ok = true;
   for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) ok = ok && isfinite(fv[p]);
   if (!ok) valid[w] = 0;
@@ -882,16 +903,28 @@ __global__ __launch_bounds__(kFactorBlock) void k_solve_stage(Params P, const Sh
                                                               const long* foff, int w0,
                                                               const double* msc,
                                                               const double* fvals, const double* K,
-                                                              double* dphi, double* ysc) {
+                                                              const double* Kminus2, double* dphi,
+                                                              double* ysc, int y_in_lds) {
+  extern __shared__ double lds[];
   int w = w0 + blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   const double* fv = fvals + foff[w];
   int c0 = P.cell_off[w];
-  double* y = ysc + c0;
+  double* y = y_in_lds ? lds : ysc + c0;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
     int i = c0 + v.order[k];
-    y[k] = charge_increment(P, K, i) * msc[i];
+    double q;
+    if (Kminus2) {
+      // the charge of K - 2 K2' field by field (the stage-2 right-hand side F1 - 2 k1)
+      double t2 = P.z[0] * (CFc(P, K, P.f_dn, i) + -2.0 * CFc(P, Kminus2, P.f_dn, i));
+      for (int sp = 1; sp < P.S; ++sp)
+        t2 = t2 + P.z[sp] * (CFc(P, K, P.f_dn + sp, i) + -2.0 * CFc(P, Kminus2, P.f_dn + sp, i));
+      q = kChargePerFmol * t2;
+    } else {
+      q = charge_increment(P, K, i);
+    }
+    y[k] = q * msc[i];
   }
   __syncthreads();
   solve_world(s, v, fv, y);
@@ -909,18 +942,25 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_only(const ShapeDev* sh
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool, masks);
-  factor_world(s, v, fvals + foff[w]);
+  extern __shared__ double lds[];
+  double* fv = fvals + foff[w];
+  for (int p = s.lds_pt + threadIdx.x; p < s.nnz; p += blockDim.x) lds[p - s.lds_pt] = fv[p];
+  __syncthreads();
+  factor_world(s, v, FactorStore{fv, lds, s.lds_pt});
+  for (int p = s.lds_pt + threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = lds[p - s.lds_pt];
 }
 
 __global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* shapes, const int* shape_of,
                                                        const int* pool, const long* foff,
                                                        const int* cell_off, const double* fvals,
-                                                       const double* rhs, double* x, double* ysc) {
+                                                       const double* rhs, double* x, double* ysc,
+                                                       int y_in_lds) {
+  extern __shared__ double shared_y[];
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   int c0 = cell_off[w];
-  double* lds = ysc + c0;
+  double* lds = y_in_lds ? shared_y : ysc + c0;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) lds[k] = rhs[c0 + v.order[k]];
   __syncthreads();
   solve_world(s, v, fvals + foff[w], lds);
@@ -930,9 +970,11 @@ __global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* sha
 // ------------------------------------------------------------------------------------------
 // ROS2 stage combinations (the W-method's k = F + coef (df/dphi) dphi on the route fields).
 
-// Adds coef (df/dphi) dphi to K's transfers and amounts (and the cell's supply contribution).
-__device__ void correct_cell(const Params& P, int i, double coef, const double* gm, const double* gj,
-                             const double* dphi, double* K, double* contrib) {
+// The W-method's correction coef (df/dphi) dphi of one cell: dn_c[s] on the amounts, dm_c[s] on
+// the membrane transfers (the bath supply takes -dm_c).
+__device__ __forceinline__ void cell_correction(const Params& P, int i, double coef, const double* gm,
+                                                const double* gj, const double* dphi, double* dn_c,
+                                                double* dm_c) {
   int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
   for (int s = 0; s < P.S; ++s) {
     double dm = coef * (gm[(long)s * P.ncell + i] * dphi[i]);
@@ -948,9 +990,20 @@ __device__ void correct_cell(const Params& P, int i, double coef, const double* 
     double cell = -dm;
     cell += out;
     cell += in;
-    CF(P, K, P.f_dn + s, i) += cell;
-    CF(P, K, P.f_mtr + s, i) += dm;
-    contrib[(long)s * P.ncell + i] = contrib[(long)s * P.ncell + i] - dm;
+    dn_c[s] = cell;
+    dm_c[s] = dm;
+  }
+}
+
+// Adds coef (df/dphi) dphi to K's transfers and amounts (and the cell's supply contribution).
+__device__ void correct_cell(const Params& P, int i, double coef, const double* gm, const double* gj,
+                             const double* dphi, double* K, double* contrib) {
+  double dn_c[4], dm_c[4];
+  cell_correction(P, i, coef, gm, gj, dphi, dn_c, dm_c);
+  for (int s = 0; s < P.S; ++s) {
+    CF(P, K, P.f_dn + s, i) += dn_c[s];
+    CF(P, K, P.f_mtr + s, i) += dm_c[s];
+    contrib[(long)s * P.ncell + i] = contrib[(long)s * P.ncell + i] - dm_c[s];
   }
 }
 
@@ -960,7 +1013,8 @@ __device__ void correct_edge(const Params& P, int e, double coef, const double* 
   for (int s = 0; s < P.S; ++s) EF(P, K, P.f_jtr + s, e) += coef * (gj[(long)s * P.nedge + e] * d);
 }
 
-// Stage 1: K1 += correction(dphi1); Y1 = y + dt K1.
+// Stage 1: K1 += correction(dphi1); Y1 = y + dt K1 on the fields the rates read (amounts,
+// gates, and the water volume, which the stage carries from the water half-step).
 __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* A, const double* B,
                                                         const int* sel, int c0, int nc, int e0,
                                                         int ne, double dt, double coef,
@@ -972,47 +1026,30 @@ __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* 
     int i = c0 + t;
     int sw = sel[P.cell_world[i]];
     const double* y = current(A, B, sw);
-    const double* yv = P.water ? next(const_cast<double*>(A), const_cast<double*>(B), sw) : y;
     correct_cell(P, i, coef, gm, gj, dphi, K1, contrib1);
-    for (int f = 0; f < P.cf; ++f) {
-      if (is_water_field(P, f)) {
-        CF(P, Y1, f, i) = CFc(P, yv, f, i);
-        continue;
-      }
-      CF(P, Y1, f, i) = CFc(P, y, f, i) + dt * CFc(P, K1, f, i);
+    for (int s = 0; s < P.S; ++s)
+      CF(P, Y1, P.f_dn + s, i) = CFc(P, y, P.f_dn + s, i) + dt * CFc(P, K1, P.f_dn + s, i);
+    if (P.water) {
+      const double* yv = next(const_cast<double*>(A), const_cast<double*>(B), sw);
+      CF(P, Y1, P.f_dvol, i) = CFc(P, yv, P.f_dvol, i);
     }
   } else if (t < nc + ne) {
     int e = e0 + (t - nc);
     const double* y = current(A, B, sel[P.edge_world[e]]);
     correct_edge(P, e, coef, gj, dphi, K1);
-    for (int f = 0; f < P.ef; ++f) EF(P, Y1, f, e) = EFc(P, y, f, e) + dt * EFc(P, K1, f, e);
+    EF(P, Y1, P.f_gate, e) = EFc(P, y, P.f_gate, e) + dt * EFc(P, K1, P.f_gate, e);
   }
 }
 
-// K2 <- K2 - 2 K1 (the rates at Y1 less twice the first stage), contributions likewise.
-__global__ __launch_bounds__(kBlock) void k_ros2_stage2(Params P, int c0, int nc, int e0, int ne,
-                                                        const double* K1, const double* contrib1,
-                                                        double* K2, double* contrib2) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nc) {
-    int i = c0 + t;
-    for (int f = 0; f < P.cf; ++f) CF(P, K2, f, i) = CFc(P, K2, f, i) + -2.0 * CFc(P, K1, f, i);
-    for (int s = 0; s < P.S; ++s)
-      contrib2[(long)s * P.ncell + i] =
-          contrib2[(long)s * P.ncell + i] + -2.0 * contrib1[(long)s * P.ncell + i];
-  } else if (t < nc + ne) {
-    int e = e0 + (t - nc);
-    for (int f = 0; f < P.ef; ++f) EF(P, K2, f, e) = EFc(P, K2, f, e) + -2.0 * EFc(P, K1, f, e);
-  }
-}
-
-// Final: K2 += correction(dphi2); y_new = (y + 1.5 dt K1) + 0.5 dt K2; domain.
+// Final: k2 = (F1 - 2 k1) + correction(dphi2), held in registers; y_new = (y + 1.5 dt k1) +
+// 0.5 dt k2; the cell's bath-supply contribution of k2; domain. F1 is read from K2.
 __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, double* B,
                                                        const int* sel, int c0, int nc, int e0,
                                                        int ne, double dt, double coef,
                                                        const double* gm, const double* gj,
                                                        const double* dphi, const double* K1,
-                                                       double* K2, double* contrib2, int* valid) {
+                                                       const double* K2, const double* contrib1,
+                                                       double* contrib2, int* valid) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   double h1 = 1.5 * dt, h2 = 0.5 * dt;
   if (t < nc) {
@@ -1020,10 +1057,19 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
     int w = P.cell_world[i];
     const double* y = current(A, B, sel[w]);
     double* o = next(A, B, sel[w]);
-    correct_cell(P, i, coef, gm, gj, dphi, K2, contrib2);
+    double dn_c[4], dm_c[4];
+    cell_correction(P, i, coef, gm, gj, dphi, dn_c, dm_c);
     for (int f = 0; f < P.cf; ++f) {
       if (is_water_field(P, f)) continue;
-      CF(P, o, f, i) = (CFc(P, y, f, i) + h1 * CFc(P, K1, f, i)) + h2 * CFc(P, K2, f, i);
+      double k1 = CFc(P, K1, f, i);
+      double k2 = CFc(P, K2, f, i) + -2.0 * k1;
+      if (f >= P.f_dn && f < P.f_dn + P.S) k2 += dn_c[f - P.f_dn];
+      if (f >= P.f_mtr && f < P.f_mtr + P.S) k2 += dm_c[f - P.f_mtr];
+      CF(P, o, f, i) = (CFc(P, y, f, i) + h1 * k1) + h2 * k2;
+    }
+    for (int s = 0; s < P.S; ++s) {
+      long q = (long)s * P.ncell + i;
+      contrib2[q] = (contrib2[q] + -2.0 * contrib1[q]) - dm_c[s];
     }
     if (!cell_domain(P, o, o, i)) valid[w] = 0;
   } else if (t < nc + ne) {
@@ -1031,9 +1077,13 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
     int w = P.edge_world[e];
     const double* y = current(A, B, sel[w]);
     double* o = next(A, B, sel[w]);
-    correct_edge(P, e, coef, gj, dphi, K2);
-    for (int f = 0; f < P.ef; ++f)
-      EF(P, o, f, e) = (EFc(P, y, f, e) + h1 * EFc(P, K1, f, e)) + h2 * EFc(P, K2, f, e);
+    double d = dphi[P.snd[e]] - dphi[P.rcv[e]];
+    for (int f = 0; f < P.ef; ++f) {
+      double k1 = EFc(P, K1, f, e);
+      double k2 = EFc(P, K2, f, e) + -2.0 * k1;
+      if (f >= P.f_jtr && f < P.f_jtr + P.S) k2 += coef * (gj[(long)(f - P.f_jtr) * P.nedge + e] * d);
+      EF(P, o, f, e) = (EFc(P, y, f, e) + h1 * k1) + h2 * k2;
+    }
     if (!edge_domain(P, o, e)) valid[w] = 0;
   }
 }
@@ -1426,13 +1476,14 @@ __global__ __launch_bounds__(kFactorBlock) void k_adj_solve(Params P, const Shap
                                                       const double* msc, const double* fvals,
                                                       const double* dbar, const double* delta,
                                                       double* wv, double* rate_bar, double* gmbar,
-                                                      double* gjbar, double* ysc) {
+                                                      double* gjbar, double* ysc, int y_in_lds) {
+  extern __shared__ double shared_y[];
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   const double* fv = fvals + foff[w];
   int c0 = P.cell_off[w], e0 = P.edge_off[w];
-  double* lds = ysc + c0;
+  double* lds = y_in_lds ? shared_y : ysc + c0;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
     int i = c0 + v.order[k];
     lds[k] = dbar[i] * msc[i];
@@ -1520,6 +1571,8 @@ struct cells_ensemble {
   int* pool = nullptr;
   unsigned long long* masks = nullptr;
   long* foff = nullptr;
+  size_t factor_lds = 0, solve_lds = 0;  // dynamic LDS bytes of the factor and solve kernels
+  int y_in_lds = 1;
   double *A = nullptr, *B = nullptr, *Y1 = nullptr, *K1 = nullptr, *K2 = nullptr;
   double *contrib1 = nullptr, *contrib2 = nullptr, *gm = nullptr, *gj = nullptr, *Gj = nullptr;
   double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *fvals = nullptr, *ysc = nullptr;
@@ -1881,6 +1934,12 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       // slot masks: for each row-structure pair (column kk, source column m), the slots of
       // column kk that column m's entries at rows >= kk reach
       d.o_mask = (long)mask_pool.size();
+      {
+        // the trailing columns that fit the factor's LDS budget
+        int64_t budget = kFactorLdsDoubles, pt = s.nnz();
+        for (int kk = s.n - 1; kk >= 0 && s.nnz() - s.column_start[kk] <= budget; --kk) pt = s.column_start[kk];
+        d.lds_pt = (int)pt;
+      }
       slot_of.assign(s.n, -1);
       for (int kk = 0; kk < s.n; ++kk) {
         int64_t c0 = s.column_start[kk], c1 = s.column_start[kk + 1];
@@ -1938,6 +1997,13 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
     if (pool.size() > (size_t)0x7fffffff) return fail(CELLS_EINVAL, "plans exceed 2^31 entries");
     e->sd_host = sd;
     e->pool_host = pool;
+    {
+      int64_t most = 0;
+      for (const ShapeDev& d : sd) most = std::max<int64_t>(most, d.nnz - d.lds_pt);
+      e->factor_lds = sizeof(double) * (size_t)std::max<int64_t>(most, 1);
+      e->y_in_lds = (size_t)e->max_n * sizeof(double) <= kSolveLdsBytes ? 1 : 0;
+      e->solve_lds = e->y_in_lds ? sizeof(double) * (size_t)std::max(e->max_n, 1) : 0;
+    }
     e->foff_host.resize(e->W + 1);
     e->foff_host[0] = 0;
     for (int k = 0; k < e->W; ++k) e->foff_host[k + 1] = e->foff_host[k] + sd[e->shape_of_host[k]].nnz;
@@ -2043,11 +2109,11 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
                        c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
     mark("k_cells");
-    hipLaunchKernelGGL(k_factor, dim3(nw), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool,
+    hipLaunchKernelGGL(k_factor, dim3(nw), dim3(kFactorBlock), e->factor_lds, st, P, e->shapes, e->shape_of, e->pool,
                        e->masks, e->foff, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
     mark("k_factor");
-    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, e->dphi, e->ysc);
+    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of,
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds);
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->contrib1, e->Y1);
@@ -2060,14 +2126,11 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, c0, c1,
                        c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
     mark("k_cells");
-    hipLaunchKernelGGL(k_ros2_stage2, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, c0, nc, ed0, ne, e->K1,
-                       e->contrib1, e->K2, e->contrib2);
-    mark("k_ros2_stage2");
-    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->dphi, e->ysc);
+    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of,
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi, e->ysc, e->y_in_lds);
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
-                       ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->K2, e->contrib2, e->valid);
+                       ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->K2, e->contrib1, e->contrib2, e->valid);
     mark("k_ros2_final");
     hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, e->contrib1,
                        1.5 * dt, e->contrib2, 0.5 * dt, e->valid);
@@ -2194,7 +2257,7 @@ extern "C" int cells_rates(cells_ensemble* e, void* stream, const double* y,
 extern "C" int cells_factor(cells_ensemble* e, void* stream, double* values) {
   if (!e || !values) return fail(CELLS_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(e->device));
-  hipLaunchKernelGGL(k_factor_only, dim3(e->W), dim3(kFactorBlock), 0, static_cast<hipStream_t>(stream),
+  hipLaunchKernelGGL(k_factor_only, dim3(e->W), dim3(kFactorBlock), e->factor_lds, static_cast<hipStream_t>(stream),
                      e->shapes, e->shape_of, e->pool, e->masks, e->foff, values);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
@@ -2204,8 +2267,8 @@ extern "C" int cells_solve(cells_ensemble* e, void* stream, const double* values
   if (!e || !values || !rhs || !x) return fail(CELLS_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(e->device));
   size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
-  hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kFactorBlock), 0, static_cast<hipStream_t>(stream), e->shapes,
-                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x, e->ysc);
+  hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kFactorBlock), e->solve_lds, static_cast<hipStream_t>(stream), e->shapes,
+                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x, e->ysc, e->y_in_lds);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }
@@ -2265,10 +2328,10 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                        c, e->K1, e->gj, e->Gj, e->valid);
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
                      e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
-  hipLaunchKernelGGL(k_factor, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool, e->masks,
+  hipLaunchKernelGGL(k_factor, dim3(W), dim3(kFactorBlock), e->factor_lds, st, P, e->shapes, e->shape_of, e->pool, e->masks,
                      e->foff, 0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
-  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K1, e->d1, e->ysc);
+  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds);
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
                      ne, dt, coef, e->gm, e->gj, e->d1, e->K1, e->contrib1, e->Y1);
   if (ne)
@@ -2276,10 +2339,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                        e->K2, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
-  hipLaunchKernelGGL(k_ros2_stage2, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, 0, nc, 0, ne, e->K1,
-                     e->contrib1, e->K2, e->contrib2);
-  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K2, e->d2, e->ysc);
+  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds);
   // reverse sweep
   hipLaunchKernelGGL(k_adj_seed, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, ybar_out, dt, ybar_in, e->k1bar,
                      e->k2bar, e->k1gbar, e->g2bar);
@@ -2288,8 +2349,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k2bar,
                      e->d2, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool, e->foff,
-                     coef, e->msc, e->fvals, e->dbar, e->d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc);
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+                     coef, e->msc, e->fvals, e->dbar, e->d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc, e->y_in_lds);
   // r2 = f(Y1) - 2 k1
   hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
                      (long)ne, e->k1gbar, e->g2bar);
@@ -2302,8 +2363,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k1bar,
                      e->d1, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), 0, st, P, e->shapes, e->shape_of, e->pool, e->foff,
-                     coef, e->msc, e->fvals, e->dbar, e->d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc);
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
+                     coef, e->msc, e->fvals, e->dbar, e->d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc, e->y_in_lds);
   // f(y) and the slopes at y
   if (ne)
     hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, c, e->k1bar, e->k1gbar,
