@@ -194,3 +194,23 @@ def test_logaddexp_is_numpys():
     for a, b in ((0.0, 0.0), (0.0, 3.0), (0.0, -40.0), (0.0, 800.0), (2.5, -1.0)):
         got = L().ref_logaddexp(ctypes.c_double(a), ctypes.c_double(b))
         assert math.isclose(got, float(np.logaddexp(a, b)), rel_tol=1e-15)
+
+
+def test_the_oracle_meets_the_committed_decimal_vectors():
+    """tests/golden/decimal_laws.json (written by tests/golden/make_decimal_vectors.py)."""
+    import json
+    import os
+
+    data = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "decimal_laws.json")))
+    lib = L()
+    for g in data["ghk"]:
+        got = lib.ref_selective_flux(*(ctypes.c_double(g[k]) for k in ("c_from", "c_to", "dv", "z")),
+                                     ctypes.c_double(data["temperature"]), ctypes.c_double(g["pa"]),
+                                     ctypes.c_double(1.0))
+        scale = g["pa"] / 1e-15 * (g["c_from"] + g["c_to"]) * (1 + abs(g["z"] * F * g["dv"] / (R * T)))
+        assert abs(got - g["flux"]) <= 2e-14 * scale, g
+    for g in data["gate"]:
+        minimum, gate = 0.1, 0.4
+        want = g["alpha"] * (1 - gate) - g["beta"] * (gate - minimum)
+        got = lib.ref_gate_rhs(*(ctypes.c_double(x) for x in (gate, g["v"], g["threshold"], minimum)))
+        assert abs(got - want) <= 1e-14 * (g["alpha"] + g["beta"]), g
