@@ -57,6 +57,61 @@ def cpu_baseline(p, seconds, dt):
     }
 
 
+def adjoint_leg(steps, snapshots):
+    """Calibration gradient on the adjoint configuration's worlds (256 worlds of 32x32 cells,
+    Na/K/Cl, ROS2 dt = 5 ms, 6 log-parameters per cell, Vmem targets every 20 steps): one loss
+    (forward sweep) and one loss-and-gradient with binomial checkpointing, timed."""
+    import torch
+
+    from cells_amd.adjoint import Calibration
+    from cells_amd.ensemble import Ensemble
+    from cells_amd.world import F, StateLayout, build, config1, initial_state
+
+    def vmem(p, y):
+        L = StateLayout.of(p)
+        q = p.q0 + 1e-15 * F * sum(p.valence[s] * L.cell(y, L.dn + s) for s in range(p.S))
+        return p.config.bath_V + q / p.cap
+
+    dt, every = 5e-3, 20
+    p = build(config1(worlds=256, cells=(1024, 1024), species=("Na", "K", "Cl"),
+                      inside_mM=(12.0, 140.0, 10.0), bath_mM=(145.0, 5.0, 110.0),
+                      permeability=(1e-10, 5e-9, 1e-9), seed=3))
+    e = Ensemble(p, "ros2")
+    y0 = initial_state(p)
+    rng = np.random.default_rng(7)
+    hidden = rng.normal(scale=0.1, size=(6, p.ncell))
+    samples = steps // every
+    cal = Calibration(e, y0, dt, steps, every, [np.zeros(p.ncell)] * samples)
+    cal.set_theta(hidden)
+    y = e.to_device(y0)
+    targets = []
+    for _ in range(samples):
+        y, _ = e.step(y, dt, nsteps=every)
+        targets.append(vmem(p, y.cpu().numpy()))
+    cal = Calibration(e, y0, dt, steps, every, targets)
+    theta = np.zeros((6, p.ncell))
+    log(f"adjoint leg: {p.W} worlds x {p.ncell // p.W} cells, {steps} steps, {snapshots} snapshots")
+    cal.loss(theta)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = cal.loss(theta)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    cal.forward_steps = cal.reverse_steps = 0
+    t0 = time.perf_counter()
+    loss2, grad = cal.loss_and_gradient(theta, snapshots=snapshots)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    log(f"adjoint leg: forward {t_fwd:.3f} s, loss+gradient {t_grad:.3f} s "
+        f"({cal.forward_steps} forward + {cal.reverse_steps} reverse steps)")
+    e.close()
+    return {"worlds": p.W, "cells_per_world": p.ncell // p.W, "steps": steps,
+            "snapshots": snapshots, "grads_per_s": 1.0 / t_grad, "forward_s": t_fwd,
+            "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
+            "forward_steps": cal.forward_steps, "reverse_steps": cal.reverse_steps,
+            "loss": loss, "grad_norm": float(np.linalg.norm(grad))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,7 +122,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--parity-worlds", type=int, default=3)
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: plain,parity,cpu")
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: plain,parity,cpu,adjoint")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="horizon of the adjoint leg (the calibration configuration runs 2000)")
+    ap.add_argument("--adjoint-snapshots", type=int, default=45)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -176,6 +234,10 @@ def main():
         parity = max(relerr(vmem(sub, got), vmem(sub, ref)), relerr(conc(sub, got), conc(sub, ref)))
         log(f"parity max relative error {parity:.3e}")
 
+    adj = None
+    if "adjoint" not in skip:
+        adj = adjoint_leg(args.adjoint_steps, args.adjoint_snapshots)
+
     cpu = None
     if "cpu" not in skip:
         log("cpu baseline")
@@ -208,6 +270,7 @@ def main():
         "kernel_ms": {kk: round(v, 4) for kk, v in per_kernel.items()},
         "plans": stats,
         "roofline": roofline,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

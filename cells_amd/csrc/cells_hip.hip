@@ -862,8 +862,8 @@ __device__ void solve_world(const ShapeDev& s, const PlanView& v, const double* 
 // Assemble (Jacobi-scaled, symmetrised; numerics/surface_imex.py:228-259) and factor
 // C + coef G for each world (workgroup per world, no LDS: the factor lives in HBM/L2).
 constexpr int kFactorBlock = 128;
-constexpr int64_t kFactorLdsDoubles = 4096;     // 32 KB of trailing factor values per world
-constexpr size_t kSolveLdsBytes = 64 * 1024;     // the solve's vector in LDS up to 8192 cells
+constexpr int64_t kFactorLdsDoubles = 0;  // trailing factor values held in LDS (0: none; occupancy wins)
+constexpr size_t kSolveLdsBytes = 0;  // the solve vector stays in HBM scratch: occupancy hides latency better
 
 __global__ __launch_bounds__(kFactorBlock) void k_factor(Params P, const ShapeDev* shapes,
                                                          const int* shape_of, const int* pool,
