@@ -57,7 +57,7 @@ def cpu_baseline(p, seconds, dt):
     }
 
 
-def adjoint_leg(steps, snapshots):
+def adjoint_leg(steps, snapshots, tape):
     """Calibration gradient on the adjoint configuration's worlds (256 worlds of 32x32 cells,
     Na/K/Cl, ROS2 dt = 5 ms, 6 log-parameters per cell, Vmem targets every 20 steps): one loss
     (forward sweep) and one loss-and-gradient with binomial checkpointing, timed."""
@@ -90,26 +90,32 @@ def adjoint_leg(steps, snapshots):
         targets.append(vmem(p, y.cpu().numpy()))
     cal = Calibration(e, y0, dt, steps, every, targets)
     theta = np.zeros((6, p.ncell))
-    log(f"adjoint leg: {p.W} worlds x {p.ncell // p.W} cells, {steps} steps, {snapshots} snapshots")
-    cal.loss(theta)  # warm
+    tape_bytes = e.reserve_tape(tape)
+    log(f"adjoint leg: {p.W} worlds x {p.ncell // p.W} cells, {steps} steps, {snapshots} snapshots, "
+        f"tape of {tape} steps ({tape_bytes / 2**30:.2f} GiB)")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    loss = cal.loss(theta)
+    loss = cal.loss(theta)  # the step graph was captured while making the targets
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
-    cal.forward_steps = cal.reverse_steps = 0
+    cal.forward_steps = cal.recorded_steps = cal.reverse_steps = 0
     t0 = time.perf_counter()
-    loss2, grad = cal.loss_and_gradient(theta, snapshots=snapshots)
+    loss2, grad = cal.loss_and_gradient(theta, snapshots=snapshots, tape=tape)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
     log(f"adjoint leg: forward {t_fwd:.3f} s, loss+gradient {t_grad:.3f} s "
-        f"({cal.forward_steps} forward + {cal.reverse_steps} reverse steps)")
+        f"({cal.forward_steps} forward + {cal.recorded_steps} recorded + {cal.reverse_steps} reverse "
+        f"steps)")
     e.close()
     return {"worlds": p.W, "cells_per_world": p.ncell // p.W, "steps": steps,
-            "snapshots": snapshots, "grads_per_s": 1.0 / t_grad, "forward_s": t_fwd,
+            "snapshots": snapshots, "tape_steps": tape, "tape_bytes": tape_bytes,
+            "grads_per_s": 1.0 / t_grad, "forward_s": t_fwd,
             "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
-            "forward_steps": cal.forward_steps, "reverse_steps": cal.reverse_steps,
-            "loss": loss, "grad_norm": float(np.linalg.norm(grad))}
+            "forward_steps": cal.forward_steps, "recorded_steps": cal.recorded_steps,
+            "reverse_steps": cal.reverse_steps, "loss": loss, "loss_from_gradient_sweep": loss2,
+            "snapshot_bytes": snapshots * 8 * int(y.numel()), "device_used_bytes": int(total - free),
+            "grad_norm": float(np.linalg.norm(grad))}
 
 
 def main():
@@ -123,9 +129,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--parity-worlds", type=int, default=3)
     ap.add_argument("--skip", default="", help="comma list of legs to skip: plain,parity,cpu,adjoint")
-    ap.add_argument("--adjoint-steps", type=int, default=200,
-                    help="horizon of the adjoint leg (the calibration configuration runs 2000)")
+    ap.add_argument("--adjoint-steps", type=int, default=2000, help="horizon of the adjoint leg")
     ap.add_argument("--adjoint-snapshots", type=int, default=45)
+    ap.add_argument("--adjoint-tape", type=int, default=45,
+                    help="recorded steps per unit of the adjoint's binomial schedule (0: none)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -236,7 +243,7 @@ def main():
 
     adj = None
     if "adjoint" not in skip:
-        adj = adjoint_leg(args.adjoint_steps, args.adjoint_snapshots)
+        adj = adjoint_leg(args.adjoint_steps, args.adjoint_snapshots, args.adjoint_tape)
 
     cpu = None
     if "cpu" not in skip:

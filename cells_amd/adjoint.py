@@ -13,13 +13,17 @@ The "leak scale" multiplies every membrane permeability of a cell (a redundant d
 three log permeabilities, declared so). The loss is the mean squared Vmem misfit over sample
 rows: L = 1/(K n) sum_k sum_i (V_i(t_k) - target_k,i)^2.
 
-Checkpointing: Griewank's binomial schedule (Griewank and Walther 2000, ACM TOMS 26:19, "revolve"):
-with c free snapshot slots and l steps to reverse, the first split keeps the right part within
-beta(c-1, r) = C(c-1+r, r) steps, r the least repetition count with beta(c, r) >= l. Snapshots
-are device copies of the state. With c >= steps every state is kept and nothing is recomputed.
+Checkpointing: Griewank's binomial schedule (Griewank and Walther 2000, ACM TOMS 26:19, "revolve")
+over units of `tape` steps: with c free snapshot slots and l units to reverse, the first split keeps
+the right part within beta(c-1, r) = C(c-1+r, r) units, r the least repetition count with
+beta(c, r) >= l. Snapshots are device copies of the state. A unit is reversed by stepping it
+forward once more while recording every step (cells_step_record: the factor, slopes, stage
+increments and stage state of each step in a device tape slot), then running the reverse steps
+from the records, last to first, with no forward recomputation inside them. With tape = 0 a unit
+is one step and each reverse step recomputes its forward internals (cells_step_vjp). The loss is
+accumulated on the sweep that first reaches each sample row, so the gradient needs no separate
+loss pass.
 
-Every step, reverse step, loss and parameter update runs in libcells_hip.so; this module only
-orders the calls and keeps the snapshots (device tensors).
 """
 
 from __future__ import annotations
@@ -89,6 +93,7 @@ class Calibration:
         self.ctl = ctl or control()
         self.base = (self.p.pam.copy(), self.p.nak_cap.copy(), self.p.paj.copy())
         self.forward_steps = 0
+        self.recorded_steps = 0
         self.reverse_steps = 0
 
     def set_theta(self, theta):
@@ -117,11 +122,14 @@ class Calibration:
             self.e.vmem_loss(y, self.targets[k], self.weight, acc)
         return float(acc.sum().cpu())
 
-    def loss_and_gradient(self, theta, snapshots=45):
-        """(loss, theta_bar [6, n]) with the binomial schedule over `snapshots` free slots."""
+    def loss_and_gradient(self, theta, snapshots=45, tape=None):
+        """(loss, theta_bar [6, n]) with the binomial schedule over `snapshots` free slots, in
+        units of `tape` recorded steps (None: the ensemble's reserved tape; 0: no tape)."""
         t = self.t
         params = self.set_theta(theta)
-        loss = self.loss(theta)
+        T = self.e.tape_slots if tape is None else int(tape)
+        if T > self.e.tape_slots:
+            raise ValueError(f"tape of {T} steps exceeds the {self.e.tape_slots} reserved slots")
         W = self.p.W
         S, n, E = self.p.S, self.p.ncell, max(self.p.nedge, 1)
         dev, f64 = self.e.device, t.float64
@@ -130,17 +138,78 @@ class Calibration:
         self.paj_bar = t.zeros((S, E), dtype=f64, device=dev)
         self.ybar = t.zeros_like(self.y0)
         self.scratch = t.zeros(W, dtype=f64, device=dev)
+        self.acc = t.zeros(W, dtype=f64, device=dev)
+        self.frontier = 0
+        self.unit = max(T, 1)
+        self.taped = T > 0
+        units = -(-self.steps // self.unit)
         store = {0: self.y0}
-        self._reverse(store, 0, self.steps, snapshots)
+        self._reverse(store, 0, units, snapshots)
         bars = (self.pam_bar.cpu().numpy(), self.cap_bar.cpu().numpy(), self.paj_bar.cpu().numpy())
-        return loss, theta_cotangent(self.p, params, bars)
+        return float(self.acc.sum().cpu()), theta_cotangent(self.p, params, bars)
+
+    def _pos(self, u):
+        return min(u * self.unit, self.steps)
+
+    def _advance_to(self, y, a, b):
+        """The state at step b from the state y at step a; adds the loss of every sample row in
+        (frontier, b] on the way."""
+        pos = a
+        for k in self.samples:
+            if max(a, self.frontier) < k <= b:
+                y = self._advance(y, k - pos)
+                pos = k
+                self.e.vmem_loss(y, self.targets[k], self.weight, self.acc)
+        y = self._advance(y, b - pos)
+        self.frontier = max(self.frontier, b)
+        return y
+
+    def _reverse_unit(self, y_a, u):
+        """ybar <- reverse of the steps of unit u, from the state y_a at its start."""
+        a, b = self._pos(u), self._pos(u + 1)
+        if not self.taped:
+            self._adjoint_step(y_a, a)
+            return
+        at = {}
+        y = y_a
+        for j in range(b - a):
+            y = self._record_step(y, j)
+            k = a + j + 1
+            if k in self.targets:
+                at[k] = y
+                if k > self.frontier:
+                    self.e.vmem_loss(y, self.targets[k], self.weight, self.acc)
+        self.frontier = max(self.frontier, b)
+        for j in range(b - a - 1, -1, -1):
+            k = a + j + 1
+            if k in at:
+                self.e.vmem_loss(at[k], self.targets[k], self.weight, self.scratch, self.ybar)
+            self._reverse_recorded(j, a + j)
+
+    def _record_step(self, y, slot):
+        y, ok = self.e.step_record(y, self.dt, slot, ctl=self.ctl)
+        if not bool(ok.cpu().numpy().all()):
+            raise FloatingPointError("a world refused a step of the calibration run")
+        self.recorded_steps += 1
+        return y
+
+    def _reverse_recorded(self, slot, a):
+        """ybar <- reverse of step a -> a+1 from its record in tape `slot`."""
+        self.ybar = self.e.step_vjp_recorded(slot, self.ybar, self.dt, ctl=self.ctl,
+                                             pam_bar=self.pam_bar, cap_bar=self.cap_bar,
+                                             paj_bar=self.paj_bar)[0]
+        self.reverse_steps += 1
 
     def _adjoint_step(self, y_a, a):
-        """ybar <- reverse of step a -> a+1 at y_a, after adding the loss cotangent at a+1."""
+        """ybar <- reverse of step a -> a+1 at y_a (forward internals recomputed), after adding
+        the loss cotangent at a+1."""
         b = a + 1
         if b in self.targets:
             y_b = self._advance(y_a, 1)
+            if b > self.frontier:
+                self.e.vmem_loss(y_b, self.targets[b], self.weight, self.acc)
             self.e.vmem_loss(y_b, self.targets[b], self.weight, self.scratch, self.ybar)
+        self.frontier = max(self.frontier, b)
         yin, _, _, _, valid = self.e.step_vjp(y_a, self.ybar, self.dt, ctl=self.ctl,
                                              pam_bar=self.pam_bar, cap_bar=self.cap_bar,
                                              paj_bar=self.paj_bar)
@@ -150,17 +219,18 @@ class Calibration:
         self.reverse_steps += 1
 
     def _reverse(self, store, a, b, free):
+        """Reverse units [a, b) from store[pos(a)] with `free` snapshot slots."""
         length = b - a
         if length == 1:
-            self._adjoint_step(store[a], a)
+            self._reverse_unit(store[self._pos(a)], a)
             return
         if free <= 0:
             for j in range(b - 1, a - 1, -1):
-                self._adjoint_step(self._advance(store[a], j - a), j)
+                self._reverse_unit(self._advance_to(store[self._pos(a)], self._pos(a), self._pos(j)), j)
             return
         right = split(length, free)
         m = b - right
-        store[m] = self._advance(store[a], m - a)
+        store[self._pos(m)] = self._advance_to(store[self._pos(a)], self._pos(a), self._pos(m))
         self._reverse(store, m, b, free - 1)
-        del store[m]
+        del store[self._pos(m)]
         self._reverse(store, a, m, free)

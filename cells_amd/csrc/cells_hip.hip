@@ -1575,8 +1575,16 @@ struct cells_ensemble {
   int y_in_lds = 1;
   double *A = nullptr, *B = nullptr, *Y1 = nullptr, *K1 = nullptr, *K2 = nullptr;
   double *contrib1 = nullptr, *contrib2 = nullptr, *gm = nullptr, *gj = nullptr, *Gj = nullptr;
-  double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *fvals = nullptr, *ysc = nullptr;
+  double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *dphi2 = nullptr, *fvals = nullptr;
+  double* ysc = nullptr;
   int *sel = nullptr, *alive = nullptr, *valid = nullptr;
+  // step records for the adjoint (cells_tape_reserve): per slot the step's input state, stage-1
+  // state, factor values, Jacobi scales, route slopes and both stage increments, field-major
+  int tape_slots = 0;
+  std::vector<void*> tape_allocations;
+  double *t_y = nullptr, *t_y1 = nullptr, *t_fv = nullptr, *t_msc = nullptr, *t_gm = nullptr;
+  double *t_gj = nullptr, *t_d1 = nullptr, *t_d2 = nullptr;
+  int* t_valid = nullptr;
   // adjoint workspace (allocated at the first cells_step_vjp)
   double *d1 = nullptr, *d2 = nullptr, *k1bar = nullptr, *k2bar = nullptr, *k1gbar = nullptr;
   double *g2bar = nullptr, *gmbar = nullptr, *gjbar = nullptr, *dbar = nullptr, *wv = nullptr;
@@ -1589,6 +1597,7 @@ struct cells_ensemble {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
     for (void* p : allocations) (void)hipFree(p);
+    for (void* p : tape_allocations) (void)hipFree(p);
   }
 };
 
@@ -1870,7 +1879,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
         (rc = alloc(e.get(), &e->contrib2, S * nc)) || (rc = alloc(e.get(), &e->gm, S * nc)) ||
         (rc = alloc(e.get(), &e->gj, std::max<size_t>(S * ne, 1))) ||
         (rc = alloc(e.get(), &e->Gj, std::max<size_t>(ne, 1))) || (rc = alloc(e.get(), &e->mdiag, nc)) ||
-        (rc = alloc(e.get(), &e->msc, nc)) || (rc = alloc(e.get(), &e->dphi, nc)))
+        (rc = alloc(e.get(), &e->msc, nc)) || (rc = alloc(e.get(), &e->dphi, nc)) ||
+        (rc = alloc(e.get(), &e->dphi2, nc)))
       return rc;
   }
   // symbolic analysis once per lattice shape (worlds with the same cell count share it)
@@ -2127,10 +2137,10 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
                        c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
     mark("k_cells");
     hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi, e->ysc, e->y_in_lds);
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds);
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
-                       ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->K2, e->contrib1, e->contrib2, e->valid);
+                       ed0, ne, dt, coef, e->gm, e->gj, e->dphi2, e->K1, e->K2, e->contrib1, e->contrib2, e->valid);
     mark("k_ros2_final");
     hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, e->contrib1,
                        1.5 * dt, e->contrib2, 0.5 * dt, e->valid);
@@ -2300,6 +2310,86 @@ int adjoint_workspace(cells_ensemble* e) {
   return CELLS_OK;
 }
 
+// What the reverse sweep of one step reads: the step's input state, its stage-1 state, the factor
+// of its stage matrix with the Jacobi scales, the route slopes at the input and both stage
+// increments. Either the ensemble's own buffers after a recomputation or a tape slot.
+struct StepRecord {
+  const double *y, *y1, *fvals, *msc, *gm, *gj, *d1, *d2;
+};
+
+int check_adjoint(const cells_ensemble* e, double dt) {
+  if (e->method != CELLS_METHOD_ROS2 || e->P.water || e->P.calcium || e->S != 3)
+    return fail(CELLS_EINVAL, "the adjoint covers ROS2 on 3-ion worlds without water");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(CELLS_EINVAL, "dt must be positive and finite");
+  return CELLS_OK;
+}
+
+// ybar_in (+)= the pullback of ybar_out through the step recorded in r; parameter cotangents
+// accumulate into pam_bar, cap_bar, paj_bar.
+int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const double* ybar_out,
+                  double dt, const Ctl& c, double* ybar_in, double* pam_bar, double* cap_bar,
+                  double* paj_bar) {
+  const Params& P = e->P;
+  const double coef = (1.0 + 1.0 / std::sqrt(2.0)) * dt;
+  const int nc = e->ncell, ne = e->nedge, W = e->W;
+  long S = e->S;
+  double* d2 = const_cast<double*>(r.d2);
+  double* d1 = const_cast<double*>(r.d1);
+  hipLaunchKernelGGL(k_adj_seed, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, ybar_out, dt, ybar_in, e->k1bar,
+                     e->k2bar, e->k1gbar, e->g2bar);
+  hipLaunchKernelGGL(k_fill, dim3(blocks(S * nc)), dim3(kBlock), 0, st, S * nc, e->gmbar, 0.0);
+  if (ne) hipLaunchKernelGGL(k_fill, dim3(blocks(S * ne)), dim3(kBlock), 0, st, S * ne, e->gjbar, 0.0);
+  // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
+  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k2bar,
+                     r.d2, e->dbar, e->gmbar, e->gjbar);
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, coef, r.msc, r.fvals, e->dbar, d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc,
+                     e->y_in_lds);
+  // r2 = f(Y1) - 2 k1
+  hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
+                     (long)ne, e->k1gbar, e->g2bar);
+  double* gate_in = ybar_in + P.edge_base + (long)P.f_gate * P.nedge;
+  if (ne)
+    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, r.y1, c, e->k2bar, e->g2bar,
+                       nullptr, e->ca_bar, e->cb_bar, e->ev_bar, gate_in, e->k1gbar, dt, paj_bar);
+  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, r.y1, c, e->k2bar, nullptr,
+                     e->ca_bar, e->cb_bar, e->ev_bar, ybar_in, e->k1bar, dt, pam_bar, cap_bar);
+  // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
+  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k1bar,
+                     r.d1, e->dbar, e->gmbar, e->gjbar);
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, coef, r.msc, r.fvals, e->dbar, d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc,
+                     e->y_in_lds);
+  // f(y) and the slopes at y
+  if (ne)
+    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, r.y, c, e->k1bar, e->k1gbar,
+                       e->gjbar, e->ca_bar, e->cb_bar, e->ev_bar, gate_in, nullptr, 0.0, paj_bar);
+  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, r.y, c, e->k1bar, e->gmbar,
+                     e->ca_bar, e->cb_bar, e->ev_bar, ybar_in, nullptr, 0.0, pam_bar, cap_bar);
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+void free_tape(cells_ensemble* e) {
+  for (void* p : e->tape_allocations) (void)hipFree(p);
+  e->tape_allocations.clear();
+  e->tape_slots = 0;
+  e->t_y = e->t_y1 = e->t_fv = e->t_msc = e->t_gm = e->t_gj = e->t_d1 = e->t_d2 = nullptr;
+  e->t_valid = nullptr;
+}
+
+template <typename T>
+int tape_alloc(cells_ensemble* e, T** dst, size_t count) {
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(CELLS_ENOMEM, "hipMalloc of the step tape failed");
+  }
+  e->tape_allocations.push_back(p);
+  *dst = static_cast<T*>(p);
+  return CELLS_OK;
+}
+
 }  // namespace
 
 extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const double* ybar_out,
@@ -2307,19 +2397,15 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                               double* pam_bar, double* cap_bar, double* paj_bar, int32_t* valid_out) {
   if (!e || !y || !ybar_out || !control || !ybar_in || !pam_bar || !cap_bar || !paj_bar)
     return fail(CELLS_EINVAL, "NULL argument");
-  if (e->method != CELLS_METHOD_ROS2 || e->P.water || e->P.calcium || e->S != 3)
-    return fail(CELLS_EINVAL, "the adjoint covers ROS2 on 3-ion worlds without water");
-  if (!(dt > 0) || !std::isfinite(dt)) return fail(CELLS_EINVAL, "dt must be positive and finite");
-  HIP_TRY(hipSetDevice(e->device));
-  int rc = adjoint_workspace(e);
+  int rc = check_adjoint(e, dt);
   if (rc) return rc;
+  HIP_TRY(hipSetDevice(e->device));
+  if ((rc = adjoint_workspace(e))) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
   const Params& P = e->P;
   const double coef = (1.0 + 1.0 / std::sqrt(2.0)) * dt;
   const int nc = e->ncell, ne = e->nedge, W = e->W;
-  size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
-  long S = e->S;
   HIP_TRY(hipMemcpyAsync(e->A, y, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_init, dim3(blocks(W)), dim3(kBlock), 0, st, W, nullptr, e->sel, e->alive, e->valid);
   // forward recomputation of the step's internals (the forward kernels, in the same order)
@@ -2341,38 +2427,85 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds);
-  // reverse sweep
-  hipLaunchKernelGGL(k_adj_seed, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, ybar_out, dt, ybar_in, e->k1bar,
-                     e->k2bar, e->k1gbar, e->g2bar);
-  hipLaunchKernelGGL(k_fill, dim3(blocks(S * nc)), dim3(kBlock), 0, st, S * nc, e->gmbar, 0.0);
-  if (ne) hipLaunchKernelGGL(k_fill, dim3(blocks(S * ne)), dim3(kBlock), 0, st, S * ne, e->gjbar, 0.0);
-  // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
-  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k2bar,
-                     e->d2, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
-                     coef, e->msc, e->fvals, e->dbar, e->d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc, e->y_in_lds);
-  // r2 = f(Y1) - 2 k1
-  hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
-                     (long)ne, e->k1gbar, e->g2bar);
-  double* gate_in = ybar_in + P.edge_base + (long)P.f_gate * P.nedge;
-  if (ne)
-    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->Y1, c, e->k2bar, e->g2bar,
-                       nullptr, e->ca_bar, e->cb_bar, e->ev_bar, gate_in, e->k1gbar, dt, paj_bar);
-  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->Y1, c, e->k2bar, nullptr,
-                     e->ca_bar, e->cb_bar, e->ev_bar, ybar_in, e->k1bar, dt, pam_bar, cap_bar);
-  // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
-  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, e->gm, e->gj, e->k1bar,
-                     e->d1, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool, e->foff,
-                     coef, e->msc, e->fvals, e->dbar, e->d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc, e->y_in_lds);
-  // f(y) and the slopes at y
-  if (ne)
-    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, c, e->k1bar, e->k1gbar,
-                       e->gjbar, e->ca_bar, e->cb_bar, e->ev_bar, gate_in, nullptr, 0.0, paj_bar);
-  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, c, e->k1bar, e->gmbar,
-                     e->ca_bar, e->cb_bar, e->ev_bar, ybar_in, nullptr, 0.0, pam_bar, cap_bar);
+  StepRecord r{e->A, e->Y1, e->fvals, e->msc, e->gm, e->gj, e->d1, e->d2};
+  if ((rc = reverse_sweep(e, st, r, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;
   if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * W, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_tape_reserve(cells_ensemble* e, int32_t slots, int64_t* bytes_out) {
+  if (!e) return fail(CELLS_EINVAL, "NULL argument");
+  if (slots < 0) return fail(CELLS_EINVAL, "negative slot count");
+  if (slots > 0 && (e->method != CELLS_METHOD_ROS2 || e->P.water || e->P.calcium || e->S != 3))
+    return fail(CELLS_EINVAL, "the adjoint covers ROS2 on 3-ion worlds without water");
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipDeviceSynchronize());
+  free_tape(e);
+  const Params& P = e->P;
+  size_t n = slots, nc = e->ncell, ne = std::max(e->nedge, 1), S = e->S;
+  size_t bytes = n * sizeof(double) * (2 * (size_t)P.total + (size_t)e->factor_total + nc * (3 + S) + S * ne) +
+                 n * sizeof(int) * e->W;
+  if (bytes_out) *bytes_out = (int64_t)bytes;
+  if (slots == 0) return CELLS_OK;
+  int rc;
+  if ((rc = tape_alloc(e, &e->t_y, n * P.total)) || (rc = tape_alloc(e, &e->t_y1, n * P.total)) ||
+      (rc = tape_alloc(e, &e->t_fv, n * e->factor_total)) || (rc = tape_alloc(e, &e->t_msc, n * nc)) ||
+      (rc = tape_alloc(e, &e->t_gm, n * S * nc)) || (rc = tape_alloc(e, &e->t_gj, n * S * ne)) ||
+      (rc = tape_alloc(e, &e->t_d1, n * nc)) || (rc = tape_alloc(e, &e->t_d2, n * nc)) ||
+      (rc = tape_alloc(e, &e->t_valid, n * e->W))) {
+    free_tape(e);
+    return rc;
+  }
+  e->tape_slots = slots;
+  return CELLS_OK;
+}
+
+extern "C" int cells_step_record(cells_ensemble* e, void* stream, const double* y_in, double* y_out,
+                                 double dt, const cells_control* control, int32_t slot, int32_t* ok_out) {
+  if (!e || !y_in || !y_out || !control) return fail(CELLS_EINVAL, "NULL argument");
+  int rc = check_adjoint(e, dt);
+  if (rc) return rc;
+  if (slot < 0 || slot >= e->tape_slots) return fail(CELLS_EINVAL, "tape slot out of range");
+  rc = cells_step(e, stream, y_in, y_out, dt, 1, control, nullptr, ok_out, 1);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const Params& P = e->P;
+  size_t q = slot, nc = e->ncell, ne = std::max(e->nedge, 1), S = e->S, tot = P.total;
+  auto copy = [&](double* dst, const double* src, size_t count) {
+    return hipMemcpyAsync(dst + q * count, src, sizeof(double) * count, hipMemcpyDeviceToDevice, st);
+  };
+  HIP_TRY(copy(e->t_y, y_in, tot));
+  HIP_TRY(copy(e->t_y1, e->Y1, tot));
+  HIP_TRY(copy(e->t_fv, e->fvals, e->factor_total));
+  HIP_TRY(copy(e->t_msc, e->msc, nc));
+  HIP_TRY(copy(e->t_gm, e->gm, S * nc));
+  HIP_TRY(copy(e->t_gj, e->gj, S * ne));
+  HIP_TRY(copy(e->t_d1, e->dphi, nc));
+  HIP_TRY(copy(e->t_d2, e->dphi2, nc));
+  HIP_TRY(hipMemcpyAsync(e->t_valid + q * e->W, e->valid, sizeof(int) * e->W, hipMemcpyDeviceToDevice, st));
+  return CELLS_OK;
+}
+
+extern "C" int cells_step_vjp_recorded(cells_ensemble* e, void* stream, int32_t slot, const double* ybar_out,
+                                       double dt, const cells_control* control, double* ybar_in,
+                                       double* pam_bar, double* cap_bar, double* paj_bar, int32_t* valid_out) {
+  if (!e || !ybar_out || !control || !ybar_in || !pam_bar || !cap_bar || !paj_bar)
+    return fail(CELLS_EINVAL, "NULL argument");
+  int rc = check_adjoint(e, dt);
+  if (rc) return rc;
+  if (slot < 0 || slot >= e->tape_slots) return fail(CELLS_EINVAL, "tape slot out of range");
+  HIP_TRY(hipSetDevice(e->device));
+  if ((rc = adjoint_workspace(e))) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
+  const Params& P = e->P;
+  size_t q = slot, nc = e->ncell, ne = std::max(e->nedge, 1), S = e->S, tot = P.total;
+  StepRecord r{e->t_y + q * tot, e->t_y1 + q * tot, e->t_fv + q * e->factor_total, e->t_msc + q * nc,
+               e->t_gm + q * S * nc, e->t_gj + q * S * ne, e->t_d1 + q * nc, e->t_d2 + q * nc};
+  if ((rc = reverse_sweep(e, st, r, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;
+  if (valid_out)
+    HIP_TRY(hipMemcpyAsync(valid_out, e->t_valid + q * e->W, sizeof(int) * e->W, hipMemcpyDeviceToDevice, st));
   return CELLS_OK;
 }
 

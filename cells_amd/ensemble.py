@@ -54,6 +54,7 @@ class Ensemble:
         _lib.check(L.cells_ensemble_create(ctypes.byref(self._world), self.method, device,
                                            ctypes.byref(handle)))
         self._h = handle
+        self.tape_slots = 0
         self.state_size = int(L.cells_state_size(handle))
         if self.state_size != self.layout.total:
             raise RuntimeError("state layout disagrees with the library's")
@@ -167,6 +168,45 @@ class Ensemble:
         _lib.check(_lib.load().cells_step_vjp(self._h, self._stream(), v(y), v(ybar), float(dt),
                                               ctypes.byref(ctl), v(yin), v(pam_bar), v(cap_bar),
                                               v(paj_bar), v(valid)))
+        return yin, pam_bar, cap_bar, paj_bar, valid
+
+    def reserve_tape(self, slots):
+        """Reserve `slots` adjoint step records on the device (0 frees them); returns bytes."""
+        nbytes = ctypes.c_int64()
+        _lib.check(_lib.load().cells_tape_reserve(self._h, int(slots), ctypes.byref(nbytes)))
+        self.tape_slots = int(slots)
+        return nbytes.value
+
+    def step_record(self, y, dt, slot, ctl=None):
+        """One batched ROS2 step (as step(..., nsteps=1)) that records itself in tape `slot`:
+        (y_next, ok [W])."""
+        t = self.torch
+        y = self.to_device(y)
+        out = t.empty_like(y)
+        ok = t.empty(self.packed.W, dtype=t.int32, device=self.device)
+        ctl = ctl or control()
+        _lib.check(_lib.load().cells_step_record(
+            self._h, self._stream(), ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+            float(dt), ctypes.byref(ctl), int(slot), ctypes.c_void_p(ok.data_ptr())))
+        return out, ok
+
+    def step_vjp_recorded(self, slot, ybar, dt, ctl=None, pam_bar=None, cap_bar=None, paj_bar=None):
+        """step_vjp of the step recorded in tape `slot`: (cotangent on y, pam_bar, cap_bar, paj_bar,
+        valid)."""
+        t = self.torch
+        p = self.packed
+        ybar = self.to_device(ybar)
+        yin = t.zeros_like(ybar)
+        dev, f64 = self.device, t.float64
+        pam_bar = t.zeros((p.S, p.ncell), dtype=f64, device=dev) if pam_bar is None else pam_bar
+        cap_bar = t.zeros(p.ncell, dtype=f64, device=dev) if cap_bar is None else cap_bar
+        paj_bar = t.zeros((p.S, max(p.nedge, 1)), dtype=f64, device=dev) if paj_bar is None else paj_bar
+        valid = t.empty(p.W, dtype=t.int32, device=dev)
+        ctl = ctl or control()
+        v = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+        _lib.check(_lib.load().cells_step_vjp_recorded(self._h, self._stream(), int(slot), v(ybar),
+                                                       float(dt), ctypes.byref(ctl), v(yin),
+                                                       v(pam_bar), v(cap_bar), v(paj_bar), v(valid)))
         return yin, pam_bar, cap_bar, paj_bar, valid
 
     def vmem_loss(self, y, target, weight, loss, ybar=None):

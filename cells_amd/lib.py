@@ -30,7 +30,8 @@ EXPORTS = (
     "cells_ensemble_destroy", "cells_state_size", "cells_step", "cells_rates",
     "cells_factor_size", "cells_factor", "cells_solve", "cells_world_positions",
     "cells_kernel_times", "cells_device_checks", "cells_step_vjp", "cells_vmem_loss",
-    "cells_set_parameters", "cells_plan_stats",
+    "cells_set_parameters", "cells_plan_stats", "cells_tape_reserve", "cells_step_record",
+    "cells_step_vjp_recorded",
 )
 
 
@@ -107,6 +108,11 @@ def load():
     lib.cells_step_vjp.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
                                    vp, vp, vp, vp, vp]
     lib.cells_vmem_loss.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, vp]
+    lib.cells_tape_reserve.argtypes = [vp, ctypes.c_int32, _i64p]
+    lib.cells_step_record.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
+                                      ctypes.c_int32, vp]
+    lib.cells_step_vjp_recorded.argtypes = [vp, vp, ctypes.c_int32, vp, ctypes.c_double,
+                                            ctypes.POINTER(ControlStruct), vp, vp, vp, vp, vp]
     lib.cells_set_parameters.argtypes = [vp, vp, vp, vp, vp]
     lib.cells_plan_stats.argtypes = [vp, _i64p, _f64p, _i32p, _i32p]
     lib.cells_world_positions.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i64p, _i64p, _i64p]

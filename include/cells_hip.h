@@ -137,6 +137,24 @@ int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const doubl
                    double dt, const cells_control* control, double* ybar_in, double* pam_bar,
                    double* cap_bar, double* paj_bar, int32_t* valid_out);
 
+/* Step tape for the adjoint (no counterpart in the reference, which has no adjoint). Reserves
+ * `slots` step records in device memory (0 frees them); each record keeps what the reverse of
+ * one ROS2 step reads: the input state, the stage-1 state, the factor values with the Jacobi
+ * scales, the route slopes and both stage increments. bytes_out (may be NULL) receives the size.
+ * Synchronises the device. */
+int cells_tape_reserve(cells_ensemble* e, int32_t slots, int64_t* bytes_out);
+
+/* One batched ROS2 step, y_in -> y_out exactly as cells_step with nsteps = 1 (same kernels, same
+ * results), that also writes its record into tape slot `slot`. */
+int cells_step_record(cells_ensemble* e, void* stream, const double* y_in, double* y_out, double dt,
+                      const cells_control* control, int32_t slot, int32_t* ok_out);
+
+/* cells_step_vjp from the record in tape slot `slot` (nothing recomputed forward: the recorded
+ * factor serves both transposed stage solves). valid_out receives the recorded step's flags. */
+int cells_step_vjp_recorded(cells_ensemble* e, void* stream, int32_t slot, const double* ybar_out,
+                            double dt, const cells_control* control, double* ybar_in,
+                            double* pam_bar, double* cap_bar, double* paj_bar, int32_t* valid_out);
+
 /* Vmem misfit of one sample row: loss_out[w] += weight sum_i (V_i - target_i)^2 over world w's
  * cells, and (when ybar is not NULL) its cotangent on the amounts added to ybar. */
 int cells_vmem_loss(cells_ensemble* e, void* stream, const double* y, const double* target,

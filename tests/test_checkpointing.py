@@ -15,11 +15,30 @@ from cells_amd import adjoint
 
 
 class Probe(adjoint.Calibration):
-    def __init__(self):  # no device
+    """State = step index; targets at every `every`-th step; tape of `unit` recorded steps
+    (unit 0: every reverse step recomputes its forward internals)."""
+
+    def __init__(self, steps, unit=0, every=7):  # no device
         self.forward_steps = 0
+        self.recorded_steps = 0
         self.reverse_steps = 0
         self.visited = []
         self.max_live = 0
+        self.steps = steps
+        self.unit = max(unit, 1)
+        self.taped = unit > 0
+        self.samples = list(range(every, steps + 1, every))
+        self.targets = {k: None for k in self.samples}
+        self.frontier = 0
+        self.loss_rows = []
+        self.cotangent_rows = []
+        self.slots = {}
+        self.e = self
+        self.weight, self.acc, self.scratch, self.ybar = 1.0, None, None, 0
+
+    # the ensemble's calls, on integer states
+    def vmem_loss(self, y, target, weight, loss, ybar=None):
+        (self.loss_rows if ybar is None else self.cotangent_rows).append(y)
 
     def _advance(self, y, n):
         self.forward_steps += n
@@ -27,11 +46,27 @@ class Probe(adjoint.Calibration):
 
     def _adjoint_step(self, y_a, a):
         assert y_a == a
+        b = a + 1
+        if b in self.targets:
+            if b > self.frontier:
+                self.loss_rows.append(b)
+            self.cotangent_rows.append(b)
+        self.frontier = max(self.frontier, b)
+        self.visited.append(a)
+
+    def _record_step(self, y, slot):
+        assert 0 <= slot < self.unit
+        self.slots[slot] = y
+        self.recorded_steps += 1
+        return y + 1
+
+    def _reverse_recorded(self, slot, a):
+        assert self.slots[slot] == a
         self.visited.append(a)
 
 
-def run(steps, snaps):
-    pr = Probe()
+def run(steps, snaps, unit=0):
+    pr = Probe(steps, unit)
     store = {0: 0}
 
     class Watch(dict):
@@ -40,7 +75,8 @@ def run(steps, snaps):
             pr.max_live = max(pr.max_live, len(s) - 1)
 
     store = Watch(store)
-    pr._reverse(store, 0, steps, snaps)
+    units = -(-steps // pr.unit)
+    pr._reverse(store, 0, units, snaps)
     return pr
 
 
@@ -49,6 +85,34 @@ def test_every_step_is_reversed_once_in_order(steps, snaps):
     pr = run(steps, snaps)
     assert pr.visited == list(range(steps - 1, -1, -1))
     assert pr.max_live <= snaps
+
+
+@pytest.mark.parametrize("steps,snaps,unit", [(1, 3, 4), (7, 2, 3), (100, 5, 8), (2000, 45, 45),
+                                              (2000, 45, 20), (500, 0, 16), (64, 64, 64)])
+def test_taped_units_reverse_every_step_once_in_order(steps, snaps, unit):
+    pr = run(steps, snaps, unit)
+    assert pr.visited == list(range(steps - 1, -1, -1))
+    assert pr.max_live <= snaps
+    # every step is recorded exactly once, just before its reversal
+    assert pr.recorded_steps == steps
+
+
+@pytest.mark.parametrize("steps,snaps,unit", [(200, 45, 0), (200, 45, 45), (2000, 45, 45),
+                                              (1000, 3, 10), (30, 0, 0)])
+def test_loss_rows_are_added_once_and_every_cotangent_once(steps, snaps, unit):
+    """The loss of each sample row is accumulated exactly once (on the sweep that first reaches
+    it) and its cotangent is seeded exactly once, in reverse order."""
+    pr = run(steps, snaps, unit)
+    assert sorted(pr.loss_rows) == pr.samples
+    assert pr.cotangent_rows == pr.samples[::-1]
+
+
+def test_a_tape_of_45_steps_meets_the_one_sweep_bound():
+    """2000 steps, 45 snapshots, 45-step tape: one plain forward sweep over all but the last unit
+    (no recomputation beyond it), one recorded sweep, one reverse sweep."""
+    pr = run(2000, 45, 45)
+    assert pr.forward_steps <= 2000 - 1
+    assert pr.recorded_steps == 2000
 
 
 @pytest.mark.parametrize("steps,snaps", [(100, 5), (2000, 45), (1000, 10)])

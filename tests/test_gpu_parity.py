@@ -333,3 +333,59 @@ def test_calibration_gradient_meets_central_differences(torch):
         an = float(np.sum(grad * d))
         assert abs(fd - an) <= 1e-6 * abs(fd), (trial, fd, an)
     e.close()
+
+
+def test_recorded_steps_reproduce_the_step_and_its_adjoint(torch):
+    """cells_step_record gives cells_step's state bit for bit, and the reverse of each recorded
+    step equals cells_step_vjp's (which recomputes the forward internals) bit for bit."""
+    p = _three_ion(worlds=3, cells=(20, 49), seed=8)
+    e = ensemble(p, "ros2")
+    e.reserve_tape(3)
+    L = StateLayout.of(p)
+    dt = 5e-3
+    y = e.to_device(initial_state(p))
+    rng = np.random.default_rng(2)
+    ys, bars = [], []
+    say("recorded steps")
+    for j in range(3):
+        ref, ok_ref = e.step(y, dt, nsteps=1)
+        got, ok = e.step_record(y, dt, j)
+        assert torch.equal(ref, got) and bool(ok.all()) and bool(ok_ref.all())
+        ys.append(y)
+        y = got
+    for j in range(3):
+        ybar = torch.as_tensor(rng.normal(size=L.total), device=y.device)
+        a = e.step_vjp(ys[j], ybar, dt)
+        b = e.step_vjp_recorded(j, ybar, dt)
+        for u, v in zip(a, b):
+            assert torch.equal(u, v)
+    e.close()
+
+
+def test_taped_calibration_gradient_equals_the_recomputing_one(torch):
+    """Binomial schedule over 6-step recorded units vs the one-step recomputing schedule: the same
+    loss (and the plain loss sweep's) and the same gradient, bit for bit."""
+    from cells_amd.adjoint import Calibration
+    from tests.helpers import vmem as vm
+
+    p = _three_ion(worlds=2, cells=(16, 25), seed=21)
+    e = ensemble(p, "ros2")
+    y0 = initial_state(p)
+    rng = np.random.default_rng(9)
+    y = e.to_device(y0)
+    targets = []
+    for _ in range(5):
+        y, _ = e.step(y, 5e-3, nsteps=8)
+        targets.append(vm(p, y.cpu().numpy()) + rng.normal(scale=1e-4, size=p.ncell))
+    cal = Calibration(e, y0, 5e-3, 40, 8, targets)
+    theta = rng.normal(scale=0.05, size=(6, p.ncell))
+    say("calibration: recomputing schedule")
+    l0, g0 = cal.loss_and_gradient(theta, snapshots=3, tape=0)
+    e.reserve_tape(6)
+    say("calibration: taped schedule")
+    l1, g1 = cal.loss_and_gradient(theta, snapshots=3, tape=6)
+    assert cal.recorded_steps == 40
+    l2 = cal.loss(theta)
+    assert l0 == l1 == l2
+    assert np.array_equal(g0, g1)
+    e.close()
