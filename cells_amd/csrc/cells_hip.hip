@@ -115,6 +115,12 @@ struct ShapeDev {
   long o_cs, o_entry, o_rowptr, o_rowent, o_lev, o_levcol, o_levent, o_node;
   long o_mask;  // offset in the mask pool (unsigned long long words)
   int lds_pt;   // first factor position held in LDS during the factor (nnz: none)
+  // multifrontal schedule: supernode records (8 ints each: first column, width, front order,
+  // update offset, children begin/end, own rel list start, pad), per level 4 boundaries (thread
+  // fronts, wavefront fronts, workgroup fronts, end), children, rel lists
+  long o_sn, o_snlev, o_child, o_rel;
+  int sn_levels;
+  long usz;  // doubles of update matrices per world
 };
 
 __device__ __forceinline__ double& CF(const Params& P, double* y, int f, int i) {
@@ -968,6 +974,155 @@ __global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* sha
 }
 
 // ------------------------------------------------------------------------------------------
+// Multifrontal supernodal factor (the default): the same factor L on the same pattern and
+// positions, with the sums grouped by the elimination tree instead of the reference's
+// adjacent-pairs fold over each entry's sources. Supernode by supernode, leaves first: the front
+// (dense, packed lower, column-major) is the supernode's assembled columns plus its children's
+// update matrices (extend-add through the children's row positions), it is factored over the
+// supernode's columns in LDS (pivot = sqrt, column times the pivot's reciprocal, right-looking
+// rank-1 updates of the trailing triangle), the columns go back to the factor and the trailing
+// triangle becomes the supernode's update matrix. One workgroup per world; a front is held by one
+// thread, one wavefront or the whole workgroup according to its size.
+
+constexpr int kMfThreads = 256;
+constexpr int kMfLds = 8192;                               // doubles of LDS per workgroup (64 KiB)
+constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 7
+constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 63
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int G>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (G == 64) wave_sync();
+  if constexpr (G == kMfThreads) __syncthreads();
+}
+
+// Packed lower column-major triangle of order m: entry (i, j), i >= j, at colbase(j, m) + i.
+__device__ __forceinline__ int colbase(int j, int m) { return j * m - (j * (j + 1)) / 2; }
+
+// Steps (b, off) -> the next entry G further on in a packed lower triangle of order T
+// (column b, row b + off).
+__device__ __forceinline__ void tri_advance(int T, int& b, int& off) {
+  while (b < T && off >= T - b) {
+    off -= T - b;
+    ++b;
+  }
+}
+
+// One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE].
+template <int G, int STRIDE>
+__device__ void mf_front(const int* rec, int q, const int* kids, const int* rel, const int* cs,
+                         double* fv, double* ub, double* F, int lane) {
+  const int* r = rec + 8 * q;
+  const int j0 = r[0], J = r[1], m = r[2], uoff = r[3], c0 = r[4], c1 = r[5];
+  const int S = m - J, tri = m * (m + 1) / 2, Pn = colbase(J, m) + J;
+  const long p0 = cs[j0];
+  // the supernode's columns are contiguous in the factor and are the front's first J columns
+  for (int e = lane; e < Pn; e += G) F[e * STRIDE] = fv[p0 + e];
+  for (int e = Pn + lane; e < tri; e += G) F[e * STRIDE] = 0.0;
+  group_sync<G>();
+  for (int x = c0; x < c1; ++x) {
+    const int* rc = rec + 8 * kids[x];
+    const int Sc = rc[2] - rc[1];
+    const int* rl = rel + rc[6];
+    const double* U = ub + rc[3];
+    int b = 0, off = lane;
+    tri_advance(Sc, b, off);
+    for (int e = lane; b < Sc; e += G) {
+      const int ra = rl[b + off], rb = rl[b];
+      F[(colbase(rb, m) + ra) * STRIDE] += U[e];
+      off += G;
+      tri_advance(Sc, b, off);
+    }
+    group_sync<G>();
+  }
+  for (int jj = 0; jj < J; ++jj) {
+    const int cb = colbase(jj, m);
+    const double d = sqrt(F[(cb + jj) * STRIDE]);
+    const double rinv = 1.0 / d;
+    for (int i = jj + 1 + lane; i < m; i += G) F[(cb + i) * STRIDE] *= rinv;
+    group_sync<G>();
+    if (lane == 0) F[(cb + jj) * STRIDE] = d;
+    const int T = m - jj - 1;
+    int b = 0, off = lane;
+    tri_advance(T, b, off);
+    while (b < T) {
+      const int k = jj + 1 + b, i = k + off;
+      F[(colbase(k, m) + i) * STRIDE] -= F[(cb + i) * STRIDE] * F[(cb + k) * STRIDE];
+      off += G;
+      tri_advance(T, b, off);
+    }
+    group_sync<G>();
+  }
+  for (int e = lane; e < Pn; e += G) fv[p0 + e] = F[e * STRIDE];
+  if (S > 0)
+    for (int e = Pn + lane; e < tri; e += G) ub[uoff + e - Pn] = F[e * STRIDE];
+  group_sync<G>();
+}
+
+__device__ void mf_factor_world(const ShapeDev& s, const int* pool, const int* cs, double* fv, double* ub,
+                                double* lds) {
+  const int* rec = pool + s.o_sn;
+  const int* lev = pool + s.o_snlev;
+  const int* kids = pool + s.o_child;
+  const int* rel = pool + s.o_rel;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  for (int L = 0; L < s.sn_levels; ++L) {
+    const int t0 = lev[4 * L], wv0 = lev[4 * L + 1], b0 = lev[4 * L + 2], e = lev[4 * L + 3];
+    for (int q = t0 + tid; q < wv0; q += kMfThreads)
+      mf_front<1, kMfThreads>(rec, q, kids, rel, cs, fv, ub, lds + tid, 0);
+    __syncthreads();
+    for (int q = wv0 + wave; q < b0; q += kMfThreads / 64)
+      mf_front<64, 1>(rec, q, kids, rel, cs, fv, ub, lds + wave * kMfWaveTri, tid & 63);
+    __syncthreads();
+    for (int q = b0; q < e; ++q) mf_front<kMfThreads, 1>(rec, q, kids, rel, cs, fv, ub, lds, tid);
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeDev* shapes,
+                                                          const int* shape_of, const int* pool,
+                                                          const long* foff, const long* uoffw, int w0,
+                                                          double coef, const double* mdiag,
+                                                          const double* msc, const double* Gj,
+                                                          double* fvals, double* ubuf, int* valid) {
+  extern __shared__ double lds[];
+  int w = w0 + blockIdx.x;
+  const ShapeDev s = shapes[shape_of[w]];
+  const PlanView v = plan_view(s, pool);
+  double* fv = fvals + foff[w];
+  int c0 = P.cell_off[w], e0 = P.edge_off[w];
+  for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = 0.0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < s.n; i += blockDim.x) fv[v.diagpos[i]] = mdiag[c0 + i];
+  for (int e = threadIdx.x; e < s.E; e += blockDim.x) {
+    double val = -(coef * Gj[e0 + e]);
+    double sa = msc[P.snd[e0 + e]], sb = msc[P.rcv[e0 + e]];
+    fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
+  }
+  __syncthreads();
+  mf_factor_world(s, pool, v.cs, fv, ubuf + uoffw[w], lds);
+  bool ok = true;
+  for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) ok = ok && isfinite(fv[p]);
+  if (!ok) valid[w] = 0;
+}
+
+// Test entry: the multifrontal factor of assembled values in place, per world.
+__global__ __launch_bounds__(kMfThreads) void k_factor_only_mf(const ShapeDev* shapes, const int* shape_of,
+                                                               const int* pool, const long* foff,
+                                                               const long* uoffw, double* fvals, double* ubuf) {
+  extern __shared__ double lds[];
+  int w = blockIdx.x;
+  const ShapeDev s = shapes[shape_of[w]];
+  const PlanView v = plan_view(s, pool);
+  mf_factor_world(s, pool, v.cs, fvals + foff[w], ubuf + uoffw[w], lds);
+}
+
+// ------------------------------------------------------------------------------------------
 // ROS2 stage combinations (the W-method's k = F + coef (df/dphi) dphi on the route fields).
 
 // The W-method's correction coef (df/dphi) dphi of one cell: dn_c[s] on the amounts, dm_c[s] on
@@ -1571,6 +1726,10 @@ struct cells_ensemble {
   int* pool = nullptr;
   unsigned long long* masks = nullptr;
   long* foff = nullptr;
+  bool pairwise = false;  // factor in the reference's summation order (slot masks), else multifrontal
+  long* uoffw = nullptr;  // per world: start of its update matrices in ubuf
+  double* ubuf = nullptr;
+  std::vector<cells::Supernodes> sn_host;
   size_t factor_lds = 0, solve_lds = 0;  // dynamic LDS bytes of the factor and solve kernels
   int y_in_lds = 1;
   double *A = nullptr, *B = nullptr, *Y1 = nullptr, *K1 = nullptr, *K2 = nullptr;
@@ -1628,6 +1787,16 @@ std::vector<int> narrow(const int64_t* a, size_t n) {
 double host_logaddexp0(double t) {
   double m = t > 0.0 ? t : 0.0;
   return m + std::log1p(std::exp(-std::fabs(0.0 - t)));
+}
+
+// Which lanes hold a front of this order in the multifrontal factor: 0 one thread, 1 one
+// wavefront, 2 the workgroup, 3 none (too large for the workgroup's LDS).
+int32_t front_class(int32_t rows) {
+  int64_t tri = (int64_t)rows * (rows + 1) / 2;
+  if (tri <= kMfThreadTri) return 0;
+  if (tri <= kMfWaveTri) return 1;
+  if (tri <= kMfLds) return 2;
+  return 3;
 }
 
 int check_world(const cells_world* w) {
@@ -1768,6 +1937,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
                           cells_ensemble** out) {
   int rc = check_world(w);
   if (rc) return rc;
+  const bool pairwise = (method & CELLS_FACTOR_PAIRWISE) != 0;
+  method &= ~CELLS_FACTOR_PAIRWISE;
   if (method != CELLS_METHOD_EULER && method != CELLS_METHOD_ROS2)
     return fail(CELLS_EINVAL, "unknown method");
   int count = 0;
@@ -1778,6 +1949,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
   std::unique_ptr<cells_ensemble> e(new cells_ensemble);
   e->device = device;
   e->method = method;
+  e->pairwise = pairwise;
   e->S = w->S;
   e->W = w->W;
   e->ncell = (int)w->ncell;
@@ -1908,6 +2080,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
           return fail(CELLS_EINVAL, "worlds of one cell count with different lattices");
     }
     e->shapes_host.resize(rep.size());
+    e->sn_host.resize(rep.size());
     std::atomic<int> next_shape{0};
     std::atomic<int> bad{0};
     std::string bad_msg;
@@ -1921,6 +2094,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
             world_pattern(w, rep[q], rows, cols);
             int n = (int)(w->cell_off[rep[q] + 1] - w->cell_off[rep[q]]);
             e->shapes_host[q] = cells::bind(n, (int64_t)rows.size(), rows.data(), cols.data(), nullptr);
+            e->sn_host[q] = cells::supernodes(e->shapes_host[q], front_class);
           } catch (const std::exception& ex) {
             bad = 1;
             bad_msg = ex.what();
@@ -1951,7 +2125,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
         d.lds_pt = (int)pt;
       }
       slot_of.assign(s.n, -1);
-      for (int kk = 0; kk < s.n; ++kk) {
+      for (int kk = 0; kk < s.n && e->pairwise; ++kk) {
         int64_t c0 = s.column_start[kk], c1 = s.column_start[kk + 1];
         if (c1 - c0 > 128) return fail(CELLS_EINVAL, "a factor column holds more than 128 entries");
         for (int64_t p2 = c0; p2 < c1; ++p2) slot_of[s.row_rank[p2]] = (int)(p2 - c0);
@@ -2002,6 +2176,38 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       std::vector<int32_t> rows, cols;
       world_pattern(w, k, rows, cols);
       for (size_t q2 = 0; q2 < rows.size(); ++q2) pool.push_back((int)cells::position(s, rows[q2], cols[q2]));
+      // the multifrontal schedule
+      const cells::Supernodes& sn = e->sn_host[q];
+      const int ns = (int)sn.first.size();
+      if (sn.update_total > 0x7fffffff) return fail(CELLS_EINVAL, "update matrices exceed 2^31 doubles");
+      d.o_sn = (long)pool.size();
+      for (int t = 0; t < ns; ++t) {
+        if (front_class(sn.rows[t]) > 2)
+          return fail(CELLS_EINVAL, "a front exceeds the workgroup's LDS (order above 127)");
+        int32_t r8[8] = {sn.first[t], sn.width[t], sn.rows[t], (int32_t)sn.update_offset[t],
+                         sn.child_ptr[t], sn.child_ptr[t + 1], sn.rel_ptr[t], 0};
+        pool.insert(pool.end(), r8, r8 + 8);
+      }
+      d.sn_levels = sn.levels;
+      d.o_snlev = (long)pool.size();
+      {
+        std::vector<int> bounds(4 * (size_t)sn.levels, 0);
+        int t = 0;
+        for (int L = 0; L < sn.levels; ++L) {
+          for (int c = 0; c < 3; ++c) {
+            bounds[4 * L + c] = t;
+            while (t < ns && sn.level[t] == L && front_class(sn.rows[t]) == c) ++t;
+          }
+          bounds[4 * L + 3] = t;
+          if (t < ns && sn.level[t] == L) return fail(CELLS_EINVAL, "front schedule out of order");
+        }
+        pool.insert(pool.end(), bounds.begin(), bounds.end());
+      }
+      d.o_child = (long)pool.size();
+      pool.insert(pool.end(), sn.child.begin(), sn.child.end());
+      d.o_rel = (long)pool.size();
+      pool.insert(pool.end(), sn.rel.begin(), sn.rel.end());
+      d.usz = sn.update_total;
       sd[q] = d;
     }
     if (pool.size() > (size_t)0x7fffffff) return fail(CELLS_EINVAL, "plans exceed 2^31 entries");
@@ -2018,12 +2224,16 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
     e->foff_host[0] = 0;
     for (int k = 0; k < e->W; ++k) e->foff_host[k + 1] = e->foff_host[k] + sd[e->shape_of_host[k]].nnz;
     e->factor_total = e->foff_host[e->W];
+    std::vector<long> uoffw(e->W + 1, 0);
+    for (int k = 0; k < e->W; ++k) uoffw[k + 1] = uoffw[k] + sd[e->shape_of_host[k]].usz;
     if ((rc = upload(e.get(), &e->shapes, sd.data(), sd.size())) ||
         (rc = upload(e.get(), &e->shape_of, e->shape_of_host.data(), e->shape_of_host.size())) ||
         (rc = upload(e.get(), &e->pool, pool.data(), pool.size())) ||
         (rc = upload(e.get(), &e->masks, mask_pool.data(), mask_pool.size())) ||
         (rc = upload(e.get(), &e->foff, e->foff_host.data(), e->foff_host.size())) ||
-        (rc = alloc(e.get(), &e->fvals, std::max<long>(e->factor_total, 1))))
+        (rc = alloc(e.get(), &e->fvals, std::max<long>(e->factor_total, 1))) ||
+        (rc = upload(e.get(), &e->uoffw, uoffw.data(), uoffw.size())) ||
+        (rc = alloc(e.get(), &e->ubuf, std::max<long>(uoffw[e->W], 1))))
       return rc;
   }
   HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
@@ -2076,6 +2286,17 @@ struct KernelTimer {
   }
 };
 
+// Assemble and factor the stage matrix C + coef G of worlds [w0, w0 + nw).
+void launch_factor(cells_ensemble* e, hipStream_t st, int w0, int nw, double coef) {
+  if (e->pairwise)
+    hipLaunchKernelGGL(k_factor, dim3(nw), dim3(kFactorBlock), e->factor_lds, st, e->P, e->shapes, e->shape_of,
+                       e->pool, e->masks, e->foff, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
+  else
+    hipLaunchKernelGGL(k_factor_mf, dim3(nw), dim3(kMfThreads), sizeof(double) * kMfLds, st, e->P, e->shapes,
+                       e->shape_of, e->pool, e->foff, e->uoffw, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals,
+                       e->ubuf, e->valid);
+}
+
 int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int world,
                 KernelTimer* tm = nullptr) {
   auto mark = [&](const char* name) {
@@ -2119,8 +2340,7 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
                        c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
     mark("k_cells");
-    hipLaunchKernelGGL(k_factor, dim3(nw), dim3(kFactorBlock), e->factor_lds, st, P, e->shapes, e->shape_of, e->pool,
-                       e->masks, e->foff, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
+    launch_factor(e, st, w0, nw, coef);
     mark("k_factor");
     hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of,
                        e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds);
@@ -2267,8 +2487,13 @@ extern "C" int cells_rates(cells_ensemble* e, void* stream, const double* y,
 extern "C" int cells_factor(cells_ensemble* e, void* stream, double* values) {
   if (!e || !values) return fail(CELLS_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(e->device));
-  hipLaunchKernelGGL(k_factor_only, dim3(e->W), dim3(kFactorBlock), e->factor_lds, static_cast<hipStream_t>(stream),
-                     e->shapes, e->shape_of, e->pool, e->masks, e->foff, values);
+  if (e->pairwise)
+    hipLaunchKernelGGL(k_factor_only, dim3(e->W), dim3(kFactorBlock), e->factor_lds, static_cast<hipStream_t>(stream),
+                       e->shapes, e->shape_of, e->pool, e->masks, e->foff, values);
+  else
+    hipLaunchKernelGGL(k_factor_only_mf, dim3(e->W), dim3(kMfThreads), sizeof(double) * kMfLds,
+                       static_cast<hipStream_t>(stream), e->shapes, e->shape_of, e->pool, e->foff, e->uoffw,
+                       values, e->ubuf);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }
@@ -2414,8 +2639,7 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                        c, e->K1, e->gj, e->Gj, e->valid);
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
                      e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
-  hipLaunchKernelGGL(k_factor, dim3(W), dim3(kFactorBlock), e->factor_lds, st, P, e->shapes, e->shape_of, e->pool, e->masks,
-                     e->foff, 0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
+  launch_factor(e, st, 0, W, coef);
   hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds);
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,

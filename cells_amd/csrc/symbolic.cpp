@@ -179,4 +179,93 @@ int64_t position(const Symbolic& s, int32_t i, int32_t j) {
   return it - s.row_rank.begin();
 }
 
+Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows)) {
+  const int32_t n = s.n;
+  auto col_len = [&](int32_t j) { return static_cast<int32_t>(s.column_start[j + 1] - s.column_start[j]); };
+  std::vector<int32_t> nchild(n, 0);
+  for (int32_t k = 0; k < n; ++k)
+    if (s.parent[k] >= 0) nchild[s.parent[k]]++;
+  // fundamental supernodes in column order
+  std::vector<int32_t> f0, fw, of(n);
+  for (int32_t k = 0; k < n;) {
+    int32_t j = k;
+    while (j + 1 < n && s.parent[j] == j + 1 && nchild[j + 1] == 1 && col_len(j) == col_len(j + 1) + 1) ++j;
+    for (int32_t q = k; q <= j; ++q) of[q] = static_cast<int32_t>(f0.size());
+    f0.push_back(k);
+    fw.push_back(j - k + 1);
+    k = j + 1;
+  }
+  const int32_t ns = static_cast<int32_t>(f0.size());
+  std::vector<int32_t> fpar(ns, -1), flev(ns, 0);
+  for (int32_t t = 0; t < ns; ++t) {
+    int32_t last = f0[t] + fw[t] - 1;
+    if (s.parent[last] >= 0) fpar[t] = of[s.parent[last]];
+  }
+  for (int32_t t = 0; t < ns; ++t)  // parents follow children in column order
+    if (fpar[t] >= 0) flev[fpar[t]] = std::max(flev[fpar[t]], flev[t] + 1);
+  // schedule: by level, then class, then first column
+  std::vector<int32_t> sched(ns);
+  for (int32_t t = 0; t < ns; ++t) sched[t] = t;
+  std::vector<int32_t> fcls(ns);
+  for (int32_t t = 0; t < ns; ++t) fcls[t] = cls(col_len(f0[t]));
+  std::sort(sched.begin(), sched.end(), [&](int32_t a, int32_t b) {
+    if (flev[a] != flev[b]) return flev[a] < flev[b];
+    if (fcls[a] != fcls[b]) return fcls[a] < fcls[b];
+    return f0[a] < f0[b];
+  });
+  std::vector<int32_t> pos(ns);
+  for (int32_t q = 0; q < ns; ++q) pos[sched[q]] = q;
+  Supernodes out;
+  out.first.resize(ns);
+  out.width.resize(ns);
+  out.rows.resize(ns);
+  out.level.resize(ns);
+  out.parent.resize(ns);
+  out.update_offset.resize(ns);
+  for (int32_t q = 0; q < ns; ++q) {
+    int32_t t = sched[q];
+    out.first[q] = f0[t];
+    out.width[q] = fw[t];
+    out.rows[q] = col_len(f0[t]);
+    out.level[q] = flev[t];
+    out.parent[q] = fpar[t] >= 0 ? pos[fpar[t]] : -1;
+    int64_t S = out.rows[q] - out.width[q];
+    out.update_offset[q] = out.update_total;
+    out.update_total += S * (S + 1) / 2;
+    out.levels = std::max(out.levels, flev[t] + 1);
+  }
+  // children, ascending by first column
+  std::vector<std::vector<int32_t>> kids(ns);
+  for (int32_t q = 0; q < ns; ++q)
+    if (out.parent[q] >= 0) kids[out.parent[q]].push_back(q);
+  out.child_ptr.assign(ns + 1, 0);
+  for (int32_t q = 0; q < ns; ++q) {
+    std::sort(kids[q].begin(), kids[q].end(), [&](int32_t a, int32_t b) { return out.first[a] < out.first[b]; });
+    out.child_ptr[q + 1] = out.child_ptr[q] + static_cast<int32_t>(kids[q].size());
+    out.child.insert(out.child.end(), kids[q].begin(), kids[q].end());
+  }
+  // each supernode's rows below its last column, as positions in its parent's front
+  out.rel_ptr.assign(ns + 1, 0);
+  std::vector<int32_t> where(n, -1);
+  for (int32_t q = 0; q < ns; ++q) {
+    int32_t last = out.first[q] + out.width[q] - 1;
+    int64_t b0 = s.column_start[last] + 1, b1 = s.column_start[last + 1];
+    int32_t p = out.parent[q];
+    if (p >= 0) {
+      int64_t c0 = s.column_start[out.first[p]], c1 = s.column_start[out.first[p] + 1];
+      for (int64_t x = c0; x < c1; ++x) where[s.row_rank[x]] = static_cast<int32_t>(x - c0);
+      for (int64_t x = b0; x < b1; ++x) {
+        int32_t r = where[s.row_rank[x]];
+        if (r < 0) throw std::runtime_error("a supernode's rows are not in its parent's front");
+        out.rel.push_back(r);
+      }
+      for (int64_t x = c0; x < c1; ++x) where[s.row_rank[x]] = -1;
+    } else if (b1 > b0) {
+      throw std::runtime_error("a root supernode has rows below it");
+    }
+    out.rel_ptr[q + 1] = static_cast<int32_t>(out.rel.size());
+  }
+  return out;
+}
+
 }  // namespace cells

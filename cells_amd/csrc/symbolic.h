@@ -24,6 +24,27 @@ struct Symbolic {
   int32_t levels() const { return level_ptr.empty() ? 0 : static_cast<int32_t>(level_ptr.size()) - 1; }
 };
 
+// Fundamental supernodes of a bound pattern and the multifrontal schedule over them. Supernode t
+// holds the consecutive columns [first, first + width) whose structures nest (column j's rows are
+// {j + 1} and column j + 1's rows, and column j + 1 has no other child), so its front is dense of
+// order rows = width + |S| with S the rows below the last column. Fronts are listed level by
+// level of the supernodal elimination tree (leaves first), children before parents.
+struct Supernodes {
+  std::vector<int32_t> first, width, rows;  // per supernode (schedule order)
+  std::vector<int32_t> level;               // supernodal tree level
+  std::vector<int32_t> parent;              // schedule index of the parent, -1 at a root
+  std::vector<int64_t> update_offset;       // start of the packed |S| x |S| update matrix
+  std::vector<int32_t> child_ptr, child;    // children (schedule indices, ascending first column)
+  std::vector<int32_t> rel_ptr, rel;        // per supernode: its rows S as positions in the
+                                            // parent's front rows
+  int64_t update_total = 0;
+  int32_t levels = 0;
+};
+
+// Supernodes of s, scheduled by level; within a level the fronts are ordered by `cls`, a class
+// of the front order (the device groups fronts of one class).
+Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows));
+
 std::vector<int32_t> minimum_degree(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols);
 Symbolic bind(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, const int32_t* order);
 int64_t position(const Symbolic& s, int32_t i, int32_t j);

@@ -40,18 +40,20 @@ def control(membrane=1.0, junction=1.0, pump=1.0, calcium_pump=1.0, water=1.0):
 class Ensemble:
     """Packed worlds bound on one GPU: parameters, per-shape plans and the stepping workspace."""
 
-    def __init__(self, packed: Packed, method: str = "euler", device: int = 0):
+    def __init__(self, packed: Packed, method: str = "euler", device: int = 0,
+                 factor: str = "supernodal"):
         torch = _torch()
         self.torch = torch
         self.packed = packed
         self.layout = StateLayout.of(packed)
         self.method = {"euler": _lib.METHOD_EULER, "ros2": _lib.METHOD_ROS2}[method]
+        flag = {"supernodal": 0, "pairwise": _lib.FACTOR_PAIRWISE}[factor]
         self.device = torch.device("cuda", device)
         L = _lib.load()
         self._world = _lib.world_struct(packed)
         handle = ctypes.c_void_p()
         torch.cuda.init()
-        _lib.check(L.cells_ensemble_create(ctypes.byref(self._world), self.method, device,
+        _lib.check(L.cells_ensemble_create(ctypes.byref(self._world), self.method | flag, device,
                                            ctypes.byref(handle)))
         self._h = handle
         self.tape_slots = 0

@@ -19,6 +19,7 @@ CELLS_ENOMEM = 3
 CELLS_ENODEV = 4
 METHOD_EULER = 0
 METHOD_ROS2 = 1
+FACTOR_PAIRWISE = 0x100  # or-ed into the method: the reference-order factor
 
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)

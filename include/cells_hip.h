@@ -23,6 +23,11 @@ extern "C" {
 
 #define CELLS_METHOD_EULER 0 /* numerics/surface_imex.py:36 imex_euler_face_electromigration@1 */
 #define CELLS_METHOD_ROS2 1  /* ROS2 (gamma = 1 + 1/sqrt 2), stage matrix C + gamma dt G */
+/* Or-ed into the method: factor each stage matrix with the reference's summation order (every
+ * entry's sources folded in adjacent pairs, numerics/sparse_cholesky.py:474-490), bit for bit
+ * the reference's factor, instead of the default multifrontal supernodal factor (same pattern and
+ * positions, sums grouped by the elimination tree; equal to rounding). */
+#define CELLS_FACTOR_PAIRWISE 0x100
 
 /* The packed worlds: lattices of cells on one held bath each, back to back. The reference's
  * counterpart is the layout pytree of models/surface_pump_v3.py:140-173 (LAYOUT_TYPE) built by
