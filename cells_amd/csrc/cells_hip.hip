@@ -116,8 +116,8 @@ struct ShapeDev {
   long o_mask;  // offset in the mask pool (unsigned long long words)
   int lds_pt;   // first factor position held in LDS during the factor (nnz: none)
   // multifrontal schedule: supernode records (8 ints each: first column, width, front order,
-  // update offset, children begin/end, own rel list start, pad), per level 4 boundaries (thread
-  // fronts, wavefront fronts, workgroup fronts, end), children, rel lists
+  // update offset, children begin/end, own rel list start, pad), per level 5 boundaries (thread,
+  // 8-lane, wavefront and workgroup fronts, end), children, rel lists
   long o_sn, o_snlev, o_child, o_rel;
   int sn_levels;
   long usz;  // doubles of update matrices per world
@@ -986,8 +986,11 @@ __global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* sha
 
 constexpr int kMfThreads = 256;
 constexpr int kMfLds = 8192;                               // doubles of LDS per workgroup (64 KiB)
+constexpr int kMfOcta = 8;                                 // lanes of a small group
 constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 7
+constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 22
 constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 63
+constexpr int kMfBatch = 8;                                // global loads in flight per lane
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -995,9 +998,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Groups of up to 64 lanes lie in one wavefront and follow one front's control flow together.
 template <int G>
 __device__ __forceinline__ void group_sync() {
-  if constexpr (G == 64) wave_sync();
+  if constexpr (G > 1 && G <= 64) wave_sync();
   if constexpr (G == kMfThreads) __syncthreads();
 }
 
@@ -1013,7 +1017,8 @@ __device__ __forceinline__ void tri_advance(int T, int& b, int& off) {
   }
 }
 
-// One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE].
+// One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE]. Global
+// loads are issued kMfBatch per lane before their values are used.
 template <int G, int STRIDE>
 __device__ void mf_front(const int* rec, int q, const int* kids, const int* rel, const int* cs,
                          double* fv, double* ub, double* F, int lane) {
@@ -1021,9 +1026,21 @@ __device__ void mf_front(const int* rec, int q, const int* kids, const int* rel,
   const int j0 = r[0], J = r[1], m = r[2], uoff = r[3], c0 = r[4], c1 = r[5];
   const int S = m - J, tri = m * (m + 1) / 2, Pn = colbase(J, m) + J;
   const long p0 = cs[j0];
-  // the supernode's columns are contiguous in the factor and are the front's first J columns
-  for (int e = lane; e < Pn; e += G) F[e * STRIDE] = fv[p0 + e];
-  for (int e = Pn + lane; e < tri; e += G) F[e * STRIDE] = 0.0;
+  // the supernode's columns are contiguous in the factor and are the front's first J columns;
+  // the update block starts at zero
+  for (int e0 = lane; e0 < tri; e0 += kMfBatch * G) {
+    double v[kMfBatch];
+#pragma unroll
+    for (int u = 0; u < kMfBatch; ++u) {
+      const int e = e0 + u * G;
+      v[u] = e < Pn ? fv[p0 + e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kMfBatch; ++u) {
+      const int e = e0 + u * G;
+      if (e < tri) F[e * STRIDE] = v[u];
+    }
+  }
   group_sync<G>();
   for (int x = c0; x < c1; ++x) {
     const int* rc = rec + 8 * kids[x];
@@ -1032,11 +1049,21 @@ __device__ void mf_front(const int* rec, int q, const int* kids, const int* rel,
     const double* U = ub + rc[3];
     int b = 0, off = lane;
     tri_advance(Sc, b, off);
-    for (int e = lane; b < Sc; e += G) {
-      const int ra = rl[b + off], rb = rl[b];
-      F[(colbase(rb, m) + ra) * STRIDE] += U[e];
-      off += G;
-      tri_advance(Sc, b, off);
+    for (int e0 = lane; b < Sc; e0 += kMfBatch * G) {
+      int ta[kMfBatch];
+      double uv[kMfBatch];
+#pragma unroll
+      for (int u = 0; u < kMfBatch; ++u) {
+        const bool in = b < Sc;
+        const int ra = in ? rl[b + off] : 0, rb = in ? rl[b] : 0;
+        uv[u] = in ? U[e0 + u * G] : 0.0;
+        ta[u] = in ? colbase(rb, m) + ra : -1;
+        off += G;
+        tri_advance(Sc, b, off);
+      }
+#pragma unroll
+      for (int u = 0; u < kMfBatch; ++u)
+        if (ta[u] >= 0) F[ta[u] * STRIDE] += uv[u];
     }
     group_sync<G>();
   }
@@ -1070,16 +1097,19 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, const int* c
   const int* lev = pool + s.o_snlev;
   const int* kids = pool + s.o_child;
   const int* rel = pool + s.o_rel;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6, octa = tid / kMfOcta;
   for (int L = 0; L < s.sn_levels; ++L) {
-    const int t0 = lev[4 * L], wv0 = lev[4 * L + 1], b0 = lev[4 * L + 2], e = lev[4 * L + 3];
-    for (int q = t0 + tid; q < wv0; q += kMfThreads)
+    const int* bd = lev + 5 * L;  // thread, 8-lane, wavefront and workgroup fronts of level L
+    for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
       mf_front<1, kMfThreads>(rec, q, kids, rel, cs, fv, ub, lds + tid, 0);
     __syncthreads();
-    for (int q = wv0 + wave; q < b0; q += kMfThreads / 64)
+    for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
+      mf_front<kMfOcta, 1>(rec, q, kids, rel, cs, fv, ub, lds + octa * kMfOctaTri, tid % kMfOcta);
+    __syncthreads();
+    for (int q = bd[2] + wave; q < bd[3]; q += kMfThreads / 64)
       mf_front<64, 1>(rec, q, kids, rel, cs, fv, ub, lds + wave * kMfWaveTri, tid & 63);
     __syncthreads();
-    for (int q = b0; q < e; ++q) mf_front<kMfThreads, 1>(rec, q, kids, rel, cs, fv, ub, lds, tid);
+    for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(rec, q, kids, rel, cs, fv, ub, lds, tid);
   }
   __syncthreads();
 }
@@ -1789,14 +1819,15 @@ double host_logaddexp0(double t) {
   return m + std::log1p(std::exp(-std::fabs(0.0 - t)));
 }
 
-// Which lanes hold a front of this order in the multifrontal factor: 0 one thread, 1 one
-// wavefront, 2 the workgroup, 3 none (too large for the workgroup's LDS).
+// Which lanes hold a front of this order in the multifrontal factor: 0 one thread, 1 eight lanes,
+// 2 one wavefront, 3 the workgroup, 4 none (too large for the workgroup's LDS).
 int32_t front_class(int32_t rows) {
   int64_t tri = (int64_t)rows * (rows + 1) / 2;
   if (tri <= kMfThreadTri) return 0;
-  if (tri <= kMfWaveTri) return 1;
-  if (tri <= kMfLds) return 2;
-  return 3;
+  if (tri <= kMfOctaTri) return 1;
+  if (tri <= kMfWaveTri) return 2;
+  if (tri <= kMfLds) return 3;
+  return 4;
 }
 
 int check_world(const cells_world* w) {
@@ -2182,7 +2213,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       if (sn.update_total > 0x7fffffff) return fail(CELLS_EINVAL, "update matrices exceed 2^31 doubles");
       d.o_sn = (long)pool.size();
       for (int t = 0; t < ns; ++t) {
-        if (front_class(sn.rows[t]) > 2)
+        if (front_class(sn.rows[t]) > 3)
           return fail(CELLS_EINVAL, "a front exceeds the workgroup's LDS (order above 127)");
         int32_t r8[8] = {sn.first[t], sn.width[t], sn.rows[t], (int32_t)sn.update_offset[t],
                          sn.child_ptr[t], sn.child_ptr[t + 1], sn.rel_ptr[t], 0};
@@ -2191,14 +2222,14 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       d.sn_levels = sn.levels;
       d.o_snlev = (long)pool.size();
       {
-        std::vector<int> bounds(4 * (size_t)sn.levels, 0);
+        std::vector<int> bounds(5 * (size_t)sn.levels, 0);
         int t = 0;
         for (int L = 0; L < sn.levels; ++L) {
-          for (int c = 0; c < 3; ++c) {
-            bounds[4 * L + c] = t;
+          for (int c = 0; c < 4; ++c) {
+            bounds[5 * L + c] = t;
             while (t < ns && sn.level[t] == L && front_class(sn.rows[t]) == c) ++t;
           }
-          bounds[4 * L + 3] = t;
+          bounds[5 * L + 4] = t;
           if (t < ns && sn.level[t] == L) return fail(CELLS_EINVAL, "front schedule out of order");
         }
         pool.insert(pool.end(), bounds.begin(), bounds.end());
