@@ -115,9 +115,10 @@ struct ShapeDev {
   long o_cs, o_entry, o_rowptr, o_rowent, o_lev, o_levcol, o_levent, o_node;
   long o_mask;  // offset in the mask pool (unsigned long long words)
   int lds_pt;   // first factor position held in LDS during the factor (nnz: none)
-  // multifrontal schedule: supernode records (8 ints each: first column, width, front order,
-  // update offset, children begin/end, own rel list start, pad), per level 5 boundaries (thread,
-  // 8-lane, wavefront and workgroup fronts, end), children, rel lists
+  // multifrontal schedule: supernode records (8 ints each: factor position of the first column,
+  // width, front order, update offset, children begin/end, pad), per level 5 boundaries (thread,
+  // 8-lane, wavefront and workgroup fronts, end), children (4 ints each: rows below, rel list
+  // start, update offset, pad), rel lists
   long o_sn, o_snlev, o_child, o_rel;
   int sn_levels;
   long usz;  // doubles of update matrices per world
@@ -1008,24 +1009,78 @@ __device__ __forceinline__ void group_sync() {
 // Packed lower column-major triangle of order m: entry (i, j), i >= j, at colbase(j, m) + i.
 __device__ __forceinline__ int colbase(int j, int m) { return j * m - (j * (j + 1)) / 2; }
 
-// Steps (b, off) -> the next entry G further on in a packed lower triangle of order T
-// (column b, row b + off).
-__device__ __forceinline__ void tri_advance(int T, int& b, int& off) {
-  while (b < T && off >= T - b) {
-    off -= T - b;
-    ++b;
+// Entry e of a packed lower column-major triangle of order T: column b, row a (a >= b). Column b
+// starts at b T - b (b - 1) / 2; the root of that quadratic, then one correction either way.
+__device__ __forceinline__ void tri_locate(int T, int e, int& b, int& a) {
+  const float h = (float)(2 * T + 1);
+  b = (int)((h - sqrtf(h * h - 8.0f * (float)e)) * 0.5f);
+  b = b < 0 ? 0 : (b > T - 1 ? T - 1 : b);
+  if (b + 1 < T && (b + 1) * T - (b + 1) * b / 2 <= e) ++b;
+  if (b * T - b * (b - 1) / 2 > e) --b;
+  a = e - (b * T - b * (b - 1) / 2) + b;
+}
+
+// Lane layout of a group of G lanes over a front's rows and columns: RW lanes down a column (row
+// lr + t RW, t < NT, covering order <= NT RW) and G / RW column groups.
+template <int G>
+struct FrontLanes {
+  static constexpr int RW = G < 64 ? G : 64;
+  static constexpr int CG = G / RW;
+  static constexpr int NT = G == 1 ? 7 : G == kMfOcta ? 3 : G == 64 ? 1 : 2;  // rows per lane
+};
+
+constexpr int kMfKB = 8;  // columns per block of a wavefront's or workgroup's partial factor
+
+typedef double mf_v4d __attribute__((ext_vector_type(4)));
+
+// Front columns k >= je, rows i >= k: F(i, k) -= sum_{jb <= l < je} F(i, l) F(k, l), on 16 x 16
+// tiles of v_mfma_f64_16x16x4f64 (lane l: A(l % 16, l / 16), B(l / 16, l % 16), result register r
+// at (l / 16 + 4 r, l % 16)); the group's wavefronts share the tiles. F packed lower, stride 1.
+template <int G>
+__device__ void mf_trailing_update(double* F, int m, int jb, int je, int lane) {
+  const int l = lane & 63, wv = lane >> 6;
+  const int lrow = l & 15, lq = l >> 4;
+  const int nt = (m - je + 15) >> 4;
+  int tile = 0;
+  for (int ti = 0; ti < nt; ++ti) {
+    for (int tk = 0; tk <= ti; ++tk, ++tile) {
+      if (tile % (G / 64) != wv) continue;
+      const int i0 = je + 16 * ti, k0 = je + 16 * tk;
+      mf_v4d c;
+      const int kc = k0 + lrow;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + lq + 4 * r;
+        c[r] = (i < m && kc < m && i >= kc) ? F[colbase(kc, m) + i] : 0.0;
+      }
+      for (int l0 = jb; l0 < je; l0 += 4) {
+        const int col = l0 + lq;
+        const int ia = i0 + lrow, kb = k0 + lrow;
+        const double a = (col < je && ia < m) ? -F[colbase(col, m) + ia] : 0.0;
+        const double b = (col < je && kb < m) ? F[colbase(col, m) + kb] : 0.0;
+        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + lq + 4 * r;
+        if (i < m && kc < m && i >= kc) F[colbase(kc, m) + i] = c[r];
+      }
+    }
   }
 }
 
-// One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE]. Global
-// loads are issued kMfBatch per lane before their values are used.
+// One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE].
 template <int G, int STRIDE>
-__device__ void mf_front(const int* rec, int q, const int* kids, const int* rel, const int* cs,
-                         double* fv, double* ub, double* F, int lane) {
-  const int* r = rec + 8 * q;
-  const int j0 = r[0], J = r[1], m = r[2], uoff = r[3], c0 = r[4], c1 = r[5];
+__device__ void mf_front(const int* rec, int q, const int* kids, const int* rel, double* fv, double* ub,
+                         double* F, int lane) {
+  using FL = FrontLanes<G>;
+  constexpr int RW = FL::RW, CG = FL::CG, NT = FL::NT;
+  const int lr = lane % RW, lc = lane / RW;
+  const int4 ra4 = reinterpret_cast<const int4*>(rec)[2 * q];
+  const int4 rb4 = reinterpret_cast<const int4*>(rec)[2 * q + 1];
+  const int J = ra4.y, m = ra4.z, uoff = ra4.w, c0 = rb4.x, c1 = rb4.y;
+  const long p0 = ra4.x;
   const int S = m - J, tri = m * (m + 1) / 2, Pn = colbase(J, m) + J;
-  const long p0 = cs[j0];
   // the supernode's columns are contiguous in the factor and are the front's first J columns;
   // the update block starts at zero
   for (int e0 = lane; e0 < tri; e0 += kMfBatch * G) {
@@ -1042,24 +1097,26 @@ __device__ void mf_front(const int* rec, int q, const int* kids, const int* rel,
     }
   }
   group_sync<G>();
+  // extend-add: child entry (a, b), a >= b, of the packed Sc x Sc update goes to front entry
+  // (rel[a], rel[b]); lanes stride over the packed entries (coalesced), kMfBatch loads in flight
   for (int x = c0; x < c1; ++x) {
-    const int* rc = rec + 8 * kids[x];
-    const int Sc = rc[2] - rc[1];
-    const int* rl = rel + rc[6];
-    const double* U = ub + rc[3];
-    int b = 0, off = lane;
-    tri_advance(Sc, b, off);
-    for (int e0 = lane; b < Sc; e0 += kMfBatch * G) {
+    const int4 kd = reinterpret_cast<const int4*>(kids)[x];  // (Sc, rel start, update offset, -)
+    const int Sc = kd.x;
+    const int* rl = rel + kd.y;
+    const double* U = ub + kd.z;
+    const int nU = Sc * (Sc + 1) / 2;
+    for (int e0 = lane; e0 < nU; e0 += kMfBatch * G) {
       int ta[kMfBatch];
       double uv[kMfBatch];
 #pragma unroll
       for (int u = 0; u < kMfBatch; ++u) {
-        const bool in = b < Sc;
-        const int ra = in ? rl[b + off] : 0, rb = in ? rl[b] : 0;
-        uv[u] = in ? U[e0 + u * G] : 0.0;
+        const int e = e0 + u * G;
+        const bool in = e < nU;
+        int b, a;
+        tri_locate(Sc, in ? e : 0, b, a);
+        const int ra = in ? rl[a] : 0, rb = in ? rl[b] : 0;
+        uv[u] = in ? U[e] : 0.0;
         ta[u] = in ? colbase(rb, m) + ra : -1;
-        off += G;
-        tri_advance(Sc, b, off);
       }
 #pragma unroll
       for (int u = 0; u < kMfBatch; ++u)
@@ -1067,23 +1124,51 @@ __device__ void mf_front(const int* rec, int q, const int* kids, const int* rel,
     }
     group_sync<G>();
   }
-  for (int jj = 0; jj < J; ++jj) {
-    const int cb = colbase(jj, m);
-    const double d = sqrt(F[(cb + jj) * STRIDE]);
-    const double rinv = 1.0 / d;
-    for (int i = jj + 1 + lane; i < m; i += G) F[(cb + i) * STRIDE] *= rinv;
-    group_sync<G>();
-    if (lane == 0) F[(cb + jj) * STRIDE] = d;
-    const int T = m - jj - 1;
-    int b = 0, off = lane;
-    tri_advance(T, b, off);
-    while (b < T) {
-      const int k = jj + 1 + b, i = k + off;
-      F[(colbase(k, m) + i) * STRIDE] -= F[(cb + i) * STRIDE] * F[(cb + k) * STRIDE];
-      off += G;
-      tri_advance(T, b, off);
+  // partial factor over the supernode's columns: lanes own rows i = jj + 1 + lr + t RW. Groups of
+  // whole wavefronts work in blocks of kMfKB columns: rank-1 steps inside the block, then one
+  // rank-kMfKB update of everything to its right on the matrix cores.
+  constexpr int KB = G >= 64 ? kMfKB : 1 << 20;
+  for (int jb = 0; jb < J; jb += KB) {
+    const int je = J < jb + KB ? J : jb + KB;
+    const int kend = G >= 64 ? je : m;  // columns the rank-1 steps update
+    for (int jj = jb; jj < je; ++jj) {
+      const int cb = colbase(jj, m);
+      const double d = sqrt(F[(cb + jj) * STRIDE]);
+      const double rinv = 1.0 / d;
+      double av[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int i = jj + 1 + lr + t * RW;
+        av[t] = i < m ? F[(cb + i) * STRIDE] * rinv : 0.0;
+      }
+      group_sync<G>();  // every lane has read the pivot and its rows
+      if (lc == 0) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int i = jj + 1 + lr + t * RW;
+          if (i < m) F[(cb + i) * STRIDE] = av[t];
+        }
+      }
+      if (lane == 0) F[(cb + jj) * STRIDE] = d;
+      group_sync<G>();  // the scaled column is in place
+#pragma unroll 4
+      for (int k = jj + 1 + lc; k < kend; k += CG) {
+        const double bk = F[(cb + k) * STRIDE];
+        const int ck = colbase(k, m);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int i = jj + 1 + lr + t * RW;
+          if (i >= k && i < m) F[(ck + i) * STRIDE] -= av[t] * bk;
+        }
+      }
+      group_sync<G>();
     }
-    group_sync<G>();
+    if constexpr (G >= 64) {
+      if (je < m) {
+        mf_trailing_update<G>(F, m, jb, je, lane);
+        group_sync<G>();
+      }
+    }
   }
   for (int e = lane; e < Pn; e += G) fv[p0 + e] = F[e * STRIDE];
   if (S > 0)
@@ -1091,8 +1176,7 @@ __device__ void mf_front(const int* rec, int q, const int* kids, const int* rel,
   group_sync<G>();
 }
 
-__device__ void mf_factor_world(const ShapeDev& s, const int* pool, const int* cs, double* fv, double* ub,
-                                double* lds) {
+__device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub, double* lds) {
   const int* rec = pool + s.o_sn;
   const int* lev = pool + s.o_snlev;
   const int* kids = pool + s.o_child;
@@ -1101,15 +1185,15 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, const int* c
   for (int L = 0; L < s.sn_levels; ++L) {
     const int* bd = lev + 5 * L;  // thread, 8-lane, wavefront and workgroup fronts of level L
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
-      mf_front<1, kMfThreads>(rec, q, kids, rel, cs, fv, ub, lds + tid, 0);
+      mf_front<1, kMfThreads>(rec, q, kids, rel, fv, ub, lds + tid, 0);
     __syncthreads();
     for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      mf_front<kMfOcta, 1>(rec, q, kids, rel, cs, fv, ub, lds + octa * kMfOctaTri, tid % kMfOcta);
+      mf_front<kMfOcta, 1>(rec, q, kids, rel, fv, ub, lds + octa * kMfOctaTri, tid % kMfOcta);
     __syncthreads();
     for (int q = bd[2] + wave; q < bd[3]; q += kMfThreads / 64)
-      mf_front<64, 1>(rec, q, kids, rel, cs, fv, ub, lds + wave * kMfWaveTri, tid & 63);
+      mf_front<64, 1>(rec, q, kids, rel, fv, ub, lds + wave * kMfWaveTri, tid & 63);
     __syncthreads();
-    for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(rec, q, kids, rel, cs, fv, ub, lds, tid);
+    for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(rec, q, kids, rel, fv, ub, lds, tid);
   }
   __syncthreads();
 }
@@ -1135,7 +1219,7 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeD
     fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
   }
   __syncthreads();
-  mf_factor_world(s, pool, v.cs, fv, ubuf + uoffw[w], lds);
+  mf_factor_world(s, pool, fv, ubuf + uoffw[w], lds);
   bool ok = true;
   for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) ok = ok && isfinite(fv[p]);
   if (!ok) valid[w] = 0;
@@ -1149,7 +1233,7 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_only_mf(const ShapeDev* s
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
-  mf_factor_world(s, pool, v.cs, fvals + foff[w], ubuf + uoffw[w], lds);
+  mf_factor_world(s, pool, fvals + foff[w], ubuf + uoffw[w], lds);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2211,12 +2295,13 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       const cells::Supernodes& sn = e->sn_host[q];
       const int ns = (int)sn.first.size();
       if (sn.update_total > 0x7fffffff) return fail(CELLS_EINVAL, "update matrices exceed 2^31 doubles");
+      while (pool.size() % 4) pool.push_back(0);  // 16-byte aligned records
       d.o_sn = (long)pool.size();
       for (int t = 0; t < ns; ++t) {
         if (front_class(sn.rows[t]) > 3)
           return fail(CELLS_EINVAL, "a front exceeds the workgroup's LDS (order above 127)");
-        int32_t r8[8] = {sn.first[t], sn.width[t], sn.rows[t], (int32_t)sn.update_offset[t],
-                         sn.child_ptr[t], sn.child_ptr[t + 1], sn.rel_ptr[t], 0};
+        int32_t r8[8] = {(int32_t)s.column_start[sn.first[t]], sn.width[t], sn.rows[t],
+                         (int32_t)sn.update_offset[t], sn.child_ptr[t], sn.child_ptr[t + 1], 0, 0};
         pool.insert(pool.end(), r8, r8 + 8);
       }
       d.sn_levels = sn.levels;
@@ -2234,8 +2319,12 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
         }
         pool.insert(pool.end(), bounds.begin(), bounds.end());
       }
+      while (pool.size() % 4) pool.push_back(0);  // 16-byte aligned child descriptors
       d.o_child = (long)pool.size();
-      pool.insert(pool.end(), sn.child.begin(), sn.child.end());
+      for (int32_t c : sn.child) {
+        int32_t c4[4] = {sn.rows[c] - sn.width[c], sn.rel_ptr[c], (int32_t)sn.update_offset[c], 0};
+        pool.insert(pool.end(), c4, c4 + 4);
+      }
       d.o_rel = (long)pool.size();
       pool.insert(pool.end(), sn.rel.begin(), sn.rel.end());
       d.usz = sn.update_total;

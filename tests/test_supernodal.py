@@ -123,3 +123,25 @@ def test_every_supernode_fits_a_workgroup_front_at_the_batched_sizes():
         s = sy.bind(n, list(snd), list(rcv))
         worst = max(b - a + len(s.structure[b - 1]) for a, b in supernodes(s))
         assert worst <= 127, (n, worst)
+
+
+def tri_locate(T, e):
+    """The device's decode of entry e of a packed lower column-major triangle of order T (float32
+    root of the column-start quadratic, then one correction either way)."""
+    h = np.float32(2 * T + 1)
+    b = int((h - np.sqrt(h * h - np.float32(8) * np.float32(e))) * np.float32(0.5))
+    b = min(max(b, 0), T - 1)
+    if b + 1 < T and (b + 1) * T - (b + 1) * b // 2 <= e:
+        b += 1
+    if b * T - b * (b - 1) // 2 > e:
+        b -= 1
+    return b, e - (b * T - b * (b - 1) // 2) + b
+
+
+def test_packed_triangle_decode_is_exact_up_to_order_200():
+    for T in range(1, 201):
+        e = 0
+        for b in range(T):
+            for a in range(b, T):
+                assert tri_locate(T, e) == (b, a), (T, e)
+                e += 1
