@@ -116,13 +116,23 @@ struct ShapeDev {
   long o_mask;  // offset in the mask pool (unsigned long long words)
   int lds_pt;   // first factor position held in LDS during the factor (nnz: none)
   // multifrontal schedule: supernode records (8 ints each: factor position of the first column,
-  // width, front order, update offset, children begin/end, pad), per level 5 boundaries (thread,
-  // 8-lane, wavefront and workgroup fronts, end), children (4 ints each: rows below, rel list
-  // start, update offset, pad), rel lists
+  // width, front order, update offset, children begin/end, update-vector offset, first column),
+  // per level 5 boundaries (thread, 8-lane, wavefront and workgroup fronts, end), children (4
+  // ints each: rows below, rel list start, update offset, update-vector offset), rel lists
   long o_sn, o_snlev, o_child, o_rel;
   int sn_levels;
   long usz;  // doubles of update matrices per world
+  long vsz;  // doubles of the solve's update vectors per world
 };
+
+// Multifrontal factor and supernodal solve: one workgroup of kMfThreads per world.
+constexpr int kMfThreads = 256;
+constexpr int kMfLds = 8192;                               // doubles of LDS per workgroup (64 KiB)
+constexpr int kMfOcta = 8;                                 // lanes of a small group
+constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 7
+constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 22
+constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 63
+constexpr int kMfBatch = 8;                                // global loads in flight per lane
 
 __device__ __forceinline__ double& CF(const Params& P, double* y, int f, int i) {
   return y[(long)f * P.ncell + i];
@@ -904,14 +914,32 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor(Params P, const ShapeDe
   if (!ok) valid[w] = 0;
 }
 
+// Which solve runs: the supernodal one (with the multifrontal factor) or the level-scheduled one
+// in the reference's summation order.
+struct SolveArgs {
+  int sn;
+  const long* voffw;  // per world: start of its update vectors in vbuf
+  double* vbuf;
+};
+
+__device__ void sv_solve_world(const ShapeDev& s, const int* pool, const double* fv, double* y, double* vec,
+                               double* lds);
+
+__device__ __forceinline__ void solve_any(const ShapeDev& s, const PlanView& v, const int* pool,
+                                          const double* fv, double* y, const SolveArgs& sa, int w,
+                                          double* lds) {
+  if (sa.sn) sv_solve_world(s, pool, fv, y, sa.vbuf + sa.voffw[w], lds);
+  else solve_world(s, v, fv, y);
+}
+
 // Solve (C + coef G) dphi = q(K) on each world's factor: rhs' = q s, dphi = x s.
-__global__ __launch_bounds__(kFactorBlock) void k_solve_stage(Params P, const ShapeDev* shapes,
-                                                              const int* shape_of, const int* pool,
-                                                              const long* foff, int w0,
-                                                              const double* msc,
-                                                              const double* fvals, const double* K,
-                                                              const double* Kminus2, double* dphi,
-                                                              double* ysc, int y_in_lds) {
+__global__ __launch_bounds__(kMfThreads) void k_solve_stage(Params P, const ShapeDev* shapes,
+                                                            const int* shape_of, const int* pool,
+                                                            const long* foff, int w0,
+                                                            const double* msc,
+                                                            const double* fvals, const double* K,
+                                                            const double* Kminus2, double* dphi,
+                                                            double* ysc, int y_in_lds, SolveArgs sa) {
   extern __shared__ double lds[];
   int w = w0 + blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
@@ -934,7 +962,8 @@ __global__ __launch_bounds__(kFactorBlock) void k_solve_stage(Params P, const Sh
     y[k] = q * msc[i];
   }
   __syncthreads();
-  solve_world(s, v, fv, y);
+  solve_any(s, v, pool, fv, y, sa, w, lds);
+  __syncthreads();
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
     int i = c0 + v.order[k];
     dphi[i] = y[k] * msc[i];
@@ -957,11 +986,11 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_only(const ShapeDev* sh
   for (int p = s.lds_pt + threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = lds[p - s.lds_pt];
 }
 
-__global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* shapes, const int* shape_of,
-                                                       const int* pool, const long* foff,
-                                                       const int* cell_off, const double* fvals,
-                                                       const double* rhs, double* x, double* ysc,
-                                                       int y_in_lds) {
+__global__ __launch_bounds__(kMfThreads) void k_solve_only(const ShapeDev* shapes, const int* shape_of,
+                                                           const int* pool, const long* foff,
+                                                           const int* cell_off, const double* fvals,
+                                                           const double* rhs, double* x, double* ysc,
+                                                           int y_in_lds, SolveArgs sa) {
   extern __shared__ double shared_y[];
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
@@ -970,7 +999,8 @@ __global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* sha
   double* lds = y_in_lds ? shared_y : ysc + c0;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) lds[k] = rhs[c0 + v.order[k]];
   __syncthreads();
-  solve_world(s, v, fvals + foff[w], lds);
+  solve_any(s, v, pool, fvals + foff[w], lds, sa, w, shared_y);
+  __syncthreads();
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) x[c0 + v.order[k]] = lds[k];
 }
 
@@ -985,13 +1015,6 @@ __global__ __launch_bounds__(kFactorBlock) void k_solve_only(const ShapeDev* sha
 // triangle becomes the supernode's update matrix. One workgroup per world; a front is held by one
 // thread, one wavefront or the whole workgroup according to its size.
 
-constexpr int kMfThreads = 256;
-constexpr int kMfLds = 8192;                               // doubles of LDS per workgroup (64 KiB)
-constexpr int kMfOcta = 8;                                 // lanes of a small group
-constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 7
-constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 22
-constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 63
-constexpr int kMfBatch = 8;                                // global loads in flight per lane
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1196,6 +1219,168 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, 
     for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(rec, q, kids, rel, fv, ub, lds, tid);
   }
   __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// Supernodal solve (the default, with the multifrontal factor): y (by rank) <- L^-T L^-1 y over
+// the same fronts. Forward, leaves first: a front's vector is its columns' right-hand sides plus
+// its children's update vectors (extend-add), its pivots run in registers (each pivot applied as
+// a multiplication by its reciprocal, as the device form of the reference's solve does,
+// numerics/sparse_cholesky.py:19-31), the solved values go to y and the rows below become the
+// update vector. Backward, roots first: the values below a front are its ancestors' solution
+// (gathered from y), one product with the front's rows below, then the pivots last to first.
+
+// Rows per lane of a solve group: one thread (order <= 7), 8 lanes (<= 24), a wavefront (<= 128).
+template <int G>
+struct SolveLanes {
+  static constexpr int NT = G == 1 ? 7 : G == kMfOcta ? 3 : 2;
+};
+// LDS of the solve's workgroup: separate regions for the thread, 8-lane and wavefront fronts (the
+// three kinds of one level run without a barrier between them)
+constexpr int kSvThreadLds = kMfThreads * 7;
+constexpr int kSvOctaLds = (kMfThreads / kMfOcta) * kMfOcta * 3;
+constexpr int kSvWaveLds = (kMfThreads / 64) * 64 * 2;
+constexpr int kSvLds = kSvThreadLds + kSvOctaLds + kSvWaveLds;
+
+template <int RW>
+__device__ __forceinline__ double group_shfl(double v, int src) {
+  if constexpr (RW == 1) return v;
+  else return __shfl(v, src, RW);
+}
+
+template <int NT>
+__device__ __forceinline__ double pick(const double (&a)[NT], int idx) {
+  double r = a[0];
+#pragma unroll
+  for (int t = 1; t < NT; ++t)
+    if (idx == t) r = a[t];
+  return r;
+}
+
+template <int G, int STRIDE>
+__device__ void sv_forward_front(const int* rec, int q, const int* kids, const int* rel,
+                                 const double* __restrict__ fv, double* __restrict__ y,
+                                 double* __restrict__ vec, double* Fv, int lane) {
+  constexpr int NT = SolveLanes<G>::NT;
+  const int4 ra4 = reinterpret_cast<const int4*>(rec)[2 * q];
+  const int4 rb4 = reinterpret_cast<const int4*>(rec)[2 * q + 1];
+  const long p0 = ra4.x;
+  const int J = ra4.y, m = ra4.z, c0 = rb4.x, c1 = rb4.y, voff = rb4.z, j0 = rb4.w;
+  for (int i = lane; i < m; i += G) Fv[i * STRIDE] = i < J ? y[j0 + i] : 0.0;
+  group_sync<G>();
+  for (int x = c0; x < c1; ++x) {
+    const int4 kd = reinterpret_cast<const int4*>(kids)[x];  // (Sc, rel start, -, vector offset)
+    const int* rl = rel + kd.y;
+    const double* u = vec + kd.w;
+    for (int a = lane; a < kd.x; a += G) Fv[rl[a] * STRIDE] += u[a];
+    group_sync<G>();
+  }
+  double fr[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int i = lane + t * G;
+    fr[t] = i < m ? Fv[i * STRIDE] : 0.0;
+  }
+#pragma unroll 4
+  for (int jj = 0; jj < J; ++jj) {
+    const long cbp = p0 + colbase(jj, m);
+    const double d = fv[cbp + jj];
+    double lv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int i = lane + t * G;
+      lv[t] = (i > jj && i < m) ? fv[cbp + i] : 0.0;
+    }
+    const double xj = group_shfl<G>(pick(fr, jj / G), jj % G) * (1.0 / d);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int i = lane + t * G;
+      if (i > jj && i < m) fr[t] -= lv[t] * xj;
+    }
+    if (lane == jj % G) y[j0 + jj] = xj;
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int i = lane + t * G;
+    if (i >= J && i < m) vec[voff + i - J] = fr[t];
+  }
+  group_sync<G>();
+}
+
+template <int G, int STRIDE>
+__device__ void sv_backward_front(const int* rec, int q, const int* rrank, const double* __restrict__ fv,
+                                  double* __restrict__ y, double* Fv, int lane) {
+  constexpr int NT = SolveLanes<G>::NT;
+  const int4 ra4 = reinterpret_cast<const int4*>(rec)[2 * q];
+  const int4 rb4 = reinterpret_cast<const int4*>(rec)[2 * q + 1];
+  const long p0 = ra4.x;
+  const int J = ra4.y, m = ra4.z, j0 = rb4.w;
+  // the solution at the rows below (the last column's rows after its diagonal)
+  const long pl = p0 + colbase(J - 1, m) + J;
+  for (int a = lane; a < m - J; a += G) Fv[(J + a) * STRIDE] = y[rrank[pl + a]];
+  group_sync<G>();
+  double xr[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int k = lane + t * G;
+    double acc = 0.0;
+    if (k < J) {
+      const long ck = p0 + colbase(k, m);
+      acc = y[j0 + k];
+      for (int i = J; i < m; ++i) acc -= fv[ck + i] * Fv[i * STRIDE];
+    }
+    xr[t] = acc;
+  }
+  for (int jj = J - 1; jj >= 0; --jj) {
+    const double d = fv[p0 + colbase(jj, m) + jj];
+    double lv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int k = lane + t * G;
+      lv[t] = k < jj ? fv[p0 + colbase(k, m) + jj] : 0.0;
+    }
+    const double zj = group_shfl<G>(pick(xr, jj / G), jj % G) * (1.0 / d);
+    if (lane == jj % G) y[j0 + jj] = zj;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int k = lane + t * G;
+      if (k < jj) xr[t] -= lv[t] * zj;
+    }
+  }
+  group_sync<G>();
+}
+
+// The solve of one world's y (by rank) on its supernodal factor; blockDim.x == kMfThreads.
+__device__ void sv_solve_world(const ShapeDev& s, const int* pool, const double* fv, double* y, double* vec,
+                               double* lds) {
+  const int* rec = pool + s.o_sn;
+  const int* lev = pool + s.o_snlev;
+  const int* kids = pool + s.o_child;
+  const int* rel = pool + s.o_rel;
+  const int* rrank = pool + s.o_entry;
+  const int tid = threadIdx.x, wave = tid >> 6, octa = tid / kMfOcta;
+  double* Fq = lds + kSvThreadLds + octa * (kMfOcta * SolveLanes<kMfOcta>::NT);
+  double* Fw = lds + kSvThreadLds + kSvOctaLds + wave * (64 * SolveLanes<64>::NT);
+  for (int L = 0; L < s.sn_levels; ++L) {
+    const int* bd = lev + 5 * L;
+    for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
+      sv_forward_front<1, kMfThreads>(rec, q, kids, rel, fv, y, vec, lds + tid, 0);
+    for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
+      sv_forward_front<kMfOcta, 1>(rec, q, kids, rel, fv, y, vec, Fq, tid % kMfOcta);
+    for (int q = bd[2] + wave; q < bd[4]; q += kMfThreads / 64)
+      sv_forward_front<64, 1>(rec, q, kids, rel, fv, y, vec, Fw, tid & 63);
+    __syncthreads();
+  }
+  for (int L = s.sn_levels - 1; L >= 0; --L) {
+    const int* bd = lev + 5 * L;
+    for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
+      sv_backward_front<1, kMfThreads>(rec, q, rrank, fv, y, lds + tid, 0);
+    for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
+      sv_backward_front<kMfOcta, 1>(rec, q, rrank, fv, y, Fq, tid % kMfOcta);
+    for (int q = bd[2] + wave; q < bd[4]; q += kMfThreads / 64)
+      sv_backward_front<64, 1>(rec, q, rrank, fv, y, Fw, tid & 63);
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeDev* shapes,
@@ -1739,13 +1924,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_dtrans(Params P, double coef, co
 
 // w = M^-1 dbar on the step's factor (workgroup per world); then rate_bar += q^T w and the
 // slopes' cotangent of M: -coef w delta^T through G = F' sum_s z_s slope_s.
-__global__ __launch_bounds__(kFactorBlock) void k_adj_solve(Params P, const ShapeDev* shapes,
-                                                      const int* shape_of, const int* pool,
-                                                      const long* foff, double coef,
-                                                      const double* msc, const double* fvals,
-                                                      const double* dbar, const double* delta,
-                                                      double* wv, double* rate_bar, double* gmbar,
-                                                      double* gjbar, double* ysc, int y_in_lds) {
+__global__ __launch_bounds__(kMfThreads) void k_adj_solve(Params P, const ShapeDev* shapes,
+                                                          const int* shape_of, const int* pool,
+                                                          const long* foff, double coef,
+                                                          const double* msc, const double* fvals,
+                                                          const double* dbar, const double* delta,
+                                                          double* wv, double* rate_bar, double* gmbar,
+                                                          double* gjbar, double* ysc, int y_in_lds,
+                                                          SolveArgs sa) {
   extern __shared__ double shared_y[];
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
@@ -1758,7 +1944,8 @@ __global__ __launch_bounds__(kFactorBlock) void k_adj_solve(Params P, const Shap
     lds[k] = dbar[i] * msc[i];
   }
   __syncthreads();
-  solve_world(s, v, fv, lds);
+  solve_any(s, v, pool, fv, lds, sa, w, shared_y);
+  __syncthreads();
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
     int i = c0 + v.order[k];
     wv[i] = lds[k] * msc[i];
@@ -1843,6 +2030,8 @@ struct cells_ensemble {
   bool pairwise = false;  // factor in the reference's summation order (slot masks), else multifrontal
   long* uoffw = nullptr;  // per world: start of its update matrices in ubuf
   double* ubuf = nullptr;
+  long* voffw = nullptr;  // per world: start of the solve's update vectors in vbuf
+  double* vbuf = nullptr;
   std::vector<cells::Supernodes> sn_host;
   size_t factor_lds = 0, solve_lds = 0;  // dynamic LDS bytes of the factor and solve kernels
   int y_in_lds = 1;
@@ -2295,13 +2484,18 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       const cells::Supernodes& sn = e->sn_host[q];
       const int ns = (int)sn.first.size();
       if (sn.update_total > 0x7fffffff) return fail(CELLS_EINVAL, "update matrices exceed 2^31 doubles");
+      // the solve's update vectors: |S| doubles per supernode
+      std::vector<int64_t> voff(ns + 1, 0);
+      for (int t = 0; t < ns; ++t) voff[t + 1] = voff[t] + (sn.rows[t] - sn.width[t]);
+      d.vsz = voff[ns];
       while (pool.size() % 4) pool.push_back(0);  // 16-byte aligned records
       d.o_sn = (long)pool.size();
       for (int t = 0; t < ns; ++t) {
         if (front_class(sn.rows[t]) > 3)
           return fail(CELLS_EINVAL, "a front exceeds the workgroup's LDS (order above 127)");
         int32_t r8[8] = {(int32_t)s.column_start[sn.first[t]], sn.width[t], sn.rows[t],
-                         (int32_t)sn.update_offset[t], sn.child_ptr[t], sn.child_ptr[t + 1], 0, 0};
+                         (int32_t)sn.update_offset[t], sn.child_ptr[t], sn.child_ptr[t + 1],
+                         (int32_t)voff[t], sn.first[t]};
         pool.insert(pool.end(), r8, r8 + 8);
       }
       d.sn_levels = sn.levels;
@@ -2322,7 +2516,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       while (pool.size() % 4) pool.push_back(0);  // 16-byte aligned child descriptors
       d.o_child = (long)pool.size();
       for (int32_t c : sn.child) {
-        int32_t c4[4] = {sn.rows[c] - sn.width[c], sn.rel_ptr[c], (int32_t)sn.update_offset[c], 0};
+        int32_t c4[4] = {sn.rows[c] - sn.width[c], sn.rel_ptr[c], (int32_t)sn.update_offset[c],
+                         (int32_t)voff[c]};
         pool.insert(pool.end(), c4, c4 + 4);
       }
       d.o_rel = (long)pool.size();
@@ -2337,15 +2532,18 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       int64_t most = 0;
       for (const ShapeDev& d : sd) most = std::max<int64_t>(most, d.nnz - d.lds_pt);
       e->factor_lds = sizeof(double) * (size_t)std::max<int64_t>(most, 1);
-      e->y_in_lds = (size_t)e->max_n * sizeof(double) <= kSolveLdsBytes ? 1 : 0;
+      e->y_in_lds = e->pairwise && (size_t)e->max_n * sizeof(double) <= kSolveLdsBytes ? 1 : 0;
       e->solve_lds = e->y_in_lds ? sizeof(double) * (size_t)std::max(e->max_n, 1) : 0;
     }
     e->foff_host.resize(e->W + 1);
     e->foff_host[0] = 0;
     for (int k = 0; k < e->W; ++k) e->foff_host[k + 1] = e->foff_host[k] + sd[e->shape_of_host[k]].nnz;
     e->factor_total = e->foff_host[e->W];
-    std::vector<long> uoffw(e->W + 1, 0);
-    for (int k = 0; k < e->W; ++k) uoffw[k + 1] = uoffw[k] + sd[e->shape_of_host[k]].usz;
+    std::vector<long> uoffw(e->W + 1, 0), voffw(e->W + 1, 0);
+    for (int k = 0; k < e->W; ++k) {
+      uoffw[k + 1] = uoffw[k] + sd[e->shape_of_host[k]].usz;
+      voffw[k + 1] = voffw[k] + sd[e->shape_of_host[k]].vsz;
+    }
     if ((rc = upload(e.get(), &e->shapes, sd.data(), sd.size())) ||
         (rc = upload(e.get(), &e->shape_of, e->shape_of_host.data(), e->shape_of_host.size())) ||
         (rc = upload(e.get(), &e->pool, pool.data(), pool.size())) ||
@@ -2353,7 +2551,9 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
         (rc = upload(e.get(), &e->foff, e->foff_host.data(), e->foff_host.size())) ||
         (rc = alloc(e.get(), &e->fvals, std::max<long>(e->factor_total, 1))) ||
         (rc = upload(e.get(), &e->uoffw, uoffw.data(), uoffw.size())) ||
-        (rc = alloc(e.get(), &e->ubuf, std::max<long>(uoffw[e->W], 1))))
+        (rc = alloc(e.get(), &e->ubuf, std::max<long>(uoffw[e->W], 1))) ||
+        (rc = upload(e.get(), &e->voffw, voffw.data(), voffw.size())) ||
+        (rc = alloc(e.get(), &e->vbuf, std::max<long>(voffw[e->W], 1))))
       return rc;
   }
   HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
@@ -2405,6 +2605,13 @@ struct KernelTimer {
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
   }
 };
+
+// The solve kernels' workgroup, LDS and mode.
+dim3 solve_block(const cells_ensemble* e) { return dim3(e->pairwise ? kFactorBlock : kMfThreads); }
+size_t solve_lds_bytes(const cells_ensemble* e) {
+  return e->pairwise ? e->solve_lds : sizeof(double) * (size_t)kSvLds;
+}
+SolveArgs solve_args(const cells_ensemble* e) { return SolveArgs{e->pairwise ? 0 : 1, e->voffw, e->vbuf}; }
 
 // Assemble and factor the stage matrix C + coef G of worlds [w0, w0 + nw).
 void launch_factor(cells_ensemble* e, hipStream_t st, int w0, int nw, double coef) {
@@ -2462,8 +2669,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     mark("k_cells");
     launch_factor(e, st, w0, nw, coef);
     mark("k_factor");
-    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds);
+    hipLaunchKernelGGL(k_solve_stage, dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds, solve_args(e));
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->contrib1, e->Y1);
@@ -2476,8 +2683,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, c0, c1,
                        c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
     mark("k_cells");
-    hipLaunchKernelGGL(k_solve_stage, dim3(nw), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds);
+    hipLaunchKernelGGL(k_solve_stage, dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds, solve_args(e));
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi2, e->K1, e->K2, e->contrib1, e->contrib2, e->valid);
@@ -2622,8 +2829,8 @@ extern "C" int cells_solve(cells_ensemble* e, void* stream, const double* values
   if (!e || !values || !rhs || !x) return fail(CELLS_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(e->device));
   size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
-  hipLaunchKernelGGL(k_solve_only, dim3(e->W), dim3(kFactorBlock), e->solve_lds, static_cast<hipStream_t>(stream), e->shapes,
-                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x, e->ysc, e->y_in_lds);
+  hipLaunchKernelGGL(k_solve_only, dim3(e->W), solve_block(e), solve_lds_bytes(e), static_cast<hipStream_t>(stream), e->shapes,
+                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x, e->ysc, e->y_in_lds, solve_args(e));
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }
@@ -2687,9 +2894,9 @@ int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const 
   // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k2bar,
                      r.d2, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, coef, r.msc, r.fvals, e->dbar, d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc,
-                     e->y_in_lds);
+                     e->y_in_lds, solve_args(e));
   // r2 = f(Y1) - 2 k1
   hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
                      (long)ne, e->k1gbar, e->g2bar);
@@ -2702,9 +2909,9 @@ int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const 
   // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k1bar,
                      r.d1, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
+  hipLaunchKernelGGL(k_adj_solve, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, coef, r.msc, r.fvals, e->dbar, d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc,
-                     e->y_in_lds);
+                     e->y_in_lds, solve_args(e));
   // f(y) and the slopes at y
   if (ne)
     hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, r.y, c, e->k1bar, e->k1gbar,
@@ -2760,8 +2967,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
                      e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
   launch_factor(e, st, 0, W, coef);
-  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds);
+  hipLaunchKernelGGL(k_solve_stage, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds, solve_args(e));
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
                      ne, dt, coef, e->gm, e->gj, e->d1, e->K1, e->contrib1, e->Y1);
   if (ne)
@@ -2769,8 +2976,8 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                        e->K2, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
-  hipLaunchKernelGGL(k_solve_stage, dim3(W), dim3(kFactorBlock), e->solve_lds, st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds);
+  hipLaunchKernelGGL(k_solve_stage, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds, solve_args(e));
   StepRecord r{e->A, e->Y1, e->fvals, e->msc, e->gm, e->gj, e->d1, e->d2};
   if ((rc = reverse_sweep(e, st, r, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;
   if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * W, hipMemcpyDeviceToDevice, st));

@@ -2,9 +2,9 @@
 
 Bars: rates within 1e-12 relative (ulp-level differences of the FP64 transcendentals); trajectories
 of Vmem, concentrations and volumes within 1e-9 relative over the plain configuration's 1e4 steps;
-the numeric factor bit for bit in the reference-order (pairwise) mode and within 1e-13 of the
-largest factor entry in the default multifrontal mode, the solves bit for bit on a given factor
-(same sums in the same order, no FMA contraction);
+the numeric factor and the solve bit for bit in the reference-order (pairwise) mode (same sums in
+the same order, no FMA contraction), and within 1e-13 (factor) and 1e-12 (solve) in the default
+supernodal mode;
 batched (one graph for all worlds) equal to plain (one world per launch) bit for bit; material
 ledgers closed to 1e-12 of their magnitudes.
 """
@@ -146,9 +146,9 @@ def test_ros2_trajectory_matches_oracle(torch):
 @pytest.mark.parametrize("mode,cells,seed", [("pairwise", (64, 300), 8), ("supernodal", (64, 300), 8),
                                              ("supernodal", (1500, 4096), 3)])
 def test_factor_and_solve_against_oracle(torch, mode, cells, seed):
-    """Pairwise mode: the factor bit for bit. Supernodal mode: the factor within 1e-13 of the
-    largest entry (Jacobi-scaled SPD stage matrices). Either way the solve on the device's factor
-    bit for bit the oracle's solve on that factor."""
+    """Pairwise mode: the factor bit for bit, and the solve on it bit for bit the oracle's solve.
+    Supernodal mode: the factor within 1e-13 of the largest entry (Jacobi-scaled SPD stage
+    matrices) and the supernodal solve on it within 1e-12 of the oracle's solve on it."""
     from cells_amd.ensemble import Ensemble
 
     p = build(config1(worlds=3, cells=cells, seed=seed))
@@ -194,7 +194,12 @@ def test_factor_and_solve_against_oracle(torch, mode, cells, seed):
         else:
             assert np.max(np.abs(got - f)) <= 1e-13 * np.max(np.abs(f))
         c0, c1 = p.cell_off[k], p.cell_off[k + 1]
-        assert xg[c0:c1].tobytes() == oracle.solve(sym[k], got, rhs[c0:c1]).tobytes()
+        want = oracle.solve(sym[k], got, rhs[c0:c1])
+        if mode == "pairwise":
+            assert xg[c0:c1].tobytes() == want.tobytes()
+        else:
+            err = np.max(np.abs(xg[c0:c1] - want)) / np.max(np.abs(want))
+            assert err <= 1e-12, err
     e.close()
 
 

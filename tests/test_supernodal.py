@@ -145,3 +145,66 @@ def test_packed_triangle_decode_is_exact_up_to_order_200():
             for a in range(b, T):
                 assert tri_locate(T, e) == (b, a), (T, e)
                 e += 1
+
+
+def supernodal_solve(s, L, b):
+    """x = L^-T L^-1 b (by rank) through the fronts: forward with update vectors passed to the
+    parent, backward with the rows below gathered from the ancestors' solution."""
+    n = s.size
+    sns = supernodes(s)
+    of = {}
+    for t, (a, bb) in enumerate(sns):
+        for j in range(a, bb):
+            of[j] = t
+    rows = [list(range(a, bb)) + list(s.structure[bb - 1]) for a, bb in sns]
+    children = {t: [] for t in range(len(sns))}
+    for t, (a, bb) in enumerate(sns):
+        p = s.parent[bb - 1]
+        if p >= 0:
+            children[of[p]].append(t)
+    y = b.copy()
+    vec = {}
+    for t, (a, bb) in enumerate(sns):  # forward, children first
+        R = rows[t]
+        J = bb - a
+        f = np.zeros(len(R))
+        f[:J] = y[a:bb]
+        pos = {r: i for i, r in enumerate(R)}
+        for c in children[t]:
+            Sc = rows[c][sns[c][1] - sns[c][0]:]
+            for aa, r in enumerate(Sc):
+                f[pos[r]] += vec[c][aa]
+        for jj in range(J):
+            x = f[jj] * (1.0 / L[a + jj, a + jj])
+            f[jj + 1:] -= L[R[jj + 1:], a + jj] * x
+            y[a + jj] = x
+        vec[t] = f[J:]
+    for t in range(len(sns) - 1, -1, -1):  # backward, parents first
+        a, bb = sns[t]
+        R = rows[t]
+        J = bb - a
+        g = y[R[J:]]
+        xr = np.array([y[a + k] - np.dot(L[R[J:], a + k], g) for k in range(J)])
+        for jj in range(J - 1, -1, -1):
+            z = xr[jj] * (1.0 / L[a + jj, a + jj])
+            y[a + jj] = z
+            xr[:jj] -= L[a + jj, a:a + jj] * z
+    return y
+
+
+@pytest.mark.parametrize("n", [7, 64, 300, 1024, 2176])
+def test_supernodal_solve_meets_a_dense_solve(n):
+    snd, rcv = lattice(n)
+    s = sy.bind(n, list(snd), list(rcv))
+    rng = np.random.default_rng(n + 1)
+    off = rng.uniform(-1, 0, len(snd))
+    A = np.zeros((n, n))
+    A[snd, rcv] = off
+    A[rcv, snd] = off
+    A[np.arange(n), np.arange(n)] = 0.5 + rng.uniform(0.5, 2.0, n) - A.sum(axis=1)
+    P = A[np.ix_(s.order, s.order)]
+    L = np.linalg.cholesky(P)
+    b = rng.uniform(-1, 1, n)
+    x = supernodal_solve(s, L, b)
+    ref = np.linalg.solve(P, b)
+    assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
