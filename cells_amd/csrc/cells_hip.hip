@@ -133,6 +133,7 @@ constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: 
 constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 22
 constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 63
 constexpr int kMfBatch = 8;                                // global loads in flight per lane
+constexpr int kRecInts = 16;                               // ints per front record in the plan pool
 
 __device__ __forceinline__ double& CF(const Params& P, double* y, int f, int i) {
   return y[(long)f * P.ncell + i];
@@ -939,7 +940,8 @@ __global__ __launch_bounds__(kMfThreads) void k_solve_stage(Params P, const Shap
                                                             const double* msc,
                                                             const double* fvals, const double* K,
                                                             const double* Kminus2, double* dphi,
-                                                            double* ysc, int y_in_lds, SolveArgs sa) {
+                                                            double* ysc, int y_in_lds, SolveArgs sa,
+                                                            int part) {
   extern __shared__ double lds[];
   int w = w0 + blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
@@ -947,26 +949,32 @@ __global__ __launch_bounds__(kMfThreads) void k_solve_stage(Params P, const Shap
   const double* fv = fvals + foff[w];
   int c0 = P.cell_off[w];
   double* y = y_in_lds ? lds : ysc + c0;
-  for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
-    int i = c0 + v.order[k];
-    double q;
-    if (Kminus2) {
-      // the charge of K - 2 K2' field by field (the stage-2 right-hand side F1 - 2 k1)
-      double t2 = P.z[0] * (CFc(P, K, P.f_dn, i) + -2.0 * CFc(P, Kminus2, P.f_dn, i));
-      for (int sp = 1; sp < P.S; ++sp)
-        t2 = t2 + P.z[sp] * (CFc(P, K, P.f_dn + sp, i) + -2.0 * CFc(P, Kminus2, P.f_dn + sp, i));
-      q = kChargePerFmol * t2;
-    } else {
-      q = charge_increment(P, K, i);
+  if (part != 2) {
+    for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
+      int i = c0 + v.order[k];
+      double q;
+      if (Kminus2) {
+        // the charge of K - 2 K2' field by field (the stage-2 right-hand side F1 - 2 k1)
+        double t2 = P.z[0] * (CFc(P, K, P.f_dn, i) + -2.0 * CFc(P, Kminus2, P.f_dn, i));
+        for (int sp = 1; sp < P.S; ++sp)
+          t2 = t2 + P.z[sp] * (CFc(P, K, P.f_dn + sp, i) + -2.0 * CFc(P, Kminus2, P.f_dn + sp, i));
+        q = kChargePerFmol * t2;
+      } else {
+        q = charge_increment(P, K, i);
+      }
+      y[k] = q * msc[i];
     }
-    y[k] = q * msc[i];
   }
-  __syncthreads();
-  solve_any(s, v, pool, fv, y, sa, w, lds);
-  __syncthreads();
-  for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
-    int i = c0 + v.order[k];
-    dphi[i] = y[k] * msc[i];
+  if (part == 0) {
+    __syncthreads();
+    solve_any(s, v, pool, fv, y, sa, w, lds);
+    __syncthreads();
+  }
+  if (part != 1) {
+    for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
+      int i = c0 + v.order[k];
+      dphi[i] = y[k] * msc[i];
+    }
   }
 }
 
@@ -990,18 +998,22 @@ __global__ __launch_bounds__(kMfThreads) void k_solve_only(const ShapeDev* shape
                                                            const int* pool, const long* foff,
                                                            const int* cell_off, const double* fvals,
                                                            const double* rhs, double* x, double* ysc,
-                                                           int y_in_lds, SolveArgs sa) {
+                                                           int y_in_lds, SolveArgs sa, int part) {
   extern __shared__ double shared_y[];
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   int c0 = cell_off[w];
   double* lds = y_in_lds ? shared_y : ysc + c0;
-  for (int k = threadIdx.x; k < s.n; k += blockDim.x) lds[k] = rhs[c0 + v.order[k]];
-  __syncthreads();
-  solve_any(s, v, pool, fvals + foff[w], lds, sa, w, shared_y);
-  __syncthreads();
-  for (int k = threadIdx.x; k < s.n; k += blockDim.x) x[c0 + v.order[k]] = lds[k];
+  if (part != 2)
+    for (int k = threadIdx.x; k < s.n; k += blockDim.x) lds[k] = rhs[c0 + v.order[k]];
+  if (part == 0) {
+    __syncthreads();
+    solve_any(s, v, pool, fvals + foff[w], lds, sa, w, shared_y);
+    __syncthreads();
+  }
+  if (part != 1)
+    for (int k = threadIdx.x; k < s.n; k += blockDim.x) x[c0 + v.order[k]] = lds[k];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1094,13 +1106,12 @@ __device__ void mf_trailing_update(double* F, int m, int jb, int je, int lane) {
 
 // One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE].
 template <int G, int STRIDE>
-__device__ void mf_front(const int* rec, int q, const int* kids, const int* rel, double* fv, double* ub,
-                         double* F, int lane) {
+__device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double* F, int lane, int* valid_w) {
   using FL = FrontLanes<G>;
   constexpr int RW = FL::RW, CG = FL::CG, NT = FL::NT;
   const int lr = lane % RW, lc = lane / RW;
-  const int4 ra4 = reinterpret_cast<const int4*>(rec)[2 * q];
-  const int4 rb4 = reinterpret_cast<const int4*>(rec)[2 * q + 1];
+  const int4 ra4 = reinterpret_cast<const int4*>(pool)[4 * qa];
+  const int4 rb4 = reinterpret_cast<const int4*>(pool)[4 * qa + 1];
   const int J = ra4.y, m = ra4.z, uoff = ra4.w, c0 = rb4.x, c1 = rb4.y;
   const long p0 = ra4.x;
   const int S = m - J, tri = m * (m + 1) / 2, Pn = colbase(J, m) + J;
@@ -1123,9 +1134,9 @@ __device__ void mf_front(const int* rec, int q, const int* kids, const int* rel,
   // extend-add: child entry (a, b), a >= b, of the packed Sc x Sc update goes to front entry
   // (rel[a], rel[b]); lanes stride over the packed entries (coalesced), kMfBatch loads in flight
   for (int x = c0; x < c1; ++x) {
-    const int4 kd = reinterpret_cast<const int4*>(kids)[x];  // (Sc, rel start, update offset, -)
+    const int4 kd = reinterpret_cast<const int4*>(pool)[x];  // (Sc, rel start, update offset, -)
     const int Sc = kd.x;
-    const int* rl = rel + kd.y;
+    const int* rl = pool + kd.y;
     const double* U = ub + kd.z;
     const int nU = Sc * (Sc + 1) / 2;
     for (int e0 = lane; e0 < nU; e0 += kMfBatch * G) {
@@ -1193,30 +1204,35 @@ __device__ void mf_front(const int* rec, int q, const int* kids, const int* rel,
       }
     }
   }
-  for (int e = lane; e < Pn; e += G) fv[p0 + e] = F[e * STRIDE];
+  bool ok = true;
+  for (int e = lane; e < Pn; e += G) {
+    const double v = F[e * STRIDE];
+    ok = ok && isfinite(v);
+    fv[p0 + e] = v;
+  }
+  if (!ok && valid_w) *valid_w = 0;  // a pivot at or below zero, or an overflow
   if (S > 0)
     for (int e = Pn + lane; e < tri; e += G) ub[uoff + e - Pn] = F[e * STRIDE];
   group_sync<G>();
 }
 
-__device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub, double* lds) {
-  const int* rec = pool + s.o_sn;
+__device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub, double* lds,
+                                int* valid_w) {
   const int* lev = pool + s.o_snlev;
-  const int* kids = pool + s.o_child;
-  const int* rel = pool + s.o_rel;
+  const int qa0 = (int)(s.o_sn / kRecInts);
   const int tid = threadIdx.x, wave = tid >> 6, octa = tid / kMfOcta;
   for (int L = 0; L < s.sn_levels; ++L) {
     const int* bd = lev + 5 * L;  // thread, 8-lane, wavefront and workgroup fronts of level L
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
-      mf_front<1, kMfThreads>(rec, q, kids, rel, fv, ub, lds + tid, 0);
+      mf_front<1, kMfThreads>(pool, qa0 + q, fv, ub, lds + tid, 0, valid_w);
     __syncthreads();
     for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      mf_front<kMfOcta, 1>(rec, q, kids, rel, fv, ub, lds + octa * kMfOctaTri, tid % kMfOcta);
+      mf_front<kMfOcta, 1>(pool, qa0 + q, fv, ub, lds + octa * kMfOctaTri, tid % kMfOcta, valid_w);
     __syncthreads();
     for (int q = bd[2] + wave; q < bd[3]; q += kMfThreads / 64)
-      mf_front<64, 1>(rec, q, kids, rel, fv, ub, lds + wave * kMfWaveTri, tid & 63);
+      mf_front<64, 1>(pool, qa0 + q, fv, ub, lds + wave * kMfWaveTri, tid & 63, valid_w);
     __syncthreads();
-    for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(rec, q, kids, rel, fv, ub, lds, tid);
+    for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(pool, qa0 + q, fv, ub, lds, tid, valid_w);
   }
   __syncthreads();
 }
@@ -1258,19 +1274,18 @@ __device__ __forceinline__ double pick(const double (&a)[NT], int idx) {
 }
 
 template <int G, int STRIDE>
-__device__ void sv_forward_front(const int* rec, int q, const int* kids, const int* rel,
-                                 const double* __restrict__ fv, double* __restrict__ y,
+__device__ void sv_forward_front(const int* pool, int qa, const double* __restrict__ fv, double* __restrict__ y,
                                  double* __restrict__ vec, double* Fv, int lane) {
   constexpr int NT = SolveLanes<G>::NT;
-  const int4 ra4 = reinterpret_cast<const int4*>(rec)[2 * q];
-  const int4 rb4 = reinterpret_cast<const int4*>(rec)[2 * q + 1];
+  const int4 ra4 = reinterpret_cast<const int4*>(pool)[4 * qa];
+  const int4 rb4 = reinterpret_cast<const int4*>(pool)[4 * qa + 1];
   const long p0 = ra4.x;
   const int J = ra4.y, m = ra4.z, c0 = rb4.x, c1 = rb4.y, voff = rb4.z, j0 = rb4.w;
   for (int i = lane; i < m; i += G) Fv[i * STRIDE] = i < J ? y[j0 + i] : 0.0;
   group_sync<G>();
   for (int x = c0; x < c1; ++x) {
-    const int4 kd = reinterpret_cast<const int4*>(kids)[x];  // (Sc, rel start, -, vector offset)
-    const int* rl = rel + kd.y;
+    const int4 kd = reinterpret_cast<const int4*>(pool)[x];  // (Sc, rel start, -, vector offset)
+    const int* rl = pool + kd.y;
     const double* u = vec + kd.w;
     for (int a = lane; a < kd.x; a += G) Fv[rl[a] * STRIDE] += u[a];
     group_sync<G>();
@@ -1308,16 +1323,16 @@ __device__ void sv_forward_front(const int* rec, int q, const int* kids, const i
 }
 
 template <int G, int STRIDE>
-__device__ void sv_backward_front(const int* rec, int q, const int* rrank, const double* __restrict__ fv,
-                                  double* __restrict__ y, double* Fv, int lane) {
+__device__ void sv_backward_front(const int* pool, int qa, const double* __restrict__ fv, double* __restrict__ y,
+                                  double* Fv, int lane) {
   constexpr int NT = SolveLanes<G>::NT;
-  const int4 ra4 = reinterpret_cast<const int4*>(rec)[2 * q];
-  const int4 rb4 = reinterpret_cast<const int4*>(rec)[2 * q + 1];
+  const int4 ra4 = reinterpret_cast<const int4*>(pool)[4 * qa];
+  const int4 rb4 = reinterpret_cast<const int4*>(pool)[4 * qa + 1];
+  const int rows_below = pool[kRecInts * qa + 8];  // ranks of the last column's rows after its diagonal
   const long p0 = ra4.x;
   const int J = ra4.y, m = ra4.z, j0 = rb4.w;
-  // the solution at the rows below (the last column's rows after its diagonal)
-  const long pl = p0 + colbase(J - 1, m) + J;
-  for (int a = lane; a < m - J; a += G) Fv[(J + a) * STRIDE] = y[rrank[pl + a]];
+  // the solution at the rows below
+  for (int a = lane; a < m - J; a += G) Fv[(J + a) * STRIDE] = y[pool[rows_below + a]];
   group_sync<G>();
   double xr[NT];
 #pragma unroll
@@ -1353,32 +1368,29 @@ __device__ void sv_backward_front(const int* rec, int q, const int* rrank, const
 // The solve of one world's y (by rank) on its supernodal factor; blockDim.x == kMfThreads.
 __device__ void sv_solve_world(const ShapeDev& s, const int* pool, const double* fv, double* y, double* vec,
                                double* lds) {
-  const int* rec = pool + s.o_sn;
   const int* lev = pool + s.o_snlev;
-  const int* kids = pool + s.o_child;
-  const int* rel = pool + s.o_rel;
-  const int* rrank = pool + s.o_entry;
+  const int qa0 = (int)(s.o_sn / kRecInts);
   const int tid = threadIdx.x, wave = tid >> 6, octa = tid / kMfOcta;
   double* Fq = lds + kSvThreadLds + octa * (kMfOcta * SolveLanes<kMfOcta>::NT);
   double* Fw = lds + kSvThreadLds + kSvOctaLds + wave * (64 * SolveLanes<64>::NT);
   for (int L = 0; L < s.sn_levels; ++L) {
     const int* bd = lev + 5 * L;
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
-      sv_forward_front<1, kMfThreads>(rec, q, kids, rel, fv, y, vec, lds + tid, 0);
+      sv_forward_front<1, kMfThreads>(pool, qa0 + q, fv, y, vec, lds + tid, 0);
     for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      sv_forward_front<kMfOcta, 1>(rec, q, kids, rel, fv, y, vec, Fq, tid % kMfOcta);
+      sv_forward_front<kMfOcta, 1>(pool, qa0 + q, fv, y, vec, Fq, tid % kMfOcta);
     for (int q = bd[2] + wave; q < bd[4]; q += kMfThreads / 64)
-      sv_forward_front<64, 1>(rec, q, kids, rel, fv, y, vec, Fw, tid & 63);
+      sv_forward_front<64, 1>(pool, qa0 + q, fv, y, vec, Fw, tid & 63);
     __syncthreads();
   }
   for (int L = s.sn_levels - 1; L >= 0; --L) {
     const int* bd = lev + 5 * L;
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
-      sv_backward_front<1, kMfThreads>(rec, q, rrank, fv, y, lds + tid, 0);
+      sv_backward_front<1, kMfThreads>(pool, qa0 + q, fv, y, lds + tid, 0);
     for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      sv_backward_front<kMfOcta, 1>(rec, q, rrank, fv, y, Fq, tid % kMfOcta);
+      sv_backward_front<kMfOcta, 1>(pool, qa0 + q, fv, y, Fq, tid % kMfOcta);
     for (int q = bd[2] + wave; q < bd[4]; q += kMfThreads / 64)
-      sv_backward_front<64, 1>(rec, q, rrank, fv, y, Fw, tid & 63);
+      sv_backward_front<64, 1>(pool, qa0 + q, fv, y, Fw, tid & 63);
     __syncthreads();
   }
 }
@@ -1404,10 +1416,7 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeD
     fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
   }
   __syncthreads();
-  mf_factor_world(s, pool, fv, ubuf + uoffw[w], lds);
-  bool ok = true;
-  for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) ok = ok && isfinite(fv[p]);
-  if (!ok) valid[w] = 0;
+  mf_factor_world(s, pool, fv, ubuf + uoffw[w], lds, valid + w);
 }
 
 // Test entry: the multifrontal factor of assembled values in place, per world.
@@ -1418,7 +1427,7 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_only_mf(const ShapeDev* s
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
-  mf_factor_world(s, pool, fvals + foff[w], ubuf + uoffw[w], lds);
+  mf_factor_world(s, pool, fvals + foff[w], ubuf + uoffw[w], lds, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1931,7 +1940,7 @@ __global__ __launch_bounds__(kMfThreads) void k_adj_solve(Params P, const ShapeD
                                                           const double* dbar, const double* delta,
                                                           double* wv, double* rate_bar, double* gmbar,
                                                           double* gjbar, double* ysc, int y_in_lds,
-                                                          SolveArgs sa) {
+                                                          SolveArgs sa, int part) {
   extern __shared__ double shared_y[];
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
@@ -1939,13 +1948,17 @@ __global__ __launch_bounds__(kMfThreads) void k_adj_solve(Params P, const ShapeD
   const double* fv = fvals + foff[w];
   int c0 = P.cell_off[w], e0 = P.edge_off[w];
   double* lds = y_in_lds ? shared_y : ysc + c0;
-  for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
-    int i = c0 + v.order[k];
-    lds[k] = dbar[i] * msc[i];
+  if (part != 2)
+    for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
+      int i = c0 + v.order[k];
+      lds[k] = dbar[i] * msc[i];
+    }
+  if (part == 0) {
+    __syncthreads();
+    solve_any(s, v, pool, fv, lds, sa, w, shared_y);
+    __syncthreads();
   }
-  __syncthreads();
-  solve_any(s, v, pool, fv, lds, sa, w, shared_y);
-  __syncthreads();
+  if (part == 1) return;
   for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
     int i = c0 + v.order[k];
     wv[i] = lds[k] * msc[i];
@@ -2480,23 +2493,30 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       std::vector<int32_t> rows, cols;
       world_pattern(w, k, rows, cols);
       for (size_t q2 = 0; q2 < rows.size(); ++q2) pool.push_back((int)cells::position(s, rows[q2], cols[q2]));
-      // the multifrontal schedule
+      // the multifrontal schedule (indices into the pool are absolute, so a front needs only its
+      // record and its world's buffers)
       const cells::Supernodes& sn = e->sn_host[q];
       const int ns = (int)sn.first.size();
       if (sn.update_total > 0x7fffffff) return fail(CELLS_EINVAL, "update matrices exceed 2^31 doubles");
-      // the solve's update vectors: |S| doubles per supernode
-      std::vector<int64_t> voff(ns + 1, 0);
+      std::vector<int64_t> voff(ns + 1, 0);  // the solve's update vectors: |S| doubles per supernode
       for (int t = 0; t < ns; ++t) voff[t + 1] = voff[t] + (sn.rows[t] - sn.width[t]);
       d.vsz = voff[ns];
-      while (pool.size() % 4) pool.push_back(0);  // 16-byte aligned records
+      while (pool.size() % kRecInts) pool.push_back(0);  // records on 64-byte boundaries
       d.o_sn = (long)pool.size();
+      const long o_lev = d.o_sn + (long)kRecInts * ns;
+      long o_child = o_lev + 5L * sn.levels;
+      o_child += (4 - o_child % 4) % 4;
+      const long o_rel = o_child + 4L * (long)sn.child.size();
+      if (o_rel + (long)sn.rel.size() > 0x7fffffffL) return fail(CELLS_EINVAL, "plans exceed 2^31 entries");
       for (int t = 0; t < ns; ++t) {
         if (front_class(sn.rows[t]) > 3)
           return fail(CELLS_EINVAL, "a front exceeds the workgroup's LDS (order above 127)");
-        int32_t r8[8] = {(int32_t)s.column_start[sn.first[t]], sn.width[t], sn.rows[t],
-                         (int32_t)sn.update_offset[t], sn.child_ptr[t], sn.child_ptr[t + 1],
-                         (int32_t)voff[t], sn.first[t]};
-        pool.insert(pool.end(), r8, r8 + 8);
+        const int64_t last = sn.first[t] + sn.width[t] - 1;
+        int32_t r16[kRecInts] = {(int32_t)s.column_start[sn.first[t]], sn.width[t], sn.rows[t],
+                                 (int32_t)sn.update_offset[t], (int32_t)(o_child / 4 + sn.child_ptr[t]),
+                                 (int32_t)(o_child / 4 + sn.child_ptr[t + 1]), (int32_t)voff[t], sn.first[t],
+                                 (int32_t)(d.o_entry + s.column_start[last] + 1)};
+        pool.insert(pool.end(), r16, r16 + kRecInts);
       }
       d.sn_levels = sn.levels;
       d.o_snlev = (long)pool.size();
@@ -2513,14 +2533,15 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
         }
         pool.insert(pool.end(), bounds.begin(), bounds.end());
       }
-      while (pool.size() % 4) pool.push_back(0);  // 16-byte aligned child descriptors
+      while ((long)pool.size() < o_child) pool.push_back(0);
       d.o_child = (long)pool.size();
       for (int32_t c : sn.child) {
-        int32_t c4[4] = {sn.rows[c] - sn.width[c], sn.rel_ptr[c], (int32_t)sn.update_offset[c],
+        int32_t c4[4] = {sn.rows[c] - sn.width[c], (int32_t)(o_rel + sn.rel_ptr[c]), (int32_t)sn.update_offset[c],
                          (int32_t)voff[c]};
         pool.insert(pool.end(), c4, c4 + 4);
       }
       d.o_rel = (long)pool.size();
+      if (d.o_rel != o_rel) return fail(CELLS_EINVAL, "plan layout mismatch");
       pool.insert(pool.end(), sn.rel.begin(), sn.rel.end());
       d.usz = sn.update_total;
       sd[q] = d;
@@ -2613,6 +2634,13 @@ size_t solve_lds_bytes(const cells_ensemble* e) {
 }
 SolveArgs solve_args(const cells_ensemble* e) { return SolveArgs{e->pairwise ? 0 : 1, e->voffw, e->vbuf}; }
 
+// A solve stage over worlds: launch(part) runs the stage kernel (part 0: right-hand side, solve
+// and result in one workgroup per world).
+template <typename Launch>
+void solve_stage(cells_ensemble*, hipStream_t, int, const double*, Launch launch) {
+  launch(0);
+}
+
 // Assemble and factor the stage matrix C + coef G of worlds [w0, w0 + nw).
 void launch_factor(cells_ensemble* e, hipStream_t st, int w0, int nw, double coef) {
   if (e->pairwise)
@@ -2669,8 +2697,10 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     mark("k_cells");
     launch_factor(e, st, w0, nw, coef);
     mark("k_factor");
+    solve_stage(e, st, nw, e->fvals, [&](int part) {
     hipLaunchKernelGGL(k_solve_stage, dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds, solve_args(e));
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds, solve_args(e), part);
+  });
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->contrib1, e->Y1);
@@ -2683,8 +2713,10 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, c0, c1,
                        c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
     mark("k_cells");
+    solve_stage(e, st, nw, e->fvals, [&](int part) {
     hipLaunchKernelGGL(k_solve_stage, dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds, solve_args(e));
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds, solve_args(e), part);
+  });
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi2, e->K1, e->K2, e->contrib1, e->contrib2, e->valid);
@@ -2829,8 +2861,10 @@ extern "C" int cells_solve(cells_ensemble* e, void* stream, const double* values
   if (!e || !values || !rhs || !x) return fail(CELLS_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(e->device));
   size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
-  hipLaunchKernelGGL(k_solve_only, dim3(e->W), solve_block(e), solve_lds_bytes(e), static_cast<hipStream_t>(stream), e->shapes,
-                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x, e->ysc, e->y_in_lds, solve_args(e));
+  solve_stage(e, static_cast<hipStream_t>(stream), e->W, values, [&](int part) {
+    hipLaunchKernelGGL(k_solve_only, dim3(e->W), solve_block(e), solve_lds_bytes(e), static_cast<hipStream_t>(stream), e->shapes,
+                     e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x, e->ysc, e->y_in_lds, solve_args(e), part);
+  });
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }
@@ -2894,9 +2928,11 @@ int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const 
   // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k2bar,
                      r.d2, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+  solve_stage(e, st, W, r.fvals, [&](int part) {
+    hipLaunchKernelGGL(k_adj_solve, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, coef, r.msc, r.fvals, e->dbar, d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc,
-                     e->y_in_lds, solve_args(e));
+                     e->y_in_lds, solve_args(e), part);
+  });
   // r2 = f(Y1) - 2 k1
   hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
                      (long)ne, e->k1gbar, e->g2bar);
@@ -2909,9 +2945,11 @@ int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const 
   // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k1bar,
                      r.d1, e->dbar, e->gmbar, e->gjbar);
-  hipLaunchKernelGGL(k_adj_solve, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+  solve_stage(e, st, W, r.fvals, [&](int part) {
+    hipLaunchKernelGGL(k_adj_solve, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, coef, r.msc, r.fvals, e->dbar, d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc,
-                     e->y_in_lds, solve_args(e));
+                     e->y_in_lds, solve_args(e), part);
+  });
   // f(y) and the slopes at y
   if (ne)
     hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, r.y, c, e->k1bar, e->k1gbar,
@@ -2967,8 +3005,10 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
                      e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
   launch_factor(e, st, 0, W, coef);
-  hipLaunchKernelGGL(k_solve_stage, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds, solve_args(e));
+  solve_stage(e, st, W, e->fvals, [&](int part) {
+    hipLaunchKernelGGL(k_solve_stage, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds, solve_args(e), part);
+  });
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
                      ne, dt, coef, e->gm, e->gj, e->d1, e->K1, e->contrib1, e->Y1);
   if (ne)
@@ -2976,8 +3016,10 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                        e->K2, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
-  hipLaunchKernelGGL(k_solve_stage, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds, solve_args(e));
+  solve_stage(e, st, W, e->fvals, [&](int part) {
+    hipLaunchKernelGGL(k_solve_stage, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds, solve_args(e), part);
+  });
   StepRecord r{e->A, e->Y1, e->fvals, e->msc, e->gm, e->gj, e->d1, e->d2};
   if ((rc = reverse_sweep(e, st, r, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;
   if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * W, hipMemcpyDeviceToDevice, st));
