@@ -25,6 +25,20 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "ros2_step_4096_worlds.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE scaled by a 1 GiB copy), or None."""
+    try:
+        for line in open(PROFILE):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 6 and cells[0] == kernel and cells[3].endswith("GB"):
+                return float(cells[3].split()[0]) * 1e9
+    except OSError:
+        pass
+    return None
 BYTES_PER_CELL_STEP = 539.0  # one read + write of the state, one read of the parameters (4 ions)
 
 
@@ -196,8 +210,10 @@ def main():
         fact_ms = float(np.mean(launches))
         achieved = stats["flops"] / (fact_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "k_factor", "launch_ms": fact_ms,
+                    "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                    "traffic": measured_traffic("k_factor_mf"),
+                    "traffic_source": "profiles/ros2_step_4096_worlds.md (rocprofv3 PMC, copy-calibrated)",
+                    "kernel": "k_factor_mf", "launch_ms": fact_ms,
                     "algorithmic": f"sum over worlds of sum_j c_j^2 = {stats['flops']:.4g} flops"}
     else:
         avg_ms = float(np.mean(launches))
