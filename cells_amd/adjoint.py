@@ -234,3 +234,33 @@ class Calibration:
         self._reverse(store, m, b, free - 1)
         del store[self._pos(m)]
         self._reverse(store, a, m, free)
+
+
+def step_function(ensemble, dt, ctl=None):
+    """One batched ROS2 step as a torch.autograd.Function of (y, pam [S, n], nak_cap [n],
+    paj [S, E]): forward replaces the ensemble's learnable parameters and steps through
+    cells_step; backward calls cells_step_vjp at the saved input. Cotangents flow through the
+    amounts and gates (the fields the step's rates read); ledger fields pass theirs through."""
+    torch = ensemble.torch
+    ctl = ctl or control()
+
+    class Step(torch.autograd.Function):
+        @staticmethod
+        def forward(fctx, y, pam, nak_cap, paj):
+            ensemble.set_parameters(pam.detach(), nak_cap.detach(), paj.detach())
+            out, ok = ensemble.step(y.detach(), dt, nsteps=1, ctl=ctl)
+            if not bool(ok.all()):
+                raise FloatingPointError("a world refused the step")
+            fctx.save_for_backward(y.detach(), pam.detach(), nak_cap.detach(), paj.detach())
+            return out
+
+        @staticmethod
+        def backward(fctx, ybar):
+            y, pam, nak_cap, paj = fctx.saved_tensors
+            ensemble.set_parameters(pam, nak_cap, paj)  # this step's parameters
+            yin, pam_bar, cap_bar, paj_bar, valid = ensemble.step_vjp(y, ybar.contiguous(), dt, ctl=ctl)
+            if not bool(valid.all()):
+                raise FloatingPointError("a world refused the step in the reverse sweep")
+            return yin, pam_bar, cap_bar, paj_bar[:, :paj.shape[1]]
+
+    return Step.apply

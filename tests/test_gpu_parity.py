@@ -409,3 +409,46 @@ def test_taped_calibration_gradient_equals_the_recomputing_one(torch):
     assert l0 == l1 == l2
     assert np.array_equal(g0, g1)
     e.close()
+
+
+def test_autograd_function_gradient_meets_central_differences(torch):
+    """torch.autograd over three steps of the Step function (cells_step forward, cells_step_vjp
+    backward): d loss / d (pam, nak_cap, paj, y0) against central differences of the HIP loss,
+    the loss a fixed weighting of the final amounts."""
+    from cells_amd.adjoint import step_function
+
+    p = _three_ion(worlds=2, cells=(16, 25), seed=31)
+    e = ensemble(p, "ros2")
+    L = StateLayout.of(p)
+    dev, f64 = torch.device("cuda", 0), torch.float64
+    rng = np.random.default_rng(12)
+    wts = np.zeros(L.total)
+    wts[:p.S * p.ncell] = rng.normal(size=p.S * p.ncell)  # the amounts' block
+    wt = torch.as_tensor(wts, device=dev)
+    step = step_function(e, 5e-3)
+    y0 = torch.as_tensor(initial_state(p), device=dev)
+    base = [torch.as_tensor(np.ascontiguousarray(a), device=dev) for a in (p.pam, p.nak_cap, p.paj)]
+
+    def loss(y, pam, cap, paj):
+        for _ in range(3):
+            y = step(y, pam, cap, paj)
+        return (wt * y).sum()
+
+    args = [y0.clone().requires_grad_(True)] + [b.clone().requires_grad_(True) for b in base]
+    say("autograd function")
+    value = loss(*args)
+    value.backward()
+    grads = [a.grad.clone() for a in args]
+    for trial in range(3):
+        dirs = [torch.as_tensor(rng.normal(size=tuple(a.shape)), device=dev) for a in args]
+        dirs[0][p.S * p.ncell:] = 0.0  # perturb only the amounts of the state
+        scale = [torch.full_like(a, 1e-4) for a in args]  # fmol for amounts
+        scale[1:] = [b.abs() for b in base]  # parameters: relative
+        hs = 1e-6
+        with torch.no_grad():
+            plus = [a.detach() + hs * s * d for a, s, d in zip(args, scale, dirs)]
+            minus = [a.detach() - hs * s * d for a, s, d in zip(args, scale, dirs)]
+            fd = (loss(*plus) - loss(*minus)).item() / (2 * hs)
+        an = sum(float((g * s * d).sum()) for g, s, d in zip(grads, scale, dirs))
+        assert abs(fd - an) <= 1e-6 * abs(fd) + 1e-18, (trial, fd, an)
+    e.close()
