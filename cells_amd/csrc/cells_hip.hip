@@ -28,6 +28,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -115,8 +116,9 @@ struct ShapeDev {
   long o_cs, o_entry, o_rowptr, o_rowent, o_lev, o_levcol, o_levent, o_node;
   long o_mask;  // offset in the mask pool (unsigned long long words)
   int lds_pt;   // first factor position held in LDS during the factor (nnz: none)
-  // multifrontal schedule: supernode records (8 ints each: factor position of the first column,
-  // width, front order, update offset, children begin/end, update-vector offset, first column),
+  // multifrontal schedule: supernode records (16 ints each: factor position of the first column,
+  // width, front order, update offset, children begin/end (pool-absolute descriptor indices),
+  // update-vector offset, first column, pool index of the last column's ranks below its diagonal),
   // per level 5 boundaries (thread, 8-lane, wavefront and workgroup fronts, end), children (4
   // ints each: rows below, rel list start, update offset, update-vector offset), rel lists
   long o_sn, o_snlev, o_child, o_rel;
@@ -2076,6 +2078,10 @@ struct cells_ensemble {
   }
 };
 
+// Drops the stationary analysis' workspace record of an ensemble (its buffers are the ensemble's
+// allocations); defined with the stationary analysis.
+void stationary_release(const cells_ensemble* e);
+
 namespace {
 
 template <typename T>
@@ -2586,6 +2592,7 @@ void cells_ensemble_destroy(cells_ensemble* e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
   (void)hipDeviceSynchronize();
+  stationary_release(e);
   delete e;
 }
 
@@ -3152,3 +3159,5 @@ extern "C" int cells_plan_stats(const cells_ensemble* e, int64_t* nnz_total, dou
   if (shapes) *shapes = (int32_t)e->shapes_host.size();
   return CELLS_OK;
 }
+
+#include "stationary.inc"

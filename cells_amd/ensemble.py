@@ -211,6 +211,54 @@ class Ensemble:
                                                        v(pam_bar), v(cap_bar), v(paj_bar), v(valid)))
         return yin, pam_bar, cap_bar, paj_bar, valid
 
+    def stationary(self, y0, ctl=None, tolerance=1e-10, max_newton=60, max_cycles=20, linear_rtol=1e-8,
+                   fp32_inner=True):
+        """Steady state of every world (damped Newton, GMRES with block-Jacobi, FP32 Krylov under
+        FP64 refinement when fp32_inner): (y, converged [W], step [W], iterations [W], report)."""
+        t = self.torch
+        y0 = self.to_device(y0)
+        out = t.empty_like(y0)
+        W = self.packed.W
+        conv = t.empty(W, dtype=t.int32, device=self.device)
+        step = t.empty(W, dtype=t.float64, device=self.device)
+        iters = t.empty(W, dtype=t.int32, device=self.device)
+        opt = _lib.StationaryOptions(float(tolerance), int(max_newton), int(max_cycles), float(linear_rtol),
+                                     1 if fp32_inner else 0)
+        rep = _lib.StationaryReport()
+        ctl = ctl or control()
+        v = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+        _lib.check(_lib.load().cells_stationary(self._h, self._stream(), v(y0), v(out), ctypes.byref(ctl),
+                                                ctypes.byref(opt), v(conv), v(step), v(iters),
+                                                ctypes.byref(rep)))
+        report = dict(converged=rep.converged, newton_iterations=rep.newton_iterations,
+                      krylov_steps=rep.krylov_steps, worst_step=rep.worst_step)
+        return out, conv, step, iters, report
+
+    def stationary_residual(self, y, ctl=None):
+        """(f(y) in coordinate units [S ncell + nedge], valid [W])."""
+        t = self.torch
+        y = self.to_device(y)
+        p = self.packed
+        r = t.empty(p.S * p.ncell + p.nedge, dtype=t.float64, device=self.device)
+        valid = t.empty(p.W, dtype=t.int32, device=self.device)
+        ctl = ctl or control()
+        v = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+        _lib.check(_lib.load().cells_stationary_residual(self._h, self._stream(), v(y), ctypes.byref(ctl),
+                                                         v(r), v(valid)))
+        return r, valid
+
+    def stationary_jvp(self, y, direction, ctl=None):
+        """J(y) direction (coordinate units) from the forward-tangent kernels."""
+        t = self.torch
+        y = self.to_device(y)
+        d = self.to_device(direction)
+        out = t.empty_like(d)
+        ctl = ctl or control()
+        v = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+        _lib.check(_lib.load().cells_stationary_jvp(self._h, self._stream(), v(y), ctypes.byref(ctl), v(d),
+                                                    v(out)))
+        return out
+
     def vmem_loss(self, y, target, weight, loss, ybar=None):
         """loss[w] += weight sum_i (V_i - target_i)^2; ybar (if given) += its cotangent."""
         v = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731

@@ -32,8 +32,19 @@ EXPORTS = (
     "cells_factor_size", "cells_factor", "cells_solve", "cells_world_positions",
     "cells_kernel_times", "cells_device_checks", "cells_step_vjp", "cells_vmem_loss",
     "cells_set_parameters", "cells_plan_stats", "cells_tape_reserve", "cells_step_record",
-    "cells_step_vjp_recorded",
+    "cells_step_vjp_recorded", "cells_stationary", "cells_stationary_residual", "cells_stationary_jvp",
 )
+
+
+class StationaryOptions(ctypes.Structure):
+    _fields_ = [("tolerance", ctypes.c_double), ("max_newton", ctypes.c_int32),
+                ("max_cycles", ctypes.c_int32), ("linear_rtol", ctypes.c_double),
+                ("fp32_inner", ctypes.c_int32)]
+
+
+class StationaryReport(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int32), ("newton_iterations", ctypes.c_int32),
+                ("krylov_steps", ctypes.c_int64), ("worst_step", ctypes.c_double)]
 
 
 class CellsError(RuntimeError):
@@ -110,6 +121,11 @@ def load():
                                    vp, vp, vp, vp, vp]
     lib.cells_vmem_loss.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, vp]
     lib.cells_tape_reserve.argtypes = [vp, ctypes.c_int32, _i64p]
+    lib.cells_stationary.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ControlStruct),
+                                     ctypes.POINTER(StationaryOptions), vp, vp, vp,
+                                     ctypes.POINTER(StationaryReport)]
+    lib.cells_stationary_residual.argtypes = [vp, vp, vp, ctypes.POINTER(ControlStruct), vp, vp]
+    lib.cells_stationary_jvp.argtypes = [vp, vp, vp, ctypes.POINTER(ControlStruct), vp, vp]
     lib.cells_step_record.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
                                       ctypes.c_int32, vp]
     lib.cells_step_vjp_recorded.argtypes = [vp, vp, ctypes.c_int32, vp, ctypes.c_double,

@@ -97,6 +97,14 @@ def config1(worlds=4096, **kw):
     return Config(**base)
 
 
+def config4(worlds=512, **kw):
+    """Stationary analysis: worlds of 64x64 = 4096 cells, Na/K/Cl/Ca, junction conductances
+    spread by a lognormal of sigma 0.3, no water."""
+    base = dict(cells=(4096, 4096), junction_sigma=0.3, seed=4)
+    base.update(kw)
+    return config1(worlds=worlds, **base)
+
+
 def lattice(n):
     """(senders, receivers) of the lattice of n cells: rows of L = floor(sqrt n) cells, the
     remaining n - L^2 cells appended to the last row; each cell's right edge, then its down edge."""

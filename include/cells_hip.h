@@ -160,6 +160,44 @@ int cells_step_vjp_recorded(cells_ensemble* e, void* stream, int32_t slot, const
                             double dt, const cells_control* control, double* ybar_in,
                             double* pam_bar, double* cap_bar, double* paj_bar, int32_t* valid_out);
 
+/* Stationary analysis (analysis/stationary.py:473-517,919-1080; the reference solves one world
+ * on the host with SciPy GMRES). Damped Newton for the steady state of every world's amounts and
+ * junction gates, in the reference's coordinates (amounts as mM over the cell volume, gates as
+ * fractions, stationary.py:117-123), stopping when a full correction is at most `tolerance` in
+ * those units (the reference's 1e-10). Linear systems: restarted GMRES(30), right-preconditioned
+ * by the per-cell S x S blocks of the Jacobian (and each gate's diagonal), Jacobian products from
+ * forward tangents of the rate laws; with fp32_inner the Krylov basis is FP32 and each restart's
+ * residual b - J x is FP64 (iterative refinement to linear_rtol). Converged worlds leave the
+ * batch between Newton iterations. Worlds without water. y_out holds y0 with the steady amounts
+ * and gates; per world converged_out (int), step_out (the last correction's max norm) and
+ * iterations_out may be NULL; all are device pointers. */
+typedef struct cells_stationary_options {
+  double tolerance;    /* Newton: full correction max norm (1e-10) */
+  int32_t max_newton;  /* Newton iterations (60) */
+  int32_t max_cycles;  /* GMRES restarts per Newton step (20) */
+  double linear_rtol;  /* |b - J x|_2 <= linear_rtol |b|_2 (1e-8) */
+  int32_t fp32_inner;  /* 1: FP32 Krylov under FP64 refinement; 0: all FP64 */
+} cells_stationary_options;
+
+typedef struct cells_stationary_report {
+  int32_t converged;          /* worlds converged */
+  int32_t newton_iterations;  /* iterations of the longest-running world */
+  int64_t krylov_steps;       /* Arnoldi steps over all worlds */
+  double worst_step;          /* largest final correction */
+} cells_stationary_report;
+
+int cells_stationary(cells_ensemble* e, void* stream, const double* y0, double* y_out,
+                     const cells_control* control, const cells_stationary_options* opt,
+                     int32_t* converged_out, double* step_out, int32_t* iterations_out,
+                     cells_stationary_report* report);
+
+/* The stationary analysis' residual f(y) (rates of the coordinates, amounts then gates, S ncell +
+ * nedge doubles) and its tangent product J(y) v, for tests. */
+int cells_stationary_residual(cells_ensemble* e, void* stream, const double* y,
+                              const cells_control* control, double* r_out, int32_t* valid_out);
+int cells_stationary_jvp(cells_ensemble* e, void* stream, const double* y, const cells_control* control,
+                         const double* v, double* out);
+
 /* Vmem misfit of one sample row: loss_out[w] += weight sum_i (V_i - target_i)^2 over world w's
  * cells, and (when ybar is not NULL) its cotangent on the amounts added to ybar. */
 int cells_vmem_loss(cells_ensemble* e, void* stream, const double* y, const double* target,
