@@ -81,24 +81,40 @@ def test_residual_is_the_oracle_rates_in_coordinates(torch):
 
 
 def test_tangent_products_meet_central_differences(torch):
+    """Amount rows and gate rows separately. The gate law depends on |V_a - V_b|, which has a
+    kink at zero: the central difference (its potential step is about 0.2 uV) is compared only on
+    the junctions with |V_a - V_b| >= 0.1 mV, where it is a derivative."""
     from cells_amd.ensemble import Ensemble
 
     p = small(seed=7)
     e = Ensemble(p, "ros2")
-    y = stepped(e, p)
+    y = stepped(e, p, steps=200)
+    L = StateLayout.of(p)
+    from tests.helpers import vmem
+
+    v_m = vmem(p, y)
+    dv = np.abs(v_m[p.snd] - v_m[p.rcv])
     rng = np.random.default_rng(3)
     na = p.S * p.ncell
-    say("stationary tangent")
+    say(f"stationary tangent (min |dV| over junctions {dv.min():.3e} V)")
+    h = 1e-6
     for trial in range(3):
         v = rng.normal(size=na + p.nedge)
         v[:na] *= 1e-2
         v[na:] *= 1e-3
         jv = e.stationary_jvp(y, v).cpu().numpy()
-        h = 1e-4
         fp, _ = e.stationary_residual(moved(p, y, v, h))
         fm, _ = e.stationary_residual(moved(p, y, v, -h))
         fd = (fp.cpu().numpy() - fm.cpu().numpy()) / (2 * h)
-        assert np.linalg.norm(jv - fd) <= 1e-6 * np.linalg.norm(fd), trial
+        smooth = np.concatenate([np.ones(na, bool), dv >= 1e-4])
+        assert smooth[na:].sum() >= p.nedge // 4
+        for name, sl in (("amounts", slice(0, na)), ("gates", slice(na, None))):
+            keep = smooth[sl]
+            err = np.abs(jv[sl] - fd[sl]) * keep
+            worst = int(np.argmax(err))
+            msg = (trial, name, float(np.linalg.norm(err) / np.linalg.norm(fd[sl])), worst,
+                   float(jv[sl][worst]), float(fd[sl][worst]))
+            assert np.linalg.norm(err) <= 1e-5 * np.linalg.norm(fd[sl] * keep), msg
     e.close()
 
 
