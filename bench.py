@@ -71,6 +71,42 @@ def cpu_baseline(p, seconds, dt):
     }
 
 
+def stationary_leg(worlds):
+    """Newton-Krylov steady state of the precision configuration (64x64-cell worlds, Na/K/Cl/Ca,
+    junction conductances lognormal sigma 0.3) from the initial state: FP32 Krylov under FP64
+    refinement, then the all-FP64 path, timed; the difference of the two steady states."""
+    import torch
+
+    from cells_amd.ensemble import Ensemble
+    from cells_amd.world import StateLayout, build, config4, initial_state
+
+    p = build(config4(worlds=worlds))
+    e = Ensemble(p, "ros2")
+    y0 = e.to_device(initial_state(p))
+    log(f"stationary leg: {p.W} worlds x {p.ncell // p.W} cells")
+    e.stationary(y0, max_newton=1)  # workspace and first touch
+    out = {}
+    states = {}
+    for name, fp32 in (("fp32_inner", True), ("fp64", False)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y, conv, step, iters, rep = e.stationary(y0, fp32_inner=fp32)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        states[name] = y.cpu().numpy()
+        out[name] = {"seconds": dt, "solves_per_s": p.W / dt, "converged": int(conv.sum()),
+                     "newton_iterations": rep["newton_iterations"], "krylov_steps": rep["krylov_steps"],
+                     "worst_step": rep["worst_step"]}
+        log(f"stationary leg ({name}): {dt:.3f} s, {out[name]}")
+    L = StateLayout.of(p)
+    a, b = states["fp32_inner"], states["fp64"]
+    diff = max(float(np.max(np.abs((L.cell(a, L.dn + s) - L.cell(b, L.dn + s)) / p.vol0))) for s in range(p.S))
+    gdiff = float(np.max(np.abs(L.edge(a, L.gate) - L.edge(b, L.gate))))
+    e.close()
+    return {"worlds": p.W, "cells_per_world": p.ncell // p.W, **out,
+            "fp32_vs_fp64_max_mM": diff, "fp32_vs_fp64_max_gate": gdiff}
+
+
 def adjoint_leg(steps, snapshots, tape):
     """Calibration gradient on the adjoint configuration's worlds (256 worlds of 32x32 cells,
     Na/K/Cl, ROS2 dt = 5 ms, 6 log-parameters per cell, Vmem targets every 20 steps): one loss
@@ -142,9 +178,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--parity-worlds", type=int, default=3)
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: plain,parity,cpu,adjoint")
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: plain,parity,cpu,adjoint,stationary")
     ap.add_argument("--adjoint-steps", type=int, default=2000, help="horizon of the adjoint leg")
     ap.add_argument("--adjoint-snapshots", type=int, default=45)
+    ap.add_argument("--stationary-worlds", type=int, default=512)
     ap.add_argument("--adjoint-tape", type=int, default=45,
                     help="recorded steps per unit of the adjoint's binomial schedule (0: none)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -257,6 +294,10 @@ def main():
         parity = max(relerr(vmem(sub, got), vmem(sub, ref)), relerr(conc(sub, got), conc(sub, ref)))
         log(f"parity max relative error {parity:.3e}")
 
+    sta = None
+    if "stationary" not in skip:
+        sta = stationary_leg(args.stationary_worlds)
+
     adj = None
     if "adjoint" not in skip:
         adj = adjoint_leg(args.adjoint_steps, args.adjoint_snapshots, args.adjoint_tape)
@@ -294,6 +335,7 @@ def main():
         "plans": stats,
         "roofline": roofline,
         "adjoint": adj,
+        "stationary": sta,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
