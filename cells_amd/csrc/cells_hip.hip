@@ -119,23 +119,27 @@ struct ShapeDev {
   // multifrontal schedule: supernode records (16 ints each: factor position of the first column,
   // width, front order, update offset, children begin/end (pool-absolute descriptor indices),
   // update-vector offset, first column, pool index of the last column's ranks below its diagonal),
-  // per level 5 boundaries (thread, 8-lane, wavefront and workgroup fronts, end), children (4
-  // ints each: rows below, rel list start, update offset, update-vector offset), rel lists
+  // per level 6 boundaries (thread, 8-lane, wavefront, workgroup-in-LDS and workgroup-in-HBM
+  // fronts, end), children (4 ints each: rows below, rel list start, update offset, update-vector
+  // offset), rel lists
   long o_sn, o_snlev, o_child, o_rel;
   int sn_levels;
   long usz;  // doubles of update matrices per world
   long vsz;  // doubles of the solve's update vectors per world
+  int big_tri;  // doubles of the largest front held in HBM scratch (0: none)
 };
 
 // Multifrontal factor and supernodal solve: one workgroup of kMfThreads per world.
 constexpr int kMfThreads = 256;
-constexpr int kMfLds = 8192;                               // doubles of LDS per workgroup (64 KiB)
+constexpr int kMfLds = 6144;                               // doubles of LDS per workgroup (48 KiB: 3 per CU)
 constexpr int kMfOcta = 8;                                 // lanes of a small group
-constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 7
-constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 22
-constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 63
+constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 6
+constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 19
+constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 54
+constexpr int kMfMaxRows = 128;                            // fronts above kMfLds (order <= 128) use HBM
 constexpr int kMfBatch = 8;                                // global loads in flight per lane
 constexpr int kRecInts = 16;                               // ints per front record in the plan pool
+constexpr int kLevelBounds = 6;                            // per level: first front of each kind, end
 
 __device__ __forceinline__ double& CF(const Params& P, double* y, int f, int i) {
   return y[(long)f * P.ncell + i];
@@ -1219,12 +1223,13 @@ __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double
 }
 
 __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub, double* lds,
-                                int* valid_w) {
+                                double* big, int* valid_w) {
   const int* lev = pool + s.o_snlev;
   const int qa0 = (int)(s.o_sn / kRecInts);
   const int tid = threadIdx.x, wave = tid >> 6, octa = tid / kMfOcta;
   for (int L = 0; L < s.sn_levels; ++L) {
-    const int* bd = lev + 5 * L;  // thread, 8-lane, wavefront and workgroup fronts of level L
+    // thread, 8-lane, wavefront, workgroup (LDS) and workgroup (HBM scratch) fronts of level L
+    const int* bd = lev + kLevelBounds * L;
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
       mf_front<1, kMfThreads>(pool, qa0 + q, fv, ub, lds + tid, 0, valid_w);
     __syncthreads();
@@ -1235,6 +1240,7 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, 
       mf_front<64, 1>(pool, qa0 + q, fv, ub, lds + wave * kMfWaveTri, tid & 63, valid_w);
     __syncthreads();
     for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(pool, qa0 + q, fv, ub, lds, tid, valid_w);
+    for (int q = bd[4]; q < bd[5]; ++q) mf_front<kMfThreads, 1>(pool, qa0 + q, fv, ub, big, tid, valid_w);
   }
   __syncthreads();
 }
@@ -1376,22 +1382,22 @@ __device__ void sv_solve_world(const ShapeDev& s, const int* pool, const double*
   double* Fq = lds + kSvThreadLds + octa * (kMfOcta * SolveLanes<kMfOcta>::NT);
   double* Fw = lds + kSvThreadLds + kSvOctaLds + wave * (64 * SolveLanes<64>::NT);
   for (int L = 0; L < s.sn_levels; ++L) {
-    const int* bd = lev + 5 * L;
+    const int* bd = lev + kLevelBounds * L;
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
       sv_forward_front<1, kMfThreads>(pool, qa0 + q, fv, y, vec, lds + tid, 0);
     for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
       sv_forward_front<kMfOcta, 1>(pool, qa0 + q, fv, y, vec, Fq, tid % kMfOcta);
-    for (int q = bd[2] + wave; q < bd[4]; q += kMfThreads / 64)
+    for (int q = bd[2] + wave; q < bd[5]; q += kMfThreads / 64)
       sv_forward_front<64, 1>(pool, qa0 + q, fv, y, vec, Fw, tid & 63);
     __syncthreads();
   }
   for (int L = s.sn_levels - 1; L >= 0; --L) {
-    const int* bd = lev + 5 * L;
+    const int* bd = lev + kLevelBounds * L;
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
       sv_backward_front<1, kMfThreads>(pool, qa0 + q, fv, y, lds + tid, 0);
     for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
       sv_backward_front<kMfOcta, 1>(pool, qa0 + q, fv, y, Fq, tid % kMfOcta);
-    for (int q = bd[2] + wave; q < bd[4]; q += kMfThreads / 64)
+    for (int q = bd[2] + wave; q < bd[5]; q += kMfThreads / 64)
       sv_backward_front<64, 1>(pool, qa0 + q, fv, y, Fw, tid & 63);
     __syncthreads();
   }
@@ -1402,7 +1408,8 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeD
                                                           const long* foff, const long* uoffw, int w0,
                                                           double coef, const double* mdiag,
                                                           const double* msc, const double* Gj,
-                                                          double* fvals, double* ubuf, int* valid) {
+                                                          double* fvals, double* ubuf, double* bigbuf,
+                                                          const long* boffw, int* valid) {
   extern __shared__ double lds[];
   int w = w0 + blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
@@ -1418,18 +1425,19 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeD
     fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
   }
   __syncthreads();
-  mf_factor_world(s, pool, fv, ubuf + uoffw[w], lds, valid + w);
+  mf_factor_world(s, pool, fv, ubuf + uoffw[w], lds, bigbuf + boffw[w], valid + w);
 }
 
 // Test entry: the multifrontal factor of assembled values in place, per world.
 __global__ __launch_bounds__(kMfThreads) void k_factor_only_mf(const ShapeDev* shapes, const int* shape_of,
                                                                const int* pool, const long* foff,
-                                                               const long* uoffw, double* fvals, double* ubuf) {
+                                                               const long* uoffw, double* fvals, double* ubuf,
+                                                               double* bigbuf, const long* boffw) {
   extern __shared__ double lds[];
   int w = blockIdx.x;
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
-  mf_factor_world(s, pool, fvals + foff[w], ubuf + uoffw[w], lds, nullptr);
+  mf_factor_world(s, pool, fvals + foff[w], ubuf + uoffw[w], lds, bigbuf + boffw[w], nullptr);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2047,6 +2055,8 @@ struct cells_ensemble {
   double* ubuf = nullptr;
   long* voffw = nullptr;  // per world: start of the solve's update vectors in vbuf
   double* vbuf = nullptr;
+  long* boffw = nullptr;  // per world: start of its HBM front scratch in bigbuf
+  double* bigbuf = nullptr;
   std::vector<cells::Supernodes> sn_host;
   size_t factor_lds = 0, solve_lds = 0;  // dynamic LDS bytes of the factor and solve kernels
   int y_in_lds = 1;
@@ -2112,14 +2122,16 @@ double host_logaddexp0(double t) {
 }
 
 // Which lanes hold a front of this order in the multifrontal factor: 0 one thread, 1 eight lanes,
-// 2 one wavefront, 3 the workgroup, 4 none (too large for the workgroup's LDS).
+// 2 one wavefront, 3 the workgroup in LDS, 4 the workgroup in HBM scratch, 5 none (the solve's
+// wavefront holds at most kMfMaxRows rows).
 int32_t front_class(int32_t rows) {
   int64_t tri = (int64_t)rows * (rows + 1) / 2;
   if (tri <= kMfThreadTri) return 0;
   if (tri <= kMfOctaTri) return 1;
   if (tri <= kMfWaveTri) return 2;
   if (tri <= kMfLds) return 3;
-  return 4;
+  if (rows <= kMfMaxRows) return 4;
+  return 5;
 }
 
 int check_world(const cells_world* w) {
@@ -2510,13 +2522,14 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       while (pool.size() % kRecInts) pool.push_back(0);  // records on 64-byte boundaries
       d.o_sn = (long)pool.size();
       const long o_lev = d.o_sn + (long)kRecInts * ns;
-      long o_child = o_lev + 5L * sn.levels;
+      long o_child = o_lev + (long)kLevelBounds * sn.levels;
       o_child += (4 - o_child % 4) % 4;
       const long o_rel = o_child + 4L * (long)sn.child.size();
       if (o_rel + (long)sn.rel.size() > 0x7fffffffL) return fail(CELLS_EINVAL, "plans exceed 2^31 entries");
       for (int t = 0; t < ns; ++t) {
-        if (front_class(sn.rows[t]) > 3)
-          return fail(CELLS_EINVAL, "a front exceeds the workgroup's LDS (order above 127)");
+        if (front_class(sn.rows[t]) > 4) return fail(CELLS_EINVAL, "a front of order above 128");
+        if (front_class(sn.rows[t]) == 4)
+          d.big_tri = std::max(d.big_tri, sn.rows[t] * (sn.rows[t] + 1) / 2);
         const int64_t last = sn.first[t] + sn.width[t] - 1;
         int32_t r16[kRecInts] = {(int32_t)s.column_start[sn.first[t]], sn.width[t], sn.rows[t],
                                  (int32_t)sn.update_offset[t], (int32_t)(o_child / 4 + sn.child_ptr[t]),
@@ -2527,14 +2540,14 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       d.sn_levels = sn.levels;
       d.o_snlev = (long)pool.size();
       {
-        std::vector<int> bounds(5 * (size_t)sn.levels, 0);
+        std::vector<int> bounds(kLevelBounds * (size_t)sn.levels, 0);
         int t = 0;
         for (int L = 0; L < sn.levels; ++L) {
-          for (int c = 0; c < 4; ++c) {
-            bounds[5 * L + c] = t;
+          for (int c = 0; c < kLevelBounds - 1; ++c) {
+            bounds[kLevelBounds * L + c] = t;
             while (t < ns && sn.level[t] == L && front_class(sn.rows[t]) == c) ++t;
           }
-          bounds[5 * L + 4] = t;
+          bounds[kLevelBounds * L + kLevelBounds - 1] = t;
           if (t < ns && sn.level[t] == L) return fail(CELLS_EINVAL, "front schedule out of order");
         }
         pool.insert(pool.end(), bounds.begin(), bounds.end());
@@ -2566,10 +2579,11 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
     e->foff_host[0] = 0;
     for (int k = 0; k < e->W; ++k) e->foff_host[k + 1] = e->foff_host[k] + sd[e->shape_of_host[k]].nnz;
     e->factor_total = e->foff_host[e->W];
-    std::vector<long> uoffw(e->W + 1, 0), voffw(e->W + 1, 0);
+    std::vector<long> uoffw(e->W + 1, 0), voffw(e->W + 1, 0), boffw(e->W + 1, 0);
     for (int k = 0; k < e->W; ++k) {
       uoffw[k + 1] = uoffw[k] + sd[e->shape_of_host[k]].usz;
       voffw[k + 1] = voffw[k] + sd[e->shape_of_host[k]].vsz;
+      boffw[k + 1] = boffw[k] + sd[e->shape_of_host[k]].big_tri;
     }
     if ((rc = upload(e.get(), &e->shapes, sd.data(), sd.size())) ||
         (rc = upload(e.get(), &e->shape_of, e->shape_of_host.data(), e->shape_of_host.size())) ||
@@ -2580,7 +2594,9 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
         (rc = upload(e.get(), &e->uoffw, uoffw.data(), uoffw.size())) ||
         (rc = alloc(e.get(), &e->ubuf, std::max<long>(uoffw[e->W], 1))) ||
         (rc = upload(e.get(), &e->voffw, voffw.data(), voffw.size())) ||
-        (rc = alloc(e.get(), &e->vbuf, std::max<long>(voffw[e->W], 1))))
+        (rc = alloc(e.get(), &e->vbuf, std::max<long>(voffw[e->W], 1))) ||
+        (rc = upload(e.get(), &e->boffw, boffw.data(), boffw.size())) ||
+        (rc = alloc(e.get(), &e->bigbuf, std::max<long>(boffw[e->W], 1))))
       return rc;
   }
   HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
@@ -2656,7 +2672,7 @@ void launch_factor(cells_ensemble* e, hipStream_t st, int w0, int nw, double coe
   else
     hipLaunchKernelGGL(k_factor_mf, dim3(nw), dim3(kMfThreads), sizeof(double) * kMfLds, st, e->P, e->shapes,
                        e->shape_of, e->pool, e->foff, e->uoffw, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals,
-                       e->ubuf, e->valid);
+                       e->ubuf, e->bigbuf, e->boffw, e->valid);
 }
 
 int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int world,
@@ -2859,7 +2875,7 @@ extern "C" int cells_factor(cells_ensemble* e, void* stream, double* values) {
   else
     hipLaunchKernelGGL(k_factor_only_mf, dim3(e->W), dim3(kMfThreads), sizeof(double) * kMfLds,
                        static_cast<hipStream_t>(stream), e->shapes, e->shape_of, e->pool, e->foff, e->uoffw,
-                       values, e->ubuf);
+                       values, e->ubuf, e->bigbuf, e->boffw);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }
