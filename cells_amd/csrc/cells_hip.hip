@@ -131,7 +131,7 @@ struct ShapeDev {
 
 // Multifrontal factor and supernodal solve: one workgroup of kMfThreads per world.
 constexpr int kMfThreads = 256;
-constexpr int kMfLds = 6144;                               // doubles of LDS per workgroup (48 KiB: 3 per CU)
+constexpr int kMfLds = 5120;                               // doubles of LDS per workgroup (40 KiB: 4 per CU)
 constexpr int kMfOcta = 8;                                 // lanes of a small group
 constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 6
 constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 19
