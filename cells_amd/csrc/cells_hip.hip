@@ -137,6 +137,13 @@ constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: 
 constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 19
 constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 54
 constexpr int kMfMaxRows = 128;                            // fronts above kMfLds (order <= 128) use HBM
+
+// The largest front order whose packed triangle fits `tri` doubles.
+constexpr int max_order(int tri) {
+  int m = 0;
+  while ((m + 1) * (m + 2) / 2 <= tri) ++m;
+  return m;
+}
 constexpr int kMfBatch = 8;                                // global loads in flight per lane
 constexpr int kRecInts = 16;                               // ints per front record in the plan pool
 constexpr int kLevelBounds = 6;                            // per level: first front of each kind, end
@@ -932,14 +939,16 @@ struct SolveArgs {
 __device__ void sv_solve_world(const ShapeDev& s, const int* pool, const double* fv, double* y, double* vec,
                                double* lds);
 
+template <bool SN>
 __device__ __forceinline__ void solve_any(const ShapeDev& s, const PlanView& v, const int* pool,
                                           const double* fv, double* y, const SolveArgs& sa, int w,
                                           double* lds) {
-  if (sa.sn) sv_solve_world(s, pool, fv, y, sa.vbuf + sa.voffw[w], lds);
+  if constexpr (SN) sv_solve_world(s, pool, fv, y, sa.vbuf + sa.voffw[w], lds);
   else solve_world(s, v, fv, y);
 }
 
 // Solve (C + coef G) dphi = q(K) on each world's factor: rhs' = q s, dphi = x s.
+template <bool SN>
 __global__ __launch_bounds__(kMfThreads) void k_solve_stage(Params P, const ShapeDev* shapes,
                                                             const int* shape_of, const int* pool,
                                                             const long* foff, int w0,
@@ -973,7 +982,7 @@ __global__ __launch_bounds__(kMfThreads) void k_solve_stage(Params P, const Shap
   }
   if (part == 0) {
     __syncthreads();
-    solve_any(s, v, pool, fv, y, sa, w, lds);
+    solve_any<SN>(s, v, pool, fv, y, sa, w, lds);
     __syncthreads();
   }
   if (part != 1) {
@@ -1000,6 +1009,7 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_only(const ShapeDev* sh
   for (int p = s.lds_pt + threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = lds[p - s.lds_pt];
 }
 
+template <bool SN>
 __global__ __launch_bounds__(kMfThreads) void k_solve_only(const ShapeDev* shapes, const int* shape_of,
                                                            const int* pool, const long* foff,
                                                            const int* cell_off, const double* fvals,
@@ -1015,7 +1025,7 @@ __global__ __launch_bounds__(kMfThreads) void k_solve_only(const ShapeDev* shape
     for (int k = threadIdx.x; k < s.n; k += blockDim.x) lds[k] = rhs[c0 + v.order[k]];
   if (part == 0) {
     __syncthreads();
-    solve_any(s, v, pool, fvals + foff[w], lds, sa, w, shared_y);
+    solve_any<SN>(s, v, pool, fvals + foff[w], lds, sa, w, shared_y);
     __syncthreads();
   }
   if (part != 1)
@@ -1067,7 +1077,9 @@ template <int G>
 struct FrontLanes {
   static constexpr int RW = G < 64 ? G : 64;
   static constexpr int CG = G / RW;
-  static constexpr int NT = G == 1 ? 7 : G == kMfOcta ? 3 : G == 64 ? 1 : 2;  // rows per lane
+  static constexpr int rows = G == 1 ? max_order(kMfThreadTri) : G == kMfOcta ? max_order(kMfOctaTri)
+                              : G == 64 ? max_order(kMfWaveTri) : kMfMaxRows;
+  static constexpr int NT = (rows + RW - 1) / RW;  // rows per lane
 };
 
 constexpr int kMfKB = 8;  // columns per block of a wavefront's or workgroup's partial factor
@@ -1254,16 +1266,21 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, 
 // update vector. Backward, roots first: the values below a front are its ancestors' solution
 // (gathered from y), one product with the front's rows below, then the pivots last to first.
 
-// Rows per lane of a solve group: one thread (order <= 7), 8 lanes (<= 24), a wavefront (<= 128).
+// Rows per lane of a solve group: one thread, 8 lanes, or a wavefront (the factor's wavefront and
+// workgroup fronts alike, order <= kMfMaxRows).
 template <int G>
 struct SolveLanes {
-  static constexpr int NT = G == 1 ? 7 : G == kMfOcta ? 3 : 2;
+  static constexpr int rows = G == 1 ? max_order(kMfThreadTri) : G == kMfOcta ? max_order(kMfOctaTri)
+                              : kMfMaxRows;
+  static constexpr int NT = (rows + G - 1) / G;
 };
+static_assert(FrontLanes<1>::NT * 1 >= max_order(kMfThreadTri) && FrontLanes<kMfThreads>::NT * 64 >= kMfMaxRows,
+              "every front row has a lane");
 // LDS of the solve's workgroup: separate regions for the thread, 8-lane and wavefront fronts (the
 // three kinds of one level run without a barrier between them)
-constexpr int kSvThreadLds = kMfThreads * 7;
-constexpr int kSvOctaLds = (kMfThreads / kMfOcta) * kMfOcta * 3;
-constexpr int kSvWaveLds = (kMfThreads / 64) * 64 * 2;
+constexpr int kSvThreadLds = kMfThreads * SolveLanes<1>::NT;
+constexpr int kSvOctaLds = (kMfThreads / kMfOcta) * kMfOcta * SolveLanes<kMfOcta>::NT;
+constexpr int kSvWaveLds = (kMfThreads / 64) * 64 * SolveLanes<64>::NT;
 constexpr int kSvLds = kSvThreadLds + kSvOctaLds + kSvWaveLds;
 
 template <int RW>
@@ -1943,6 +1960,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_dtrans(Params P, double coef, co
 
 // w = M^-1 dbar on the step's factor (workgroup per world); then rate_bar += q^T w and the
 // slopes' cotangent of M: -coef w delta^T through G = F' sum_s z_s slope_s.
+template <bool SN>
 __global__ __launch_bounds__(kMfThreads) void k_adj_solve(Params P, const ShapeDev* shapes,
                                                           const int* shape_of, const int* pool,
                                                           const long* foff, double coef,
@@ -1965,7 +1983,7 @@ __global__ __launch_bounds__(kMfThreads) void k_adj_solve(Params P, const ShapeD
     }
   if (part == 0) {
     __syncthreads();
-    solve_any(s, v, pool, fv, lds, sa, w, shared_y);
+    solve_any<SN>(s, v, pool, fv, lds, sa, w, shared_y);
     __syncthreads();
   }
   if (part == 1) return;
@@ -2721,7 +2739,7 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     launch_factor(e, st, w0, nw, coef);
     mark("k_factor");
     solve_stage(e, st, nw, e->fvals, [&](int part) {
-    hipLaunchKernelGGL(k_solve_stage, dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
+    hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
                        e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds, solve_args(e), part);
   });
     mark("k_solve_stage");
@@ -2737,7 +2755,7 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
                        c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
     mark("k_cells");
     solve_stage(e, st, nw, e->fvals, [&](int part) {
-    hipLaunchKernelGGL(k_solve_stage, dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
+    hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
                        e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds, solve_args(e), part);
   });
     mark("k_solve_stage");
@@ -2885,7 +2903,7 @@ extern "C" int cells_solve(cells_ensemble* e, void* stream, const double* values
   HIP_TRY(hipSetDevice(e->device));
   size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
   solve_stage(e, static_cast<hipStream_t>(stream), e->W, values, [&](int part) {
-    hipLaunchKernelGGL(k_solve_only, dim3(e->W), solve_block(e), solve_lds_bytes(e), static_cast<hipStream_t>(stream), e->shapes,
+    hipLaunchKernelGGL((e->pairwise ? k_solve_only<false> : k_solve_only<true>), dim3(e->W), solve_block(e), solve_lds_bytes(e), static_cast<hipStream_t>(stream), e->shapes,
                      e->shape_of, e->pool, e->foff, e->P.cell_off, values, rhs, x, e->ysc, e->y_in_lds, solve_args(e), part);
   });
   HIP_TRY(hipGetLastError());
@@ -2952,7 +2970,7 @@ int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const 
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k2bar,
                      r.d2, e->dbar, e->gmbar, e->gjbar);
   solve_stage(e, st, W, r.fvals, [&](int part) {
-    hipLaunchKernelGGL(k_adj_solve, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+    hipLaunchKernelGGL((e->pairwise ? k_adj_solve<false> : k_adj_solve<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, coef, r.msc, r.fvals, e->dbar, d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc,
                      e->y_in_lds, solve_args(e), part);
   });
@@ -2969,7 +2987,7 @@ int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const 
   hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k1bar,
                      r.d1, e->dbar, e->gmbar, e->gjbar);
   solve_stage(e, st, W, r.fvals, [&](int part) {
-    hipLaunchKernelGGL(k_adj_solve, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+    hipLaunchKernelGGL((e->pairwise ? k_adj_solve<false> : k_adj_solve<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, coef, r.msc, r.fvals, e->dbar, d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc,
                      e->y_in_lds, solve_args(e), part);
   });
@@ -3029,7 +3047,7 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                      e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
   launch_factor(e, st, 0, W, coef);
   solve_stage(e, st, W, e->fvals, [&](int part) {
-    hipLaunchKernelGGL(k_solve_stage, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+    hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds, solve_args(e), part);
   });
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
@@ -3040,7 +3058,7 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
   solve_stage(e, st, W, e->fvals, [&](int part) {
-    hipLaunchKernelGGL(k_solve_stage, dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
+    hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds, solve_args(e), part);
   });
   StepRecord r{e->A, e->Y1, e->fvals, e->msc, e->gm, e->gj, e->d1, e->d2};
