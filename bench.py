@@ -252,11 +252,23 @@ def main():
                     "traffic_source": "profiles/ros2_step_4096_worlds.md (rocprofv3 PMC, copy-calibrated)",
                     "kernel": "k_factor_mf", "launch_ms": fact_ms,
                     "algorithmic": f"sum over worlds of sum_j c_j^2 = {stats['flops']:.4g} flops"}
+    elif dominant == "k_solve_stage":
+        # one stage solve: the factor read by both sweeps (2 x 8 x nnz(L)) plus the right-hand side's
+        # amounts (S per cell), the Jacobi scale, the result and the rank-ordered vector (read and
+        # written by each sweep): 8 (S + 6) bytes per cell
+        avg_ms = float(np.mean(launches))
+        algo = 2 * 8 * stats["nnz"] + 8 * (p.S + 6) * p.ncell
+        achieved = algo / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_solve_stage"),
+                    "traffic_source": "profiles/ros2_step_4096_worlds.md (rocprofv3 PMC, copy-calibrated)",
+                    "kernel": "k_solve_stage", "launch_ms": avg_ms,
+                    "algorithmic": f"2 x 8 x nnz(L) + 8 (S + 6) ncell = {algo:.4g} bytes per launch"}
     else:
         avg_ms = float(np.mean(launches))
         achieved = BYTES_PER_CELL_STEP * p.ncell / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant), "kernel": dominant,
                     "launch_ms": avg_ms}
     step_gbs = BYTES_PER_CELL_STEP * p.ncell / (ms_per_step * 1e-3) / 1e9
 
