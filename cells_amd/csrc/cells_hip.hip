@@ -1653,11 +1653,10 @@ __device__ __forceinline__ double ratio_of(double residual, double bound) {
   return isnan(r) ? INFINITY : r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_device_checks(Params P, const double* prev, const double* y,
-                                                          double steps, double material_atol,
-                                                          int* failed, double* ratios) {
-  __shared__ double sh[kBlock];
-  int w = blockIdx.x;
+// World w's ledger checks of the state y against its predecessor prev (whole workgroup); the
+// failure bits and the four bound ratios on thread 0.
+__device__ int ledger_checks(const Params& P, const double* prev, const double* y, int w, double steps,
+                           double material_atol, double* sh, double* rat) {
   int c0 = P.cell_off[w], c1 = P.cell_off[w + 1], e0 = P.edge_off[w], e1 = P.edge_off[w + 1];
   double rounding = kLedgerStepRounding * kEps * steps;
   bool domain = true, dissip = false;
@@ -1740,12 +1739,51 @@ __global__ __launch_bounds__(kBlock) void k_device_checks(Params P, const double
     if (!(rq <= 1.0)) bits |= 8;
     if (di > 0) bits |= 16;
     if (!(rwat <= 1.0)) bits |= 32;
-    failed[w] = bits;
-    ratios[4 * w + 0] = rmat;
-    ratios[4 * w + 1] = rtot;
-    ratios[4 * w + 2] = rq;
-    ratios[4 * w + 3] = rwat;
+    rat[0] = rmat;
+    rat[1] = rtot;
+    rat[2] = rq;
+    rat[3] = rwat;
+    return bits;
   }
+  return 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_device_checks(Params P, const double* prev, const double* y,
+                                                          double steps, double material_atol,
+                                                          int* failed, double* ratios) {
+  __shared__ double sh[kBlock];
+  const int w = blockIdx.x;
+  double rat[4];
+  const int bits = ledger_checks(P, prev, y, w, steps, material_atol, sh, rat);
+  if (threadIdx.x == 0) {
+    failed[w] = bits;
+    for (int k = 0; k < 4; ++k) ratios[4 * w + k] = rat[k];
+  }
+}
+
+// The checks as the step's epilogue: each live world's candidate state (the buffer the commit
+// would select) against its current one; a failing world refuses the step (the commit keeps its
+// state and stops it) and its bits are or-ed into bits_out. steps = the ledgers' step count
+// after this step (the device counter).
+__global__ __launch_bounds__(kBlock) void k_step_checks(Params P, const double* A, const double* B, const int* sel,
+                                                        const int* alive, int* valid, const long long* counter,
+                                                        double material_atol, int w0, int* bits_out) {
+  __shared__ double sh[kBlock];
+  const int w = w0 + blockIdx.x;
+  if (!alive[w] || !valid[w]) return;
+  const int sw = sel[w];
+  const double* cur = current(A, B, sw);
+  const double* cand = next(const_cast<double*>(A), const_cast<double*>(B), sw);
+  double rat[4];
+  const int bits = ledger_checks(P, cur, cand, w, (double)counter[0], material_atol, sh, rat);
+  if (threadIdx.x == 0 && bits) {
+    valid[w] = 0;
+    bits_out[w] |= bits;
+  }
+}
+
+__global__ void k_tick(long long* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += 1;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2083,6 +2121,13 @@ struct cells_ensemble {
   double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *dphi2 = nullptr, *fvals = nullptr;
   double* ysc = nullptr;
   int *sel = nullptr, *alive = nullptr, *valid = nullptr;
+  // the ledger checks as the step's epilogue (cells_step_checks)
+  bool checks_on = false;
+  double check_atol = 1e-12;
+  long long check_steps = 0;  // the ledgers' step count of the next cells_step's input
+  long long* step_counter = nullptr;
+  int* check_bits = nullptr;
+  bool graph_checks = false;
   // step records for the adjoint (cells_tape_reserve): per slot the step's input state, stage-1
   // state, factor values, Jacobi scales, route slopes and both stage increments, field-major
   int tape_slots = 0;
@@ -2397,7 +2442,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       (rc = alloc(e.get(), &e->K1, total)) || (rc = alloc(e.get(), &e->contrib1, S * nc)) ||
       (rc = alloc(e.get(), &e->ysc, nc)) ||
       (rc = alloc(e.get(), &e->sel, e->W)) || (rc = alloc(e.get(), &e->alive, e->W)) ||
-      (rc = alloc(e.get(), &e->valid, e->W)))
+      (rc = alloc(e.get(), &e->valid, e->W)) || (rc = alloc(e.get(), &e->step_counter, 1)) ||
+      (rc = alloc(e.get(), &e->check_bits, e->W)))
     return rc;
   if (method == CELLS_METHOD_ROS2) {
     if ((rc = alloc(e.get(), &e->Y1, total)) || (rc = alloc(e.get(), &e->K2, total)) ||
@@ -2701,6 +2747,7 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
   const Params& P = e->P;
   int w0 = world < 0 ? 0 : world;
   int nw = world < 0 ? e->W : 1;
+  if (e->checks_on && world <= 0) hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, e->step_counter);
   int c0 = (int)e->cell_off[w0], c1 = (int)e->cell_off[w0 + nw];
   int ed0 = (int)e->edge_off[w0], ed1 = (int)e->edge_off[w0 + nw];
   int nc = c1 - c0, ne = ed1 - ed0;
@@ -2770,6 +2817,11 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_water, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, 1, 0.5 * dt, c, e->valid);
     mark("k_water");
   }
+  if (e->checks_on) {
+    hipLaunchKernelGGL(k_step_checks, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->alive, e->valid,
+                       e->step_counter, e->check_atol, w0, e->check_bits);
+    mark("k_step_checks");
+  }
   hipLaunchKernelGGL(k_commit, dim3(blocks(nw)), dim3(kBlock), 0, st, w0, nw, e->sel, e->alive, e->valid);
   mark("k_commit");
   return CELLS_OK;
@@ -2792,9 +2844,16 @@ extern "C" int cells_step(cells_ensemble* e, void* stream, const double* y_in, d
   // water world, world fields) must hold the current values in both.
   HIP_TRY(hipMemcpyAsync(e->B, y_in, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_init, dim3(blocks(e->W)), dim3(kBlock), 0, st, e->W, active, e->sel, e->alive, e->valid);
+  if (e->checks_on) {
+    HIP_TRY(hipMemcpyAsync(e->step_counter, &e->check_steps, sizeof(long long), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(e->check_bits, 0, sizeof(int) * e->W, st));
+    HIP_TRY(hipStreamSynchronize(st));  // the counter's host copy is read before it advances
+    e->check_steps += nsteps;
+  }
   HIP_TRY(hipGetLastError());
   if (batched) {
-    bool same = e->graph && e->graph_dt == dt && std::memcmp(&e->graph_ctl, &c, sizeof(Ctl)) == 0;
+    bool same = e->graph && e->graph_dt == dt && std::memcmp(&e->graph_ctl, &c, sizeof(Ctl)) == 0 &&
+                e->graph_checks == e->checks_on;
     if (!same && nsteps > 0) {
       if (e->graph) {
         HIP_TRY(hipGraphExecDestroy(e->graph));
@@ -2810,6 +2869,7 @@ extern "C" int cells_step(cells_ensemble* e, void* stream, const double* y_in, d
       if (inst != hipSuccess) return fail(CELLS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
       e->graph_dt = dt;
       e->graph_ctl = c;
+      e->graph_checks = e->checks_on;
     }
     for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(e->graph, st));
   } else {
@@ -2919,6 +2979,24 @@ extern "C" int cells_device_checks(cells_ensemble* e, void* stream, const double
   hipLaunchKernelGGL(k_device_checks, dim3(e->W), dim3(kBlock), 0, static_cast<hipStream_t>(stream), e->P,
                      previous, y, steps, material_atol, failed_out, ratios_out);
   HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_step_checks(cells_ensemble* e, int32_t enabled, double material_atol, int64_t steps_before) {
+  if (!e) return fail(CELLS_EINVAL, "NULL argument");
+  if (!(material_atol >= 0) || steps_before < 0) return fail(CELLS_EINVAL, "tolerance and step count must be >= 0");
+  HIP_TRY(hipSetDevice(e->device));
+  e->checks_on = enabled != 0;
+  e->check_atol = material_atol;
+  e->check_steps = steps_before;
+  return CELLS_OK;
+}
+
+extern "C" int cells_step_check_bits(cells_ensemble* e, void* stream, int32_t* bits_out) {
+  if (!e || !bits_out) return fail(CELLS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipMemcpyAsync(bits_out, e->check_bits, sizeof(int) * e->W, hipMemcpyDeviceToDevice,
+                         static_cast<hipStream_t>(stream)));
   return CELLS_OK;
 }
 

@@ -172,6 +172,18 @@ class Ensemble:
                                               v(paj_bar), v(valid)))
         return yin, pam_bar, cap_bar, paj_bar, valid
 
+    def step_checks(self, enabled=True, material_atol=1e-12, steps_before=0):
+        """Run the ledger checks as each step's epilogue (a failing world refuses the step)."""
+        _lib.check(_lib.load().cells_step_checks(self._h, 1 if enabled else 0, float(material_atol),
+                                                 int(steps_before)))
+
+    def check_bits(self):
+        """Failure bits per world of the last step call with checks on (device int32 [W])."""
+        t = self.torch
+        bits = t.empty(self.packed.W, dtype=t.int32, device=self.device)
+        _lib.check(_lib.load().cells_step_check_bits(self._h, self._stream(), ctypes.c_void_p(bits.data_ptr())))
+        return bits
+
     def reserve_tape(self, slots):
         """Reserve `slots` adjoint step records on the device (0 frees them); returns bytes."""
         nbytes = ctypes.c_int64()

@@ -32,7 +32,7 @@ EXPORTS = (
     "cells_factor_size", "cells_factor", "cells_solve", "cells_world_positions",
     "cells_kernel_times", "cells_device_checks", "cells_step_vjp", "cells_vmem_loss",
     "cells_set_parameters", "cells_plan_stats", "cells_tape_reserve", "cells_step_record",
-    "cells_step_vjp_recorded", "cells_stationary", "cells_stationary_residual", "cells_stationary_jvp",
+    "cells_step_vjp_recorded", "cells_step_checks", "cells_step_check_bits", "cells_stationary", "cells_stationary_residual", "cells_stationary_jvp",
 )
 
 
@@ -121,6 +121,8 @@ def load():
                                    vp, vp, vp, vp, vp]
     lib.cells_vmem_loss.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, vp]
     lib.cells_tape_reserve.argtypes = [vp, ctypes.c_int32, _i64p]
+    lib.cells_step_checks.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int64]
+    lib.cells_step_check_bits.argtypes = [vp, vp, vp]
     lib.cells_stationary.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ControlStruct),
                                      ctypes.POINTER(StationaryOptions), vp, vp, vp,
                                      ctypes.POINTER(StationaryReport)]

@@ -142,6 +142,18 @@ int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const doubl
                    double dt, const cells_control* control, double* ybar_in, double* pam_bar,
                    double* cap_bar, double* paj_bar, int32_t* valid_out);
 
+/* The ledger checks of cells_device_checks as each step's epilogue (execution/vector_windows.py:
+ * 374-410 with models/surface_pump_checks.py:112): for the cells_step calls that follow, every
+ * live world's candidate state is checked against its current one before the commit, and a world
+ * that fails refuses the step (keeps its state and stops, as a domain refusal does) with its
+ * failure bits (the bits of cells_device_checks) or-ed for the call. steps_before is the ledgers'
+ * step count of the next input state (the rounding allowance grows with it); it advances by
+ * nsteps with each call. enabled = 0 turns the epilogue off. */
+int cells_step_checks(cells_ensemble* e, int32_t enabled, double material_atol, int64_t steps_before);
+
+/* The failure bits per world (int32, device) of the last cells_step call with checks on. */
+int cells_step_check_bits(cells_ensemble* e, void* stream, int32_t* bits_out);
+
 /* Step tape for the adjoint (no counterpart in the reference, which has no adjoint). Reserves
  * `slots` step records in device memory (0 frees them); each record keeps what the reverse of
  * one ROS2 step reads: the input state, the stage-1 state, the factor values with the Jacobi

@@ -452,3 +452,38 @@ def test_autograd_function_gradient_meets_central_differences(torch):
         an = sum(float((g * s * d).sum()) for g, s, d in zip(grads, scale, dirs))
         assert abs(fd - an) <= 1e-6 * abs(fd) + 1e-18, (trial, fd, an)
     e.close()
+
+
+def test_fused_ledger_checks_refuse_only_the_corrupted_world(torch):
+    """With the checks as the step's epilogue: a clean run is bit for bit the unchecked one with no
+    bits; a world whose material ledger is corrupted refuses its step (state kept, ok = 0, the
+    material bit set) while the others advance exactly as without checks."""
+    p = build(config1(worlds=4, cells=(30, 120), seed=17))
+    L = StateLayout.of(p)
+    e = ensemble(p, "ros2")
+    y0 = initial_state(p)
+    ref, ok_ref = e.step(y0, 5e-3, nsteps=4)
+    e.step_checks(True, material_atol=1e-12, steps_before=0)
+    say("fused checks, clean")
+    got, ok = e.step(y0, 5e-3, nsteps=4)
+    assert torch.equal(ref, got) and bool(ok.all()) and bool(ok_ref.all())
+    assert int(e.check_bits().abs().sum()) == 0
+    # corrupt world 2's first cell's membrane transfer of K
+    bad = y0.copy()
+    i = int(p.cell_off[2])
+    L.cell(bad, L.mtr + 1)[i] += 1e-3
+    e.step_checks(True, material_atol=1e-12, steps_before=0)
+    say("fused checks, planted ledger error")
+    got, ok = e.step(bad, 5e-3, nsteps=4)
+    bits = e.check_bits().cpu().numpy()
+    okn = ok.cpu().numpy()
+    assert okn.tolist() == [1, 1, 0, 1] and bits[2] & 2 and bits[[0, 1, 3]].sum() == 0
+    g = got.cpu().numpy()
+    r = ref.cpu().numpy()
+    for w in (0, 1, 3):
+        c0, c1 = p.cell_off[w], p.cell_off[w + 1]
+        assert np.array_equal(L.cell(g, L.dn)[c0:c1], L.cell(r, L.dn)[c0:c1])
+    c0, c1 = p.cell_off[2], p.cell_off[3]
+    assert np.array_equal(L.cell(g, L.dn)[c0:c1], L.cell(bad, L.dn)[c0:c1])
+    e.step_checks(False)
+    e.close()
