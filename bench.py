@@ -224,6 +224,18 @@ def main():
     all_ok = bool(ok.cpu().numpy().all())
     final = y.cpu().numpy()
     log(f"batched: {ms_per_step:.3f} ms/step, {cell_steps:.4g} cell-steps/s, all accepted {all_ok}")
+    # the same steps with the ledger checks as each step's epilogue
+    e.step_checks(True, material_atol=1e-12, steps_before=args.warmup)
+    e.step(yw, dt, nsteps=1, out=out, ok=ok)  # captures the checked graph
+    torch.cuda.synchronize()
+    e.step_checks(True, material_atol=1e-12, steps_before=args.warmup)
+    t0 = time.perf_counter()
+    yc, okc = e.step(yw, dt, nsteps=args.steps, out=out, ok=ok)
+    torch.cuda.synchronize()
+    checked_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    checked_bits = int(e.check_bits().abs().sum())
+    e.step_checks(False)
+    log(f"with the ledger checks as epilogue: {checked_ms:.3f} ms/step, failure bits {checked_bits}")
 
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 8
@@ -338,6 +350,8 @@ def main():
                    "method": "ros2"},
         "plain_value": plain,
         "plain_worlds": min(args.plain_worlds, p.W) if plain else None,
+        "ms_per_step_with_checks": checked_ms,
+        "check_failures": checked_bits,
         "batched_over_plain": (cell_steps / plain) if plain else None,
         "all_worlds_accepted": all_ok,
         "parity_max_rel_err": parity,
