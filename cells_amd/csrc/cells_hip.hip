@@ -1122,11 +1122,12 @@ __device__ void mf_trailing_update(double* F, int m, int jb, int je, int lane) {
   }
 }
 
-// One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE].
-template <int G, int STRIDE>
+// One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE]; fronts of at
+// most MR rows.
+template <int G, int STRIDE, int MR = FrontLanes<G>::rows>
 __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double* F, int lane, int* valid_w) {
   using FL = FrontLanes<G>;
-  constexpr int RW = FL::RW, CG = FL::CG, NT = FL::NT;
+  constexpr int RW = FL::RW, CG = FL::CG, NT = (MR + RW - 1) / RW;
   const int lr = lane % RW, lc = lane / RW;
   const int4 ra4 = reinterpret_cast<const int4*>(pool)[4 * qa];
   const int4 rb4 = reinterpret_cast<const int4*>(pool)[4 * qa + 1];
@@ -1234,6 +1235,8 @@ __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double
   group_sync<G>();
 }
 
+// The factor of one world; fronts in HBM scratch hold at most BIGROWS rows.
+template <int BIGROWS = kMfMaxRows>
 __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub, double* lds,
                                 double* big, int* valid_w) {
   const int* lev = pool + s.o_snlev;
@@ -1252,7 +1255,7 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, 
       mf_front<64, 1>(pool, qa0 + q, fv, ub, lds + wave * kMfWaveTri, tid & 63, valid_w);
     __syncthreads();
     for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(pool, qa0 + q, fv, ub, lds, tid, valid_w);
-    for (int q = bd[4]; q < bd[5]; ++q) mf_front<kMfThreads, 1>(pool, qa0 + q, fv, ub, big, tid, valid_w);
+    for (int q = bd[4]; q < bd[5]; ++q) mf_front<kMfThreads, 1, BIGROWS>(pool, qa0 + q, fv, ub, big, tid, valid_w);
   }
   __syncthreads();
 }
@@ -3273,3 +3276,4 @@ extern "C" int cells_plan_stats(const cells_ensemble* e, int64_t* nnz_total, dou
 }
 
 #include "stationary.inc"
+#include "field.inc"

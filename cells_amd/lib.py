@@ -32,7 +32,10 @@ EXPORTS = (
     "cells_factor_size", "cells_factor", "cells_solve", "cells_world_positions",
     "cells_kernel_times", "cells_device_checks", "cells_step_vjp", "cells_vmem_loss",
     "cells_set_parameters", "cells_plan_stats", "cells_tape_reserve", "cells_step_record",
-    "cells_step_vjp_recorded", "cells_step_checks", "cells_step_check_bits", "cells_stationary", "cells_stationary_residual", "cells_stationary_jvp",
+    "cells_step_vjp_recorded", "cells_step_checks", "cells_step_check_bits", "cells_stationary",
+    "cells_stationary_residual", "cells_stationary_jvp", "cells_field_create", "cells_field_destroy",
+    "cells_field_state_size", "cells_field_step", "cells_field_refresh", "cells_field_plan_stats",
+    "cells_field_kernel_times",
 )
 
 
@@ -74,6 +77,12 @@ class WorldStruct(ctypes.Structure):
         ("snd", _i64p), ("rcv", _i64p), ("paj", _f64p),
         ("inc_ptr", _i64p), ("inc_edge", _i64p), ("inc_out", _i32p),
     ]
+
+
+class FieldGridStruct(ctypes.Structure):
+    _fields_ = [("side", ctypes.c_int32), ("pad", ctypes.c_int32), ("cell_node", _i32p),
+                ("node_volume", ctypes.c_double), ("face_factor", ctypes.c_double * 4),
+                ("node_n0", _f64p), ("node_q0", _f64p)]
 
 
 class ControlStruct(ctypes.Structure):
@@ -142,6 +151,17 @@ def load():
     lib.cells_symbolic_export.argtypes = [ctypes.c_void_p, _i32p, _i32p, _i64p, _i32p, _i32p,
                                           _i32p, _i32p]
     lib.cells_symbolic_free.argtypes = [ctypes.c_void_p]
+    lib.cells_field_create.argtypes = [ctypes.POINTER(WorldStruct), ctypes.POINTER(FieldGridStruct),
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    lib.cells_field_destroy.argtypes = [vp]
+    lib.cells_field_state_size.argtypes = [vp]
+    lib.cells_field_state_size.restype = ctypes.c_int64
+    lib.cells_field_step.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_int32,
+                                     ctypes.POINTER(ControlStruct), vp, vp]
+    lib.cells_field_refresh.argtypes = [vp]
+    lib.cells_field_plan_stats.argtypes = [vp, _i64p, _f64p, _i32p, _i32p, _i32p]
+    lib.cells_field_kernel_times.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
+                                             ctypes.c_int32, ctypes.c_int32, _f64p, ctypes.c_char_p, _i32p]
     _lib = lib
     return lib
 
@@ -185,3 +205,20 @@ def world_struct(p):
     w.inc_out = ptr(arr("inc_out", np.int32), _i32p)
     w._keep = keep
     return w
+
+
+def field_grid_struct(fp):
+    """The cells_field_grid view of a cells_amd.field.FieldPacked (arrays kept alive on it)."""
+    keep = [np.ascontiguousarray(fp.cell_node, dtype=np.int32),
+            np.ascontiguousarray(fp.node_n0, dtype=np.float64),
+            np.ascontiguousarray(fp.node_q0, dtype=np.float64)]
+    g = FieldGridStruct()
+    g.side = fp.side
+    g.cell_node = ptr(keep[0], _i32p)
+    g.node_volume = fp.node_volume
+    for s in range(fp.S):
+        g.face_factor[s] = fp.face_factor[s]
+    g.node_n0 = ptr(keep[1], _f64p)
+    g.node_q0 = ptr(keep[2], _f64p)
+    g._keep = keep
+    return g

@@ -75,6 +75,8 @@ class Config:
     bath_pressure: float = 0.0
     bath_V: float = 0.0
     seed: int = 0
+    # Rows of this many cells (the last row holds the rest); 0: the square rule of `lattice`.
+    row_width: int = 0
 
 
 def config0(**kw):
@@ -105,14 +107,25 @@ def config4(worlds=512, **kw):
     return config1(worlds=worlds, **base)
 
 
-def lattice(n):
-    """(senders, receivers) of the lattice of n cells: rows of L = floor(sqrt n) cells, the
-    remaining n - L^2 cells appended to the last row; each cell's right edge, then its down edge."""
+def row_widths(n, row_width=0):
+    """Cells per lattice row: L = floor(sqrt n) rows of L with the remaining n - L^2 appended to
+    the last row, or (row_width > 0) full rows of row_width and the rest in a last, shorter row."""
     if n < 1:
         raise ValueError("A lattice needs at least one cell")
+    if row_width > 0:
+        full, rest = divmod(n, row_width)
+        return np.array([row_width] * full + ([rest] if rest else []), dtype=np.int64)
     side = math.isqrt(n)
     widths = np.full(side, side, dtype=np.int64)
     widths[-1] += n - side * side
+    return widths
+
+
+def lattice(n, row_width=0):
+    """(senders, receivers) of the lattice of n cells in rows of `row_widths(n, row_width)`; each
+    cell's right edge, then its down edge."""
+    widths = row_widths(n, row_width)
+    side = len(widths)
     start = np.concatenate(([0], np.cumsum(widths)))
     snd, rcv = [], []
     for r in range(side):
@@ -197,7 +210,7 @@ def build(cfg: Config) -> Packed:
     snd_l, rcv_l, esz = [], [], []
     for w, n in enumerate(sizes.tolist()):
         if n not in lattices:
-            lattices[n] = lattice(n)
+            lattices[n] = lattice(n, cfg.row_width)
         s, r = lattices[n]
         snd_l.append(s + cell_off[w])
         rcv_l.append(r + cell_off[w])

@@ -232,6 +232,64 @@ int cells_solve(cells_ensemble* e, void* stream, const double* values, const dou
  * distinct lattice shapes. */
 int cells_plan_stats(const cells_ensemble* e, int64_t* nnz_total, double* flops_total,
                      int32_t* max_levels, int32_t* shapes);
+
+/* ---- spatial worlds: the cells' lattice in an extracellular finite-volume grid ----
+ * The reference's counterpart is a resolved world with outside squares and faces
+ * (numerics/surface_transport.py's face routes, numerics/surface_field.py:371 _bind_sparse's
+ * operator pattern) stepped by numerics/surface_imex.py:36 imex_euler_face_electromigration@1
+ * (models/surface_pump.py:292-362); the reference's sparse field stops at 65 536 nodes
+ * (surface_field.py:84), below a 256 x 256 grid. Every world shares one grid of side x side nodes
+ * (row-major, the outer ring held at the bath: cells_world's bath_c, potential 0) with faces
+ * between 4-neighbours: horizontal faces r (side - 1) + c between (r, c) and (r, c + 1), then
+ * vertical faces side (side - 1) + r side + c between (r, c) and (r + 1, c), the first node the
+ * sender. Each cell's membrane faces one grid node instead of the bath. */
+typedef struct cells_field_grid {
+    int32_t side;
+    int32_t pad;
+    const int32_t* cell_node;  /* [ncell] world-local grid node of each cell's membrane */
+    double node_volume;        /* pL, every node */
+    double face_factor[4];     /* D_s A / L of every face, m^3/s */
+    const double* node_n0;     /* [S][side*side] reference amounts, fmol (every world's) */
+    const double* node_q0;     /* [W][side*side] fixed node charges, C */
+} cells_field_grid;
+
+typedef struct cells_field cells_field;
+
+/* Binds spatial worlds (3 species, no water, no Ca-ATPase; one lattice shared by every world):
+ * the cells as cells_ensemble_create binds them, and the operator's plan: the interior grid
+ * nodes' 5-point pattern (each cell's capacitor eliminated into its node: the operator's node
+ * rows (C + dt L) phi = q become dt L phi = q_node + q_cell), ordered by nested dissection,
+ * fundamental supernodes, a multifrontal schedule. The operator (face weights, factor) is rebuilt
+ * at a world's first step and then every refresh_every steps (the reference rebuilds it every
+ * step, models/surface_pump.py:300-306). */
+int cells_field_create(const cells_world* cells, const cells_field_grid* grid, int32_t refresh_every,
+                       int32_t device, cells_field** out);
+void cells_field_destroy(cells_field* f);
+/* Doubles in one spatial state: cell fields (amount increments [S], membrane transfers [S],
+ * membrane heat, Na/K cycles, pump heat, metabolic work) x ncell, edge fields (junction transfers
+ * [S], junction heat, gate) x nedge, node fields (amount increments [S], potential) x W side^2,
+ * face fields (transfers [S], heat) x W F, world fields (ring supplies [S], reservoir work) x W. */
+int64_t cells_field_state_size(const cells_field* f);
+/* nsteps IMEX Euler steps with implicit face electromigration of every world (models/
+ * surface_pump.py:292-362): explicit membranes, junctions, pumps, gates and face diffusion at the
+ * step-start potentials (the state's node potentials: the last accepted step's operator
+ * potentials); the stage charge; the operator solve; the face drift with the operator's weights;
+ * face heat and reservoir work; domain checks; where(valid, new, old). Buffers and refusals as
+ * cells_step. */
+int cells_field_step(cells_field* f, void* stream, const double* y_in, double* y_out, double dt,
+                     int32_t nsteps, const cells_control* control, const int32_t* active,
+                     int32_t* ok_out);
+/* The next step rebuilds the operator. */
+int cells_field_refresh(cells_field* f);
+/* The operator plan: factor entries per world, flops per factor (sum of column counts squared),
+ * supernodes, largest front, supernodal tree levels. */
+int cells_field_plan_stats(const cells_field* f, int64_t* nnz, double* flops, int32_t* fronts,
+                           int32_t* max_front, int32_t* levels);
+/* Per-kernel timing of `reps` steps from y (operator refreshed at the first), as
+ * cells_kernel_times. */
+int cells_field_kernel_times(cells_field* f, void* stream, const double* y, double dt,
+                             const cells_control* control, int32_t reps, int32_t capacity,
+                             double* ms_out, char* names_out, int32_t* count_out);
 /* Host copy of world w's plan positions: diag_pos [n] by node, edge_pos [E] by edge, and the
  * offset of its factor slice. */
 int cells_world_positions(const cells_ensemble* e, int32_t w, int64_t* diag_pos,

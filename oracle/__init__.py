@@ -222,3 +222,114 @@ def ros2_vjp(p, y, dt, sym, ybar, ctl=None, world=None):
     if not ok:
         raise ValueError("the step at y is refused, or the world has water or calcium")
     return out, pam_bar, cap_bar, paj_bar
+
+
+# ---------------------------------------------------------------- spatial worlds
+
+class FieldGrid(ctypes.Structure):
+    _fields_ = [("side", ctypes.c_int32), ("pad", ctypes.c_int32), ("cell_node", _i32p),
+                ("node_volume", ctypes.c_double), ("face_factor", ctypes.c_double * 4),
+                ("node_n0", _f64p), ("node_q0", _f64p)]
+
+
+def field_grid_struct(fp):
+    g = FieldGrid()
+    keep = [np.ascontiguousarray(fp.cell_node, dtype=np.int32),
+            np.ascontiguousarray(fp.node_n0, dtype=np.float64),
+            np.ascontiguousarray(fp.node_q0, dtype=np.float64)]
+    g.side = fp.side
+    g.cell_node = _p(keep[0], _i32p)
+    g.node_volume = fp.node_volume
+    for s in range(fp.S):
+        g.face_factor[s] = fp.face_factor[s]
+    g.node_n0 = _p(keep[1], _f64p)
+    g.node_q0 = _p(keep[2], _f64p)
+    g._keep = keep
+    return g
+
+
+class FieldOperator:
+    """The oracle's operator system of a spatial world (cells, then interior nodes row-major):
+    its minimum-degree symbolic (oracle.symbolic.bind, the reference's sparse solver), the
+    entry positions, and the frozen operator (weights, scales, factors) the steps carry."""
+
+    def __init__(self, fp):
+        from oracle.symbolic import bind
+
+        G, n = fp.side, fp.config.cells
+        ni = (G - 2) * (G - 2)
+        self.nsys = n + ni
+        local = fp.cell_node[:n].astype(np.int64)
+
+        def interior(node):
+            r, c = np.divmod(node, G)
+            return (r - 1) * (G - 2) + (c - 1)
+
+        a, b = fp.face_nodes()
+        ring = fp.ring()
+        inner = ~ring[a] & ~ring[b]
+        rows = np.concatenate((np.arange(n), n + interior(a[inner])))
+        cols = np.concatenate((n + interior(local), n + interior(b[inner])))
+        s = bind(self.nsys, rows, cols)
+        row_rank = s.rank[s.row]
+        pos = {}
+        for kk in range(self.nsys):
+            for q in range(int(s.column_start[kk]), int(s.column_start[kk + 1])):
+                pos[(int(row_rank[q]), kk)] = q
+
+        def at(u, v):
+            ru, rv = int(s.rank[u]), int(s.rank[v])
+            return pos[(max(ru, rv), min(ru, rv))]
+
+        diag_pos = s.column_start[s.rank[np.arange(self.nsys)]]
+        cap_pos = [at(i, n + interior(local[i])) for i in range(n)]
+        face_pos = np.full(len(a), -1, dtype=np.int64)
+        for q in np.nonzero(inner)[0]:
+            face_pos[q] = at(n + interior(a[q]), n + interior(b[q]))
+        edge_pos = np.concatenate((np.asarray(cap_pos, dtype=np.int64), face_pos))
+        self.symbolic = s
+        self.nnz = int(s.column_start[-1])
+        arrays = [np.ascontiguousarray(x, dtype=np.int64) for x in
+                  (s.column_start, row_rank, s.order, diag_pos, edge_pos)]
+        self._keep = arrays
+        self.structs = (Symbolic * fp.W)()
+        for k in range(fp.W):
+            st = self.structs[k]
+            st.n, st.nnz = self.nsys, self.nnz
+            st.column_start, st.row_rank, st.order, st.diag_pos, st.edge_pos = [_p(x, _i64p) for x in arrays]
+        self.weight = np.zeros(fp.W * fp.S * fp.faces)
+        self.scale = np.zeros(fp.W * self.nsys)
+        self.factor = np.zeros(fp.W * self.nnz)
+
+
+def field_step(fp, y, dt, op, refresh, ctl=None, world=None, grid=None):
+    """One oracle step of every spatial world: (state, valid [W]). refresh rebuilds op's frozen
+    operator from y first."""
+    w = world if world is not None else world_struct(fp.cells)
+    g = grid if grid is not None else field_grid_struct(fp)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(y)
+    valid = np.zeros(fp.W, dtype=np.int32)
+    lib().ref_field_step(ctypes.byref(w), ctypes.byref(g), op.structs, _p(y, _f64p), _p(out, _f64p),
+                         ctypes.c_double(dt), ctypes.byref(ctl), int(bool(refresh)),
+                         _p(op.weight, _f64p), _p(op.scale, _f64p), _p(op.factor, _f64p),
+                         _p(valid, _i32p))
+    return out, valid.astype(bool)
+
+
+def field_run(fp, y, dt, steps, refresh_every, op=None, ctl=None):
+    """steps oracle steps, the operator refreshed every refresh_every steps (from step 0)."""
+    op = op or FieldOperator(fp)
+    w, g = world_struct(fp.cells), field_grid_struct(fp)
+    ok = np.ones(fp.W, dtype=bool)
+    for t in range(steps):
+        y, v = field_step(fp, y, dt, op, t % refresh_every == 0, ctl, w, g)
+        ok &= v
+    return y, ok
+
+
+def logmean(left, right):
+    m, lg = ctypes.c_double(), ctypes.c_double()
+    lib().ref_logmean(ctypes.c_double(left), ctypes.c_double(right), ctypes.byref(m), ctypes.byref(lg))
+    return m.value, lg.value

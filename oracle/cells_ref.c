@@ -1186,3 +1186,418 @@ int ref_ros2_vjp(const cells_world* w, const cells_symbolic* sym, const double* 
     free(dbar); free(x1bar); free(g1bar);
     return ok;
 }
+
+/* ---------------------------------------------------------------- spatial worlds
+ *
+ * A lattice of cells in an extracellular finite-volume grid, stepped by the reference's IMEX
+ * Euler with implicit face electromigration (models/surface_pump.py:292-362, tableau
+ * numerics/surface_imex.py:59-61): explicit membrane, junction, pump, gate and face-diffusion
+ * rates at the step-start potentials (models/surface_pump.py:158-240, face_weights
+ * surface_imex.py:123-132 with logmean numerics/extracellular_field.py:48-62); the stage charge;
+ * the operator (C + dt L) phi = q on every node but the held ring, Jacobi-scaled and factored
+ * (surface_imex.py:228-259, the face Laplacian :142-162), its solve and charge residual
+ * (:269-278, models/surface_pump.py:325-330); the face drift with the operator's weights
+ * (surface_imex.py:135-139, models/surface_pump.py:243-256); the applied face flux's heat and the
+ * reservoir work (models/surface_pump.py:333-359); domain_valid and where(valid, new, old).
+ * The step-start potentials are the previous accepted step's operator potentials (the state's
+ * node potentials; a cell's is its node's plus its charge over its capacitance). */
+
+void ref_logmean(double left, double right, double* mean, double* log_ratio) {
+    /* numerics/extracellular_field.py:48-62 */
+    double middle = 0.5 * left + 0.5 * right;
+    double difference = left - right;
+    double ratio = (0.5 * difference) / middle;
+    int near = fabs(ratio) < 0.5;
+    double lg = near ? 2 * atanh(ratio) : log(left) - log(right);
+    int small = fabs(ratio) < 1e-4;
+    double square = ratio * ratio;
+    *mean = small ? middle * (1 - square / 3 - 4 * square * square / 45) : difference / lg;
+    *log_ratio = lg;
+}
+
+void ref_field_layout(const cells_world* w, const cells_field_grid* g, cells_field_layout* L) {
+    int32_t S = w->S, f = 0;
+    memset(L, 0, sizeof(*L));
+    L->dn = f; f += S;
+    L->mtr = f; f += S;
+    L->mdiss = f++;
+    L->pcyc = f++;
+    L->pdiss = f++;
+    L->pwork = f++;
+    L->cell_fields = f;
+    f = 0;
+    L->jtr = f; f += S;
+    L->jdiss = f++;
+    L->gate = f++;
+    L->edge_fields = f;
+    f = 0;
+    L->dno = f; f += S;
+    L->phi = f++;
+    L->node_fields = f;
+    f = 0;
+    L->ftr = f; f += S;
+    L->fheat = f++;
+    L->face_fields = f;
+    f = 0;
+    L->sup = f; f += S;
+    L->rwork = f++;
+    L->world_fields = f;
+    L->nodes = (int64_t)g->side * g->side;
+    L->faces = 2 * (int64_t)g->side * (g->side - 1);
+    L->edge_base = (int64_t)L->cell_fields * w->ncell;
+    L->node_base = L->edge_base + (int64_t)L->edge_fields * w->nedge;
+    L->face_base = L->node_base + (int64_t)L->node_fields * L->nodes * w->W;
+    L->world_base = L->face_base + (int64_t)L->face_fields * L->faces * w->W;
+    L->total = L->world_base + (int64_t)L->world_fields * w->W;
+}
+
+#define FCF(L, w, y, f, i) (y)[(int64_t)(f) * (w)->ncell + (i)]
+#define FEF(L, w, y, f, e) (y)[(L)->edge_base + (int64_t)(f) * (w)->nedge + (e)]
+#define FNF(L, w, y, f, k, g) (y)[(L)->node_base + ((int64_t)(f) * (w)->W + (k)) * (L)->nodes + (g)]
+#define FFF(L, w, y, f, k, q) (y)[(L)->face_base + ((int64_t)(f) * (w)->W + (k)) * (L)->faces + (q)]
+#define FWF(L, w, y, f, k) (y)[(L)->world_base + (int64_t)(f) * (w)->W + (k)]
+
+static void face_nodes(int32_t side, int64_t q, int64_t* a, int64_t* b) {
+    int64_t nh = (int64_t)side * (side - 1);
+    if (q < nh) {
+        int64_t r = q / (side - 1), c = q % (side - 1);
+        *a = r * side + c;
+        *b = *a + 1;
+    } else {
+        int64_t v = q - nh;
+        *a = v;
+        *b = v + side;
+    }
+}
+
+static int on_ring(int32_t side, int64_t node) {
+    int64_t r = node / side, c = node % side;
+    return r == 0 || c == 0 || r == side - 1 || c == side - 1;
+}
+
+/* Interior nodes in row-major order: the operator system's nodes after the cells. */
+static int64_t interior_index(int32_t side, int64_t node) {
+    int64_t r = node / side, c = node % side;
+    return (r - 1) * (side - 2) + (c - 1);
+}
+
+/* World k's slices of every field, src -> dst. */
+static void copy_field_world(const cells_world* w, const cells_field_layout* L, int32_t k,
+                             const double* src, double* dst) {
+    for (int f = 0; f < L->cell_fields; ++f)
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) FCF(L, w, dst, f, i) = FCF(L, w, src, f, i);
+    for (int f = 0; f < L->edge_fields; ++f)
+        for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) FEF(L, w, dst, f, e) = FEF(L, w, src, f, e);
+    for (int f = 0; f < L->node_fields; ++f)
+        for (int64_t n = 0; n < L->nodes; ++n) FNF(L, w, dst, f, k, n) = FNF(L, w, src, f, k, n);
+    for (int f = 0; f < L->face_fields; ++f)
+        for (int64_t q = 0; q < L->faces; ++q) FFF(L, w, dst, f, k, q) = FFF(L, w, src, f, k, q);
+    for (int f = 0; f < L->world_fields; ++f) FWF(L, w, dst, f, k) = FWF(L, w, src, f, k);
+}
+
+static double field_charge(const cells_world* w, const double* y_dn, int64_t stride, int64_t i) {
+    double t = w->valence[0] * y_dn[i];
+    for (int s = 1; s < w->S; ++s) t = t + w->valence[s] * y_dn[(int64_t)s * stride + i];
+    return CHARGE_PER_FMOL * t;
+}
+
+int ref_field_step(const cells_world* w, const cells_field_grid* g, const cells_symbolic* sym,
+                   const double* y, double* out, double dt, const cells_control* c, int32_t refresh,
+                   double* op_weight, double* op_scale, double* op_factor, int32_t* valid_out) {
+    cells_field_layout L;
+    ref_field_layout(w, g, &L);
+    const int S = w->S;
+    const int32_t side = g->side;
+    const int64_t NN = L.nodes, NF = L.faces, ni = (int64_t)(side - 2) * (side - 2);
+    const double T = w->temperature, RT = R_CONST * T;
+    double* r = (double*)calloc((size_t)L.total, sizeof(double));
+    double* known = (double*)malloc(sizeof(double) * (size_t)L.total);
+    double* inflow = (double*)calloc((size_t)S * NN, sizeof(double));
+    double* drift = (double*)calloc((size_t)S * NF, sizeof(double));
+    memcpy(known, y, sizeof(double) * (size_t)L.total);
+    memcpy(out, y, sizeof(double) * (size_t)L.total);
+    int all = 1;
+    int64_t fpos = 0;
+    for (int32_t k = 0; k < w->W; ++k) {
+        const int64_t c0 = w->cell_off[k], c1 = w->cell_off[k + 1], nc = c1 - c0;
+        const int64_t e0 = w->edge_off[k], e1 = w->edge_off[k + 1];
+        const cells_symbolic* sk = &sym[k];
+        const int64_t nsys = nc + ni;
+        double* fk = op_factor + fpos;
+        fpos += sk->nnz;
+        double* wk = op_weight + (int64_t)k * S * NF;
+        double* sck = op_scale + (int64_t)k * nsys;
+        int valid = 1;
+        const double ctl[5] = {c->membrane, c->junction, c->pump, c->calcium_pump, c->water};
+        for (int q = 0; q < 5; ++q) valid = valid && isfinite(ctl[q]) && ctl[q] >= 0 && ctl[q] <= 1;
+        /* concentrations */
+        double* co = (double*)malloc(sizeof(double) * (size_t)S * NN);
+        for (int s = 0; s < S; ++s)
+            for (int64_t n = 0; n < NN; ++n) {
+                double amount = g->node_n0[(int64_t)s * NN + n] + FNF(&L, w, y, L.dno + s, k, n);
+                co[(int64_t)s * NN + n] = amount / g->node_volume;
+                valid = valid && isfinite(co[(int64_t)s * NN + n]) && co[(int64_t)s * NN + n] > 0;
+            }
+        /* step-start potentials: nodes from the state, cells their node's plus q / C */
+        double* phi = (double*)malloc(sizeof(double) * (size_t)(nc + NN));
+        for (int64_t n = 0; n < NN; ++n) phi[nc + n] = FNF(&L, w, y, L.phi, k, n);
+        for (int64_t i = c0; i < c1; ++i) {
+            double q = w->q0[i] + field_charge(w, &FCF(&L, w, y, L.dn, 0), w->ncell, i);
+            phi[i - c0] = phi[nc + g->cell_node[i]] + q / w->cap[i];
+        }
+        /* junction routes (surface_transport.py:310-367) */
+        for (int64_t e = e0; e < e1; ++e) {
+            int64_t a = w->snd[e], b = w->rcv[e];
+            double dv = phi[a - c0] - phi[b - c0];
+            double gt = FEF(&L, w, y, L.gate, e);
+            double avail = c->junction * gt;
+            double diss = 0.0;
+            for (int s = 0; s < S; ++s) {
+                double ca = (w->n0[(int64_t)s * w->ncell + a] + FCF(&L, w, y, L.dn + s, a)) / w->vol0[a];
+                double cb = (w->n0[(int64_t)s * w->ncell + b] + FCF(&L, w, y, L.dn + s, b)) / w->vol0[b];
+                double z = w->valence[s];
+                double pa = w->paj[(int64_t)s * w->nedge + e];
+                valid = valid && isfinite(ca) && ca > 0 && isfinite(cb) && cb > 0;
+                double J = ref_selective_flux(ca, cb, dv, z, T, pa, avail);
+                double mu = RT * (log(ca) - log(cb)) + z * F_CONST * dv;
+                diss = s == 0 ? J * mu : diss + J * mu;
+                valid = valid && fabs(z * F_CONST * dv / RT) <= MAX_ABS_X && isfinite(J);
+                FEF(&L, w, r, L.jtr + s, e) = J;
+            }
+            FEF(&L, w, r, L.jdiss, e) = 1e-3 * diss;
+            FEF(&L, w, r, L.gate, e) = ref_gate_rhs(gt, dv, w->gate_threshold, w->gate_minimum);
+            valid = valid && gate_domain(gt, dv, w->gate_threshold, w->gate_minimum, c->junction);
+        }
+        /* membrane routes into the cells' nodes, Na/K-ATPase, owner sums */
+        for (int s = 0; s < S; ++s)
+            for (int64_t n = 0; n < NN; ++n) inflow[(int64_t)s * NN + n] = 0.0;
+        for (int64_t i = c0; i < c1; ++i) {
+            int64_t node = g->cell_node[i];
+            double dv = phi[i - c0] - phi[nc + node];
+            double ci[4], Jm[4], mu[4];
+            double diss = 0.0;
+            for (int s = 0; s < S; ++s) {
+                double z = w->valence[s], cout = co[(int64_t)s * NN + node];
+                ci[s] = (w->n0[(int64_t)s * w->ncell + i] + FCF(&L, w, y, L.dn + s, i)) / w->vol0[i];
+                valid = valid && isfinite(ci[s]) && ci[s] > 0;
+                double pa = w->pam[(int64_t)s * w->ncell + i];
+                Jm[s] = ref_selective_flux(ci[s], cout, dv, z, T, pa, c->membrane);
+                mu[s] = RT * (log(ci[s]) - log(cout)) + z * F_CONST * dv;
+                diss = s == 0 ? Jm[s] * mu[s] : diss + Jm[s] * mu[s];
+                valid = valid && fabs(z * F_CONST * dv / RT) <= MAX_ABS_X && isfinite(Jm[s]);
+            }
+            double cyc, energy, drive;
+            double kout = co[1 * NN + node];
+            ref_nak_kinetics(w, ci[0], kout, mu[0], mu[1], w->nak_cap[i], c->pump, &cyc, &energy, &drive);
+            valid = valid && ref_nak_domain(w, ci[0], kout, mu[0], mu[1], w->nak_cap[i], c->pump);
+            double extra[4] = {3.0 * cyc, -2.0 * cyc, -0.0 * cyc, -0.0 * cyc};
+            int64_t p0 = w->inc_ptr[i], p1 = w->inc_ptr[i + 1], pm = p0 + w->inc_out[i];
+            for (int s = 0; s < S; ++s) {
+                double outs = 0.0, ins = 0.0;
+                for (int64_t p = p0; p < pm; ++p) outs += -FEF(&L, w, r, L.jtr + s, w->inc_edge[p]);
+                for (int64_t p = pm; p < p1; ++p) ins += FEF(&L, w, r, L.jtr + s, w->inc_edge[p]);
+                double cell = -Jm[s];
+                cell += outs;
+                cell += ins;
+                FCF(&L, w, r, L.dn + s, i) = cell - extra[s];
+                FCF(&L, w, r, L.mtr + s, i) = Jm[s];
+                inflow[(int64_t)s * NN + node] += Jm[s] + extra[s];
+            }
+            FCF(&L, w, r, L.mdiss, i) = 1e-3 * diss;
+            FCF(&L, w, r, L.pcyc, i) = cyc;
+            FCF(&L, w, r, L.pdiss, i) = -1e-3 * cyc * energy;
+            FCF(&L, w, r, L.pwork, i) = -1e-3 * cyc * drive;
+        }
+        /* faces: weights and explicit diffusion (surface_imex.py:123-132) */
+        for (int64_t q = 0; q < NF; ++q) {
+            int64_t a, b;
+            face_nodes(side, q, &a, &b);
+            for (int s = 0; s < S; ++s) {
+                double mean, lg;
+                ref_logmean(co[(int64_t)s * NN + a], co[(int64_t)s * NN + b], &mean, &lg);
+                double weight = g->face_factor[s] * mean / MOL_PER_FMOL;
+                double lr = log(co[(int64_t)s * NN + a]) - log(co[(int64_t)s * NN + b]);
+                double d = weight * lr;
+                valid = valid && isfinite(d);
+                FFF(&L, w, r, L.ftr + s, k, q) = d;
+                inflow[(int64_t)s * NN + a] -= d;
+                inflow[(int64_t)s * NN + b] += d;
+                if (refresh) wk[(int64_t)s * NF + q] = weight;
+            }
+        }
+        /* node rates: the ring's inflow is its supply (surface_transport.py reservoirs) */
+        double supply[4] = {0, 0, 0, 0};
+        for (int64_t n = 0; n < NN; ++n) {
+            int ring = on_ring(side, n);
+            for (int s = 0; s < S; ++s) {
+                FNF(&L, w, r, L.dno + s, k, n) = ring ? 0.0 : inflow[(int64_t)s * NN + n];
+                if (ring) supply[s] -= inflow[(int64_t)s * NN + n];
+            }
+        }
+        /* the operator at the step start (Euler's operator time 0), when refreshed */
+        double* cond = (double*)malloc(sizeof(double) * (size_t)NF);
+        for (int64_t q = 0; q < NF; ++q) {
+            double t = 0.0;
+            for (int s = 0; s < S; ++s)
+                t = s == 0 ? w->valence[s] * w->valence[s] * wk[(int64_t)s * NF + q]
+                           : t + w->valence[s] * w->valence[s] * wk[(int64_t)s * NF + q];
+            cond[q] = CHARGE_PER_FMOL * F_CONST / RT * t;
+        }
+        if (refresh) {
+            /* M = C + dt L on the system nodes (cells, then interior nodes); scale 1/sqrt(diag) */
+            double* diag = (double*)calloc((size_t)nsys, sizeof(double));
+            for (int64_t i = c0; i < c1; ++i) {
+                diag[i - c0] += w->cap[i];
+                diag[nc + interior_index(side, g->cell_node[i])] += w->cap[i];
+            }
+            for (int64_t q = 0; q < NF; ++q) {
+                int64_t a, b;
+                face_nodes(side, q, &a, &b);
+                if (!on_ring(side, a)) diag[nc + interior_index(side, a)] += dt * cond[q];
+                if (!on_ring(side, b)) diag[nc + interior_index(side, b)] += dt * cond[q];
+            }
+            for (int64_t x = 0; x < nsys; ++x) {
+                valid = valid && diag[x] > 0;
+                sck[x] = 1.0 / sqrt(diag[x] > 0 ? diag[x] : 1.0);
+            }
+            for (int64_t p = 0; p < sk->nnz; ++p) fk[p] = 0.0;
+            for (int64_t x = 0; x < nsys; ++x) fk[sk->diag_pos[x]] = (diag[x] * sck[x] * sck[x] + diag[x] * sck[x] * sck[x]) / 2;
+            for (int64_t i = c0; i < c1; ++i) {
+                int64_t xn = nc + interior_index(side, g->cell_node[i]);
+                double v = -w->cap[i], si = sck[i - c0], sn = sck[xn];
+                fk[sk->edge_pos[i - c0]] = (v * si * sn + v * sn * si) / 2;
+            }
+            for (int64_t q = 0; q < NF; ++q) {
+                int64_t p = sk->edge_pos[nc + q];
+                if (p < 0) continue;
+                int64_t a, b;
+                face_nodes(side, q, &a, &b);
+                double v = -(dt * cond[q]);
+                double sa = sck[nc + interior_index(side, a)], sb = sck[nc + interior_index(side, b)];
+                fk[p] = (v * sa * sb + v * sb * sa) / 2;
+            }
+            free(diag);
+            ref_factor(sk, fk);
+            for (int64_t p = 0; p < sk->nnz; ++p) valid = valid && isfinite(fk[p]);
+        }
+        /* known = y + dt r (every field but the potentials) */
+        for (int f = 0; f < L.cell_fields; ++f)
+            for (int64_t i = c0; i < c1; ++i) FCF(&L, w, known, f, i) += dt * FCF(&L, w, r, f, i);
+        for (int f = 0; f < L.edge_fields; ++f)
+            for (int64_t e = e0; e < e1; ++e) FEF(&L, w, known, f, e) += dt * FEF(&L, w, r, f, e);
+        for (int s = 0; s < S; ++s)
+            for (int64_t n = 0; n < NN; ++n) FNF(&L, w, known, L.dno + s, k, n) += dt * FNF(&L, w, r, L.dno + s, k, n);
+        for (int s = 0; s < S; ++s)
+            for (int64_t q = 0; q < NF; ++q) FFF(&L, w, known, L.ftr + s, k, q) += dt * FFF(&L, w, r, L.ftr + s, k, q);
+        /* stage charges and the operator solve */
+        double* q = (double*)malloc(sizeof(double) * (size_t)nsys);
+        double* x = (double*)malloc(sizeof(double) * (size_t)nsys);
+        for (int64_t i = c0; i < c1; ++i)
+            q[i - c0] = w->q0[i] + field_charge(w, &FCF(&L, w, known, L.dn, 0), w->ncell, i);
+        for (int64_t n = 0; n < NN; ++n) {
+            if (on_ring(side, n)) continue;
+            double t = w->valence[0] * FNF(&L, w, known, L.dno, k, n);
+            for (int s = 1; s < S; ++s) t = t + w->valence[s] * FNF(&L, w, known, L.dno + s, k, n);
+            q[nc + interior_index(side, n)] = g->node_q0[(int64_t)k * NN + n] + CHARGE_PER_FMOL * t;
+        }
+        double* rhs = (double*)malloc(sizeof(double) * (size_t)nsys);
+        for (int64_t xx = 0; xx < nsys; ++xx) rhs[xx] = q[xx] * sck[xx];
+        ref_solve(sk, fk, rhs, x);
+        double* pot = (double*)malloc(sizeof(double) * (size_t)(nc + NN));
+        for (int64_t xx = 0; xx < nc; ++xx) pot[xx] = x[xx] * sck[xx];
+        for (int64_t n = 0; n < NN; ++n)
+            pot[nc + n] = on_ring(side, n) ? 0.0 : x[nc + interior_index(side, n)] * sck[nc + interior_index(side, n)];
+        /* charge residual (C + dt L) phi - q on the system nodes */
+        double* mphi = (double*)calloc((size_t)nsys, sizeof(double));
+        for (int64_t i = c0; i < c1; ++i) {
+            int64_t n = g->cell_node[i], xn = nc + interior_index(side, n);
+            double cv = w->cap[i] * (pot[i - c0] - pot[nc + n]);
+            mphi[i - c0] += cv;
+            mphi[xn] -= cv;
+        }
+        for (int64_t qq = 0; qq < NF; ++qq) {
+            int64_t a, b;
+            face_nodes(side, qq, &a, &b);
+            double cur = dt * cond[qq] * (pot[nc + a] - pot[nc + b]);
+            if (!on_ring(side, a)) mphi[nc + interior_index(side, a)] += cur;
+            if (!on_ring(side, b)) mphi[nc + interior_index(side, b)] -= cur;
+        }
+        for (int64_t xx = 0; xx < nsys; ++xx) {
+            double scale = fabs(q[xx]) > fabs(mphi[xx]) ? fabs(q[xx]) : fabs(mphi[xx]);
+            valid = valid && isfinite(x[xx]) && fabs(mphi[xx] - q[xx]) <= 3e-25 + 2e-12 * scale;
+        }
+        /* drift with the operator's weights; stage = known + dt drift */
+        double dsup[4] = {0, 0, 0, 0};
+        for (int s = 0; s < S; ++s)
+            for (int64_t n = 0; n < NN; ++n) inflow[(int64_t)s * NN + n] = 0.0;
+        for (int64_t qq = 0; qq < NF; ++qq) {
+            int64_t a, b;
+            face_nodes(side, qq, &a, &b);
+            double dvf = pot[nc + a] - pot[nc + b];
+            for (int s = 0; s < S; ++s) {
+                double fl = wk[(int64_t)s * NF + qq] * (w->valence[s] * F_CONST / RT) * dvf;
+                drift[(int64_t)s * NF + qq] = fl;
+                inflow[(int64_t)s * NN + a] -= fl;
+                inflow[(int64_t)s * NN + b] += fl;
+            }
+        }
+        copy_field_world(w, &L, k, known, out);
+        for (int64_t n = 0; n < NN; ++n) {
+            int ring = on_ring(side, n);
+            for (int s = 0; s < S; ++s) {
+                if (ring) dsup[s] -= inflow[(int64_t)s * NN + n];
+                else FNF(&L, w, out, L.dno + s, k, n) += dt * inflow[(int64_t)s * NN + n];
+            }
+            FNF(&L, w, out, L.phi, k, n) = pot[nc + n];
+        }
+        for (int64_t qq = 0; qq < NF; ++qq) {
+            double heat = 0.0;
+            for (int s = 0; s < S; ++s) {
+                double wt = wk[(int64_t)s * NF + qq];
+                double fl = FFF(&L, w, r, L.ftr + s, k, qq) + drift[(int64_t)s * NF + qq];
+                FFF(&L, w, out, L.ftr + s, k, qq) += dt * drift[(int64_t)s * NF + qq];
+                double term = wt > 0 ? fl * fl / wt : 0.0;
+                heat = s == 0 ? term : heat + term;
+            }
+            FFF(&L, w, out, L.fheat, k, qq) += dt * (1e-3 * RT * heat);
+        }
+        double rw = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double applied = dt * supply[s] + dt * dsup[s];
+            FWF(&L, w, out, L.sup + s, k) += applied;
+            double mu = RT * log(w->bath_c[s]);
+            rw = s == 0 ? mu * applied : rw + mu * applied;
+        }
+        FWF(&L, w, out, L.rwork, k) += 1e-3 * rw;
+        /* domain_valid: finite fields, positive amounts, gates in [minimum, 1] */
+        for (int64_t i = c0; i < c1; ++i) {
+            for (int f = 0; f < L.cell_fields; ++f) valid = valid && isfinite(FCF(&L, w, out, f, i));
+            for (int s = 0; s < S; ++s) {
+                double nn = w->n0[(int64_t)s * w->ncell + i] + FCF(&L, w, out, L.dn + s, i);
+                valid = valid && isfinite(nn) && nn > 0;
+            }
+        }
+        for (int64_t e = e0; e < e1; ++e) {
+            for (int f = 0; f < L.edge_fields; ++f) valid = valid && isfinite(FEF(&L, w, out, f, e));
+            double gt = FEF(&L, w, out, L.gate, e);
+            valid = valid && gt >= w->gate_minimum && gt <= 1.0;
+        }
+        for (int64_t n = 0; n < NN; ++n)
+            for (int s = 0; s < S; ++s) {
+                double nn = g->node_n0[(int64_t)s * NN + n] + FNF(&L, w, out, L.dno + s, k, n);
+                valid = valid && isfinite(nn) && nn > 0 && isfinite(FNF(&L, w, out, L.phi, k, n));
+            }
+        for (int64_t qq = 0; qq < NF; ++qq)
+            for (int f = 0; f < L.face_fields; ++f) valid = valid && isfinite(FFF(&L, w, out, f, k, qq));
+        for (int f = 0; f < L.world_fields; ++f) valid = valid && isfinite(FWF(&L, w, out, f, k));
+        if (!valid) copy_field_world(w, &L, k, y, out); /* the world keeps its old state */
+        valid_out[k] = valid;
+        all = all && valid;
+        free(co); free(phi); free(cond); free(q); free(x); free(rhs); free(pot); free(mphi);
+    }
+    free(r);
+    free(known);
+    free(inflow);
+    free(drift);
+    return all;
+}

@@ -107,6 +107,50 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym /* [W] */, con
 void ref_factor(const cells_symbolic* s, double* values);
 void ref_solve(const cells_symbolic* s, const double* factor, const double* rhs_node, double* x_node);
 
+/* ---- spatial worlds: the cells' lattice embedded in an extracellular finite-volume grid ----
+ * Every world shares one square grid of side x side nodes (row-major; the outer ring is held at
+ * the bath: w->bath_c, potential 0) with faces between 4-neighbours: horizontal faces
+ * r (side - 1) + c between (r, c) and (r, c + 1), then vertical faces side (side - 1) + r side + c
+ * between (r, c) and (r + 1, c); the first node of a face is its sender. Each cell's membrane
+ * faces one grid node instead of the bath. */
+typedef struct {
+    int32_t side;
+    int32_t pad;
+    const int32_t* cell_node;  /* [ncell] world-local grid node of each cell's membrane */
+    double node_volume;        /* pL, every node */
+    double face_factor[4];     /* D_s A / L of every face, m^3/s */
+    const double* node_n0;     /* [S][side*side] reference amounts, fmol (shared by the worlds) */
+    const double* node_q0;     /* [W][side*side] fixed node charges, C */
+} cells_field_grid;
+
+/* Field state layout: cells (amount increments [S], membrane transfers [S], membrane heat, Na/K
+ * cycles, pump heat, metabolic work), edges (junction transfers [S], junction heat, gate), nodes
+ * (amount increments [S], potential) over W side^2, faces (transfers [S], heat) over W F, worlds
+ * (ring supplies [S], reservoir work). */
+typedef struct {
+    int32_t cell_fields, edge_fields, node_fields, face_fields, world_fields;
+    int32_t dn, mtr, mdiss, pcyc, pdiss, pwork, jtr, jdiss, gate, dno, phi, ftr, fheat, sup, rwork;
+    int64_t nodes, faces; /* per world */
+    int64_t edge_base, node_base, face_base, world_base, total;
+} cells_field_layout;
+
+void ref_field_layout(const cells_world* w, const cells_field_grid* g, cells_field_layout* out);
+
+/* One IMEX Euler step with implicit face electromigration of every spatial world
+ * (models/surface_pump.py:292-362 with the Euler tableau). sym[k] is world k's operator system,
+ * cells then interior nodes (row-major), whose diag_pos lists every node's diagonal entry and
+ * whose edge_pos lists each cell's capacitor entry (cells in order) then each face's entry (-1
+ * where a face touches the ring). op_weight [W][S][F] and op_factor (the worlds' factors back to
+ * back, sym[k].nnz each) hold the frozen operator: refresh = 1 rebuilds them from y's
+ * concentrations (step-start weights) before the solve. out receives the new state or the old
+ * one where a world refuses; valid[W] the acceptance. */
+int ref_field_step(const cells_world* w, const cells_field_grid* g, const cells_symbolic* sym,
+                   const double* y, double* out, double dt, const cells_control* c, int32_t refresh,
+                   double* op_weight, double* op_scale, double* op_factor, int32_t* valid);
+
+/* The reference's log-mean (numerics/extracellular_field.py:48-62): mean and log ratio. */
+void ref_logmean(double left, double right, double* mean, double* log_ratio);
+
 /* Point laws, exposed for closed-form tests. */
 double ref_bernoulli(double x);
 double ref_bernoulli_slope(double x);
