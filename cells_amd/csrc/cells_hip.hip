@@ -1350,7 +1350,8 @@ __device__ void sv_forward_front(const int* pool, int qa, const double* __restri
   group_sync<G>();
 }
 
-template <int G, int STRIDE>
+// UNROLL: loads in flight per lane in the products with the rows below.
+template <int G, int STRIDE, int UNROLL = 1>
 __device__ void sv_backward_front(const int* pool, int qa, const double* __restrict__ fv, double* __restrict__ y,
                                   double* Fv, int lane) {
   constexpr int NT = SolveLanes<G>::NT;
@@ -1370,6 +1371,7 @@ __device__ void sv_backward_front(const int* pool, int qa, const double* __restr
     if (k < J) {
       const long ck = p0 + colbase(k, m);
       acc = y[j0 + k];
+#pragma unroll UNROLL
       for (int i = J; i < m; ++i) acc -= fv[ck + i] * Fv[i * STRIDE];
     }
     xr[t] = acc;

@@ -285,8 +285,8 @@ int cells_field_refresh(cells_field* f);
  * supernodes, largest front, supernodal tree levels. */
 int cells_field_plan_stats(const cells_field* f, int64_t* nnz, double* flops, int32_t* fronts,
                            int32_t* max_front, int32_t* levels);
-/* Per-kernel timing of `reps` steps from y (operator refreshed at the first), as
- * cells_kernel_times. */
+/* Per-kernel timing of `reps` steps from y (operator refreshed at the first): each kernel's total
+ * per step over the steps that ran it (HIP events), in first-launch order. */
 int cells_field_kernel_times(cells_field* f, void* stream, const double* y, double dt,
                              const cells_control* control, int32_t reps, int32_t capacity,
                              double* ms_out, char* names_out, int32_t* count_out);
