@@ -46,3 +46,25 @@ def test_no_gpu_means_an_error_not_a_fallback():
 
     with pytest.raises(RuntimeError):
         Ensemble(build(config0()))
+
+
+def test_spatial_worlds_without_a_gpu_are_an_error():
+    """cells_field_create checks its arguments, then refuses with CELLS_ENODEV (no fallback)."""
+    import ctypes
+
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from cells_amd import lib
+    from cells_amd.field import build_field, small_field
+
+    fp = build_field(small_field())
+    w, g = lib.world_struct(fp.cells), lib.field_grid_struct(fp)
+    h = ctypes.c_void_p()
+    L = lib.load()
+    assert L.cells_field_create(ctypes.byref(w), ctypes.byref(g), 0, 0, ctypes.byref(h)) == lib.CELLS_EINVAL
+    assert L.cells_field_create(ctypes.byref(w), ctypes.byref(g), 50, 0, ctypes.byref(h)) == lib.CELLS_ENODEV
+    fp.cell_node[1] = 0  # a cell on the held ring
+    g = lib.field_grid_struct(fp)
+    assert L.cells_field_create(ctypes.byref(w), ctypes.byref(g), 50, 0, ctypes.byref(h)) == lib.CELLS_EINVAL

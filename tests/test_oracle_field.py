@@ -1,0 +1,166 @@
+"""The oracle's spatial step (cells in an extracellular grid, IMEX Euler with implicit face drift),
+pinned by what the mathematics fixes, on the CPU.
+
+- The log-mean (numerics/extracellular_field.py:48-62) against Decimal arithmetic on all three
+  branches, as the reference's own log-mean test does (tests/unit/test_surface_reference.py:16).
+- Face diffusion W (ln a - ln b) with W = D A/L c_log is Fick's D A/L (a - b) (the log-mean times
+  the log ratio is the difference): the face transfers of one step against that closed form.
+- The operator solve: with every route closed and the grid at the bath, the step's node potentials
+  are the dense solution of (C + dt L) phi = q, L the face-conductance Laplacian
+  (numerics/surface_imex.py:142-153) and C the cells' capacitors, assembled here from the
+  definitions.
+- Conservation: per species, cells plus interior nodes change by exactly the ring supply; after
+  every accepted step each interior node with its cell holds no net charge (C phi = Q, the
+  stiffly accurate operator, surface_imex.py:14-17).
+- Rest: with every route closed and the grid at the bath, nothing moves.
+- Refusal: an availability outside [0, 1] refuses the step and keeps the state.
+"""
+
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+import oracle
+from cells_amd.field import (FieldConfig, FieldLayout, amounts, build_field, charges,
+                             initial_field_state, small_field)
+from cells_amd.world import F, R
+
+getcontext().prec = 50
+
+
+def decimal_logmean(a, b):
+    a, b = Decimal(a), Decimal(b)
+    if a == b:
+        return float(a)
+    return float((a - b) / (a.ln() - b.ln()))
+
+
+@pytest.mark.parametrize("a,b", [(145.0, 145.0), (145.0, 145.0 * (1 + 3e-5)), (5.0, 5.0001),
+                                 (110.0, 80.0), (12.0, 140.0), (1e-3, 2.0), (3.0, 2.999)])
+def test_logmean_meets_decimal_on_every_branch(a, b):
+    mean, lg = oracle.logmean(a, b)
+    want = decimal_logmean(a, b)
+    assert abs(mean - want) <= 4e-16 * want, (mean, want)
+    want_log = float(Decimal(a).ln() - Decimal(b).ln())
+    assert abs(lg - want_log) <= 1e-15 * max(abs(want_log), 1e-300) + 1e-300
+
+
+def closed(fp):
+    return oracle.control(membrane=0.0, junction=0.0, pump=0.0)
+
+
+def test_face_diffusion_is_ficks_law():
+    """One step from a randomly perturbed grid with every route closed: each face's transfer over
+    dt is Fick's D A/L (a - b) at the start concentrations plus the drift W z F/RT (phi_a - phi_b)
+    with the step's own node potentials and weights."""
+    fp = build_field(small_field(worlds=1, seed=11))
+    L = FieldLayout.of(fp)
+    y = initial_field_state(fp)
+    rng = np.random.default_rng(2)
+    interior = np.nonzero(~fp.ring())[0]
+    for s in range(fp.S):
+        L.node(y, L.dno + s)[0, interior] = rng.uniform(-0.05, 0.05, interior.size) * fp.node_n0[s, interior]
+    op = oracle.FieldOperator(fp)
+    out, ok = oracle.field_step(fp, y, 1e-3, op, True, closed(fp))
+    assert ok.all()
+    a, b = fp.face_nodes()
+    phi = L.node(out, L.phi)[0]
+    z = fp.cells.valence
+    T = fp.cells.config.temperature
+    for s in range(fp.S):
+        c = (fp.node_n0[s] + L.node(y, L.dno + s)[0]) / fp.node_volume
+        fick = fp.face_factor[s] * (c[a] - c[b]) / 1e-15
+        weight = op.weight.reshape(fp.S, -1)[s]
+        drift = weight * (z[s] * F / (R * T)) * (phi[a] - phi[b])
+        got = L.face(out, L.ftr + s)[0] / 1e-3
+        scale = np.max(np.abs(fick))
+        assert np.max(np.abs(got - (fick + drift))) <= 1e-12 * scale
+
+
+def test_operator_potentials_are_the_dense_solution():
+    fp = build_field(FieldConfig(worlds=1, cells=6, row_width=3, side=12, spacing=3, seed=12))
+    L = FieldLayout.of(fp)
+    y = initial_field_state(fp)
+    rng = np.random.default_rng(4)
+    n = fp.config.cells
+    L.cell(y, L.dn)[:] = rng.uniform(-1e-4, 1e-4, n)  # charge the cells (fmol of Na)
+    dt = 1e-3
+    op = oracle.FieldOperator(fp)
+    out, ok = oracle.field_step(fp, y, dt, op, True, closed(fp))
+    assert ok.all()
+    # the system: cells, then interior nodes row-major; capacitors cell-node, faces node-node
+    ring = fp.ring()
+    inner = np.nonzero(~ring)[0]
+    index = {int(g): n + k for k, g in enumerate(inner)}
+    N = n + inner.size
+    M = np.zeros((N, N))
+    cap = fp.cells.cap[:n]
+    for i in range(n):
+        g = index[int(fp.cell_node[i])]
+        M[i, i] += cap[i]
+        M[g, g] += cap[i]
+        M[i, g] -= cap[i]
+        M[g, i] -= cap[i]
+    T = fp.cells.config.temperature
+    z = fp.cells.valence
+    cond = 1e-15 * F * F / (R * T) * sum(z[s] ** 2 * fp.face_factor[s] * fp.cells.bath_c[s] / 1e-15
+                                          for s in range(fp.S))
+    a, b = fp.face_nodes()
+    for q in range(len(a)):
+        for u, v in ((a[q], b[q]), (b[q], a[q])):
+            if not ring[u]:
+                M[index[int(u)], index[int(u)]] += dt * cond
+                if not ring[v]:
+                    M[index[int(u)], index[int(v)]] -= dt * cond
+    cq, nq = charges(fp, y)
+    q = np.concatenate((cq[:n], nq[0, inner]))
+    phi = np.linalg.solve(M, q)
+    got = L.node(out, L.phi)[0, inner]
+    assert np.max(np.abs(got - phi[n:])) <= 1e-10 * np.max(np.abs(phi[n:]))
+    assert np.all(L.node(out, L.phi)[0, ring] == 0.0)
+
+
+def test_material_closes_and_every_node_with_its_cell_is_neutral():
+    fp = build_field(FieldConfig(worlds=2, cells=9, row_width=3, side=16, spacing=4, seed=13))
+    L = FieldLayout.of(fp)
+    y0 = initial_field_state(fp)
+    op = oracle.FieldOperator(fp)
+    y = y0
+    ring = fp.ring()
+    c0, n0 = amounts(fp, y0)
+    w, g = oracle.world_struct(fp.cells), oracle.field_grid_struct(fp)
+    for t in range(120):
+        y, ok = oracle.field_step(fp, y, 1e-3, op, t % 10 == 0, None, w, g)
+        assert ok.all()
+        if t % 20 != 19:
+            continue
+        cq, nq = charges(fp, y)
+        c1, n1 = amounts(fp, y)
+        for k in range(fp.W):
+            cs = slice(int(fp.cells.cell_off[k]), int(fp.cells.cell_off[k + 1]))
+            qn = nq[k].copy()
+            qn[fp.cell_node[cs]] += cq[cs]
+            assert np.max(np.abs(qn[~ring])) <= 1e-12 * np.max(np.abs(cq[cs]))
+            for s in range(fp.S):
+                before = c0[s, cs].sum() + n0[s, k, ~ring].sum()
+                after = c1[s, cs].sum() + n1[s, k, ~ring].sum()
+                assert abs(after - before - L.world(y, L.sup + s)[k]) <= 1e-13 * before
+
+
+def test_rest_stays_put_and_bad_controls_refuse():
+    fp = build_field(small_field(worlds=2, seed=14))
+    y0 = initial_field_state(fp)
+    op = oracle.FieldOperator(fp)
+    out, ok = oracle.field_step(fp, y0, 1e-3, op, True, closed(fp))
+    assert ok.all()
+    L = FieldLayout.of(fp)
+    # nothing moves: the cells' fields stay at zero and the nodes' amounts within rounding of the
+    # operator's potentials (a part in 1e15 of the amounts)
+    for f in range(L.cell_fields):
+        assert np.all(L.cell(out, f) == 0.0)
+    for s in range(fp.S):
+        assert np.max(np.abs(L.node(out, L.dno + s))) <= 1e-15 * fp.node_n0[s].max()
+    bad, ok = oracle.field_step(fp, y0, 1e-3, op, True, oracle.control(pump=1.5))
+    assert not ok.any()
+    assert np.array_equal(bad, y0)
