@@ -107,6 +107,50 @@ def stationary_leg(worlds):
             "fp32_vs_fp64_max_mM": diff, "fp32_vs_fp64_max_gate": gdiff}
 
 
+def spatial_leg(steps, refresh):
+    """The spatial configuration: 64 worlds of 2048 cells (a 45 x 46 lattice) in a 256 x 256
+    extracellular grid with a held ring, Na/K/Cl, IMEX Euler dt = 1 ms with the implicit face drift
+    through the nested-dissection operator, rebuilt every `refresh` steps. Times `steps` steps
+    (a multiple of refresh: the factor cost is amortised as it is in a run) and one step's kernels;
+    prices the operator solve against its algorithmic bytes (both sweeps read the factor once,
+    2 x 8 x nnz per world) and the factor against the FP64 matrix peak (sum of column counts
+    squared per world)."""
+    import torch
+
+    from cells_amd.field import FieldConfig, FieldEnsemble, build_field, initial_field_state
+
+    fp = build_field(FieldConfig())
+    e = FieldEnsemble(fp, refresh_every=refresh)
+    plan = e.plan_stats()
+    log(f"spatial leg: {fp.W} worlds x {fp.config.cells} cells in {fp.side}^2 grids, plan {plan}")
+    y = e.to_device(initial_field_state(fp))
+    kt = e.kernel_times(y, 1e-3, reps=3)
+    y, ok = e.step(y, 1e-3, nsteps=refresh)  # warm: one operator refresh and its steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y, ok = e.step(y, 1e-3, nsteps=steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms = el / steps * 1e3
+    solve_ms = sum(v for k, v in kt if k.startswith("k_fd_solve"))
+    factor_ms = sum(v for k, v in kt if k.startswith("k_fd_factor") or k == "k_fd_assemble")
+    solve_bytes = 2 * 8 * plan["nnz"] * fp.W
+    achieved = solve_bytes / (solve_ms * 1e-3) / 1e9
+    tflops = plan["flops"] * fp.W / (factor_ms * 1e-3) / 1e12  # the reference's cost model
+    log(f"spatial leg: {ms:.3f} ms/step, solve {solve_ms:.3f} ms ({achieved:.0f} GB/s), "
+        f"factor {factor_ms:.2f} ms per refresh")
+    e.close()
+    return {"worlds": fp.W, "cells_per_world": fp.config.cells, "grid": f"{fp.side}x{fp.side}",
+            "steps": steps, "refresh_every": refresh, "ms_per_step": ms,
+            "cell_steps_per_s": fp.W * fp.config.cells * steps / el,
+            "all_worlds_accepted": bool(ok.all()), "plan": plan,
+            "kernel_ms": {k: round(v, 4) for k, v in kt},
+            "solve": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": achieved / HBM_PEAK_GBS, "ms_per_step": solve_ms},
+            "factor": {"bound": "mfma", "achieved": tflops, "peak": 78.6, "unit": "TFLOP/s",
+                       "frac": tflops / 78.6, "ms_per_refresh": factor_ms}}
+
+
 def adjoint_leg(steps, snapshots, tape):
     """Calibration gradient on the adjoint configuration's worlds (256 worlds of 32x32 cells,
     Na/K/Cl, ROS2 dt = 5 ms, 6 log-parameters per cell, Vmem targets every 20 steps): one loss
@@ -178,7 +222,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--parity-worlds", type=int, default=3)
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: plain,parity,cpu,adjoint,stationary")
+    ap.add_argument("--skip", default="",
+                    help="comma list of legs to skip: plain,parity,cpu,adjoint,stationary,spatial")
+    ap.add_argument("--spatial-steps", type=int, default=100)
     ap.add_argument("--adjoint-steps", type=int, default=2000, help="horizon of the adjoint leg")
     ap.add_argument("--adjoint-snapshots", type=int, default=45)
     ap.add_argument("--stationary-worlds", type=int, default=512)
@@ -322,6 +368,10 @@ def main():
     if "stationary" not in skip:
         sta = stationary_leg(args.stationary_worlds)
 
+    spatial = None
+    if "spatial" not in skip:
+        spatial = spatial_leg(args.spatial_steps, 50)
+
     adj = None
     if "adjoint" not in skip:
         adj = adjoint_leg(args.adjoint_steps, args.adjoint_snapshots, args.adjoint_tape)
@@ -362,6 +412,7 @@ def main():
         "roofline": roofline,
         "adjoint": adj,
         "stationary": sta,
+        "spatial": spatial,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
