@@ -131,7 +131,7 @@ struct ShapeDev {
 
 // Multifrontal factor and supernodal solve: one workgroup of kMfThreads per world.
 constexpr int kMfThreads = 256;
-constexpr int kMfLds = 5120;                               // doubles of LDS per workgroup (40 KiB: 4 per CU)
+constexpr int kMfLds = 4096;                               // doubles of LDS per workgroup (32 KiB: 5 per CU)
 constexpr int kMfOcta = 8;                                 // lanes of a small group
 constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 6
 constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 19
@@ -949,7 +949,7 @@ __device__ __forceinline__ void solve_any(const ShapeDev& s, const PlanView& v, 
 
 // Solve (C + coef G) dphi = q(K) on each world's factor: rhs' = q s, dphi = x s.
 template <bool SN>
-__global__ __launch_bounds__(kMfThreads) void k_solve_stage(Params P, const ShapeDev* shapes,
+__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_solve_stage(Params P, const ShapeDev* shapes,
                                                             const int* shape_of, const int* pool,
                                                             const long* foff, int w0,
                                                             const double* msc,
@@ -2004,7 +2004,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_dtrans(Params P, double coef, co
 // w = M^-1 dbar on the step's factor (workgroup per world); then rate_bar += q^T w and the
 // slopes' cotangent of M: -coef w delta^T through G = F' sum_s z_s slope_s.
 template <bool SN>
-__global__ __launch_bounds__(kMfThreads) void k_adj_solve(Params P, const ShapeDev* shapes,
+__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_adj_solve(Params P, const ShapeDev* shapes,
                                                           const int* shape_of, const int* pool,
                                                           const long* foff, double coef,
                                                           const double* msc, const double* fvals,
