@@ -235,6 +235,7 @@ __device__ __forceinline__ double* next(double* A, double* B, int s) { return s 
 // Junction routes (numerics/surface_transport.py:310-367, junction branch; gate rhs
 // junction_voltage_relaxation.py:41-59; slopes models/surface_pump_v3.py:2747-2755).
 
+template <bool SLOPES>
 __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, const double* B,
                                                   const int* sel, const double* src_override,
                                                   int vol_next, int e0, int e1, Ctl c, double* R,
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, con
   double avail = c.junction * g;
   bool ok = true;
   double diss = 0.0, G = 0.0;
-  bool slopes = gj != nullptr;
+  constexpr bool slopes = SLOPES;  // gj, Gj non-null exactly when SLOPES
   for (int s = 0; s < P.S; ++s) {
     double z = P.z[s];
     double pa = P.paj[(long)s * P.nedge + e];
@@ -288,7 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, con
 // With coef > 0 also the stage matrix's diagonal and its Jacobi scale
 // (numerics/surface_imex.py:228-259 on C + coef G).
 
-__global__ __launch_bounds__(kBlock) void k_cells(Params P, const double* A, const double* B,
+template <bool SLOPES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ? 4 : 5))) void k_cells(Params P, const double* A, const double* B,
                                                   const int* sel, const double* src_override,
                                                   int vol_next, int c0, int c1, Ctl c, double* R,
                                                   double* contrib, double* gm, const double* Gj,
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_cells(Params P, const double* A, con
   cell_concentrations(P, y, yv, i, ci, vol);
   double dv = potential(P, y, i) - P.Vb;
   bool ok = true;
-  bool slopes = gm != nullptr;
+  constexpr bool slopes = SLOPES;  // gm, Gj, mdiag, msc non-null exactly when SLOPES
   double Jm[4], mu[4];
   double diss = 0.0, Gm = 0.0;
   for (int s = 0; s < P.S; ++s) {
@@ -2763,11 +2765,11 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
   int vol_next = P.water ? 1 : 0;
   if (e->method == CELLS_METHOD_EULER) {
     if (ne) {
-      hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+      hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
                          ed0, ed1, c, e->K1, nullptr, nullptr, e->valid);
       mark("k_edges");
     }
-    hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+    hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
                        c0, c1, c, e->K1, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
     mark("k_cells");
     hipLaunchKernelGGL(k_euler_update, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
@@ -2781,11 +2783,11 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     const double coef = gamma * dt;
     size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
     if (ne) {
-      hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+      hipLaunchKernelGGL(k_edges<true>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
                          ed0, ed1, c, e->K1, e->gj, e->Gj, e->valid);
       mark("k_edges");
     }
-    hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+    hipLaunchKernelGGL(k_cells<true>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
                        c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
     mark("k_cells");
     launch_factor(e, st, w0, nw, coef);
@@ -2799,11 +2801,11 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
                        ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->contrib1, e->Y1);
     mark("k_ros2_stage1");
     if (ne) {
-      hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, ed0,
+      hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, ed0,
                          ed1, c, e->K2, nullptr, nullptr, e->valid);
       mark("k_edges");
     }
-    hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, c0, c1,
+    hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, c0, c1,
                        c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
     mark("k_cells");
     solve_stage(e, st, nw, e->fvals, [&](int part) {
@@ -2939,9 +2941,9 @@ extern "C" int cells_rates(cells_ensemble* e, void* stream, const double* y,
   HIP_TRY(hipMemcpyAsync(e->A, y, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_init, dim3(blocks(e->W)), dim3(kBlock), 0, st, e->W, nullptr, e->sel, e->alive, e->valid);
   if (e->nedge)
-    hipLaunchKernelGGL(k_edges, dim3(blocks(e->nedge)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
+    hipLaunchKernelGGL(k_edges<false>, dim3(blocks(e->nedge)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
                        e->nedge, c, rates_out, nullptr, nullptr, e->valid);
-  hipLaunchKernelGGL(k_cells, dim3(blocks(e->ncell)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
+  hipLaunchKernelGGL(k_cells<false>, dim3(blocks(e->ncell)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
                      e->ncell, c, rates_out, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_world_rates, dim3(e->W), dim3(kBlock), 0, st, P, e->contrib1, rates_out);
   if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * e->W, hipMemcpyDeviceToDevice, st));
@@ -3124,9 +3126,9 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   hipLaunchKernelGGL(k_init, dim3(blocks(W)), dim3(kBlock), 0, st, W, nullptr, e->sel, e->alive, e->valid);
   // forward recomputation of the step's internals (the forward kernels, in the same order)
   if (ne)
-    hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, ne,
+    hipLaunchKernelGGL(k_edges<true>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, ne,
                        c, e->K1, e->gj, e->Gj, e->valid);
-  hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
+  hipLaunchKernelGGL(k_cells<true>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
                      e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
   launch_factor(e, st, 0, W, coef);
   solve_stage(e, st, W, e->fvals, [&](int part) {
@@ -3136,9 +3138,9 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
                      ne, dt, coef, e->gm, e->gj, e->d1, e->K1, e->contrib1, e->Y1);
   if (ne)
-    hipLaunchKernelGGL(k_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, ne, c,
+    hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, ne, c,
                        e->K2, nullptr, nullptr, e->valid);
-  hipLaunchKernelGGL(k_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
+  hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
   solve_stage(e, st, W, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
