@@ -83,3 +83,19 @@ def test_a_refusing_world_stops_alone_or_ends_a_batch_run(torch, ens):
     assert np.array_equal(L.edge(got, L.gate)[e0:e1], L.edge(y0, L.gate)[e0:e1])
     with pytest.raises(NumericalError, match=f"world {bad} refused step 0"):
         run_windowed(ens, y0, 5e-3, 12, WindowPolicy(5, refusals="batch"))
+
+
+def test_resume_from_checkpoint_files_is_bitwise(torch, ens, tmp_path):
+    """A run written to checkpoint directories (the reference's container) and read back into a
+    new start state continues bit for bit."""
+    from cells_amd.snapshot import read_cohort, write_cohort
+
+    p = ens.packed
+    y0 = initial_state(p)
+    whole, _ = ens.step(y0, 5e-3, nsteps=20)
+    half, _ = ens.step(y0, 5e-3, nsteps=10)
+    write_cohort(tmp_path / "cohort", p, half, 10 * 5e-3, 10)
+    back, times, steps = read_cohort(tmp_path / "cohort", p)
+    assert steps.tolist() == [10] * p.W
+    rest, _ = ens.step(back, 5e-3, nsteps=10)
+    assert torch.equal(rest, whole)
