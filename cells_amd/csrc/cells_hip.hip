@@ -98,6 +98,9 @@ struct Params {
   const int* inc_ptr;
   const int* inc_edge;
   const int* inc_out;
+  // worlds by cell count, largest first: the order in which whole-ensemble launches of the
+  // per-world factor and solves take them (the largest start first, the smallest fill the end)
+  const int* world_order;
   // state layout
   int cf, ef, wf;
   int f_dn, f_mtr, f_mdiss, f_pcyc, f_pdiss, f_pwork, f_ccyc, f_cdiss, f_cwork, f_dvol, f_wtr, f_wdiss;
@@ -147,6 +150,12 @@ constexpr int max_order(int tri) {
 constexpr int kMfBatch = 8;                                // global loads in flight per lane
 constexpr int kRecInts = 16;                               // ints per front record in the plan pool
 constexpr int kLevelBounds = 6;                            // per level: first front of each kind, end
+
+// The world a workgroup of a per-world launch takes: by size, largest first, when the launch
+// covers every world.
+__device__ __forceinline__ int launch_world(const Params& P, int w0) {
+  return (w0 == 0 && (int)gridDim.x == P.W) ? P.world_order[blockIdx.x] : w0 + (int)blockIdx.x;
+}
 
 __device__ __forceinline__ double& CF(const Params& P, double* y, int f, int i) {
   return y[(long)f * P.ncell + i];
@@ -960,7 +969,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(8, 8
                                                             double* ysc, int y_in_lds, SolveArgs sa,
                                                             int part) {
   extern __shared__ double lds[];
-  int w = w0 + blockIdx.x;
+  int w = launch_world(P, w0);
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   const double* fv = fvals + foff[w];
@@ -1435,7 +1444,7 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeD
                                                           double* fvals, double* ubuf, double* bigbuf,
                                                           const long* boffw, int* valid) {
   extern __shared__ double lds[];
-  int w = w0 + blockIdx.x;
+  int w = launch_world(P, w0);
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   double* fv = fvals + foff[w];
@@ -2015,7 +2024,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(8, 8
                                                           double* gjbar, double* ysc, int y_in_lds,
                                                           SolveArgs sa, int part) {
   extern __shared__ double shared_y[];
-  int w = blockIdx.x;
+  int w = launch_world(P, 0);
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   const double* fv = fvals + foff[w];
@@ -2431,6 +2440,14 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
   UP_I(inc_edge, iedge);
   if ((rc = upload(e.get(), &d_i, w->inc_out, nc))) return rc;
   P.inc_out = d_i;
+  {
+    std::vector<int> order(e->W);
+    for (int k = 0; k < e->W; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return w->cell_off[a + 1] - w->cell_off[a] > w->cell_off[b + 1] - w->cell_off[b];
+    });
+    UP_I(world_order, order);
+  }
   UP_D(vol0, w->vol0, nc);
   UP_D(cap, w->cap, nc);
   UP_D(pam, w->pam, S * nc);
