@@ -107,6 +107,20 @@ def stationary_leg(worlds):
             "fp32_vs_fp64_max_mM": diff, "fp32_vs_fp64_max_gate": gdiff}
 
 
+def copy_probe(mib):
+    """A plain device copy of known size (the calibration of the HBM byte counters)."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: {mib} MiB read + {mib} MiB written per launch")
+    del src, dst
+
+
 def spatial_leg(steps, refresh):
     """The spatial configuration: 64 worlds of 2048 cells (a 45 x 46 lattice) in a 256 x 256
     extracellular grid with a held ring, Na/K/Cl, IMEX Euler dt = 1 ms with the implicit face drift
@@ -225,6 +239,8 @@ def main():
     ap.add_argument("--skip", default="",
                     help="comma list of legs to skip: plain,parity,cpu,adjoint,stationary,spatial")
     ap.add_argument("--spatial-steps", type=int, default=100)
+    ap.add_argument("--spatial-only", action="store_true",
+                    help="run only the spatial leg and print its JSON (for profiling it alone)")
     ap.add_argument("--adjoint-steps", type=int, default=2000, help="horizon of the adjoint leg")
     ap.add_argument("--adjoint-snapshots", type=int, default=45)
     ap.add_argument("--stationary-worlds", type=int, default=512)
@@ -238,6 +254,11 @@ def main():
                           "GPUs with no exchange, and no multi-GPU figure is measured"}))
         sys.exit(2)
     skip = set(filter(None, args.skip.split(",")))
+    if args.spatial_only:
+        if args.copy_probe:
+            copy_probe(args.copy_probe)
+        print(json.dumps({"spatial": spatial_leg(args.spatial_steps, 50)}), flush=True)
+        return
 
     import torch
 
@@ -284,14 +305,7 @@ def main():
     log(f"with the ledger checks as epilogue: {checked_ms:.3f} ms/step, failure bits {checked_bits}")
 
     if args.copy_probe:
-        n = args.copy_probe * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device=y0.device)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written per launch")
-        del src, dst
+        copy_probe(args.copy_probe)
 
     log("kernel times")
     times = e.kernel_times(yw, dt, reps=3)
