@@ -124,41 +124,56 @@ def copy_probe(mib):
 def spatial_leg(steps, refresh):
     """The spatial configuration: 64 worlds of 2048 cells (a 45 x 46 lattice) in a 256 x 256
     extracellular grid with a held ring, Na/K/Cl, IMEX Euler dt = 1 ms with the implicit face drift
-    through the nested-dissection operator, rebuilt every `refresh` steps. Times `steps` steps
-    (a multiple of refresh: the factor cost is amortised as it is in a run) and one step's kernels;
-    prices the operator solve against its algorithmic bytes (both sweeps read the factor once,
-    2 x 8 x nnz per world) and the factor against the FP64 matrix peak (sum of column counts
-    squared per world)."""
+    through the nested-dissection operator, rebuilt every `refresh` steps; each step starts from
+    the field solve at its state (chord steps on the operator's factor). Times `steps` steps (a
+    multiple of refresh: the factor cost is amortised as it is in a run) and one step's kernels,
+    then the same steps from the carried potentials; prices the operator solve against its
+    algorithmic bytes (both sweeps read the factor once, 2 x 8 x nnz per world) and the factor
+    against the FP64 matrix peak (sum of column counts squared per world)."""
+    import numpy as np
     import torch
 
     from cells_amd.field import FieldConfig, FieldEnsemble, build_field, initial_field_state
 
     fp = build_field(FieldConfig())
-    e = FieldEnsemble(fp, refresh_every=refresh)
-    plan = e.plan_stats()
-    log(f"spatial leg: {fp.W} worlds x {fp.config.cells} cells in {fp.side}^2 grids, plan {plan}")
-    y = e.to_device(initial_field_state(fp))
-    kt = e.kernel_times(y, 1e-3, reps=3)
-    y, ok = e.step(y, 1e-3, nsteps=refresh)  # warm: one operator refresh and its steps
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    y, ok = e.step(y, 1e-3, nsteps=steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    ms = el / steps * 1e3
-    solve_ms = sum(v for k, v in kt if k.startswith("k_fd_solve"))
+    out = {}
+    for start in ("solve", "carried"):
+        e = FieldEnsemble(fp, refresh_every=refresh, start=start)
+        plan = e.plan_stats()
+        log(f"spatial leg ({start}): {fp.W} worlds x {fp.config.cells} cells in {fp.side}^2 grids, plan {plan}")
+        y = e.to_device(initial_field_state(fp))
+        kt = e.kernel_times(y, 1e-3, reps=3)
+        y, ok = e.step(y, 1e-3, nsteps=refresh)  # warm: one operator refresh and its steps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y, ok = e.step(y, 1e-3, nsteps=steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms = el / steps * 1e3
+        log(f"spatial leg ({start}): {ms:.3f} ms/step")
+        out[start] = (kt, ms, el, bool(ok.all()), plan)
+        if start == "solve":
+            _, chord, _ = e.potentials(y, 1e-3)
+            chord = chord.cpu().numpy()
+        e.close()
+    kt, ms, el, accepted, plan = out["solve"]
+    solve_ms = sum(v for k, v in kt if k.startswith("k_fd_solve") and "field" not in k)
+    field_ms = sum(v for k, v in kt if k.startswith("k_fd_k") or k.startswith("k_fd_solve field"))
     factor_ms = sum(v for k, v in kt if k.startswith("k_fd_factor") or k == "k_fd_assemble")
     solve_bytes = 2 * 8 * plan["nnz"] * fp.W
     achieved = solve_bytes / (solve_ms * 1e-3) / 1e9
     tflops = plan["flops"] * fp.W / (factor_ms * 1e-3) / 1e12  # the reference's cost model
-    log(f"spatial leg: {ms:.3f} ms/step, solve {solve_ms:.3f} ms ({achieved:.0f} GB/s), "
-        f"factor {factor_ms:.2f} ms per refresh")
-    e.close()
+    log(f"spatial leg: operator solve {solve_ms:.3f} ms ({achieved:.0f} GB/s), field solve {field_ms:.3f} ms "
+        f"({float(np.mean(chord)):.2f} chord steps), factor {factor_ms:.2f} ms per refresh")
     return {"worlds": fp.W, "cells_per_world": fp.config.cells, "grid": f"{fp.side}x{fp.side}",
-            "steps": steps, "refresh_every": refresh, "ms_per_step": ms,
-            "cell_steps_per_s": fp.W * fp.config.cells * steps / el,
-            "all_worlds_accepted": bool(ok.all()), "plan": plan,
+            "steps": steps, "refresh_every": refresh, "start_potentials": "field solve",
+            "ms_per_step": ms, "cell_steps_per_s": fp.W * fp.config.cells * steps / el,
+            "all_worlds_accepted": accepted and out["carried"][3], "plan": plan,
             "kernel_ms": {k: round(v, 4) for k, v in kt},
+            "field_solve": {"ms_per_step": field_ms, "chord_steps_mean": float(np.mean(chord)),
+                            "chord_steps_max": int(np.max(chord))},
+            "carried_start": {"ms_per_step": out["carried"][1],
+                              "cell_steps_per_s": fp.W * fp.config.cells * steps / out["carried"][2]},
             "solve": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": achieved / HBM_PEAK_GBS, "ms_per_step": solve_ms},
             "factor": {"bound": "mfma", "achieved": tflops, "peak": 78.6, "unit": "TFLOP/s",

@@ -233,9 +233,12 @@ def charges(fp: FieldPacked, y):
 class FieldEnsemble:
     """Spatial worlds bound on one GPU (cells_field_create): the cells, the grid, the operator's
     nested-dissection plan and the stepping workspace. The operator is rebuilt at the first step
-    and then every refresh_every steps."""
+    and then every refresh_every steps. start="solve" (the reference's): each step starts from the
+    field solve at its state (numerics/surface_field.py:614-855), by at most max_iterations chord
+    steps; start="carried": from the state's node potentials (the last operator solve's)."""
 
-    def __init__(self, fp: FieldPacked, refresh_every: int = 50, device: int = 0):
+    def __init__(self, fp: FieldPacked, refresh_every: int = 50, device: int = 0, start: str = "solve",
+                 max_iterations: int = 8):
         import ctypes
 
         from cells_amd import lib as _lib
@@ -256,6 +259,10 @@ class FieldEnsemble:
         self.state_size = int(_lib.load().cells_field_state_size(handle))
         if self.state_size != self.layout.total:
             raise RuntimeError("spatial state layout disagrees with the library's")
+        if start not in ("solve", "carried"):
+            raise ValueError('start must be "solve" or "carried"')
+        self.start = start
+        _lib.check(_lib.load().cells_field_set_start(handle, int(start == "solve"), int(max_iterations)))
 
     def close(self):
         from cells_amd import lib as _lib
@@ -308,6 +315,29 @@ class FieldEnsemble:
         from cells_amd import lib as _lib
 
         _lib.check(_lib.load().cells_field_refresh(self._h))
+
+    def potentials(self, y, dt, ctl=None):
+        """The field solve at y (cells_field_potentials): (node potentials [W, side^2], chord steps
+        [W], ok [W]) on the device; a cell's potential is its node's plus its charge over its
+        capacitance."""
+        import ctypes
+
+        from cells_amd import lib as _lib
+        from cells_amd.ensemble import control
+
+        t = self.torch
+        y = self.to_device(y)
+        if y.numel() != self.state_size:
+            raise ValueError("state size disagrees with the worlds")
+        W, nodes = self.packed.W, self.packed.side * self.packed.side
+        phi = t.empty((W, nodes), dtype=t.float64, device=self.device)
+        it = t.empty(W, dtype=t.int32, device=self.device)
+        ok = t.empty(W, dtype=t.int32, device=self.device)
+        ctl = ctl or control()
+        _lib.check(_lib.load().cells_field_potentials(
+            self._h, self._stream(), ctypes.c_void_p(y.data_ptr()), float(dt), ctypes.byref(ctl),
+            ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(it.data_ptr()), ctypes.c_void_p(ok.data_ptr())))
+        return phi, it, ok
 
     def plan_stats(self):
         """dict(nnz, flops, fronts, max_front, levels) of the operator's factor plan (per world)."""

@@ -35,7 +35,7 @@ EXPORTS = (
     "cells_step_vjp_recorded", "cells_step_checks", "cells_step_check_bits", "cells_stationary",
     "cells_stationary_residual", "cells_stationary_jvp", "cells_field_create", "cells_field_destroy",
     "cells_field_state_size", "cells_field_step", "cells_field_refresh", "cells_field_plan_stats",
-    "cells_field_kernel_times",
+    "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials",
 )
 
 
@@ -159,6 +159,8 @@ def load():
     lib.cells_field_step.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_int32,
                                      ctypes.POINTER(ControlStruct), vp, vp]
     lib.cells_field_refresh.argtypes = [vp]
+    lib.cells_field_set_start.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    lib.cells_field_potentials.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct), vp, vp, vp]
     lib.cells_field_plan_stats.argtypes = [vp, _i64p, _f64p, _i32p, _i32p, _i32p]
     lib.cells_field_kernel_times.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
                                              ctypes.c_int32, ctypes.c_int32, _f64p, ctypes.c_char_p, _i32p]

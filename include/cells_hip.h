@@ -271,16 +271,32 @@ void cells_field_destroy(cells_field* f);
  * face fields (transfers [S], heat) x W F, world fields (ring supplies [S], reservoir work) x W. */
 int64_t cells_field_state_size(const cells_field* f);
 /* nsteps IMEX Euler steps with implicit face electromigration of every world (models/
- * surface_pump.py:292-362): explicit membranes, junctions, pumps, gates and face diffusion at the
- * step-start potentials (the state's node potentials: the last accepted step's operator
- * potentials); the stage charge; the operator solve; the face drift with the operator's weights;
- * face heat and reservoir work; domain checks; where(valid, new, old). Buffers and refusals as
- * cells_step. */
+ * surface_pump.py:292-362): the step-start potentials (the field solve, cells_field_potentials,
+ * or with the field solve off the state's node potentials: the last accepted step's operator
+ * potentials); explicit membranes, junctions, pumps, gates and face diffusion at them; the stage
+ * charge; the operator solve; the face drift with the operator's weights; face heat and
+ * reservoir work; domain checks; where(valid, new, old). A world whose field solve misses its
+ * target refuses the step. Buffers and refusals as cells_step. */
 int cells_field_step(cells_field* f, void* stream, const double* y_in, double* y_out, double dt,
                      int32_t nsteps, const cells_control* control, const int32_t* active,
                      int32_t* ok_out);
 /* The next step rebuilds the operator. */
 int cells_field_refresh(cells_field* f);
+/* The step-start potentials: field_solve = 1 (the default) solves the field at each step's start
+ * as the reference's readouts do (numerics/surface_field.py:614-855 solve: each cell and its node
+ * a floating capacitor component, q_i / C_i on top of one offset per interior node, the offsets
+ * by current continuity through the faces and junctions to max |r| <= 0.01 (1e-23 + 2e-12 max
+ * current)), by at most max_iterations (1..64, default 8) chord steps with the operator's factor;
+ * field_solve = 0 takes the state's node potentials instead (the last operator solve's). */
+int cells_field_set_start(cells_field* f, int32_t field_solve, int32_t max_iterations);
+/* The field solve at state y (device), alone: phi_out [W][side^2] the node potentials (a cell's is
+ * its node's plus q_i / C_i), iterations_out [W] the chord steps taken, ok_out [W] 1 where the
+ * solve met its target and the faces' and nodes' domain held (each may be NULL but phi_out). The
+ * reference's counterpart is models/surface_tissue.py:306 field_values (its potentials). Builds
+ * the operator when the next step would; the refresh schedule does not advance. */
+int cells_field_potentials(cells_field* f, void* stream, const double* y, double dt,
+                           const cells_control* control, double* phi_out, int32_t* iterations_out,
+                           int32_t* ok_out);
 /* The operator plan: factor entries per world, flops per factor (sum of column counts squared),
  * supernodes, largest front, supernodal tree levels. */
 int cells_field_plan_stats(const cells_field* f, int64_t* nnz, double* flops, int32_t* fronts,

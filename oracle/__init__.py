@@ -62,7 +62,7 @@ def lib():
         if not os.path.exists(LIB):
             build()
         _lib = ctypes.CDLL(LIB)
-        for name in ("ref_bernoulli", "ref_bernoulli_slope", "ref_logaddexp"):
+        for name in ("ref_bernoulli", "ref_bernoulli_slope", "ref_bernoulli_derivative", "ref_logaddexp"):
             getattr(_lib, name).restype = ctypes.c_double
         for name in ("ref_selective_flux", "ref_route_slope", "ref_gate_rhs"):
             getattr(_lib, name).restype = ctypes.c_double
@@ -300,33 +300,99 @@ class FieldOperator:
         self.weight = np.zeros(fp.W * fp.S * fp.faces)
         self.scale = np.zeros(fp.W * self.nsys)
         self.factor = np.zeros(fp.W * self.nnz)
+        self.kirchhoff = self._offset_pattern(fp, interior, a, b, inner)
+
+    @staticmethod
+    def _offset_pattern(fp, interior, a, b, inner):
+        """The field solve's offset system (one offset per interior node, row-major): the face
+        Laplacian's pattern plus each junction's pair of cell nodes; edge_pos lists each face's
+        entry (-1 where it touches the ring), then each junction's (world-local order). Every
+        world shares the grid, the placement and the lattice, so one pattern serves them all."""
+        from oracle.symbolic import bind
+
+        p, G, n = fp.cells, fp.side, fp.config.cells
+        ni = (G - 2) * (G - 2)
+        e0, e1 = int(p.edge_off[0]), int(p.edge_off[1])
+        for k in range(1, fp.W):
+            f0, f1 = int(p.edge_off[k]), int(p.edge_off[k + 1])
+            c0 = int(p.cell_off[k])
+            if not (np.array_equal(p.snd[f0:f1] - c0, p.snd[e0:e1]) and
+                    np.array_equal(p.rcv[f0:f1] - c0, p.rcv[e0:e1])):
+                raise ValueError("the worlds' junction graphs differ")
+        local = fp.cell_node[:n].astype(np.int64)
+        ja, jb = interior(local[p.snd[e0:e1]]), interior(local[p.rcv[e0:e1]])
+        rows = np.concatenate((interior(a[inner]), ja))
+        cols = np.concatenate((interior(b[inner]), jb))
+        s = bind(ni, rows, cols)
+        row_rank = s.rank[s.row]
+        pos = {}
+        for kk in range(ni):
+            for q in range(int(s.column_start[kk]), int(s.column_start[kk + 1])):
+                pos[(int(row_rank[q]), kk)] = q
+
+        def at(u, v):
+            ru, rv = int(s.rank[u]), int(s.rank[v])
+            return pos[(max(ru, rv), min(ru, rv))]
+
+        diag_pos = s.column_start[s.rank[np.arange(ni)]]
+        face_pos = np.full(len(a), -1, dtype=np.int64)
+        for q in np.nonzero(inner)[0]:
+            face_pos[q] = at(interior(a[q]), interior(b[q]))
+        junction_pos = np.array([at(u, v) for u, v in zip(ja, jb)], dtype=np.int64)
+        arrays = [np.ascontiguousarray(x, dtype=np.int64) for x in
+                  (s.column_start, row_rank, s.order, diag_pos, np.concatenate((face_pos, junction_pos)))]
+        structs = (Symbolic * fp.W)()
+        for k in range(fp.W):
+            st = structs[k]
+            st.n, st.nnz = ni, int(s.column_start[-1])
+            st.column_start, st.row_rank, st.order, st.diag_pos, st.edge_pos = [_p(x, _i64p) for x in arrays]
+        structs._keep = arrays
+        return structs
 
 
-def field_step(fp, y, dt, op, refresh, ctl=None, world=None, grid=None):
+def field_step(fp, y, dt, op, refresh, ctl=None, world=None, grid=None, start="solve"):
     """One oracle step of every spatial world: (state, valid [W]). refresh rebuilds op's frozen
-    operator from y first."""
+    operator from y first. start="solve": the step-start potentials are the field solve at y (the
+    reference's); "carried": the state's node potentials."""
+    if start not in ("solve", "carried"):
+        raise ValueError('start must be "solve" or "carried"')
     w = world if world is not None else world_struct(fp.cells)
     g = grid if grid is not None else field_grid_struct(fp)
     ctl = ctl or control()
     y = np.ascontiguousarray(y, dtype=np.float64)
     out = np.empty_like(y)
     valid = np.zeros(fp.W, dtype=np.int32)
-    lib().ref_field_step(ctypes.byref(w), ctypes.byref(g), op.structs, _p(y, _f64p), _p(out, _f64p),
+    lib().ref_field_step(ctypes.byref(w), ctypes.byref(g), op.structs,
+                         op.kirchhoff if start == "solve" else None, _p(y, _f64p), _p(out, _f64p),
                          ctypes.c_double(dt), ctypes.byref(ctl), int(bool(refresh)),
                          _p(op.weight, _f64p), _p(op.scale, _f64p), _p(op.factor, _f64p),
                          _p(valid, _i32p))
     return out, valid.astype(bool)
 
 
-def field_run(fp, y, dt, steps, refresh_every, op=None, ctl=None):
+def field_run(fp, y, dt, steps, refresh_every, op=None, ctl=None, start="solve"):
     """steps oracle steps, the operator refreshed every refresh_every steps (from step 0)."""
     op = op or FieldOperator(fp)
     w, g = world_struct(fp.cells), field_grid_struct(fp)
     ok = np.ones(fp.W, dtype=bool)
     for t in range(steps):
-        y, v = field_step(fp, y, dt, op, t % refresh_every == 0, ctl, w, g)
+        y, v = field_step(fp, y, dt, op, t % refresh_every == 0, ctl, w, g, start)
         ok &= v
     return y, ok
+
+
+def field_start(fp, y, op, world, ctl=None):
+    """The field solve of one spatial world at y: (cell potentials [n], node potentials
+    [side^2], Newton iterations, converged)."""
+    w, g = world_struct(fp.cells), field_grid_struct(fp)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n = fp.config.cells
+    phi = np.zeros(n + fp.side * fp.side)
+    it = ctypes.c_int32()
+    ok = lib().ref_field_start(ctypes.byref(w), ctypes.byref(g), ctypes.byref(op.kirchhoff[world]), int(world),
+                               _p(y, _f64p), ctypes.byref(ctl), _p(phi, _f64p), ctypes.byref(it))
+    return phi[:n], phi[n:], int(it.value), bool(ok)
 
 
 def logmean(left, right):

@@ -69,6 +69,13 @@ double ref_bernoulli_slope(double x) {
     return (em - 1.0 - x * em) / (d * d);
 }
 
+double ref_bernoulli_derivative(double x) {
+    /* numerics/tissue_field.py:186-190 (the field Newton's junction slope) */
+    if (fabs(x) < 1e-4) return -0.5 + x / 6 - x * x * x / 180;
+    double em1 = expm1(x);
+    return (em1 - x * exp(x)) / (em1 * em1);
+}
+
 double ref_selective_flux(double c_from, double c_to, double dv, double z, double T, double pa,
                           double avail) {
     /* laws/junctions/selective.py:28-50 with residual 0, gate 1, multiplier avail. */
@@ -1199,8 +1206,10 @@ int ref_ros2_vjp(const cells_world* w, const cells_symbolic* sym, const double* 
  * (:269-278, models/surface_pump.py:325-330); the face drift with the operator's weights
  * (surface_imex.py:135-139, models/surface_pump.py:243-256); the applied face flux's heat and the
  * reservoir work (models/surface_pump.py:333-359); domain_valid and where(valid, new, old).
- * The step-start potentials are the previous accepted step's operator potentials (the state's
- * node potentials; a cell's is its node's plus its charge over its capacitance). */
+ * The step-start potentials are the field solve at the state (ref_field_start below, the
+ * reference's readouts), or, in the carried form, the previous accepted step's operator
+ * potentials (the state's node potentials; a cell's is its node's plus its charge over its
+ * capacitance). */
 
 void ref_logmean(double left, double right, double* mean, double* log_ratio) {
     /* numerics/extracellular_field.py:48-62 */
@@ -1301,9 +1310,270 @@ static double field_charge(const cells_world* w, const double* y_dn, int64_t str
     return CHARGE_PER_FMOL * t;
 }
 
+/* ---- the field solve at a state (numerics/surface_field.py:614-855)
+ *
+ * The capacitor components of a spatial world are each cell with its grid node (joined by the
+ * membrane capacitor) and every other interior node alone; the ring nodes are held at potential 0
+ * (reservoir clamps), so every component floats. The capacitor solve fixes each component's
+ * representative (its node) at 0: base = q_i / C_i at a cell, 0 at the nodes (_capacitor_solve,
+ * :598-602). The offsets, one per interior node, solve Kirchhoff current continuity: each
+ * component's net outflow through its faces (log-mean electrodiffusion, numerics/
+ * surface_transport.py:340-353) and its cell's junctions (GHK) is zero. Start: the faces' linear
+ * balance at the base (:668-672); then the damped Newton with the junctions' slopes (:737-775):
+ * at most 16 iterations, 12 halvings, a trial accepted when its largest residual falls or its norm
+ * max |r| / (1e-23 + 2e-12 max current) reaches 0.01. The offset systems (the face conductance
+ * Laplacian plus the junction slopes on the interior nodes) are scaled by their largest entry as
+ * _offset_solve (:604-612) scales them, and solved with the sparse Cholesky on ks, world k's
+ * pattern of interior nodes with edge_pos = each face's entry (-1 where it touches the ring), then
+ * each junction's (world-local order). */
+
+#define NEWTON_TARGET 0.01
+#define CURRENT_ATOL_A 1e-23
+#define CURRENT_RTOL 2e-12
+#define NEWTON_ITERATIONS 16
+#define NEWTON_BACKTRACKS 12
+
+typedef struct {
+    const cells_world* w;
+    const cells_field_grid* g;
+    const cells_symbolic* ks;
+    const cells_control* c;
+    int32_t k, side;
+    int64_t c0, nc, e0, ne, NN, NF, ni;
+    double T, RT;
+    double *ci, *co, *base, *open, *phi, *fcur, *jcur, *res, *slope, *cond, *mean, *lg;
+} field_newton;
+
+static double newton_evaluate(field_newton* K, const double* offset, int* valid_out, double* resmax_out) {
+    const cells_world* w = K->w;
+    const int S = w->S;
+    const int32_t side = K->side;
+    int valid = 1;
+    for (int64_t n = 0; n < K->NN; ++n)
+        K->phi[K->nc + n] = K->base[K->nc + n] + (on_ring(side, n) ? 0.0 : offset[interior_index(side, n)]);
+    for (int64_t i = 0; i < K->nc; ++i)
+        K->phi[i] = K->base[i] + offset[interior_index(side, K->g->cell_node[K->c0 + i])];
+    for (int64_t x = 0; x < K->nc + K->NN; ++x) valid = valid && isfinite(K->phi[x]);
+    /* membrane routes: inside each component, checked for the transport's domain only */
+    for (int64_t i = 0; i < K->nc; ++i) {
+        int64_t node = K->g->cell_node[K->c0 + i];
+        double dv = K->phi[i] - K->phi[K->nc + node];
+        for (int s = 0; s < S; ++s) {
+            double z = w->valence[s];
+            double J = ref_selective_flux(K->ci[(int64_t)s * K->nc + i], K->co[(int64_t)s * K->NN + node], dv, z,
+                                          K->T, w->pam[(int64_t)s * w->ncell + K->c0 + i], K->c->membrane);
+            valid = valid && fabs(z * F_CONST * dv / K->RT) <= MAX_ABS_X && isfinite(J);
+        }
+    }
+    double jmax = 0.0, fmax = 0.0;
+    for (int64_t j = 0; j < K->ne; ++j) {
+        int64_t e = K->e0 + j, a = w->snd[e] - K->c0, b = w->rcv[e] - K->c0;
+        double dv = K->phi[a] - K->phi[b];
+        double cur = 0.0, sl = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double z = w->valence[s], pa = w->paj[(int64_t)s * w->nedge + e];
+            double ca = K->ci[(int64_t)s * K->nc + a], cb = K->ci[(int64_t)s * K->nc + b];
+            double J = ref_selective_flux(ca, cb, dv, z, K->T, pa, K->open[j]);
+            cur = s == 0 ? z * J : cur + z * J;
+            double x = z * F_CONST * dv / K->RT;
+            double d = z * z * pa * (-ca * ref_bernoulli_derivative(-x) - cb * ref_bernoulli_derivative(x));
+            sl = s == 0 ? d : sl + d;
+            valid = valid && fabs(x) <= MAX_ABS_X && isfinite(J);
+        }
+        K->jcur[j] = CHARGE_PER_FMOL * cur;
+        K->slope[j] = F_CONST * F_CONST / K->RT * K->open[j] * sl;
+        valid = valid && isfinite(K->slope[j]) && K->slope[j] >= 0;
+        if (fabs(K->jcur[j]) > jmax) jmax = fabs(K->jcur[j]); /* a NaN current fails isfinite(J) */
+    }
+    for (int64_t q = 0; q < K->NF; ++q) {
+        int64_t a, b;
+        face_nodes(side, q, &a, &b);
+        double dv = K->phi[K->nc + a] - K->phi[K->nc + b];
+        double cur = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double z = w->valence[s];
+            double ca = K->co[(int64_t)s * K->NN + a], cb = K->co[(int64_t)s * K->NN + b];
+            double mu = K->RT * (log(ca) - log(cb)) + z * F_CONST * dv;
+            double J = K->g->face_factor[s] * K->mean[(int64_t)s * K->NF + q] * mu / (K->RT * MOL_PER_FMOL);
+            cur = s == 0 ? z * J : cur + z * J;
+            valid = valid && fabs(z * F_CONST * dv / K->RT) <= MAX_ABS_X && isfinite(J);
+        }
+        K->fcur[q] = CHARGE_PER_FMOL * cur;
+        if (fabs(K->fcur[q]) > fmax) fmax = fabs(K->fcur[q]);
+    }
+    /* each component's net outflow: face_incidence @ face currents + junction_incidence @ junction currents */
+    for (int64_t x = 0; x < K->ni; ++x) K->res[x] = 0.0;
+    for (int64_t q = 0; q < K->NF; ++q) {
+        int64_t a, b;
+        face_nodes(side, q, &a, &b);
+        if (!on_ring(side, a)) K->res[interior_index(side, a)] += K->fcur[q];
+        if (!on_ring(side, b)) K->res[interior_index(side, b)] -= K->fcur[q];
+    }
+    for (int64_t j = 0; j < K->ne; ++j) {
+        int64_t e = K->e0 + j;
+        K->res[interior_index(side, K->g->cell_node[w->snd[e]])] += K->jcur[j];
+        K->res[interior_index(side, K->g->cell_node[w->rcv[e]])] -= K->jcur[j];
+    }
+    double rmax = 0.0;
+    for (int64_t x = 0; x < K->ni; ++x)
+        if (fabs(K->res[x]) > rmax) rmax = fabs(K->res[x]);
+    double scale = fmax > jmax ? fmax : jmax;
+    double norm = rmax / (CURRENT_ATOL_A + CURRENT_RTOL * scale);
+    valid = valid && isfinite(norm);
+    *valid_out = valid;
+    *resmax_out = rmax;
+    return norm;
+}
+
+/* _offset_solve: the face conductance Laplacian on the interior nodes (+ the junction slopes when
+ * slope is given) scaled by its largest entry; x = matrix^-1 rhs. */
+static void newton_offset_solve(field_newton* K, const double* slope, const double* rhs, double* x) {
+    const cells_symbolic* ks = K->ks;
+    const int32_t side = K->side;
+    double* v = (double*)calloc((size_t)ks->nnz, sizeof(double));
+    double* diag = (double*)calloc((size_t)K->ni, sizeof(double));
+    double* b = (double*)malloc(sizeof(double) * (size_t)K->ni);
+    for (int64_t q = 0; q < K->NF; ++q) {
+        int64_t a, bb;
+        face_nodes(side, q, &a, &bb);
+        if (!on_ring(side, a)) diag[interior_index(side, a)] += K->cond[q];
+        if (!on_ring(side, bb)) diag[interior_index(side, bb)] += K->cond[q];
+        if (ks->edge_pos[q] >= 0) v[ks->edge_pos[q]] -= K->cond[q];
+    }
+    if (slope)
+        for (int64_t j = 0; j < K->ne; ++j) {
+            int64_t e = K->e0 + j;
+            diag[interior_index(side, K->g->cell_node[K->w->snd[e]])] += slope[j];
+            diag[interior_index(side, K->g->cell_node[K->w->rcv[e]])] += slope[j];
+            v[ks->edge_pos[K->NF + j]] -= slope[j];
+        }
+    for (int64_t xx = 0; xx < K->ni; ++xx) v[ks->diag_pos[xx]] = diag[xx];
+    double scale = 0.0;
+    for (int64_t p = 0; p < ks->nnz; ++p)
+        if (fabs(v[p]) > scale) scale = fabs(v[p]);
+    if (!(scale > 0)) scale = 1.0;
+    for (int64_t p = 0; p < ks->nnz; ++p) v[p] = v[p] / scale;
+    for (int64_t xx = 0; xx < K->ni; ++xx) b[xx] = rhs[xx] / scale;
+    ref_factor(ks, v);
+    ref_solve(ks, v, b, x);
+    free(v);
+    free(diag);
+    free(b);
+}
+
+int ref_field_start(const cells_world* w, const cells_field_grid* g, const cells_symbolic* ks, int32_t k,
+                    const double* y, const cells_control* c, double* phi, int32_t* iterations) {
+    cells_field_layout L;
+    ref_field_layout(w, g, &L);
+    const int S = w->S;
+    field_newton K;
+    K.w = w; K.g = g; K.ks = ks; K.c = c; K.k = k; K.side = g->side;
+    K.c0 = w->cell_off[k]; K.nc = w->cell_off[k + 1] - K.c0;
+    K.e0 = w->edge_off[k]; K.ne = w->edge_off[k + 1] - K.e0;
+    K.NN = L.nodes; K.NF = L.faces; K.ni = (int64_t)(g->side - 2) * (g->side - 2);
+    K.T = w->temperature; K.RT = R_CONST * K.T;
+    K.ci = (double*)malloc(sizeof(double) * (size_t)(S * K.nc));
+    K.co = (double*)malloc(sizeof(double) * (size_t)(S * K.NN));
+    K.base = (double*)malloc(sizeof(double) * (size_t)(K.nc + K.NN));
+    K.open = (double*)malloc(sizeof(double) * (size_t)(K.ne > 0 ? K.ne : 1));
+    K.phi = phi;
+    K.fcur = (double*)malloc(sizeof(double) * (size_t)K.NF);
+    K.jcur = (double*)malloc(sizeof(double) * (size_t)(K.ne > 0 ? K.ne : 1));
+    K.res = (double*)malloc(sizeof(double) * (size_t)K.ni);
+    K.slope = (double*)malloc(sizeof(double) * (size_t)(K.ne > 0 ? K.ne : 1));
+    K.cond = (double*)malloc(sizeof(double) * (size_t)K.NF);
+    K.mean = (double*)malloc(sizeof(double) * (size_t)(S * K.NF));
+    K.lg = (double*)malloc(sizeof(double) * (size_t)(S * K.NF));
+    double* offset = (double*)malloc(sizeof(double) * (size_t)K.ni);
+    double* proposal = (double*)malloc(sizeof(double) * (size_t)K.ni);
+    double* delta = (double*)malloc(sizeof(double) * (size_t)K.ni);
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)K.ni);
+    int valid = 1;
+    for (int s = 0; s < S; ++s) {
+        for (int64_t i = 0; i < K.nc; ++i) {
+            double v = (w->n0[(int64_t)s * w->ncell + K.c0 + i] + FCF(&L, w, y, L.dn + s, K.c0 + i)) / w->vol0[K.c0 + i];
+            K.ci[(int64_t)s * K.nc + i] = v;
+            valid = valid && isfinite(v) && v > 0;
+        }
+        for (int64_t n = 0; n < K.NN; ++n) {
+            double v = (g->node_n0[(int64_t)s * K.NN + n] + FNF(&L, w, y, L.dno + s, k, n)) / g->node_volume;
+            K.co[(int64_t)s * K.NN + n] = v;
+            valid = valid && isfinite(v) && v > 0;
+        }
+    }
+    /* base: the capacitor solve with each component's node fixed at 0 */
+    for (int64_t n = 0; n < K.NN; ++n) K.base[K.nc + n] = 0.0;
+    for (int64_t i = 0; i < K.nc; ++i) {
+        double q = w->q0[K.c0 + i] + field_charge(w, &FCF(&L, w, y, L.dn, 0), w->ncell, K.c0 + i);
+        K.base[i] = q / w->cap[K.c0 + i];
+    }
+    for (int64_t j = 0; j < K.ne; ++j) K.open[j] = c->junction * FEF(&L, w, y, L.gate, K.e0 + j);
+    /* the faces' conductances and their currents at the base (dv = 0 between nodes) */
+    for (int64_t q = 0; q < K.NF; ++q) {
+        int64_t a, b;
+        face_nodes(g->side, q, &a, &b);
+        double G = 0.0, f0 = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double z = w->valence[s], m, lg;
+            ref_logmean(K.co[(int64_t)s * K.NN + a], K.co[(int64_t)s * K.NN + b], &m, &lg);
+            K.mean[(int64_t)s * K.NF + q] = m;
+            K.lg[(int64_t)s * K.NF + q] = lg;
+            double weight = g->face_factor[s] * m;
+            G = s == 0 ? z * z * weight : G + z * z * weight;
+            f0 = s == 0 ? z * weight * lg : f0 + z * weight * lg;
+        }
+        K.cond[q] = F_CONST * F_CONST / K.RT * G;
+        K.fcur[q] = F_CONST * f0;
+    }
+    for (int64_t x = 0; x < K.ni; ++x) rhs[x] = 0.0;
+    for (int64_t q = 0; q < K.NF; ++q) {
+        int64_t a, b;
+        face_nodes(g->side, q, &a, &b);
+        if (!on_ring(g->side, a)) rhs[interior_index(g->side, a)] -= K.fcur[q];
+        if (!on_ring(g->side, b)) rhs[interior_index(g->side, b)] += K.fcur[q];
+    }
+    newton_offset_solve(&K, NULL, rhs, offset);
+    int ok;
+    double rmax;
+    double norm = newton_evaluate(&K, offset, &ok, &rmax);
+    int accepted = ok && valid;
+    int32_t count = 0;
+    while (count < NEWTON_ITERATIONS && norm > NEWTON_TARGET && accepted) {
+        norm = newton_evaluate(&K, offset, &ok, &rmax);
+        for (int64_t x = 0; x < K.ni; ++x) rhs[x] = -K.res[x];
+        newton_offset_solve(&K, K.slope, rhs, delta);
+        double before = rmax;
+        int step = 0;
+        accepted = 0;
+        while (step < NEWTON_BACKTRACKS && !accepted) {
+            double f = pow(0.5, step);
+            for (int64_t x = 0; x < K.ni; ++x) proposal[x] = offset[x] + delta[x] * f;
+            norm = newton_evaluate(&K, proposal, &ok, &rmax);
+            accepted = ok && (rmax < before || norm <= NEWTON_TARGET);
+            ++step;
+        }
+        ++count;
+        if (accepted) memcpy(offset, proposal, sizeof(double) * (size_t)K.ni);
+    }
+    norm = newton_evaluate(&K, offset, &ok, &rmax);
+    int converged = valid && ok && accepted && norm <= NEWTON_TARGET;
+    /* the cells' capacitor rows (solved rows; each node's row carries its component's charge) */
+    for (int64_t i = 0; i < K.nc; ++i) {
+        double q = w->q0[K.c0 + i] + field_charge(w, &FCF(&L, w, y, L.dn, 0), w->ncell, K.c0 + i);
+        double cq = w->cap[K.c0 + i] * (phi[i] - phi[K.nc + g->cell_node[K.c0 + i]]);
+        double sc = fabs(q) > fabs(cq) ? fabs(q) : fabs(cq);
+        converged = converged && fabs(cq - q) <= 3e-25 + 2e-12 * sc;
+    }
+    if (iterations) *iterations = count;
+    free(K.ci); free(K.co); free(K.base); free(K.open); free(K.fcur); free(K.jcur); free(K.res);
+    free(K.slope); free(K.cond); free(K.mean); free(K.lg);
+    free(offset); free(proposal); free(delta); free(rhs);
+    return converged;
+}
+
 int ref_field_step(const cells_world* w, const cells_field_grid* g, const cells_symbolic* sym,
-                   const double* y, double* out, double dt, const cells_control* c, int32_t refresh,
-                   double* op_weight, double* op_scale, double* op_factor, int32_t* valid_out) {
+                   const cells_symbolic* ksym, const double* y, double* out, double dt, const cells_control* c,
+                   int32_t refresh, double* op_weight, double* op_scale, double* op_factor, int32_t* valid_out) {
     cells_field_layout L;
     ref_field_layout(w, g, &L);
     const int S = w->S;
@@ -1338,12 +1608,17 @@ int ref_field_step(const cells_world* w, const cells_field_grid* g, const cells_
                 co[(int64_t)s * NN + n] = amount / g->node_volume;
                 valid = valid && isfinite(co[(int64_t)s * NN + n]) && co[(int64_t)s * NN + n] > 0;
             }
-        /* step-start potentials: nodes from the state, cells their node's plus q / C */
+        /* step-start potentials: the field solve at y (ksym given), else the state's node
+         * potentials, cells their node's plus q / C */
         double* phi = (double*)malloc(sizeof(double) * (size_t)(nc + NN));
-        for (int64_t n = 0; n < NN; ++n) phi[nc + n] = FNF(&L, w, y, L.phi, k, n);
-        for (int64_t i = c0; i < c1; ++i) {
-            double q = w->q0[i] + field_charge(w, &FCF(&L, w, y, L.dn, 0), w->ncell, i);
-            phi[i - c0] = phi[nc + g->cell_node[i]] + q / w->cap[i];
+        if (ksym) {
+            valid = ref_field_start(w, g, &ksym[k], k, y, c, phi, NULL) && valid;
+        } else {
+            for (int64_t n = 0; n < NN; ++n) phi[nc + n] = FNF(&L, w, y, L.phi, k, n);
+            for (int64_t i = c0; i < c1; ++i) {
+                double q = w->q0[i] + field_charge(w, &FCF(&L, w, y, L.dn, 0), w->ncell, i);
+                phi[i - c0] = phi[nc + g->cell_node[i]] + q / w->cap[i];
+            }
         }
         /* junction routes (surface_transport.py:310-367) */
         for (int64_t e = e0; e < e1; ++e) {

@@ -136,8 +136,19 @@ typedef struct {
 
 void ref_field_layout(const cells_world* w, const cells_field_grid* g, cells_field_layout* out);
 
+/* The field solve of spatial world k at state y (numerics/surface_field.py:614-855): the
+ * capacitor components' base and the damped Newton over their floating offsets for current
+ * continuity. ks is world k's offset pattern over the interior nodes (row-major): diag_pos of each,
+ * edge_pos each face's entry (-1 where it touches the ring) then each junction's (world-local
+ * order). phi [ncell_k + side^2] receives the cells' then the nodes' potentials; iterations (may
+ * be NULL) the Newton count. Returns the reference's `converged`. */
+int ref_field_start(const cells_world* w, const cells_field_grid* g, const cells_symbolic* ks, int32_t k,
+                    const double* y, const cells_control* c, double* phi, int32_t* iterations);
+
 /* One IMEX Euler step with implicit face electromigration of every spatial world
- * (models/surface_pump.py:292-362 with the Euler tableau). sym[k] is world k's operator system,
+ * (models/surface_pump.py:292-362 with the Euler tableau). The step-start potentials are
+ * ref_field_start's where ksym [W] is given, else the state's node potentials (the carried form).
+ * sym[k] is world k's operator system,
  * cells then interior nodes (row-major), whose diag_pos lists every node's diagonal entry and
  * whose edge_pos lists each cell's capacitor entry (cells in order) then each face's entry (-1
  * where a face touches the ring). op_weight [W][S][F] and op_factor (the worlds' factors back to
@@ -145,8 +156,8 @@ void ref_field_layout(const cells_world* w, const cells_field_grid* g, cells_fie
  * concentrations (step-start weights) before the solve. out receives the new state or the old
  * one where a world refuses; valid[W] the acceptance. */
 int ref_field_step(const cells_world* w, const cells_field_grid* g, const cells_symbolic* sym,
-                   const double* y, double* out, double dt, const cells_control* c, int32_t refresh,
-                   double* op_weight, double* op_scale, double* op_factor, int32_t* valid);
+                   const cells_symbolic* ksym, const double* y, double* out, double dt, const cells_control* c,
+                   int32_t refresh, double* op_weight, double* op_scale, double* op_factor, int32_t* valid);
 
 /* The reference's log-mean (numerics/extracellular_field.py:48-62): mean and log ratio. */
 void ref_logmean(double left, double right, double* mean, double* log_ratio);
@@ -154,6 +165,7 @@ void ref_logmean(double left, double right, double* mean, double* log_ratio);
 /* Point laws, exposed for closed-form tests. */
 double ref_bernoulli(double x);
 double ref_bernoulli_slope(double x);
+double ref_bernoulli_derivative(double x);
 double ref_selective_flux(double c_from, double c_to, double dv, double z, double T, double pa,
                           double avail);
 double ref_route_slope(double c_from, double c_to, double dv, double z, double T, double pa,

@@ -2,9 +2,12 @@
 
 - Parity: the library's steps against the oracle's (tests restate the reference's IMEX Euler with
   implicit face electromigration on the cells-plus-free-nodes operator, minimum-degree ordered and
-  factored in the reference's order) from the same seeded worlds, with the operator rebuilt every
-  step and every 5 steps: every field within 1e-9 of its scale. Grids up to 130 x 130, whose
-  largest fronts (above 100 rows) run in HBM scratch and through the workgroup solve.
+  factored in the reference's order, each step from the field solve at its state) from the same
+  seeded worlds, with the operator rebuilt every step and every 5 steps: every field within 1e-9
+  of its scale; once more from the carried potentials. Grids up to 130 x 130, whose largest fronts
+  (above 100 rows) run in HBM scratch and through the workgroup solve.
+- The field solve alone (cells_field_potentials) against the oracle's Newton at states some steps
+  into a run: node and cell potentials within 1e-9 of their scale.
 - At the full 256 x 256 grid with 2048 cells: material closes per species against the ring
   supplies (1e-12 of the amounts), every interior node with its cell is neutral after each step
   (1e-12 of the cells' charge), and the refusal path keeps a world's state.
@@ -66,28 +69,65 @@ def compare(fp, got, want, tol=1e-9):
     return worst, bad
 
 
-@pytest.mark.parametrize("cfg,steps,refresh", [
-    (small_field(worlds=2, seed=5), 60, 1),
-    (small_field(worlds=2, seed=6), 60, 5),
-    (FieldConfig(worlds=2, cells=36, row_width=6, side=40, spacing=5, seed=7), 20, 5),
-    (FieldConfig(worlds=2, cells=100, row_width=10, side=130, spacing=11, seed=8), 8, 4),
-], ids=["14-grid-refresh1", "14-grid-refresh5", "40-grid", "130-grid"])
-def test_field_steps_match_the_oracle(torch, cfg, steps, refresh):
+@pytest.mark.parametrize("cfg,steps,refresh,start", [
+    (small_field(worlds=2, seed=5), 60, 1, "solve"),
+    (small_field(worlds=2, seed=6), 60, 5, "solve"),
+    (small_field(worlds=2, seed=6), 60, 5, "carried"),
+    (FieldConfig(worlds=2, cells=36, row_width=6, side=40, spacing=5, seed=7), 20, 5, "solve"),
+    (FieldConfig(worlds=2, cells=100, row_width=10, side=130, spacing=11, seed=8), 8, 4, "solve"),
+], ids=["14-grid-refresh1", "14-grid-refresh5", "14-grid-carried", "40-grid", "130-grid"])
+def test_field_steps_match_the_oracle(torch, cfg, steps, refresh, start):
     from cells_amd.field import FieldEnsemble
 
     fp = build_field(cfg)
     say(f"field parity: side {cfg.side}, {cfg.cells} cells, {cfg.worlds} worlds, {steps} steps, "
-        f"refresh every {refresh}")
+        f"refresh every {refresh}, start {start}")
     y0 = initial_field_state(fp)
-    want, ok_ref = oracle.field_run(fp, y0, 1e-3, steps, refresh)
+    want, ok_ref = oracle.field_run(fp, y0, 1e-3, steps, refresh, start=start)
     assert ok_ref.all()
-    e = FieldEnsemble(fp, refresh_every=refresh)
+    e = FieldEnsemble(fp, refresh_every=refresh, start=start)
     say(f"plan {e.plan_stats()}")
     got, ok = e.step(y0, 1e-3, nsteps=steps)
     assert bool(ok.all())
     worst, bad = compare(fp, got.cpu().numpy(), want)
     say(f"worst {max(worst.values()):.3e}")
     assert not bad, bad
+    e.close()
+
+
+@pytest.mark.parametrize("cfg,steps", [
+    (small_field(worlds=2, seed=18), 12),
+    (FieldConfig(worlds=2, cells=36, row_width=6, side=40, spacing=5, seed=19), 10),
+    (FieldConfig(worlds=2, cells=100, row_width=10, side=130, spacing=11, seed=20), 4),
+], ids=["14-grid", "40-grid", "130-grid"])
+def test_field_solve_matches_the_oracle(torch, cfg, steps):
+    """The step-start potentials (cells_field_potentials, the chord iteration on the operator's
+    factor) against the oracle's damped Newton (numerics/surface_field.py:614-855) at a state some
+    steps into a run: the same root, met to the same target."""
+    from cells_amd.field import FieldEnsemble
+
+    fp = build_field(cfg)
+    say(f"field solve parity: side {cfg.side}, {cfg.cells} cells, {steps} steps")
+    y, ok_ref = oracle.field_run(fp, initial_field_state(fp), 1e-3, steps, 3)
+    assert ok_ref.all()
+    op = oracle.FieldOperator(fp)
+    e = FieldEnsemble(fp, refresh_every=3)
+    e.step(initial_field_state(fp), 1e-3, nsteps=1)  # an operator one step old: the chord's matrix
+    phi, iterations, ok = e.potentials(y, 1e-3)
+    assert bool(ok.all())
+    phi = phi.cpu().numpy()
+    say(f"chord steps {iterations.cpu().tolist()}")
+    n = fp.config.cells
+    cq, _ = charges(fp, y)
+    for k in range(fp.W):
+        cells, nodes, newton, conv = oracle.field_start(fp, y, op, k)
+        assert conv
+        scale = np.max(np.abs(nodes))
+        assert scale > 0
+        assert np.max(np.abs(phi[k] - nodes)) <= 1e-9 * scale, (k, np.max(np.abs(phi[k] - nodes)) / scale)
+        c0 = int(fp.cells.cell_off[k])
+        got_cells = phi[k][fp.cell_node[c0:c0 + n]] + cq[c0:c0 + n] / fp.cells.cap[c0:c0 + n]
+        assert np.max(np.abs(got_cells - cells)) <= 1e-9 * np.max(np.abs(cells))
     e.close()
 
 
@@ -142,6 +182,9 @@ def test_full_grid_closes_material_and_charge(torch):
             assert np.max(np.abs(q[~ring])) <= 1e-12 * np.max(np.abs(cq[cs]))
         phi = L.node(yh, L.phi)
         say(f"after {20 * (chunk + 1)} steps: node potentials {phi.min():.3e} .. {phi.max():.3e} V")
+    start, iterations, ok = e.potentials(y, 1e-3)
+    assert bool(ok.all())
+    say(f"field solve at the end: chord steps {iterations.cpu().tolist()}")
     e.close()
 
 

@@ -53,7 +53,11 @@ def closed(fp):
 def test_face_diffusion_is_ficks_law():
     """One step from a randomly perturbed grid with every route closed: each face's transfer over
     dt is Fick's D A/L (a - b) at the start concentrations plus the drift W z F/RT (phi_a - phi_b)
-    with the step's own node potentials and weights."""
+    with the step's own node potentials and weights. The step starts from the carried potentials:
+    with every route closed the face laws do not see the start potentials, and a grid perturbed in
+    every species at random carries net face currents that are a 1e-4 residue of the species'
+    currents, below what the field solve's target can resolve (the reference's sparse field solve
+    refuses such states too; its floor variant, numerics/surface_field.py:30-44, exists for them)."""
     fp = build_field(small_field(worlds=1, seed=11))
     L = FieldLayout.of(fp)
     y = initial_field_state(fp)
@@ -62,7 +66,7 @@ def test_face_diffusion_is_ficks_law():
     for s in range(fp.S):
         L.node(y, L.dno + s)[0, interior] = rng.uniform(-0.05, 0.05, interior.size) * fp.node_n0[s, interior]
     op = oracle.FieldOperator(fp)
-    out, ok = oracle.field_step(fp, y, 1e-3, op, True, closed(fp))
+    out, ok = oracle.field_step(fp, y, 1e-3, op, True, closed(fp), start="carried")
     assert ok.all()
     a, b = fp.face_nodes()
     phi = L.node(out, L.phi)[0]
@@ -164,3 +168,157 @@ def test_rest_stays_put_and_bad_controls_refuse():
     bad, ok = oracle.field_step(fp, y0, 1e-3, op, True, oracle.control(pump=1.5))
     assert not ok.any()
     assert np.array_equal(bad, y0)
+
+
+# ---- the field solve at a state (numerics/surface_field.py:614-855) ----------------------------
+#
+# Pinned by what current continuity fixes, with the currents restated here from the route laws
+# (numerics/surface_transport.py:340-353 faces, laws/junctions/selective.py:28-50 junctions):
+# - at rest (bath everywhere, equal cells) every current is zero and the solve stops at its start;
+# - with the junctions closed the offsets solve a linear system, assembled densely here;
+# - with them open, the returned potentials leave each component's net outflow within the
+#   Newton's target, and leaving the junctions out of that sum breaks it (they matter there).
+
+
+def log_mean(a, b):
+    """(a - b) / ln(a / b), with ln(a / b) = log1p((a - b) / b) (a - b exact for close a, b)."""
+    d = a - b
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(d == 0, a, d / np.log1p(d / b))
+
+
+def bernoulli(x):
+    x = np.asarray(x, dtype=float)
+    return np.where(np.abs(x) < 1e-4, 1 - x / 2 + x * x / 12, x / np.expm1(np.where(x == 0, 1.0, x)))
+
+
+def field_currents(fp, y, node_phi, ctl_junction=1.0):
+    """(face currents [F], junction currents [E_0]) of world 0 at node potentials node_phi, A."""
+    p = fp.cells
+    L = FieldLayout.of(fp)
+    T = p.config.temperature
+    z = p.valence[:fp.S]
+    n = fp.config.cells
+    a, b = fp.face_nodes()
+    co = np.stack([(fp.node_n0[s] + L.node(y, L.dno + s)[0]) / fp.node_volume for s in range(fp.S)])
+    la, lb = co[:, a], co[:, b]
+    mean = log_mean(la, lb)
+    dv = node_phi[a] - node_phi[b]
+    mu = R * T * (np.log(la) - np.log(lb)) + z[:, None] * F * dv
+    face = F * (z[:, None] * fp.face_factor[:fp.S, None] * mean * mu / (R * T)).sum(axis=0)
+    cq, _ = charges(fp, y)
+    cell_phi = node_phi[fp.cell_node[:n]] + cq[:n] / p.cap[:n]
+    e1 = int(p.edge_off[1])
+    snd, rcv = p.snd[:e1], p.rcv[:e1]
+    n0 = np.asarray(p.n0).reshape(fp.S, -1)
+    ci = np.stack([(n0[s, :n] + L.cell(y, L.dn + s)[:n]) / p.vol0[:n] for s in range(fp.S)])
+    x = z[:, None] * F * (cell_phi[snd] - cell_phi[rcv]) / (R * T)
+    pa = p.paj.reshape(fp.S, -1)[:, :e1]
+    avail = ctl_junction * L.edge(y, L.gate)[:e1]
+    J = pa * avail * (ci[:, snd] * bernoulli(-x) - ci[:, rcv] * bernoulli(x))
+    return face, F * (z[:, None] * J).sum(axis=0)
+
+
+def net_outflow(fp, face, junction):
+    n = fp.config.cells
+    a, b = fp.face_nodes()
+    r = np.zeros(fp.side * fp.side)
+    np.add.at(r, a, face)
+    np.add.at(r, b, -face)
+    p = fp.cells
+    e1 = int(p.edge_off[1])
+    np.add.at(r, fp.cell_node[p.snd[:e1]], junction)
+    np.add.at(r, fp.cell_node[p.rcv[:e1]], -junction)
+    return r[~fp.ring()]
+
+
+def test_bernoulli_derivative_is_the_slope_of_bernoulli():
+    assert oracle.lib().ref_bernoulli_derivative(ctypes_double(0.0)) == -0.5
+    for x in (-30.0, -2.0, -0.3, -5e-5, 3e-5, 0.01, 0.7, 4.0, 25.0):
+        h = 1e-6 * max(1.0, abs(x))
+        fd = (oracle.lib().ref_bernoulli(ctypes_double(x + h)) - oracle.lib().ref_bernoulli(ctypes_double(x - h))) / (2 * h)
+        got = oracle.lib().ref_bernoulli_derivative(ctypes_double(x))
+        assert abs(got - fd) <= 1e-7 * max(abs(fd), 1e-3), (x, got, fd)
+
+
+def ctypes_double(v):
+    import ctypes
+
+    return ctypes.c_double(v)
+
+
+def test_field_solve_at_rest_is_the_capacitor_solve():
+    fp = build_field(small_field(worlds=2, seed=15))
+    y = initial_field_state(fp)
+    op = oracle.FieldOperator(fp)
+    cq, _ = charges(fp, y)
+    n = fp.config.cells
+    for k in range(fp.W):
+        cells, nodes, iterations, ok = oracle.field_start(fp, y, op, k)
+        assert ok and iterations == 0
+        assert np.all(nodes == 0.0)
+        c0 = int(fp.cells.cell_off[k])
+        assert np.allclose(cells, cq[c0:c0 + n] / fp.cells.cap[c0:c0 + n], rtol=1e-15, atol=0)
+
+
+def test_field_solve_with_closed_junctions_is_the_linear_balance():
+    """Junctions closed: the faces alone balance each node, a linear system in the node potentials
+    (conductances F^2/RT sum_s z_s^2 D_s A/L c_log, diffusion currents F sum_s z_s D_s A/L c_log
+    ln(a/b)), solved densely here from those definitions."""
+    fp = build_field(small_field(worlds=1, seed=16))
+    L = FieldLayout.of(fp)
+    y = initial_field_state(fp)
+    rng = np.random.default_rng(7)
+    interior = np.nonzero(~fp.ring())[0]
+    L.node(y, L.dno + 1)[0, interior] = rng.uniform(-0.05, 0.05, interior.size) * fp.node_n0[1, interior]
+    op = oracle.FieldOperator(fp)
+    cells, nodes, iterations, ok = oracle.field_start(fp, y, op, 0, closed(fp))
+    assert ok
+    T = fp.cells.config.temperature
+    z = fp.cells.valence[:fp.S]
+    a, b = fp.face_nodes()
+    co = np.stack([(fp.node_n0[s] + L.node(y, L.dno + s)[0]) / fp.node_volume for s in range(fp.S)])
+    la, lb = co[:, a], co[:, b]
+    mean = log_mean(la, lb)
+    weight = fp.face_factor[:fp.S, None] * mean
+    G = F * F / (R * T) * (z[:, None] ** 2 * weight).sum(axis=0)
+    I0 = F * (z[:, None] * weight * (np.log(la) - np.log(lb))).sum(axis=0)
+    index = -np.ones(fp.side * fp.side, dtype=int)
+    index[interior] = np.arange(interior.size)
+    M = np.zeros((interior.size, interior.size))
+    rhs = np.zeros(interior.size)
+    for q in range(len(a)):
+        ia, ib = index[a[q]], index[b[q]]
+        if ia >= 0:
+            M[ia, ia] += G[q]
+            rhs[ia] -= I0[q]
+            if ib >= 0:
+                M[ia, ib] -= G[q]
+        if ib >= 0:
+            M[ib, ib] += G[q]
+            rhs[ib] += I0[q]
+            if ia >= 0:
+                M[ib, ia] -= G[q]
+    want = np.linalg.solve(M, rhs)
+    assert np.max(np.abs(nodes[interior] - want)) <= 1e-11 * np.max(np.abs(want))
+    assert np.all(nodes[fp.ring()] == 0.0)
+
+
+def test_field_solve_balances_faces_and_junctions():
+    fp = build_field(small_field(worlds=1, seed=17))
+    y = initial_field_state(fp)
+    op = oracle.FieldOperator(fp)
+    w, g = oracle.world_struct(fp.cells), oracle.field_grid_struct(fp)
+    for t in range(8):
+        y, ok = oracle.field_step(fp, y, 1e-3, op, True, None, w, g)
+        assert ok.all()
+    cells, nodes, iterations, ok = oracle.field_start(fp, y, op, 0)
+    assert ok and iterations >= 1
+    face, junction = field_currents(fp, y, nodes)
+    r = net_outflow(fp, face, junction)
+    bound = 0.01 * (1e-23 + 2e-12 * max(np.abs(face).max(), np.abs(junction).max()))
+    assert np.abs(r).max() <= 10 * bound, (np.abs(r).max(), bound)
+    assert np.abs(net_outflow(fp, face, 0 * junction)).max() > 1e3 * bound
+    cq, _ = charges(fp, y)
+    n = fp.config.cells
+    assert np.allclose(cells, nodes[fp.cell_node[:n]] + cq[:n] / fp.cells.cap[:n], rtol=1e-14, atol=0)
