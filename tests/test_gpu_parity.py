@@ -107,6 +107,45 @@ def test_plain_config_trajectory_matches_oracle(torch):
     e.close()
 
 
+@pytest.mark.parametrize("method,cfg,dt,steps", [
+    ("ros2", config1(worlds=12, cells=(1, 7), seed=3), 5e-3, 40),
+    ("euler", config0(worlds=5, cells=(1, 4), seed=4), 1e-3, 200),
+], ids=["ros2-1-to-7-cells", "euler-water-1-to-4-cells"])
+def test_tiny_and_ragged_worlds_match_the_oracle(torch, method, cfg, dt, steps):
+    """Worlds of one cell (no junctions: a 1 x 1 stage matrix) up to a few cells, mixed in one
+    batch: the trajectory against the oracle (1e-9), batched against plain bit for bit, material
+    closed."""
+    p = build(cfg)
+    say(f"tiny worlds {method}: cells {np.diff(p.cell_off).tolist()}, junctions {np.diff(p.edge_off).tolist()}")
+    e = ensemble(p, method)
+    y0 = initial_state(p)
+    got, ok = e.step(y0, dt, nsteps=steps)
+    assert ok.cpu().numpy().all()
+    got = got.cpu().numpy()
+    want = y0
+    w = oracle.world_struct(p)
+    if method == "ros2":
+        _, sym = oracle.world_symbolics(p)
+        for _ in range(steps):
+            want, okr = oracle.step_ros2(p, want, dt, sym, world=w)
+            assert okr.all()
+    else:
+        for _ in range(steps):
+            want, okr = oracle.step_euler(p, want, dt, world=w)
+            assert okr.all()
+    err = max(relerr(vmem(p, got), vmem(p, want)), relerr(conc(p, got), conc(p, want)))
+    if p.water:
+        err = max(err, relerr(volume(p, got), volume(p, want)))
+    say(f"tiny worlds worst relative error {err:.3e}")
+    assert err <= 1e-9
+    assert material_residual(p, got) <= 1e-12
+    a, oka = e.step(y0, dt, nsteps=5, batched=True)
+    b, okb = e.step(y0, dt, nsteps=5, batched=False)
+    assert oka.cpu().numpy().all() and okb.cpu().numpy().all()
+    assert torch.equal(a, b)
+    e.close()
+
+
 def test_batched_equals_plain_bitwise(torch):
     for method, cfg, dt in (("euler", config0(worlds=4, cells=(20, 70), seed=5), 1e-3),
                             ("ros2", config1(worlds=4, cells=(30, 90), seed=6), 5e-3)):
