@@ -140,6 +140,8 @@ constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: 
 constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 19
 constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 54
 constexpr int kMfMaxRows = 128;                            // fronts above kMfLds (order <= 128) use HBM
+constexpr int kRelaxRows = 90;                             // relaxed supernodes join up to this order,
+constexpr int kRelaxZeros = 16;                            // with at most this many zeros per column
 
 // The largest front order whose packed triangle fits `tri` doubles.
 constexpr int max_order(int tri) {
@@ -2110,7 +2112,9 @@ struct cells_ensemble {
   int S = 0, W = 0, ncell = 0, nedge = 0, max_n = 0;
   std::vector<int64_t> cell_off, edge_off;
   std::vector<int> shape_of_host;
-  std::vector<cells::Symbolic> shapes_host;
+  std::vector<cells::Symbolic> shapes_host;      // the plans' patterns (relaxed in the multifrontal mode)
+  std::vector<int64_t> exact_nnz_host;           // per shape: nnz(L) of the bound pattern itself
+  std::vector<double> exact_flops_host;          // per shape: its sum of column counts squared
   std::vector<long> foff_host;
   std::vector<ShapeDev> sd_host;
   std::vector<int> pool_host;
@@ -2503,6 +2507,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
           return fail(CELLS_EINVAL, "worlds of one cell count with different lattices");
     }
     e->shapes_host.resize(rep.size());
+    e->exact_nnz_host.resize(rep.size());
+    e->exact_flops_host.resize(rep.size());
     e->sn_host.resize(rep.size());
     std::atomic<int> next_shape{0};
     std::atomic<int> bad{0};
@@ -2516,7 +2522,29 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
           try {
             world_pattern(w, rep[q], rows, cols);
             int n = (int)(w->cell_off[rep[q] + 1] - w->cell_off[rep[q]]);
-            e->shapes_host[q] = cells::bind(n, (int64_t)rows.size(), rows.data(), cols.data(), nullptr);
+            cells::Symbolic s = cells::bind(n, (int64_t)rows.size(), rows.data(), cols.data(), nullptr);
+            e->exact_nnz_host[q] = s.nnz();
+            double f = 0.0;
+            for (int32_t k = 0; k < s.n; ++k) {
+              const double c = (double)(s.column_start[k + 1] - s.column_start[k]);
+              f += c * c;
+            }
+            e->exact_flops_host[q] = f;
+            if (!e->pairwise) {
+              // relaxed supernodes: the multifrontal plan factors the pattern with the chains'
+              // explicit zeros (the reference-order mode keeps the bound pattern itself)
+              const std::vector<int32_t> pad = cells::relaxed_padding(s, kRelaxRows, kRelaxZeros);
+              if (!pad.empty()) {
+                for (size_t x = 0; x < pad.size(); x += 2) {
+                  rows.push_back(pad[x]);
+                  cols.push_back(pad[x + 1]);
+                }
+                cells::Symbolic r = cells::bind(n, (int64_t)rows.size(), rows.data(), cols.data(), s.order.data());
+                if (r.parent != s.parent) throw std::runtime_error("relaxed supernodes changed the elimination tree");
+                s = std::move(r);
+              }
+            }
+            e->shapes_host[q] = std::move(s);
             e->sn_host[q] = cells::supernodes(e->shapes_host[q], front_class);
           } catch (const std::exception& ex) {
             bad = 1;
@@ -2713,6 +2741,22 @@ int cells_world_positions(const cells_ensemble* e, int32_t w, int64_t* diag_pos,
   if (edge_pos)
     for (int q = 0; q < d.E; ++q) edge_pos[q] = node[2 * d.n + q];
   if (factor_offset) *factor_offset = e->foff_host[w];
+  return CELLS_OK;
+}
+
+int cells_world_factor_positions(const cells_ensemble* e, int32_t w, const int32_t* rows, const int32_t* cols,
+                                 int64_t count, int64_t* out) {
+  if (!e || w < 0 || w >= e->W) return fail(CELLS_EINVAL, "world out of range");
+  if (count < 0 || (count > 0 && (!rows || !cols || !out))) return fail(CELLS_EINVAL, "bad argument");
+  const cells::Symbolic& s = e->shapes_host[e->shape_of_host[w]];
+  for (int64_t x = 0; x < count; ++x) {
+    if (rows[x] < 0 || rows[x] >= s.n || cols[x] < 0 || cols[x] >= s.n) return fail(CELLS_EINVAL, "node out of range");
+    try {
+      out[x] = cells::position(s, rows[x], cols[x]);
+    } catch (const std::out_of_range&) {
+      out[x] = -1;
+    }
+  }
   return CELLS_OK;
 }
 
@@ -3273,21 +3317,11 @@ extern "C" int cells_plan_stats(const cells_ensemble* e, int64_t* nnz_total, dou
   int64_t nnz = 0;
   double flops = 0.0;
   int32_t lev = 0;
-  std::vector<double> shape_flops(e->shapes_host.size(), 0.0);
-  for (size_t q = 0; q < e->shapes_host.size(); ++q) {
-    const cells::Symbolic& s = e->shapes_host[q];
-    double f = 0.0;
-    for (int32_t k = 0; k < s.n; ++k) {
-      double c = (double)(s.column_start[k + 1] - s.column_start[k]);
-      f += c * c;
-    }
-    shape_flops[q] = f;
-    lev = std::max(lev, s.levels());
-  }
+  for (size_t q = 0; q < e->shapes_host.size(); ++q) lev = std::max(lev, e->shapes_host[q].levels());
   for (int w = 0; w < e->W; ++w) {
     int q = e->shape_of_host[w];
-    nnz += e->shapes_host[q].nnz();
-    flops += shape_flops[q];
+    nnz += e->exact_nnz_host[q];
+    flops += e->exact_flops_host[q];
   }
   if (nnz_total) *nnz_total = nnz;
   if (flops_total) *flops_total = flops;

@@ -268,4 +268,64 @@ Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows)) {
   return out;
 }
 
+std::vector<int32_t> relaxed_padding(const Symbolic& s, int32_t max_rows, int32_t max_zeros) {
+  const int32_t n = s.n;
+  auto col_len = [&](int32_t j) { return static_cast<int32_t>(s.column_start[j + 1] - s.column_start[j]); };
+  std::vector<int32_t> nchild(n, 0);
+  for (int32_t k = 0; k < n; ++k)
+    if (s.parent[k] >= 0) nchild[s.parent[k]]++;
+  std::vector<int32_t> f0, fw, of(n);
+  for (int32_t k = 0; k < n;) {
+    int32_t j = k;
+    while (j + 1 < n && s.parent[j] == j + 1 && nchild[j + 1] == 1 && col_len(j) == col_len(j + 1) + 1) ++j;
+    for (int32_t q = k; q <= j; ++q) of[q] = static_cast<int32_t>(f0.size());
+    f0.push_back(k);
+    fw.push_back(j - k + 1);
+    k = j + 1;
+  }
+  const int32_t ns = static_cast<int32_t>(f0.size());
+  std::vector<int32_t> fpar(ns, -1), kids(ns, 0);
+  for (int32_t t = 0; t < ns; ++t) {
+    const int32_t last = f0[t] + fw[t] - 1;
+    if (s.parent[last] >= 0) fpar[t] = of[s.parent[last]];
+    if (fpar[t] >= 0) kids[fpar[t]]++;
+  }
+  // chains from the top down: top[t] is the top supernode of t's front, order[top] its order
+  std::vector<int32_t> top(ns), order(ns);
+  for (int32_t t = ns - 1; t >= 0; --t) {
+    top[t] = t;
+    order[t] = col_len(f0[t]);
+    const int32_t p = fpar[t];
+    if (p >= 0 && kids[p] == 1 && f0[t] + fw[t] == f0[p] && order[top[p]] + fw[t] <= max_rows) {
+      // zeros of t's first column in the joined front (its later columns add no more): the
+      // front's order less the column's own count
+      if (order[top[p]] + fw[t] - col_len(f0[t]) <= max_zeros) {
+        top[t] = top[p];
+        order[top[p]] += fw[t];
+      }
+    }
+  }
+  std::vector<int32_t> pad;
+  std::vector<char> in(n, 0);
+  for (int32_t t = 0; t < ns; ++t) {
+    if (top[t] == t) continue;
+    const int32_t T = top[t], end = f0[T] + fw[T];  // the front's columns end at the top's end
+    const int32_t last = end - 1;
+    for (int32_t j = f0[t]; j < f0[t] + fw[t]; ++j) {
+      // the trapezoid's rows below column j: columns j+1 .. last, then the top's rows below
+      for (int64_t p = s.column_start[j] + 1; p < s.column_start[j + 1]; ++p) in[s.row_rank[p]] = 1;
+      auto want = [&](int32_t r) {
+        if (!in[r]) {
+          pad.push_back(s.order[r]);
+          pad.push_back(s.order[j]);
+        }
+      };
+      for (int32_t r = j + 1; r <= last; ++r) want(r);
+      for (int64_t p = s.column_start[last] + 1; p < s.column_start[last + 1]; ++p) want(s.row_rank[p]);
+      for (int64_t p = s.column_start[j] + 1; p < s.column_start[j + 1]; ++p) in[s.row_rank[p]] = 0;
+    }
+  }
+  return pad;
+}
+
 }  // namespace cells

@@ -45,6 +45,15 @@ struct Supernodes {
 // of the front order (the device groups fronts of one class).
 Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows));
 
+// Relaxed supernodes: a fundamental supernode whose parent supernode starts at its next column
+// and has no other child joins its parent's front when the joined front stays within max_rows
+// and gives its first column at most max_zeros explicit zeros (chains joined from the top down).
+// Each joined column's rows become the dense trapezoid's; the etree is unchanged.
+// Returns those zeros as pattern entries (node pairs, flattened i0, j0, i1, j1, ...): binding the
+// pattern plus them with s's order gives the relaxed symbolic, whose fundamental supernodes are
+// the joined fronts.
+std::vector<int32_t> relaxed_padding(const Symbolic& s, int32_t max_rows, int32_t max_zeros);
+
 std::vector<int32_t> minimum_degree(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols);
 Symbolic bind(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, const int32_t* order);
 int64_t position(const Symbolic& s, int32_t i, int32_t j);

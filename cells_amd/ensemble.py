@@ -22,6 +22,7 @@ from cells_amd import lib as _lib
 from cells_amd.world import Packed, StateLayout
 
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
 
 
 def _torch():
@@ -126,6 +127,16 @@ class Ensemble:
         _lib.check(_lib.load().cells_world_positions(self._h, w, d.ctypes.data_as(_i64p),
                                                      e.ctypes.data_as(_i64p), ctypes.byref(off)))
         return d, e[:E], off.value
+
+    def factor_positions(self, w, rows, cols):
+        """Positions in world w's factor slice of the entries (rows, cols) (world-local nodes), -1
+        where the plan's pattern has none."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        out = np.empty(max(r.size, 1), np.int64)
+        _lib.check(_lib.load().cells_world_factor_positions(
+            self._h, w, r.ctypes.data_as(_i32p), c.ctypes.data_as(_i32p), r.size, out.ctypes.data_as(_i64p)))
+        return out[:r.size]
 
     def factor(self, values):
         v = self.to_device(values).clone()

@@ -36,6 +36,7 @@ EXPORTS = (
     "cells_stationary_residual", "cells_stationary_jvp", "cells_field_create", "cells_field_destroy",
     "cells_field_state_size", "cells_field_step", "cells_field_refresh", "cells_field_plan_stats",
     "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials",
+    "cells_world_factor_positions",
 )
 
 
@@ -144,6 +145,8 @@ def load():
     lib.cells_set_parameters.argtypes = [vp, vp, vp, vp, vp]
     lib.cells_plan_stats.argtypes = [vp, _i64p, _f64p, _i32p, _i32p]
     lib.cells_world_positions.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i64p, _i64p, _i64p]
+    lib.cells_world_factor_positions.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p, ctypes.c_int64,
+                                                 _i64p]
     lib.cells_minimum_degree.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p]
     lib.cells_bind.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p,
                                ctypes.POINTER(ctypes.c_void_p)]

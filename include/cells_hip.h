@@ -222,14 +222,17 @@ int cells_set_parameters(cells_ensemble* e, void* stream, const double* pam, con
 
 /* The batched numeric factor and solve on the stage pattern (numerics/sparse_cholesky.py:474-514):
  * values: device [factor size] (each world's assembled entries at its factor positions), factored
- * in place; rhs/x: device [ncell] in node order, one system per world. */
+ * in place; rhs/x: device [ncell] in node order, one system per world. The factor's layout is its
+ * plan's: the bound pattern's columns (the reference's) in the reference-order mode; in the
+ * multifrontal mode the relaxed pattern's, whose chains of supernodes carry explicit zeros
+ * (cells_world_factor_positions locates any entry). */
 int64_t cells_factor_size(const cells_ensemble* e);
 int cells_factor(cells_ensemble* e, void* stream, double* values);
 int cells_solve(cells_ensemble* e, void* stream, const double* values, const double* rhs,
                 double* x);
-/* Totals of the bound plans: factor entries over all worlds, sum over worlds of sum_j c_j^2 (the
- * reference's factorisation cost, c_j the entries of column j), the most levels, and the number of
- * distinct lattice shapes. */
+/* Totals of the bound patterns (without the relaxed supernodes' zeros): factor entries over all
+ * worlds, sum over worlds of sum_j c_j^2 (the reference's factorisation cost, c_j the entries of
+ * column j), the most column levels of the plans, and the number of distinct lattice shapes. */
 int cells_plan_stats(const cells_ensemble* e, int64_t* nnz_total, double* flops_total,
                      int32_t* max_levels, int32_t* shapes);
 
@@ -310,6 +313,10 @@ int cells_field_kernel_times(cells_field* f, void* stream, const double* y, doub
  * offset of its factor slice. */
 int cells_world_positions(const cells_ensemble* e, int32_t w, int64_t* diag_pos,
                           int64_t* edge_pos, int64_t* factor_offset);
+/* Positions in world w's factor slice of the entries (rows[x], cols[x]) (world-local nodes, either
+ * triangle), -1 where the plan's pattern has none. */
+int cells_world_factor_positions(const cells_ensemble* e, int32_t w, const int32_t* rows,
+                                 const int32_t* cols, int64_t count, int64_t* out);
 
 #ifdef __cplusplus
 }

@@ -187,7 +187,9 @@ def test_ros2_trajectory_matches_oracle(torch):
 def test_factor_and_solve_against_oracle(torch, mode, cells, seed):
     """Pairwise mode: the factor bit for bit, and the solve on it bit for bit the oracle's solve.
     Supernodal mode: the factor within 1e-13 of the largest entry (Jacobi-scaled SPD stage
-    matrices) and the supernodal solve on it within 1e-12 of the oracle's solve on it."""
+    matrices) and the supernodal solve on it within 1e-12 of the oracle's solve on it. The
+    pairwise plan's layout is the bound pattern's; the supernodal plan's is the relaxed pattern's,
+    whose extra entries (the joined chains' explicit zeros) stay zero."""
     from cells_amd.ensemble import Ensemble
 
     p = build(config1(worlds=3, cells=cells, seed=seed))
@@ -196,7 +198,7 @@ def test_factor_and_solve_against_oracle(torch, mode, cells, seed):
     rng = np.random.default_rng(9)
     values = np.zeros(e.factor_size)
     rhs = rng.uniform(-1, 1, p.ncell)
-    expect_f = []
+    expect_f, where, extra = [], [], []
     for k in range(p.W):
         d, ep, off = e.positions(k)
         c0, c1 = p.cell_off[k], p.cell_off[k + 1]
@@ -211,23 +213,33 @@ def test_factor_and_solve_against_oracle(torch, mode, cells, seed):
         off_diag = off_diag * sc[p.snd[e0:e1] - c0] * sc[p.rcv[e0:e1] - c0]
         diag = np.ones(n)
         nnz = int(syms[k].column_start[-1])
-        v = np.zeros(nnz)
+        size = (e.positions(k + 1)[2] if k + 1 < p.W else e.factor_size) - off
+        # the plan's position of every entry of the bound pattern
+        pos = e.factor_positions(k, syms[k].row, syms[k].column)
+        assert np.all(pos >= 0) and np.unique(pos).size == nnz
+        if mode == "pairwise":
+            assert size == nnz and np.array_equal(pos, np.arange(nnz))
+        v = np.zeros(size)
         v[d] = diag
         v[ep] = off_diag
-        values[off:off + nnz] = v
+        values[off:off + size] = v
         # the oracle's positions are the same pattern: check, then factor there
         vo = np.zeros(nnz)
         vo[np.asarray([sym[k].diag_pos[i] for i in range(n)])] = diag
         vo[np.asarray([sym[k].edge_pos[q] for q in range(e1 - e0)])] = off_diag
-        assert np.array_equal(v, vo)
-        expect_f.append(oracle.factor(sym[k], v))
-    say(f"factor ({mode}, cells {cells})")
+        assert np.array_equal(v[pos], vo) and np.count_nonzero(v) == np.count_nonzero(vo)
+        expect_f.append(oracle.factor(sym[k], vo))
+        where.append(off + pos)
+        rest = np.ones(size, bool)
+        rest[pos] = False
+        extra.append(off + np.nonzero(rest)[0])
+    say(f"factor ({mode}, cells {cells}, {sum(x.size for x in extra)} explicit zeros)")
     fg = e.factor(values).cpu().numpy()
     xg = e.solve(fg, rhs).cpu().numpy()
     for k in range(p.W):
-        _, _, off = e.positions(k)
         f = expect_f[k]
-        got = fg[off:off + f.size]
+        got = fg[where[k]]
+        assert np.all(fg[extra[k]] == 0.0)
         if mode == "pairwise":
             assert got.tobytes() == f.tobytes()
         else:
