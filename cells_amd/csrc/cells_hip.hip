@@ -140,8 +140,8 @@ constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: 
 constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 19
 constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 54
 constexpr int kMfMaxRows = 128;                            // fronts above kMfLds (order <= 128) use HBM
-constexpr int kRelaxRows = 90;                             // relaxed supernodes join up to this order,
-constexpr int kRelaxZeros = 16;                            // with at most this many zeros per column
+constexpr int kRelaxRows = kMfMaxRows;                     // relaxed supernodes join up to this order,
+constexpr int kRelaxZeros = 64;                            // with at most this many zeros per column
 
 // The largest front order whose packed triangle fits `tri` doubles.
 constexpr int max_order(int tri) {

@@ -210,3 +210,27 @@ def test_a_refused_world_keeps_its_state(torch):
     assert np.max(np.abs(L.cell(got, L.dn)[:n] - L.cell(want, L.dn)[:n])) <= 1e-9 * np.max(
         np.abs(L.cell(want, L.dn)[:n]))
     e.close()
+
+
+def test_an_unconverged_field_solve_refuses_the_step(torch):
+    """A world whose field solve misses the Newton target within its chord-step bound refuses the
+    step and keeps its state, as the reference refuses an unconverged field (models/
+    surface_kinetic.py readouts: valid = converged & ...). The bound is forced from the host: one
+    chord step, where these states need more."""
+    from cells_amd.field import FieldEnsemble
+
+    fp = build_field(small_field(worlds=2, seed=21))
+    y, ok_ref = oracle.field_run(fp, initial_field_state(fp), 1e-3, 6, 1)
+    assert ok_ref.all()
+    full = FieldEnsemble(fp, refresh_every=1)
+    _, chord, ok = full.potentials(y, 1e-3)
+    assert bool(ok.all()) and int(chord.min()) >= 2
+    full.close()
+    short = FieldEnsemble(fp, refresh_every=1, max_iterations=1)
+    y0 = short.to_device(y)
+    y1, ok = short.step(y0, 1e-3, nsteps=1)
+    assert not bool(ok.any())
+    assert torch.equal(y1, y0)
+    _, chord, ok = short.potentials(y0, 1e-3)
+    assert not bool(ok.any()) and chord.cpu().tolist() == [1, 1]
+    short.close()
