@@ -300,8 +300,9 @@ int cells_field_set_start(cells_field* f, int32_t field_solve, int32_t max_itera
 int cells_field_potentials(cells_field* f, void* stream, const double* y, double dt,
                            const cells_control* control, double* phi_out, int32_t* iterations_out,
                            int32_t* ok_out);
-/* The operator plan: factor entries per world, flops per factor (sum of column counts squared),
- * supernodes, largest front, supernodal tree levels. */
+/* The operator plan: factor entries per world and flops per factor (sum of column counts squared)
+ * of the bound pattern, and the supernodes, largest front and supernodal tree levels of the plan
+ * (relaxed supernodes: the chains joined with their explicit zeros). */
 int cells_field_plan_stats(const cells_field* f, int64_t* nnz, double* flops, int32_t* fronts,
                            int32_t* max_front, int32_t* levels);
 /* Per-kernel timing of `reps` steps from y (operator refreshed at the first): each kernel's total
