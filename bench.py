@@ -121,7 +121,7 @@ def copy_probe(mib):
     del src, dst
 
 
-def spatial_leg(steps, refresh):
+def spatial_leg(steps, refresh, starts=("solve", "carried")):
     """The spatial configuration: 64 worlds of 2048 cells (a 45 x 46 lattice) in a 256 x 256
     extracellular grid with a held ring, Na/K/Cl, IMEX Euler dt = 1 ms with the implicit face drift
     through the nested-dissection operator, rebuilt every `refresh` steps; each step starts from
@@ -137,7 +137,7 @@ def spatial_leg(steps, refresh):
 
     fp = build_field(FieldConfig())
     out = {}
-    for start in ("solve", "carried"):
+    for start in starts:
         e = FieldEnsemble(fp, refresh_every=refresh, start=start)
         plan = e.plan_stats()
         log(f"spatial leg ({start}): {fp.W} worlds x {fp.config.cells} cells in {fp.side}^2 grids, plan {plan}")
@@ -156,7 +156,9 @@ def spatial_leg(steps, refresh):
             _, chord, _ = e.potentials(y, 1e-3)
             chord = chord.cpu().numpy()
         e.close()
-    kt, ms, el, accepted, plan = out["solve"]
+    kt, ms, el, accepted, plan = out[starts[0]]
+    if starts[0] != "solve":
+        chord = np.zeros(1)
     solve_ms = sum(v for k, v in kt if k.startswith("k_fd_solve") and "field" not in k)
     field_ms = sum(v for k, v in kt if k.startswith("k_fd_k") or k.startswith("k_fd_solve field"))
     factor_ms = sum(v for k, v in kt if k.startswith("k_fd_factor") or k == "k_fd_assemble")
@@ -166,14 +168,16 @@ def spatial_leg(steps, refresh):
     log(f"spatial leg: operator solve {solve_ms:.3f} ms ({achieved:.0f} GB/s), field solve {field_ms:.3f} ms "
         f"({float(np.mean(chord)):.2f} chord steps), factor {factor_ms:.2f} ms per refresh")
     return {"worlds": fp.W, "cells_per_world": fp.config.cells, "grid": f"{fp.side}x{fp.side}",
-            "steps": steps, "refresh_every": refresh, "start_potentials": "field solve",
+            "steps": steps, "refresh_every": refresh,
+            "start_potentials": "field solve" if starts[0] == "solve" else "carried",
             "ms_per_step": ms, "cell_steps_per_s": fp.W * fp.config.cells * steps / el,
-            "all_worlds_accepted": accepted and out["carried"][3], "plan": plan,
+            "all_worlds_accepted": all(v[3] for v in out.values()), "plan": plan,
             "kernel_ms": {k: round(v, 4) for k, v in kt},
             "field_solve": {"ms_per_step": field_ms, "chord_steps_mean": float(np.mean(chord)),
                             "chord_steps_max": int(np.max(chord))},
-            "carried_start": {"ms_per_step": out["carried"][1],
-                              "cell_steps_per_s": fp.W * fp.config.cells * steps / out["carried"][2]},
+            "carried_start": ({"ms_per_step": out["carried"][1],
+                               "cell_steps_per_s": fp.W * fp.config.cells * steps / out["carried"][2]}
+                              if "carried" in out and starts[0] == "solve" else None),
             "solve": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": achieved / HBM_PEAK_GBS, "ms_per_step": solve_ms},
             "factor": {"bound": "mfma", "achieved": tflops, "peak": 78.6, "unit": "TFLOP/s",
@@ -254,6 +258,8 @@ def main():
     ap.add_argument("--skip", default="",
                     help="comma list of legs to skip: plain,parity,cpu,adjoint,stationary,spatial")
     ap.add_argument("--spatial-steps", type=int, default=100)
+    ap.add_argument("--spatial-start", choices=("both", "solve", "carried"), default="both",
+                    help="the spatial leg's step-start potentials: the field solve, the carried ones, or both")
     ap.add_argument("--spatial-only", action="store_true",
                     help="run only the spatial leg and print its JSON (for profiling it alone)")
     ap.add_argument("--adjoint-steps", type=int, default=2000, help="horizon of the adjoint leg")
@@ -272,7 +278,8 @@ def main():
     if args.spatial_only:
         if args.copy_probe:
             copy_probe(args.copy_probe)
-        print(json.dumps({"spatial": spatial_leg(args.spatial_steps, 50)}), flush=True)
+        starts = ("solve", "carried") if args.spatial_start == "both" else (args.spatial_start,)
+        print(json.dumps({"spatial": spatial_leg(args.spatial_steps, 50, starts)}), flush=True)
         return
 
     import torch
