@@ -1431,9 +1431,9 @@ __device__ void sv_solve_world(const ShapeDev& s, const int* pool, const double*
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
       sv_backward_front<1, kMfThreads>(pool, qa0 + q, fv, y, lds + tid, 0);
     for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      sv_backward_front<kMfOcta, 1>(pool, qa0 + q, fv, y, Fq, tid % kMfOcta);
+      sv_backward_front<kMfOcta, 1, 4>(pool, qa0 + q, fv, y, Fq, tid % kMfOcta);
     for (int q = bd[2] + wave; q < bd[5]; q += kMfThreads / 64)
-      sv_backward_front<64, 1>(pool, qa0 + q, fv, y, Fw, tid & 63);
+      sv_backward_front<64, 1, 8>(pool, qa0 + q, fv, y, Fw, tid & 63);
     __syncthreads();
   }
 }
