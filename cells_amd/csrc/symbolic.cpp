@@ -296,7 +296,7 @@ std::vector<int32_t> relaxed_padding(const Symbolic& s, int32_t max_rows, int32_
     top[t] = t;
     order[t] = col_len(f0[t]);
     const int32_t p = fpar[t];
-    if (p >= 0 && kids[p] == 1 && f0[t] + fw[t] == f0[p] && order[top[p]] + fw[t] <= max_rows) {
+    if (p >= 0 && kids[p] == 1 && f0[t] + fw[t] == f0[p] && s.parent[f0[p] - 1] == f0[p] && order[top[p]] + fw[t] <= max_rows) {
       // zeros of t's first column in the joined front (its later columns add no more): the
       // front's order less the column's own count
       if (order[top[p]] + fw[t] - col_len(f0[t]) <= max_zeros) {
