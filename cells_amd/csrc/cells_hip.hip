@@ -2351,6 +2351,18 @@ int cells_symbolic_export(const cells_symbolic* sym, int32_t* order, int32_t* ra
 
 void cells_symbolic_free(cells_symbolic* s) { delete s; }
 
+int cells_relaxed_padding(const cells_symbolic* s, int32_t max_rows, int32_t max_zeros, int32_t* pairs,
+                          int64_t capacity, int64_t* count) {
+  if (!s || !count || max_rows < 1 || max_zeros < 0) return fail(CELLS_EINVAL, "bad argument");
+  const std::vector<int32_t> pad = cells::relaxed_padding(s->s, max_rows, max_zeros);
+  *count = (int64_t)pad.size() / 2;
+  if (pairs) {
+    if ((int64_t)pad.size() > 2 * capacity) return fail(CELLS_EINVAL, "pairs buffer too small");
+    std::copy(pad.begin(), pad.end(), pairs);
+  }
+  return CELLS_OK;
+}
+
 int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
                           cells_ensemble** out) {
   int rc = check_world(w);

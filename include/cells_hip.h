@@ -86,6 +86,17 @@ int cells_symbolic_export(const cells_symbolic* s, int32_t* order, int32_t* rank
                           int64_t* column_start, int32_t* row_node, int32_t* column_node,
                           int32_t* parent, int32_t* level);
 void cells_symbolic_free(cells_symbolic* s);
+/* Relaxed supernodes of a bound pattern (no counterpart in the reference, whose factor is
+ * column by column): a fundamental supernode whose parent supernode starts at its parent column
+ * (its next column) and has no other child joins its parent's front when the front's order stays
+ * within max_rows and the supernode's columns gain at most max_zeros explicit zeros; chains join
+ * from the top down. The joined columns take the front's dense trapezoid of rows: pairs receives
+ * those entries (node pairs i, j, flattened) and count their number; binding the pattern plus
+ * them with s's order gives the same elimination tree with the joined fronts as its fundamental
+ * supernodes (the multifrontal plan of cells_ensemble_create uses max_rows 128, max_zeros 64).
+ * pairs may be NULL to ask for the count; capacity is in pairs. Host only. */
+int cells_relaxed_padding(const cells_symbolic* s, int32_t max_rows, int32_t max_zeros, int32_t* pairs,
+                          int64_t capacity, int64_t* count);
 
 /* ---- device ensemble: checked_step under the execution layer's loops ---- */
 

@@ -266,3 +266,85 @@ def test_a_matrix_that_is_not_positive_definite_gives_a_non_finite_factor():
     with np.errstate(invalid="ignore"):
         values = oracle.factor(structs[0], a[s.row, s.column])
     assert not np.all(np.isfinite(values))
+
+
+# ---- relaxed supernodes (the multifrontal plan's pattern) ---------------------------------------
+#
+# The plan joins chains of supernodes with explicit zeros (cells_relaxed_padding). Pinned by what
+# the elimination fixes: the padded pattern keeps the order's elimination tree; every padded entry
+# lies outside the filled pattern, so the Cholesky factor of any SPD matrix on the pattern is zero
+# there (checked against numpy's dense Cholesky); the oracle's factor on the padded pattern equals
+# its factor on the bound one at every true entry; the joined fronts are the padded pattern's
+# fundamental supernodes and fewer levels deep.
+
+
+def _supernode_levels(s):
+    n = s.size
+    col_len = np.diff(s.column_start)
+    nchild = np.bincount(s.parent[s.parent >= 0], minlength=n)
+    first, of = [], np.zeros(n, int)
+    k = 0
+    while k < n:
+        j = k
+        while j + 1 < n and s.parent[j] == j + 1 and nchild[j + 1] == 1 and col_len[j] == col_len[j + 1] + 1:
+            j += 1
+        of[k:j + 1] = len(first)
+        first.append((k, j + 1))
+        k = j + 1
+    lev = np.zeros(len(first), int)
+    for t, (a, b) in enumerate(first):
+        p = s.parent[b - 1]
+        if p >= 0:
+            lev[of[p]] = max(lev[of[p]], lev[t] + 1)
+    return len(first), int(lev.max()) + 1
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 300, 1216, 2176])
+def test_relaxed_supernodes_keep_the_tree_and_pad_only_structural_zeros(n):
+    snd, rcv = lattice(n)
+    exact = product.bind(n, snd, rcv)
+    order, pr, pc = product.relaxed_padding(n, snd, rcv)
+    assert np.array_equal(order, exact.order)
+    relaxed = product.bind(n, np.concatenate((snd, pr)), np.concatenate((rcv, pc)), order=order)
+    assert np.array_equal(relaxed.parent, exact.parent)
+    assert int(relaxed.column_start[-1]) == int(exact.column_start[-1]) + pr.size
+    true = set(zip(exact.row.tolist(), exact.column.tolist()))
+    assert not any((int(a), int(b)) in true or (int(b), int(a)) in true for a, b in zip(pr, pc))
+    assert true <= set(zip(relaxed.row.tolist(), relaxed.column.tolist()))
+    fronts0, levels0 = _supernode_levels(exact)
+    fronts1, levels1 = _supernode_levels(relaxed)
+    assert fronts1 <= fronts0 and levels1 <= levels0
+    if n >= 1216:
+        assert levels1 < levels0 / 2  # the chains were what made these trees deep
+    # an SPD matrix on the pattern: its dense factor vanishes at the padded entries
+    rng = np.random.default_rng(n)
+    A = np.zeros((n, n))
+    w = rng.uniform(0.1, 1.0, snd.size)
+    A[snd, rcv] = A[rcv, snd] = -w
+    A[np.arange(n), np.arange(n)] = -A.sum(axis=1) + rng.uniform(0.5, 1.5, n)
+    P = A[np.ix_(order, order)]
+    Ld = np.linalg.cholesky(P)
+    rank = np.empty(n, int)
+    rank[order] = np.arange(n)
+    hi, lo = np.maximum(rank[pr], rank[pc]), np.minimum(rank[pr], rank[pc])
+    assert np.all(np.abs(Ld[hi, lo]) <= 1e-14 * np.abs(Ld).max())
+    # the oracle's factor on either pattern agrees at the true entries and is zero at the padding
+    syms = [ref.bind(n, snd, rcv, order=order),
+            ref.bind(n, np.concatenate((snd, pr)), np.concatenate((rcv, pc)), order=order)]
+    factors = []
+    for s in syms:
+        st = oracle.Symbolic()
+        keep = [np.ascontiguousarray(x, dtype=np.int64) for x in
+                (s.column_start, s.rank[s.row], s.order, s.column_start[s.rank[np.arange(n)]], np.zeros(1))]
+        st.n, st.nnz = n, int(s.column_start[-1])
+        st.column_start, st.row_rank, st.order, st.diag_pos, st.edge_pos = [
+            x.ctypes.data_as(oracle._i64p) for x in keep]
+        v = P[s.rank[s.row], s.rank[s.column]]
+        factors.append((oracle.factor(st, v), s, keep))
+    (f0, s0, _), (f1, s1, _) = factors
+    pos1 = {(int(r), int(c)): q for q, (r, c) in enumerate(zip(s1.row, s1.column))}
+    at = np.array([pos1[(int(r), int(c))] for r, c in zip(s0.row, s0.column)])
+    assert np.max(np.abs(f1[at] - f0)) <= 1e-13 * np.max(np.abs(f0))
+    rest = np.ones(f1.size, bool)
+    rest[at] = False
+    assert np.all(f1[rest] == 0.0)
