@@ -121,7 +121,7 @@ def copy_probe(mib):
     del src, dst
 
 
-def spatial_leg(steps, refresh, starts=("solve", "carried")):
+def spatial_leg(steps, refresh, starts=("solve", "carried"), tune_plan=True):
     """The spatial configuration: 64 worlds of 2048 cells (a 45 x 46 lattice) in a 256 x 256
     extracellular grid with a held ring, Na/K/Cl, IMEX Euler dt = 1 ms with the implicit face drift
     through the nested-dissection operator, rebuilt every `refresh` steps; each step starts from
@@ -136,6 +136,12 @@ def spatial_leg(steps, refresh, starts=("solve", "carried")):
     from cells_amd.field import FieldConfig, FieldEnsemble, build_field, initial_field_state
 
     fp = build_field(FieldConfig())
+    tune = None
+    if tune_plan:
+        from cells_amd.autotune import autotune_field
+
+        tune = autotune_field(fp, steps=10, refresh_every=refresh)
+        log(f"spatial leg: autotuned grid plan options {tune.best} ({tune.ms_per_step:.2f} ms/step over 10 steps)")
     out = {}
     for start in starts:
         e = FieldEnsemble(fp, refresh_every=refresh, start=start)
@@ -168,7 +174,7 @@ def spatial_leg(steps, refresh, starts=("solve", "carried")):
     log(f"spatial leg: operator solve {solve_ms:.3f} ms ({achieved:.0f} GB/s), field solve {field_ms:.3f} ms "
         f"({float(np.mean(chord)):.2f} chord steps), factor {factor_ms:.2f} ms per refresh")
     return {"worlds": fp.W, "cells_per_world": fp.config.cells, "grid": f"{fp.side}x{fp.side}",
-            "steps": steps, "refresh_every": refresh,
+            "steps": steps, "refresh_every": refresh, "autotune": tune.summary() if tune else None,
             "start_potentials": "field solve" if starts[0] == "solve" else "carried",
             "ms_per_step": ms, "cell_steps_per_s": fp.W * fp.config.cells * steps / el,
             "all_worlds_accepted": all(v[3] for v in out.values()), "plan": plan,
@@ -256,7 +262,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--parity-worlds", type=int, default=3)
     ap.add_argument("--skip", default="",
-                    help="comma list of legs to skip: plain,parity,cpu,adjoint,stationary,spatial")
+                    help="comma list of legs to skip: tune,plain,parity,cpu,adjoint,stationary,spatial")
     ap.add_argument("--spatial-steps", type=int, default=100)
     ap.add_argument("--spatial-start", choices=("both", "solve", "carried"), default="both",
                     help="the spatial leg's step-start potentials: the field solve, the carried ones, or both")
@@ -279,7 +285,8 @@ def main():
         if args.copy_probe:
             copy_probe(args.copy_probe)
         starts = ("solve", "carried") if args.spatial_start == "both" else (args.spatial_start,)
-        print(json.dumps({"spatial": spatial_leg(args.spatial_steps, 50, starts)}), flush=True)
+        skip = set(filter(None, args.skip.split(",")))
+        print(json.dumps({"spatial": spatial_leg(args.spatial_steps, 50, starts, "tune" not in skip)}), flush=True)
         return
 
     import torch
@@ -292,6 +299,13 @@ def main():
     dt = 5e-3
     log("building worlds")
     p = build(config1(worlds=args.worlds))
+    tune = None
+    if "tune" not in skip:
+        from cells_amd.autotune import autotune_ensemble
+
+        tune = autotune_ensemble(p, "ros2", dt, steps=3)
+        log(f"autotuned plan options {tune.best} ({tune.ms_per_step:.2f} ms/step); candidates "
+            + ", ".join(f"{c['relax_zeros']}: {m:.2f}" for c, m in tune.table))
     log(f"{p.W} worlds, {p.ncell} cells, {p.nedge} edges; binding")
     t0 = time.perf_counter()
     e = Ensemble(p, "ros2")
@@ -406,7 +420,7 @@ def main():
 
     spatial = None
     if "spatial" not in skip:
-        spatial = spatial_leg(args.spatial_steps, 50)
+        spatial = spatial_leg(args.spatial_steps, 50, tune_plan="tune" not in skip)
 
     adj = None
     if "adjoint" not in skip:
@@ -449,6 +463,7 @@ def main():
         "adjoint": adj,
         "stationary": sta,
         "spatial": spatial,
+        "autotune": tune.summary() if tune is not None else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

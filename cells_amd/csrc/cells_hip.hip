@@ -143,6 +143,16 @@ constexpr int kMfMaxRows = 128;                            // fronts above kMfLd
 constexpr int kRelaxRows = kMfMaxRows;                     // relaxed supernodes join up to this order,
 constexpr int kRelaxZeros = 64;                            // with at most this many zeros per column
 
+// Plan options read when an ensemble or a field is created (cells_set_plan_options; defaults from
+// sweeps on the batched and spatial configurations). Process-wide; set them between creates.
+struct PlanOptions {
+  int relax_rows = kRelaxRows;   // lattice plans: largest joined front
+  int relax_zeros = kRelaxZeros; // lattice plans: explicit zeros per joined column
+  int grid_relax_zeros = 3;      // grid plans: explicit zeros per joined column
+  int grid_subtrees = 32;        // grid plans: fronts at the depth where the tree is cut (at least)
+};
+PlanOptions g_plan;
+
 // The largest front order whose packed triangle fits `tri` doubles.
 constexpr int max_order(int tri) {
   int m = 0;
@@ -2301,6 +2311,26 @@ extern "C" {
 
 const char* cells_last_error(void) { return g_error.c_str(); }
 
+int cells_set_plan_options(int32_t relax_rows, int32_t relax_zeros, int32_t grid_relax_zeros,
+                           int32_t grid_subtrees) {
+  if (relax_rows < 1 || relax_rows > kMfMaxRows || relax_zeros < 0 || grid_relax_zeros < 0 || grid_subtrees < 1)
+    return fail(CELLS_EINVAL, "plan options out of range");
+  g_plan.relax_rows = relax_rows;
+  g_plan.relax_zeros = relax_zeros;
+  g_plan.grid_relax_zeros = grid_relax_zeros;
+  g_plan.grid_subtrees = grid_subtrees;
+  return CELLS_OK;
+}
+
+int cells_get_plan_options(int32_t* relax_rows, int32_t* relax_zeros, int32_t* grid_relax_zeros,
+                           int32_t* grid_subtrees) {
+  if (relax_rows) *relax_rows = g_plan.relax_rows;
+  if (relax_zeros) *relax_zeros = g_plan.relax_zeros;
+  if (grid_relax_zeros) *grid_relax_zeros = g_plan.grid_relax_zeros;
+  if (grid_subtrees) *grid_subtrees = g_plan.grid_subtrees;
+  return CELLS_OK;
+}
+
 int cells_minimum_degree(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols,
                          int32_t* order_out) {
   try {
@@ -2527,6 +2557,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
     std::string bad_msg;
     unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> pool_threads;
+    const PlanOptions plan = g_plan;  // read once: the worker threads share it
     for (unsigned t = 0; t < hw; ++t)
       pool_threads.emplace_back([&]() {
         std::vector<int32_t> rows, cols;
@@ -2545,7 +2576,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
             if (!e->pairwise) {
               // relaxed supernodes: the multifrontal plan factors the pattern with the chains'
               // explicit zeros (the reference-order mode keeps the bound pattern itself)
-              const std::vector<int32_t> pad = cells::relaxed_padding(s, kRelaxRows, kRelaxZeros);
+              const std::vector<int32_t> pad = cells::relaxed_padding(s, plan.relax_rows, plan.relax_zeros);
               if (!pad.empty()) {
                 for (size_t x = 0; x < pad.size(); x += 2) {
                   rows.push_back(pad[x]);

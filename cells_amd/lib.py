@@ -36,7 +36,8 @@ EXPORTS = (
     "cells_stationary_residual", "cells_stationary_jvp", "cells_field_create", "cells_field_destroy",
     "cells_field_state_size", "cells_field_step", "cells_field_refresh", "cells_field_plan_stats",
     "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials",
-    "cells_world_factor_positions", "cells_relaxed_padding",
+    "cells_world_factor_positions", "cells_relaxed_padding", "cells_set_plan_options",
+    "cells_get_plan_options",
 )
 
 
@@ -154,6 +155,8 @@ def load():
     lib.cells_symbolic_export.argtypes = [ctypes.c_void_p, _i32p, _i32p, _i64p, _i32p, _i32p,
                                           _i32p, _i32p]
     lib.cells_symbolic_free.argtypes = [ctypes.c_void_p]
+    lib.cells_set_plan_options.argtypes = [ctypes.c_int32] * 4
+    lib.cells_get_plan_options.argtypes = [_i32p] * 4
     lib.cells_relaxed_padding.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p, ctypes.c_int64,
                                           _i64p]
     lib.cells_field_create.argtypes = [ctypes.POINTER(WorldStruct), ctypes.POINTER(FieldGridStruct),

@@ -67,6 +67,17 @@ typedef struct cells_control {
 } cells_control;
 
 typedef struct cells_symbolic cells_symbolic;
+
+/* Plan options read by cells_ensemble_create and cells_field_create (no counterpart in the
+ * reference; the schedule knobs the autotuner of cells_amd/autotune.py sweeps): the lattice
+ * plans' relaxed supernodes (largest joined front, 1..128, default 128; explicit zeros per
+ * joined column, default 64) and the grid plans' (explicit zeros per joined column, default 3;
+ * the depth of the tree's cut: the first holding at least grid_subtrees fronts, default 32).
+ * Process-wide; set them between creates, not during one. */
+int cells_set_plan_options(int32_t relax_rows, int32_t relax_zeros, int32_t grid_relax_zeros,
+                           int32_t grid_subtrees);
+int cells_get_plan_options(int32_t* relax_rows, int32_t* relax_zeros, int32_t* grid_relax_zeros,
+                           int32_t* grid_subtrees);
 typedef struct cells_ensemble cells_ensemble;
 
 const char* cells_last_error(void);
@@ -93,7 +104,8 @@ void cells_symbolic_free(cells_symbolic* s);
  * from the top down. The joined columns take the front's dense trapezoid of rows: pairs receives
  * those entries (node pairs i, j, flattened) and count their number; binding the pattern plus
  * them with s's order gives the same elimination tree with the joined fronts as its fundamental
- * supernodes (the multifrontal plan of cells_ensemble_create uses max_rows 128, max_zeros 64).
+ * supernodes (the multifrontal plan of cells_ensemble_create uses cells_set_plan_options'
+ * relax_rows and relax_zeros).
  * pairs may be NULL to ask for the count; capacity is in pairs. Host only. */
 int cells_relaxed_padding(const cells_symbolic* s, int32_t max_rows, int32_t max_zeros, int32_t* pairs,
                           int64_t capacity, int64_t* count);
