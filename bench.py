@@ -352,14 +352,29 @@ def main():
     dominant = max(per_kernel, key=per_kernel.get)
     launches = [ms for name, ms in times if name == dominant]
     if dominant == "k_factor":
+        # the factor's algorithmic bytes: L written once (8 nnz of the bound pattern) and the stage
+        # matrix it assembles read once (diagonal, junction conductances, Jacobi scale: 8 (2 n + E));
+        # its flops, sum_j c_j^2, per byte put it below the FP64 matrix ridge (78.6 TFLOP/s over
+        # 8 TB/s), so the roofline that bounds it is HBM's
         fact_ms = float(np.mean(launches))
-        achieved = stats["flops"] / (fact_ms * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                    "traffic": measured_traffic("k_factor_mf"),
-                    "traffic_source": "profiles/ros2_step_4096_worlds.md (rocprofv3 PMC, copy-calibrated)",
-                    "kernel": "k_factor_mf", "launch_ms": fact_ms,
-                    "algorithmic": f"sum over worlds of sum_j c_j^2 = {stats['flops']:.4g} flops"}
+        algo = 8 * stats["nnz"] + 8 * (2 * p.ncell + p.nedge)
+        intensity = stats["flops"] / algo
+        ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+        if intensity < ridge:
+            achieved = algo / (fact_ms * 1e-3) / 1e9
+            roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS}
+        else:
+            achieved = stats["flops"] / (fact_ms * 1e-3) / 1e12
+            roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS}
+        roofline.update({"traffic": measured_traffic("k_factor_mf"),
+                         "traffic_source": "profiles/ros2_step_4096_worlds.md (rocprofv3 PMC, copy-calibrated)",
+                         "kernel": "k_factor_mf", "launch_ms": fact_ms,
+                         "algorithmic": f"8 nnz(L) + 8 (2 ncell + nedge) = {algo:.4g} bytes and sum over worlds "
+                                        f"of sum_j c_j^2 = {stats['flops']:.4g} flops per launch",
+                         "intensity_flop_per_byte": intensity, "ridge_flop_per_byte": ridge,
+                         "fp64_tflops": stats["flops"] / (fact_ms * 1e-3) / 1e12})
     elif dominant == "k_solve_stage":
         # one stage solve: the factor read by both sweeps (2 x 8 x nnz(L)) plus the right-hand side's
         # amounts (S per cell), the Jacobi scale, the result and the rank-ordered vector (read and
