@@ -1147,8 +1147,24 @@ __device__ void mf_trailing_update(double* F, int m, int jb, int je, int lane) {
 
 // One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE]; fronts of at
 // most MR rows.
-template <int G, int STRIDE, int MR = FrontLanes<G>::rows>
-__device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double* F, int lane, int* valid_w) {
+// The stage matrix's entries scattered straight into the fronts (the lattice factor): each front
+// starts at zero, takes its columns' diagonal mdiag and its columns' junction entries
+// -coef G_e Jacobi-scaled and symmetrised (the values the assembly would have stored), so the
+// factor array is neither cleared nor read before it is written.
+struct MfAssembly {
+  const double* mdiag;  // per cell, the stage matrix's scaled diagonal
+  const double* msc;    // per cell, the Jacobi scale
+  const double* Gj;     // per junction, the conductance slope
+  const int* snd;
+  const int* rcv;
+  const int* order;     // the world's rank -> node
+  double coef;
+  int c0, e0;           // the world's first cell and junction
+};
+
+template <int G, int STRIDE, int MR = FrontLanes<G>::rows, bool ASM = false>
+__device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double* F, int lane, int* valid_w,
+                         const MfAssembly* am = nullptr) {
   using FL = FrontLanes<G>;
   constexpr int RW = FL::RW, CG = FL::CG, NT = (MR + RW - 1) / RW;
   const int lr = lane % RW, lc = lane / RW;
@@ -1157,19 +1173,34 @@ __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double
   const int J = ra4.y, m = ra4.z, uoff = ra4.w, c0 = rb4.x, c1 = rb4.y;
   const long p0 = ra4.x;
   const int S = m - J, tri = m * (m + 1) / 2, Pn = colbase(J, m) + J;
-  // the supernode's columns are contiguous in the factor and are the front's first J columns;
-  // the update block starts at zero
-  for (int e0 = lane; e0 < tri; e0 += kMfBatch * G) {
-    double v[kMfBatch];
-#pragma unroll
-    for (int u = 0; u < kMfBatch; ++u) {
-      const int e = e0 + u * G;
-      v[u] = e < Pn ? fv[p0 + e] : 0.0;
+  if constexpr (ASM) {
+    const int2 ec = reinterpret_cast<const int2*>(pool)[(kRecInts * qa + 10) / 2];  // junction list
+    for (int e = lane; e < tri; e += G) F[e * STRIDE] = 0.0;
+    group_sync<G>();
+    for (int jj = lane; jj < J; jj += G) F[(colbase(jj, m) + jj) * STRIDE] = am->mdiag[am->c0 + am->order[rb4.w + jj]];
+    const int2* el = reinterpret_cast<const int2*>(pool + ec.x);
+    for (int x = lane; x < ec.y; x += G) {
+      const int2 le = el[x];  // (front-local packed position, world-local junction)
+      const int e = am->e0 + le.y;
+      const double val = -(am->coef * am->Gj[e]);
+      const double sa = am->msc[am->snd[e]], sb = am->msc[am->rcv[e]];
+      F[le.x * STRIDE] = (val * sa * sb + val * sb * sa) / 2;
     }
+  } else {
+    // the supernode's columns are contiguous in the factor and are the front's first J columns;
+    // the update block starts at zero
+    for (int e0 = lane; e0 < tri; e0 += kMfBatch * G) {
+      double v[kMfBatch];
 #pragma unroll
-    for (int u = 0; u < kMfBatch; ++u) {
-      const int e = e0 + u * G;
-      if (e < tri) F[e * STRIDE] = v[u];
+      for (int u = 0; u < kMfBatch; ++u) {
+        const int e = e0 + u * G;
+        v[u] = e < Pn ? fv[p0 + e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kMfBatch; ++u) {
+        const int e = e0 + u * G;
+        if (e < tri) F[e * STRIDE] = v[u];
+      }
     }
   }
   group_sync<G>();
@@ -1259,9 +1290,9 @@ __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double
 }
 
 // The factor of one world; fronts in HBM scratch hold at most BIGROWS rows.
-template <int BIGROWS = kMfMaxRows>
+template <int BIGROWS = kMfMaxRows, bool ASM = false>
 __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub, double* lds,
-                                double* big, int* valid_w) {
+                                double* big, int* valid_w, const MfAssembly* am = nullptr) {
   const int* lev = pool + s.o_snlev;
   const int qa0 = (int)(s.o_sn / kRecInts);
   const int tid = threadIdx.x, wave = tid >> 6, octa = tid / kMfOcta;
@@ -1269,16 +1300,20 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, 
     // thread, 8-lane, wavefront, workgroup (LDS) and workgroup (HBM scratch) fronts of level L
     const int* bd = lev + kLevelBounds * L;
     for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
-      mf_front<1, kMfThreads>(pool, qa0 + q, fv, ub, lds + tid, 0, valid_w);
+      mf_front<1, kMfThreads, FrontLanes<1>::rows, ASM>(pool, qa0 + q, fv, ub, lds + tid, 0, valid_w, am);
     __syncthreads();
     for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      mf_front<kMfOcta, 1>(pool, qa0 + q, fv, ub, lds + octa * kMfOctaTri, tid % kMfOcta, valid_w);
+      mf_front<kMfOcta, 1, FrontLanes<kMfOcta>::rows, ASM>(pool, qa0 + q, fv, ub, lds + octa * kMfOctaTri,
+                                                           tid % kMfOcta, valid_w, am);
     __syncthreads();
     for (int q = bd[2] + wave; q < bd[3]; q += kMfThreads / 64)
-      mf_front<64, 1>(pool, qa0 + q, fv, ub, lds + wave * kMfWaveTri, tid & 63, valid_w);
+      mf_front<64, 1, FrontLanes<64>::rows, ASM>(pool, qa0 + q, fv, ub, lds + wave * kMfWaveTri, tid & 63, valid_w,
+                                                 am);
     __syncthreads();
-    for (int q = bd[3]; q < bd[4]; ++q) mf_front<kMfThreads, 1>(pool, qa0 + q, fv, ub, lds, tid, valid_w);
-    for (int q = bd[4]; q < bd[5]; ++q) mf_front<kMfThreads, 1, BIGROWS>(pool, qa0 + q, fv, ub, big, tid, valid_w);
+    for (int q = bd[3]; q < bd[4]; ++q)
+      mf_front<kMfThreads, 1, FrontLanes<kMfThreads>::rows, ASM>(pool, qa0 + q, fv, ub, lds, tid, valid_w, am);
+    for (int q = bd[4]; q < bd[5]; ++q)
+      mf_front<kMfThreads, 1, BIGROWS, ASM>(pool, qa0 + q, fv, ub, big, tid, valid_w, am);
   }
   __syncthreads();
 }
@@ -1448,7 +1483,7 @@ __device__ void sv_solve_world(const ShapeDev& s, const int* pool, const double*
   }
 }
 
-__global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeDev* shapes,
+__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_factor_mf(Params P, const ShapeDev* shapes,
                                                           const int* shape_of, const int* pool,
                                                           const long* foff, const long* uoffw, int w0,
                                                           double coef, const double* mdiag,
@@ -1460,17 +1495,8 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_mf(Params P, const ShapeD
   const ShapeDev s = shapes[shape_of[w]];
   const PlanView v = plan_view(s, pool);
   double* fv = fvals + foff[w];
-  int c0 = P.cell_off[w], e0 = P.edge_off[w];
-  for (int p = threadIdx.x; p < s.nnz; p += blockDim.x) fv[p] = 0.0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < s.n; i += blockDim.x) fv[v.diagpos[i]] = mdiag[c0 + i];
-  for (int e = threadIdx.x; e < s.E; e += blockDim.x) {
-    double val = -(coef * Gj[e0 + e]);
-    double sa = msc[P.snd[e0 + e]], sb = msc[P.rcv[e0 + e]];
-    fv[v.edgepos[e]] = (val * sa * sb + val * sb * sa) / 2;
-  }
-  __syncthreads();
-  mf_factor_world(s, pool, fv, ubuf + uoffw[w], lds, bigbuf + boffw[w], valid + w);
+  const MfAssembly am{mdiag, msc, Gj, P.snd, P.rcv, v.order, coef, P.cell_off[w], P.edge_off[w]};
+  mf_factor_world<kMfMaxRows, true>(s, pool, fv, ubuf + uoffw[w], lds, bigbuf + boffw[w], valid + w, &am);
 }
 
 // Test entry: the multifrontal factor of assembled values in place, per world.
@@ -2684,7 +2710,22 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       long o_child = o_lev + (long)kLevelBounds * sn.levels;
       o_child += (4 - o_child % 4) % 4;
       const long o_rel = o_child + 4L * (long)sn.child.size();
-      if (o_rel + (long)sn.rel.size() > 0x7fffffffL) return fail(CELLS_EINVAL, "plans exceed 2^31 entries");
+      // each front's junction entries: (front-local packed position, world-local junction)
+      std::vector<int> front_of(s.n);
+      for (int t = 0; t < ns; ++t)
+        for (int c = sn.first[t]; c < sn.first[t] + sn.width[t]; ++c) front_of[c] = t;
+      std::vector<std::vector<int>> fedges(ns);
+      for (size_t q2 = 0; q2 < rows.size(); ++q2) {
+        const int64_t pe = cells::position(s, rows[q2], cols[q2]);
+        const int t = front_of[s.col_rank[pe]];
+        fedges[t].push_back((int)(pe - s.column_start[sn.first[t]]));
+        fedges[t].push_back((int)q2);
+      }
+      long o_fedge = o_rel + (long)sn.rel.size();
+      o_fedge += o_fedge % 2;  // int2 aligned
+      std::vector<long> fstart(ns + 1, 0);
+      for (int t = 0; t < ns; ++t) fstart[t + 1] = fstart[t] + (long)fedges[t].size();
+      if (o_fedge + fstart[ns] > 0x7fffffffL) return fail(CELLS_EINVAL, "plans exceed 2^31 entries");
       for (int t = 0; t < ns; ++t) {
         if (front_class(sn.rows[t]) > 4) return fail(CELLS_EINVAL, "a front of order above 128");
         if (front_class(sn.rows[t]) == 4)
@@ -2693,7 +2734,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
         int32_t r16[kRecInts] = {(int32_t)s.column_start[sn.first[t]], sn.width[t], sn.rows[t],
                                  (int32_t)sn.update_offset[t], (int32_t)(o_child / 4 + sn.child_ptr[t]),
                                  (int32_t)(o_child / 4 + sn.child_ptr[t + 1]), (int32_t)voff[t], sn.first[t],
-                                 (int32_t)(d.o_entry + s.column_start[last] + 1)};
+                                 (int32_t)(d.o_entry + s.column_start[last] + 1), 0,
+                                 (int32_t)(o_fedge + fstart[t]), (int32_t)(fedges[t].size() / 2)};
         pool.insert(pool.end(), r16, r16 + kRecInts);
       }
       d.sn_levels = sn.levels;
@@ -2721,6 +2763,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       d.o_rel = (long)pool.size();
       if (d.o_rel != o_rel) return fail(CELLS_EINVAL, "plan layout mismatch");
       pool.insert(pool.end(), sn.rel.begin(), sn.rel.end());
+      while ((long)pool.size() < o_fedge) pool.push_back(0);
+      for (int t = 0; t < ns; ++t) pool.insert(pool.end(), fedges[t].begin(), fedges[t].end());
       d.usz = sn.update_total;
       sd[q] = d;
     }
