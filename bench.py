@@ -277,7 +277,7 @@ def main():
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: independent worlds shard across "
+        print(json.dumps({"error": "the benchmark runs on one GPU: independent worlds shard across "
                           "GPUs with no exchange, and no multi-GPU figure is measured"}))
         sys.exit(2)
     skip = set(filter(None, args.skip.split(",")))

@@ -21,7 +21,7 @@
  *   models/pump_composite.py:1769-1815        Strang split: water half, ionic step, water half
  *   models/surface_tissue.py:269-420          charge_increment, capacitor_change, domain_valid
  * The ROS2 stage driver (Verwer, Spee, Blom, Hundsdorfer 1999, SIAM J. Sci. Comput. 20:1456,
- * gamma = 1 + 1/sqrt 2) is this build's own method: the reference has none. Its stage matrix is
+ * gamma = 1 + 1/sqrt 2) is not the reference's: the reference has no Rosenbrock method. Its stage matrix is
  * C + gamma dt G with G the routes' slope-conductance Laplacian, as N2's operator
  * (surface_pump_v3.py:2758-2837).
  */

@@ -3,7 +3,7 @@
 The reference's own whole-step checks restated here (tests/unit/test_surface_pump.py of the
 reference): material ledgers closed, observed order (Euler 1, second-order methods 2) against a
 fine reference, restart bitwise (:339), refusals keep the state (:339, :359), and water ledgers
-(tests/unit/test_osmotic_water.py). ROS2 is this build's own method; its order 2 holds for any
+(tests/unit/test_osmotic_water.py). ROS2 is not among the reference's methods; its order 2 holds for any
 stage Jacobian (a W-method), which these tests confirm on the coupled tissue.
 """
 
