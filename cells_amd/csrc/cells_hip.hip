@@ -140,6 +140,8 @@ constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: 
 constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 19
 constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 54
 constexpr int kMfMaxRows = 128;                            // fronts above kMfLds (order <= 128) use HBM
+constexpr int kMfBlockLds = 5120;  // a lattice workgroup front's LDS (40 KiB, order <= 100): the factor's
+                                   // 4 workgroups per CU (its 4 waves per SIMD) fill the CU's 160 KiB
 constexpr int kRelaxRows = kMfMaxRows;                     // relaxed supernodes join up to this order,
 constexpr int kRelaxZeros = 64;                            // with at most this many zeros per column
 
@@ -2248,7 +2250,7 @@ int32_t front_class(int32_t rows) {
   if (tri <= kMfThreadTri) return 0;
   if (tri <= kMfOctaTri) return 1;
   if (tri <= kMfWaveTri) return 2;
-  if (tri <= kMfLds) return 3;
+  if (tri <= kMfBlockLds) return 3;
   if (rows <= kMfMaxRows) return 4;
   return 5;
 }
@@ -2615,6 +2617,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
             }
             e->shapes_host[q] = std::move(s);
             e->sn_host[q] = cells::supernodes(e->shapes_host[q], front_class);
+            cells::share_update_storage(e->sn_host[q]);
           } catch (const std::exception& ex) {
             bad = 1;
             bad_msg = ex.what();
@@ -2889,7 +2892,7 @@ void launch_factor(cells_ensemble* e, hipStream_t st, int w0, int nw, double coe
     hipLaunchKernelGGL(k_factor, dim3(nw), dim3(kFactorBlock), e->factor_lds, st, e->P, e->shapes, e->shape_of,
                        e->pool, e->masks, e->foff, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals, e->valid);
   else
-    hipLaunchKernelGGL(k_factor_mf, dim3(nw), dim3(kMfThreads), sizeof(double) * kMfLds, st, e->P, e->shapes,
+    hipLaunchKernelGGL(k_factor_mf, dim3(nw), dim3(kMfThreads), sizeof(double) * kMfBlockLds, st, e->P, e->shapes,
                        e->shape_of, e->pool, e->foff, e->uoffw, w0, coef, e->mdiag, e->msc, e->Gj, e->fvals,
                        e->ubuf, e->bigbuf, e->boffw, e->valid);
 }
@@ -3106,7 +3109,7 @@ extern "C" int cells_factor(cells_ensemble* e, void* stream, double* values) {
     hipLaunchKernelGGL(k_factor_only, dim3(e->W), dim3(kFactorBlock), e->factor_lds, static_cast<hipStream_t>(stream),
                        e->shapes, e->shape_of, e->pool, e->masks, e->foff, values);
   else
-    hipLaunchKernelGGL(k_factor_only_mf, dim3(e->W), dim3(kMfThreads), sizeof(double) * kMfLds,
+    hipLaunchKernelGGL(k_factor_only_mf, dim3(e->W), dim3(kMfThreads), sizeof(double) * kMfBlockLds,
                        static_cast<hipStream_t>(stream), e->shapes, e->shape_of, e->pool, e->foff, e->uoffw,
                        values, e->ubuf, e->bigbuf, e->boffw);
   HIP_TRY(hipGetLastError());

@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <queue>
 #include <stdexcept>
 #include <tuple>
@@ -266,6 +268,70 @@ Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows)) {
     out.rel_ptr[q + 1] = static_cast<int32_t>(out.rel.size());
   }
   return out;
+}
+
+void share_update_storage(Supernodes& sn) {
+  const int32_t ns = static_cast<int32_t>(sn.first.size());
+  auto size = [&](int32_t t) {
+    const int64_t S = sn.rows[t] - sn.width[t];
+    return sn.parent[t] >= 0 ? S * (S + 1) / 2 : 0;
+  };
+  std::map<int64_t, int64_t> free_blocks;  // start -> end of each free block below the top
+  std::vector<std::vector<int32_t>> release(sn.levels + 1);
+  int64_t top = 0;
+  int32_t L = -1;
+  sn.update_total = 0;
+  for (int32_t t = 0; t < ns; ++t) {
+    for (; L < sn.level[t];) {  // a level's start frees the updates read at the level before it
+      ++L;
+      for (int32_t c : release[L]) {
+        int64_t a = sn.update_offset[c], b = a + size(c);
+        auto next = free_blocks.lower_bound(a);
+        if (next != free_blocks.begin()) {
+          auto prev = std::prev(next);
+          if (prev->second == a) {
+            a = prev->first;
+            free_blocks.erase(prev);
+          }
+        }
+        if (next != free_blocks.end() && next->first == b) {
+          b = next->second;
+          free_blocks.erase(next);
+        }
+        if (b == top) top = a;
+        else free_blocks[a] = b;
+      }
+    }
+    const int64_t need = size(t);
+    if (need == 0) {
+      sn.update_offset[t] = 0;
+      continue;
+    }
+    auto it = free_blocks.begin();
+    while (it != free_blocks.end() && it->second - it->first < need) ++it;
+    if (it != free_blocks.end()) {
+      sn.update_offset[t] = it->first;
+      const int64_t rest = it->first + need, end = it->second;
+      free_blocks.erase(it);
+      if (rest < end) free_blocks[rest] = end;
+    } else {
+      sn.update_offset[t] = top;
+      top += need;
+    }
+    sn.update_total = std::max(sn.update_total, top);
+    release[sn.level[sn.parent[t]] + 1].push_back(t);
+  }
+  // check: at every level the live updates are disjoint
+  std::vector<std::vector<std::pair<int64_t, int64_t>>> live(sn.levels);
+  for (int32_t t = 0; t < ns; ++t)
+    if (size(t) > 0)
+      for (int32_t v = sn.level[t]; v <= sn.level[sn.parent[t]]; ++v)
+        live[v].emplace_back(sn.update_offset[t], sn.update_offset[t] + size(t));
+  for (auto& blocks : live) {
+    std::sort(blocks.begin(), blocks.end());
+    for (size_t x = 1; x < blocks.size(); ++x)
+      if (blocks[x].first < blocks[x - 1].second) throw std::runtime_error("live update matrices overlap");
+  }
 }
 
 std::vector<int32_t> relaxed_padding(const Symbolic& s, int32_t max_rows, int32_t max_zeros) {

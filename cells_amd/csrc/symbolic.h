@@ -45,6 +45,12 @@ struct Supernodes {
 // of the front order (the device groups fronts of one class).
 Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows));
 
+// Update matrices sharing storage: a supernode's update is written when its level runs and read
+// when its parent's level runs, so it lives over the levels [level, parent's level]; updates whose
+// lives do not meet take the same doubles (first fit, in schedule order, freed blocks coalesced).
+// Rewrites update_offset and sets update_total to the peak. Throws if two live updates overlap.
+void share_update_storage(Supernodes& sn);
+
 // Relaxed supernodes: a fundamental supernode whose parent supernode starts at its next column
 // and has no other child joins its parent's front when the joined front stays within max_rows
 // and gives its first column at most max_zeros explicit zeros (chains joined from the top down).
