@@ -134,14 +134,15 @@ struct ShapeDev {
 
 // Multifrontal factor and supernodal solve: one workgroup of kMfThreads per world.
 constexpr int kMfThreads = 256;
-constexpr int kMfLds = 4096;                               // doubles of LDS per workgroup (32 KiB: 5 per CU)
+constexpr int kMfLds = 4096;                               // doubles of LDS per workgroup front (32 KiB, order <= 90)
 constexpr int kMfOcta = 8;                                 // lanes of a small group
-constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 6
-constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 19
-constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 54
+constexpr int kMfThreadTri = kMfLds / kMfThreads;          // a thread's front: order <= 5
+constexpr int kMfOctaTri = kMfLds / (kMfThreads / kMfOcta);  // an 8-lane group's front: order <= 15
+constexpr int kMfWaveTri = kMfLds / (kMfThreads / 64);     // a wavefront's front: order <= 44
 constexpr int kMfMaxRows = 128;                            // fronts above kMfLds (order <= 128) use HBM
 constexpr int kMfBlockLds = 5120;  // a lattice workgroup front's LDS (40 KiB, order <= 100): the factor's
                                    // 4 workgroups per CU (its 4 waves per SIMD) fill the CU's 160 KiB
+constexpr int kMfBlockWaveTri = kMfBlockLds / (kMfThreads / 64);  // a lattice wavefront's front: order <= 50
 constexpr int kRelaxRows = kMfMaxRows;                     // relaxed supernodes join up to this order,
 constexpr int kRelaxZeros = 64;                            // with at most this many zeros per column
 
@@ -1309,7 +1310,7 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, 
                                                            tid % kMfOcta, valid_w, am);
     __syncthreads();
     for (int q = bd[2] + wave; q < bd[3]; q += kMfThreads / 64)
-      mf_front<64, 1, FrontLanes<64>::rows, ASM>(pool, qa0 + q, fv, ub, lds + wave * kMfWaveTri, tid & 63, valid_w,
+      mf_front<64, 1, max_order(kMfBlockWaveTri), ASM>(pool, qa0 + q, fv, ub, lds + wave * kMfBlockWaveTri, tid & 63, valid_w,
                                                  am);
     __syncthreads();
     for (int q = bd[3]; q < bd[4]; ++q)
@@ -2249,7 +2250,7 @@ int32_t front_class(int32_t rows) {
   int64_t tri = (int64_t)rows * (rows + 1) / 2;
   if (tri <= kMfThreadTri) return 0;
   if (tri <= kMfOctaTri) return 1;
-  if (tri <= kMfWaveTri) return 2;
+  if (tri <= kMfBlockWaveTri) return 2;
   if (tri <= kMfBlockLds) return 3;
   if (rows <= kMfMaxRows) return 4;
   return 5;
