@@ -1523,20 +1523,29 @@ __device__ __forceinline__ void cell_correction(const Params& P, int i, double c
                                                 const double* gj, const double* dphi, double* dn_c,
                                                 double* dm_c) {
   int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
-  for (int s = 0; s < P.S; ++s) {
-    double dm = coef * (gm[(long)s * P.ncell + i] * dphi[i]);
-    double out = 0.0, in = 0.0;
-    for (int p = p0; p < pm; ++p) {
-      int e = P.inc_edge[p];
-      out += -(coef * (gj[(long)s * P.nedge + e] * (dphi[P.snd[e]] - dphi[P.rcv[e]])));
+  // edges outer, species inner: each edge's index and potential drop are loaded once; every
+  // species still sums its outgoing, then incoming, edges in incidence order
+  double out[4] = {0.0, 0.0, 0.0, 0.0}, in[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int p = p0; p < p1; ++p) {
+    const int e = P.inc_edge[p];
+    const double d = dphi[P.snd[e]] - dphi[P.rcv[e]];
+    const bool outgoing = p < pm;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s >= P.S) break;
+      const double t = coef * (gj[(long)s * P.nedge + e] * d);
+      if (outgoing) out[s] += -t;
+      else in[s] += t;
     }
-    for (int p = pm; p < p1; ++p) {
-      int e = P.inc_edge[p];
-      in += coef * (gj[(long)s * P.nedge + e] * (dphi[P.snd[e]] - dphi[P.rcv[e]]));
-    }
+  }
+  const double dp = dphi[i];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
+    double dm = coef * (gm[(long)s * P.ncell + i] * dp);
     double cell = -dm;
-    cell += out;
-    cell += in;
+    cell += out[s];
+    cell += in[s];
     dn_c[s] = cell;
     dm_c[s] = dm;
   }
