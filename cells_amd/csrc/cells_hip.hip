@@ -385,13 +385,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     extra[3] = extra[3] - (-1.0) * ccyc;
   }
   int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
+  // edges outer (each index loaded once), species inner; each species sums its outgoing, then
+  // incoming, junction transfers in incidence order
+  double jout[4] = {0.0, 0.0, 0.0, 0.0}, jin[4] = {0.0, 0.0, 0.0, 0.0};
+  double g = kChargePerFmol * Gm;  // with slopes: the diagonal's conductances, in incidence order
+  for (int p = p0; p < p1; ++p) {
+    const int e = P.inc_edge[p];
+    if (slopes) g += Gj[e];
+    const bool outgoing = p < pm;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s >= P.S) break;
+      const double t = EFc(P, R, P.f_jtr + s, e);
+      if (outgoing) jout[s] += -t;
+      else jin[s] += t;
+    }
+  }
   for (int s = 0; s < P.S; ++s) {
-    double out = 0.0, in = 0.0;
-    for (int p = p0; p < pm; ++p) out += -EFc(P, R, P.f_jtr + s, P.inc_edge[p]);
-    for (int p = pm; p < p1; ++p) in += EFc(P, R, P.f_jtr + s, P.inc_edge[p]);
     double cell = -Jm[s];
-    cell += out;
-    cell += in;
+    cell += jout[s];
+    cell += jin[s];
     double r = cell - extra[s];
     CF(P, R, P.f_dn + s, i) = r;
     CF(P, R, P.f_mtr + s, i) = Jm[s];
@@ -416,8 +429,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     CF(P, R, P.f_wdiss, i) = 0.0;
   }
   if (slopes) {
-    double g = kChargePerFmol * Gm;
-    for (int p = p0; p < p1; ++p) g += Gj[P.inc_edge[p]];
     double d = P.cap[i] + coef * g;
     ok = ok && d > 0;
     double sc = 1.0 / sqrt(d > 0 ? d : 1.0);
