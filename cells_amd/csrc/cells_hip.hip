@@ -1745,23 +1745,27 @@ __device__ int ledger_checks(const Params& P, const double* prev, const double* 
     double cyc = CFc(P, y, P.f_pcyc, i);
     int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
     double zq = 0.0, zm = 0.0;
+    // the junction net per species, edges outer (each index loaded once), each species summed in
+    // incidence order
+    double nets[4] = {0.0, 0.0, 0.0, 0.0}, netms[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int p = p0; p < p1; ++p) {
+      const int e = P.inc_edge[p];
+      const bool outgoing = p < pm;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        if (s >= P.S) break;
+        const double j = EFc(P, y, P.f_jtr + s, e);
+        if (outgoing) nets[s] -= j;
+        else nets[s] += j;
+        netms[s] += fabs(j);
+      }
+    }
     for (int s = 0; s < P.S; ++s) {
       double dn = CFc(P, y, P.f_dn + s, i), mtr = CFc(P, y, P.f_mtr + s, i);
       double pump = s == 0 ? 3.0 * cyc : s == 1 ? -2.0 * cyc : 0.0;
       if (s == 3 && P.calcium) pump += CFc(P, y, P.f_ccyc, i);
-      double net = 0.0, netm = 0.0;
-      for (int p = p0; p < pm; ++p) {
-        double j = EFc(P, y, P.f_jtr + s, P.inc_edge[p]);
-        net -= j;
-        netm += fabs(j);
-      }
-      for (int p = pm; p < p1; ++p) {
-        double j = EFc(P, y, P.f_jtr + s, P.inc_edge[p]);
-        net += j;
-        netm += fabs(j);
-      }
-      double res = dn + mtr + pump - net;
-      double mag = fabs(dn) + fabs(mtr) + fabs(pump) + netm;
+      double res = dn + mtr + pump - nets[s];
+      double mag = fabs(dn) + fabs(mtr) + fabs(pump) + netms[s];
       rmat = fmax(rmat, ratio_of(res, material_atol + rounding * mag));
       tot[s] += dn;
       totmag[s] += fabs(dn);
