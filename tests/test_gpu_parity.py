@@ -183,11 +183,12 @@ def test_ros2_trajectory_matches_oracle(torch):
 
 
 @pytest.mark.parametrize("mode,cells,seed", [("pairwise", (64, 300), 8), ("supernodal", (64, 300), 8),
-                                             ("supernodal", (1500, 4096), 3)])
+                                             ("supernodal", (1500, 4096), 3), ("supernodal", (4096, 4096), 5)])
 def test_factor_and_solve_against_oracle(torch, mode, cells, seed):
     """Pairwise mode: the factor bit for bit, and the solve on it bit for bit the oracle's solve.
     Supernodal mode: the factor within 1e-13 of the largest entry (Jacobi-scaled SPD stage
-    matrices) and the supernodal solve on it within 1e-12 of the oracle's solve on it. The
+    matrices) and the supernodal solve on it within 1e-12 of the oracle's solve on it; worlds of
+    4096 cells hold workgroup fronts of order 91-100 (in LDS) and 101-128 (in HBM scratch). The
     pairwise plan's layout is the bound pattern's; the supernodal plan's is the relaxed pattern's,
     whose extra entries (the joined chains' explicit zeros) stay zero."""
     from cells_amd.ensemble import Ensemble
