@@ -282,6 +282,7 @@ void share_update_storage(Supernodes& sn) {
   int32_t L = -1;
   sn.update_total = 0;
   for (int32_t t = 0; t < ns; ++t) {
+    if (sn.level[t] < L) throw std::runtime_error("supernodes not in level order");
     for (; L < sn.level[t];) {  // a level's start frees the updates read at the level before it
       ++L;
       for (int32_t c : release[L]) {
