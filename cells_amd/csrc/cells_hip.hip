@@ -63,6 +63,15 @@ constexpr double kCaMaxAffinity = 80.0;
 constexpr double kCaMaxVoltage = 0.15;
 constexpr double kGateMaxVoltage = 0.120;
 constexpr double kWaterMaxRelaxation = 0.05;
+constexpr double kExplicitStepBound = 1.0;   // models/surface_pump.py:116 EXPLICIT_STEP_BOUND
+constexpr double kGateStepBound = 0.125;     // models/surface_pump.py:114 GATE_STEP_BOUND
+constexpr double kChargeAtol = 3e-25;        // numerics/surface_field.py CHARGE_ATOL_C
+constexpr double kChargeRtol = 2e-12;        // numerics/surface_field.py CHARGE_RTOL
+// Refusal reasons or-ed into a world's failure bits by the step's own kernels (the ledger checks'
+// bits are the low ones): a stage solve's charge residual above its bound, an explicit stage's
+// membrane rate above the explicit-step bound.
+constexpr int kRefuseResidual = 1 << 16;
+constexpr int kRefuseExplicitRate = 1 << 17;
 constexpr int kBlock = 256;
 
 // ------------------------------------------------------------------------------------------
@@ -101,6 +110,8 @@ struct Params {
   // worlds by cell count, largest first: the order in which whole-ensemble launches of the
   // per-world factor and solves take them (the largest start first, the smallest fill the end)
   const int* world_order;
+  // per world: refusal reasons of the current cells_step call (kRefuse* bits), or-ed atomically
+  int* refuse_bits;
   // state layout
   int cf, ef, wf;
   int f_dn, f_mtr, f_mdiss, f_pcyc, f_pdiss, f_pwork, f_ccyc, f_cdiss, f_cwork, f_dvol, f_wtr, f_wdiss;
@@ -319,7 +330,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
                                                   int vol_next, int c0, int c1, Ctl c, double* R,
                                                   double* contrib, double* gm, const double* Gj,
                                                   double coef, double* mdiag, double* msc,
-                                                  int* valid) {
+                                                  double rate_dt, int* valid) {
   int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c1) return;
   int w = P.cell_world[i];
@@ -434,6 +445,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     double sc = 1.0 / sqrt(d > 0 ? d : 1.0);
     mdiag[i] = (d * sc * sc + d * sc * sc) / 2;
     msc[i] = sc;
+  }
+  if (rate_dt > 0.0) {
+    // the explicit stage's membrane charge relaxation (models/surface_pump_v3.py:4588-4638
+    // membrane_rate with one patch per cell and no kinds): F^2/RT sum_s z^2 (A P)_s max(c_in, c_out)
+    // times the availability, over the cell's capacitance; dt x rate <= 1 (surface_pump.py:116)
+    double t = 0.0;
+    for (int s = 0; s < P.S; ++s) {
+      const double z2 = P.z[s] * P.z[s];
+      const double term = z2 * P.pam[(long)s * P.ncell + i] * fmax(ci[s], P.cb[s]);
+      t = s == 0 ? term : t + term;
+    }
+    const double rate = kF * kF / P.RT * (t * c.membrane) / P.cap[i];
+    if (!(rate_dt * rate <= kExplicitStepBound)) {
+      ok = false;
+      atomicOr(P.refuse_bits + w, kRefuseExplicitRate);
+    }
   }
   if (!ok) valid[w] = 0;
 }
@@ -1529,18 +1556,31 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_only_mf(const ShapeDev* s
 // ROS2 stage combinations (the W-method's k = F + coef (df/dphi) dphi on the route fields).
 
 // The W-method's correction coef (df/dphi) dphi of one cell: dn_c[s] on the amounts, dm_c[s] on
-// the membrane transfers (the bath supply takes -dm_c).
+// the membrane transfers (the bath supply takes -dm_c). With Gj (RES) also the stage matrix's
+// row times dphi, (C + coef G) dphi at the cell (its diagonal summed as k_cells sums it), and the
+// row's magnitude (the sum of the absolute terms of that product).
+template <bool RES>
 __device__ __forceinline__ void cell_correction(const Params& P, int i, double coef, const double* gm,
-                                                const double* gj, const double* dphi, double* dn_c,
-                                                double* dm_c) {
+                                                const double* gj, const double* Gj, const double* dphi,
+                                                double* dn_c, double* dm_c, double* mx = nullptr,
+                                                double* mag = nullptr) {
   int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
   // edges outer, species inner: each edge's index and potential drop are loaded once; every
   // species still sums its outgoing, then incoming, edges in incidence order
   double out[4] = {0.0, 0.0, 0.0, 0.0}, in[4] = {0.0, 0.0, 0.0, 0.0};
+  double gsum = 0.0, off = 0.0, offmag = 0.0;
   for (int p = p0; p < p1; ++p) {
     const int e = P.inc_edge[p];
-    const double d = dphi[P.snd[e]] - dphi[P.rcv[e]];
+    const double ds = dphi[P.snd[e]], dr = dphi[P.rcv[e]];
+    const double d = ds - dr;
     const bool outgoing = p < pm;
+    if constexpr (RES) {
+      const double ge = Gj[e];
+      const double t = ge * (outgoing ? dr : ds);
+      gsum += ge;
+      off += t;
+      offmag += fabs(t);
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       if (s >= P.S) break;
@@ -1550,28 +1590,49 @@ __device__ __forceinline__ void cell_correction(const Params& P, int i, double c
     }
   }
   const double dp = dphi[i];
+  double Gm = 0.0;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (s >= P.S) break;
-    double dm = coef * (gm[(long)s * P.ncell + i] * dp);
+    const double g = gm[(long)s * P.ncell + i];
+    if constexpr (RES) Gm = s == 0 ? P.z[s] * g : Gm + P.z[s] * g;
+    double dm = coef * (g * dp);
     double cell = -dm;
     cell += out[s];
     cell += in[s];
     dn_c[s] = cell;
     dm_c[s] = dm;
   }
+  if constexpr (RES) {
+    const double diag = P.cap[i] + coef * (kChargePerFmol * Gm + gsum);
+    *mx = diag * dp - coef * off;
+    *mag = fabs(diag * dp) + fabs(coef) * offmag;
+  }
 }
 
-// Adds coef (df/dphi) dphi to K's transfers and amounts (and the cell's supply contribution).
-__device__ void correct_cell(const Params& P, int i, double coef, const double* gm, const double* gj,
-                             const double* dphi, double* K, double* contrib) {
-  double dn_c[4], dm_c[4];
-  cell_correction(P, i, coef, gm, gj, dphi, dn_c, dm_c);
+// The stage solve's charge residual at a cell (numerics/surface_imex.py:269-278 solve_operator's
+// (C + a L) phi - q, checked as models/surface_pump.py:325-330 checks it): |M dphi - q| <=
+// atol + rtol scale. The stage equation here is in rate form (q is the charge rate of a stage's
+// rates, which can vanish at a cell whose neighbours' terms do not), so the scale is the largest
+// of |q|, |M dphi| and the row's absolute terms.
+__device__ __forceinline__ bool stage_residual_ok(double q, double mx, double mag, double atol, double rtol) {
+  const double scale = fmax(fmax(fabs(q), fabs(mx)), mag);
+  return isfinite(mx) && fabs(mx - q) <= atol + rtol * scale;
+}
+
+// Adds coef (df/dphi) dphi to K's transfers and amounts (and the cell's supply contribution);
+// returns whether the stage solve's residual at the cell holds (q: the solve's right-hand side).
+__device__ bool correct_cell(const Params& P, int i, double coef, const double* gm, const double* gj,
+                             const double* Gj, const double* dphi, double q, double atol, double rtol,
+                             double* K, double* contrib) {
+  double dn_c[4], dm_c[4], mx, mag;
+  cell_correction<true>(P, i, coef, gm, gj, Gj, dphi, dn_c, dm_c, &mx, &mag);
   for (int s = 0; s < P.S; ++s) {
     CF(P, K, P.f_dn + s, i) += dn_c[s];
     CF(P, K, P.f_mtr + s, i) += dm_c[s];
     contrib[(long)s * P.ncell + i] = contrib[(long)s * P.ncell + i] - dm_c[s];
   }
+  return stage_residual_ok(q, mx, mag, atol, rtol);
 }
 
 __device__ void correct_edge(const Params& P, int e, double coef, const double* gj, const double* dphi,
@@ -1580,20 +1641,27 @@ __device__ void correct_edge(const Params& P, int e, double coef, const double* 
   for (int s = 0; s < P.S; ++s) EF(P, K, P.f_jtr + s, e) += coef * (gj[(long)s * P.nedge + e] * d);
 }
 
-// Stage 1: K1 += correction(dphi1); Y1 = y + dt K1 on the fields the rates read (amounts,
+// Stage 1: the stage solve's residual (its right-hand side is K1's charge rate before the
+// correction); K1 += correction(dphi1); Y1 = y + dt K1 on the fields the rates read (amounts,
 // gates, and the water volume, which the stage carries from the water half-step).
 __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* A, const double* B,
                                                         const int* sel, int c0, int nc, int e0,
                                                         int ne, double dt, double coef,
                                                         const double* gm, const double* gj,
-                                                        const double* dphi, double* K1,
-                                                        double* contrib1, double* Y1) {
+                                                        const double* Gj, const double* dphi,
+                                                        double atol, double rtol, double* K1,
+                                                        double* contrib1, double* Y1, int* valid) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nc) {
     int i = c0 + t;
-    int sw = sel[P.cell_world[i]];
+    int w = P.cell_world[i];
+    int sw = sel[w];
     const double* y = current(A, B, sw);
-    correct_cell(P, i, coef, gm, gj, dphi, K1, contrib1);
+    const double q = charge_increment(P, K1, i);
+    if (!correct_cell(P, i, coef, gm, gj, Gj, dphi, q, atol, rtol, K1, contrib1)) {
+      valid[w] = 0;
+      atomicOr(P.refuse_bits + w, kRefuseResidual);
+    }
     for (int s = 0; s < P.S; ++s)
       CF(P, Y1, P.f_dn + s, i) = CFc(P, y, P.f_dn + s, i) + dt * CFc(P, K1, P.f_dn + s, i);
     if (P.water) {
@@ -1608,13 +1676,15 @@ __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* 
   }
 }
 
-// Final: k2 = (F1 - 2 k1) + correction(dphi2), held in registers; y_new = (y + 1.5 dt k1) +
-// 0.5 dt k2; the cell's bath-supply contribution of k2; domain. F1 is read from K2.
+// Final: k2 = (F1 - 2 k1) + correction(dphi2), held in registers; the stage-2 solve's residual
+// (its right-hand side is the charge rate of F1 - 2 k1); y_new = (y + 1.5 dt k1) + 0.5 dt k2; the
+// cell's bath-supply contribution of k2; domain. F1 is read from K2.
 __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, double* B,
                                                        const int* sel, int c0, int nc, int e0,
                                                        int ne, double dt, double coef,
                                                        const double* gm, const double* gj,
-                                                       const double* dphi, const double* K1,
+                                                       const double* Gj, const double* dphi,
+                                                       double atol, double rtol, const double* K1,
                                                        const double* K2, const double* contrib1,
                                                        double* contrib2, int* valid) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1624,13 +1694,18 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
     int w = P.cell_world[i];
     const double* y = current(A, B, sel[w]);
     double* o = next(A, B, sel[w]);
-    double dn_c[4], dm_c[4];
-    cell_correction(P, i, coef, gm, gj, dphi, dn_c, dm_c);
+    double dn_c[4], dm_c[4], mx, mag;
+    cell_correction<true>(P, i, coef, gm, gj, Gj, dphi, dn_c, dm_c, &mx, &mag);
+    double qt = 0.0;  // the charge of F1 - 2 k1, summed as the stage-2 solve sums it
     for (int f = 0; f < P.cf; ++f) {
       if (is_water_field(P, f)) continue;
       double k1 = CFc(P, K1, f, i);
       double k2 = CFc(P, K2, f, i) + -2.0 * k1;
-      if (f >= P.f_dn && f < P.f_dn + P.S) k2 += dn_c[f - P.f_dn];
+      if (f >= P.f_dn && f < P.f_dn + P.S) {
+        const int sp = f - P.f_dn;
+        qt = sp == 0 ? P.z[0] * k2 : qt + P.z[sp] * k2;
+        k2 += dn_c[sp];
+      }
       if (f >= P.f_mtr && f < P.f_mtr + P.S) k2 += dm_c[f - P.f_mtr];
       CF(P, o, f, i) = (CFc(P, y, f, i) + h1 * k1) + h2 * k2;
     }
@@ -1638,7 +1713,12 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
       long q = (long)s * P.ncell + i;
       contrib2[q] = (contrib2[q] + -2.0 * contrib1[q]) - dm_c[s];
     }
-    if (!cell_domain(P, o, o, i)) valid[w] = 0;
+    bool ok = cell_domain(P, o, o, i);
+    if (!stage_residual_ok(kChargePerFmol * qt, mx, mag, atol, rtol)) {
+      ok = false;
+      atomicOr(P.refuse_bits + w, kRefuseResidual);
+    }
+    if (!ok) valid[w] = 0;
   } else if (t < nc + ne) {
     int e = e0 + (t - nc);
     int w = P.edge_world[e];
@@ -2209,8 +2289,12 @@ struct cells_ensemble {
   double check_atol = 1e-12;
   long long check_steps = 0;  // the ledgers' step count of the next cells_step's input
   long long* step_counter = nullptr;
-  int* check_bits = nullptr;
-  bool graph_checks = false;
+  int* check_bits = nullptr;  // per world: the ledger checks' bits and the step's refusal reasons
+  // the stage solves' residual bound (numerics/surface_field.py:47-48 CHARGE_ATOL_C, CHARGE_RTOL)
+  double res_atol = kChargeAtol, res_rtol = kChargeRtol;
+  // rate bounds of the bound worlds at their declared state (1/s; cells_step_bounds): the explicit
+  // rate, the junction gates' and the Ca-ATPase's
+  double bound_explicit = 0.0, bound_gate = 0.0, bound_calcium = 0.0;
   // step records for the adjoint (cells_tape_reserve): per slot the step's input state, stage-1
   // state, factor values, Jacobi scales, route slopes and both stage increments, field-major
   int tape_slots = 0;
@@ -2224,8 +2308,19 @@ struct cells_ensemble {
   double *ca_bar = nullptr, *cb_bar = nullptr, *ev_bar = nullptr, *loss_w = nullptr;
   hipStream_t capture_stream = nullptr;
   hipGraphExec_t graph = nullptr;
-  double graph_dt = 0.0;
-  Ctl graph_ctl{};
+  // what the captured step graph bakes in: it is replayed only while all of it is unchanged
+  struct GraphKey {
+    double dt = 0.0;
+    Ctl ctl{};
+    bool checks = false;
+    double check_atol = 0.0, res_atol = 0.0, res_rtol = 0.0;
+    bool operator==(const GraphKey& o) const {
+      return dt == o.dt && ctl.membrane == o.ctl.membrane && ctl.junction == o.ctl.junction &&
+             ctl.pump == o.ctl.pump && ctl.calcium_pump == o.ctl.calcium_pump && ctl.water == o.ctl.water &&
+             checks == o.checks && check_atol == o.check_atol && res_atol == o.res_atol &&
+             res_rtol == o.res_rtol;
+    }
+  } graph_key;
   ~cells_ensemble() {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -2300,6 +2395,18 @@ int check_world(const cells_world* w) {
         return fail(CELLS_EINVAL, "an edge leaves its world or joins a cell to itself");
     }
   }
+  {
+    // The stage matrix takes one off-diagonal entry per pair of cells; the reference sums the
+    // conductances of every junction between a pair into it (numerics/surface_field.py:413-418),
+    // which the plans here do not: a repeated pair is refused rather than mis-assembled.
+    std::vector<std::pair<int64_t, int64_t>> pairs;
+    pairs.reserve((size_t)w->nedge);
+    for (int64_t e = 0; e < w->nedge; ++e)
+      pairs.emplace_back(std::min(w->snd[e], w->rcv[e]), std::max(w->snd[e], w->rcv[e]));
+    std::sort(pairs.begin(), pairs.end());
+    if (std::adjacent_find(pairs.begin(), pairs.end()) != pairs.end())
+      return fail(CELLS_EINVAL, "two junctions join the same pair of cells (merge them into one)");
+  }
   if (w->inc_ptr[0] != 0 || w->inc_ptr[w->ncell] != 2 * w->nedge)
     return fail(CELLS_EINVAL, "incidence lists disagree with the edges");
   for (int64_t i = 0; i < w->ncell; ++i) {
@@ -2345,6 +2452,94 @@ void fill_layout(Params& P, const cells_world* w) {
   P.edge_base = (long)P.cf * w->ncell;
   P.world_base = P.edge_base + (long)P.ef * w->nedge;
   P.total = P.world_base + (long)P.wf * w->W;
+}
+
+// Rate bounds (1/s) of the worlds at their declared state, from the definition alone as the
+// reference's step checks take them (models/surface_pump.py:402-445 StepChecks; held bath, one
+// patch per cell, no faces, no floating nodes):
+// - explicit: IMEX Euler keeps the membranes and junctions explicit, explicit_rate_bound
+//   (surface_pump.py:464-555): the larger of 2 max_i sum_j G_j / C_i over the junctions and
+//   max_i G_m / C_i over the membranes, G = F^2/RT sum_s z_s^2 (A P)_s max(c_a, c_b), plus the
+//   neutral species' material term (neutral_material_rate, :621); ROS2 puts the route slopes in
+//   the stage matrix as N2 does, so what stays explicit is the material relaxation of
+//   implicit_conductance_rate_bound (surface_pump_v3.py:4981-5017): 2 max over species and cells
+//   of (A P_m B(-x) + sum_j A P_j) / V, x = z F V_0 / RT at the declared membrane voltage;
+// - gate: junction_gate_rate_bound (surface_pump.py:447-461), 1.3 exp(77 threshold) + 20 where a
+//   gate can close (minimum below 1), checked for IMEX Euler only (surface_pump_v3.py:5161-5164);
+// - calcium: calcium_rate_bound (surface_pump_v2.py:583-588), 2 max_i cap_i / (Km V_i).
+void rate_bounds(const cells_world* w, int method, double* bexp, double* bgate, double* bca) {
+  const int S = w->S;
+  const int64_t nc = w->ncell, ne = w->nedge;
+  const double RT = kR * w->temperature, slope = kF * kF / RT;
+  auto conc = [&](int s, int64_t i) { return w->n0[(int64_t)s * nc + i] / w->vol0[i]; };
+  auto host_bernoulli = [](double x) {
+    if (std::fabs(x) < 1e-4) {
+      const double x2 = x * x;
+      return 1.0 - x / 2.0 + x2 / 12.0 - (x2 * x2) / 720.0;
+    }
+    return x / std::expm1(x);
+  };
+  auto rows = [](double g, double c) { return c > 0 ? g / c : (g == 0 ? 0.0 : INFINITY); };
+  double best = 0.0;
+  if (method == CELLS_METHOD_EULER) {
+    std::vector<double> jrow(nc, 0.0);
+    for (int64_t e = 0; e < ne; ++e) {
+      const int64_t a = w->snd[e], b = w->rcv[e];
+      double t = 0.0;
+      for (int s = 0; s < S; ++s)
+        t += w->valence[s] * w->valence[s] * w->paj[(int64_t)s * ne + e] * std::max(conc(s, a), conc(s, b));
+      const double g = slope * t;
+      jrow[a] += g;
+      jrow[b] += g;
+    }
+    double jmax = 0.0, mmax = 0.0;
+    for (int64_t i = 0; i < nc; ++i) {
+      double t = 0.0;
+      for (int s = 0; s < S; ++s)
+        t += w->valence[s] * w->valence[s] * w->pam[(int64_t)s * nc + i] * std::max(conc(s, i), w->bath_c[s]);
+      jmax = std::max(jmax, rows(jrow[i], w->cap[i]));
+      mmax = std::max(mmax, rows(slope * t, w->cap[i]));
+    }
+    best = std::max(2.0 * jmax, mmax);
+  }
+  // material relaxation through the routes: every species under ROS2, the neutral ones under Euler
+  std::vector<double> mat((size_t)S * nc, 0.0);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int s = 0; s < S; ++s) {
+      mat[(size_t)s * nc + w->snd[e]] += w->paj[(int64_t)s * ne + e];
+      mat[(size_t)s * nc + w->rcv[e]] += w->paj[(int64_t)s * ne + e];
+    }
+  double material = 0.0;
+  for (int s = 0; s < S; ++s) {
+    if (method == CELLS_METHOD_EULER && w->valence[s] != 0.0) continue;
+    for (int64_t i = 0; i < nc; ++i) {
+      const double v0 = w->q0[i] / w->cap[i];
+      const double x = w->valence[s] * kF * v0 / RT;
+      const double pm = w->pam[(int64_t)s * nc + i];
+      const double route = method == CELLS_METHOD_EULER ? pm : pm * host_bernoulli(-x);
+      material = std::max(material, (route + mat[(size_t)s * nc + i]) / (w->vol0[i] * 1e-15));
+    }
+  }
+  *bexp = std::max(best, 2.0 * material);
+  *bgate = (method == CELLS_METHOD_EULER && ne > 0 && w->gate_minimum < 1.0)
+               ? 1.3 * std::exp(77.0 * w->gate_threshold) + 20.0 : 0.0;
+  double ca = 0.0;
+  if (w->calcium)
+    for (int64_t i = 0; i < nc; ++i)
+      if (w->ca_cap[i] > 0) ca = std::max(ca, w->ca_cap[i] / w->ca_km_ca / w->vol0[i]);
+  *bca = 2.0 * ca;
+}
+
+// The step checks of one step size (models/surface_pump.py:431-444, surface_pump_v2.py:612-617,
+// surface_pump_v3.py:5156-5164): ConfigError's messages as CELLS_EINVAL.
+int check_step_size(const cells_ensemble* e, double dt) {
+  if (dt * e->bound_explicit > kExplicitStepBound)
+    return fail(CELLS_EINVAL, "IMEX step exceeds the explicit-rate stability bound");
+  if (dt * e->bound_calcium > kExplicitStepBound)
+    return fail(CELLS_EINVAL, "IMEX step exceeds the explicit calcium pump rate bound");
+  if (dt * e->bound_gate > kGateStepBound)
+    return fail(CELLS_EINVAL, "Resolved junction step exceeds conservative kinetic bound");
+  return CELLS_OK;
 }
 
 // The lattice pattern of world k in local indices.
@@ -2559,6 +2754,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
   UP_D(paj, w->paj, S * ne);
 #undef UP_I
 #undef UP_D
+  rate_bounds(w, method, &e->bound_explicit, &e->bound_gate, &e->bound_calcium);
   // workspace
   long total = P.total;
   if ((rc = alloc(e.get(), &e->A, total)) || (rc = alloc(e.get(), &e->B, total)) ||
@@ -2568,6 +2764,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       (rc = alloc(e.get(), &e->valid, e->W)) || (rc = alloc(e.get(), &e->step_counter, 1)) ||
       (rc = alloc(e.get(), &e->check_bits, e->W)))
     return rc;
+  P.refuse_bits = e->check_bits;
   if (method == CELLS_METHOD_ROS2) {
     if ((rc = alloc(e.get(), &e->Y1, total)) || (rc = alloc(e.get(), &e->K2, total)) ||
         (rc = alloc(e.get(), &e->contrib2, S * nc)) || (rc = alloc(e.get(), &e->gm, S * nc)) ||
@@ -2606,8 +2803,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
     e->exact_flops_host.resize(rep.size());
     e->sn_host.resize(rep.size());
     std::atomic<int> next_shape{0};
-    std::atomic<int> bad{0};
-    std::string bad_msg;
+    std::vector<std::string> bad_msg(rep.size());  // per shape: the failure of its analysis, if any
     unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> pool_threads;
     const PlanOptions plan = g_plan;  // read once: the worker threads share it
@@ -2644,13 +2840,14 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
             e->sn_host[q] = cells::supernodes(e->shapes_host[q], front_class);
             cells::share_update_storage(e->sn_host[q]);
           } catch (const std::exception& ex) {
-            bad = 1;
-            bad_msg = ex.what();
+            bad_msg[q] = ex.what();
+            if (bad_msg[q].empty()) bad_msg[q] = "symbolic analysis failed";
           }
         }
       });
     for (auto& t : pool_threads) t.join();
-    if (bad) return fail(CELLS_EINVAL, bad_msg);
+    for (const std::string& m : bad_msg)  // the first failing shape's message, whatever the timing
+      if (!m.empty()) return fail(CELLS_EINVAL, m);
     // device plans, concatenated
     std::vector<ShapeDev> sd(rep.size());
     std::vector<int> pool;
@@ -2946,7 +3143,7 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
       mark("k_edges");
     }
     hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
-                       c0, c1, c, e->K1, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+                       c0, c1, c, e->K1, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, dt, e->valid);
     mark("k_cells");
     hipLaunchKernelGGL(k_euler_update, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
                        ed0, ne, dt, e->K1, e->valid);
@@ -2964,7 +3161,7 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
       mark("k_edges");
     }
     hipLaunchKernelGGL(k_cells<true>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
-                       c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
+                       c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, 0.0, e->valid);
     mark("k_cells");
     launch_factor(e, st, w0, nw, coef);
     mark("k_factor");
@@ -2974,7 +3171,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
   });
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
-                       ed0, ne, dt, coef, e->gm, e->gj, e->dphi, e->K1, e->contrib1, e->Y1);
+                       ed0, ne, dt, coef, e->gm, e->gj, e->Gj, e->dphi, e->res_atol, e->res_rtol, e->K1,
+                       e->contrib1, e->Y1, e->valid);
     mark("k_ros2_stage1");
     if (ne) {
       hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, ed0,
@@ -2982,7 +3180,7 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
       mark("k_edges");
     }
     hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, c0, c1,
-                       c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+                       c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, e->valid);
     mark("k_cells");
     solve_stage(e, st, nw, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
@@ -2990,7 +3188,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
   });
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
-                       ed0, ne, dt, coef, e->gm, e->gj, e->dphi2, e->K1, e->K2, e->contrib1, e->contrib2, e->valid);
+                       ed0, ne, dt, coef, e->gm, e->gj, e->Gj, e->dphi2, e->res_atol, e->res_rtol, e->K1, e->K2,
+                       e->contrib1, e->contrib2, e->valid);
     mark("k_ros2_final");
     hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, e->contrib1,
                        1.5 * dt, e->contrib2, 0.5 * dt, e->valid);
@@ -3018,6 +3217,7 @@ extern "C" int cells_step(cells_ensemble* e, void* stream, const double* y_in, d
   if (!e || !y_in || !y_out || !control) return fail(CELLS_EINVAL, "NULL argument");
   if (nsteps < 0) return fail(CELLS_EINVAL, "negative step count");
   if (!(dt > 0) || !std::isfinite(dt)) return fail(CELLS_EINVAL, "dt must be positive and finite");
+  if (int rc = check_step_size(e, dt)) return rc;
   HIP_TRY(hipSetDevice(e->device));
   hipStream_t st = static_cast<hipStream_t>(stream);
   Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
@@ -3027,16 +3227,22 @@ extern "C" int cells_step(cells_ensemble* e, void* stream, const double* y_in, d
   // water world, world fields) must hold the current values in both.
   HIP_TRY(hipMemcpyAsync(e->B, y_in, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_init, dim3(blocks(e->W)), dim3(kBlock), 0, st, e->W, active, e->sel, e->alive, e->valid);
+  HIP_TRY(hipMemsetAsync(e->check_bits, 0, sizeof(int) * e->W, st));
   if (e->checks_on) {
     HIP_TRY(hipMemcpyAsync(e->step_counter, &e->check_steps, sizeof(long long), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemsetAsync(e->check_bits, 0, sizeof(int) * e->W, st));
     HIP_TRY(hipStreamSynchronize(st));  // the counter's host copy is read before it advances
     e->check_steps += nsteps;
   }
   HIP_TRY(hipGetLastError());
   if (batched) {
-    bool same = e->graph && e->graph_dt == dt && std::memcmp(&e->graph_ctl, &c, sizeof(Ctl)) == 0 &&
-                e->graph_checks == e->checks_on;
+    cells_ensemble::GraphKey key;
+    key.dt = dt;
+    key.ctl = c;
+    key.checks = e->checks_on;
+    key.check_atol = e->checks_on ? e->check_atol : 0.0;
+    key.res_atol = e->res_atol;
+    key.res_rtol = e->res_rtol;
+    bool same = e->graph && e->graph_key == key;
     if (!same && nsteps > 0) {
       if (e->graph) {
         HIP_TRY(hipGraphExecDestroy(e->graph));
@@ -3050,9 +3256,7 @@ extern "C" int cells_step(cells_ensemble* e, void* stream, const double* y_in, d
       hipError_t inst = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (inst != hipSuccess) return fail(CELLS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
-      e->graph_dt = dt;
-      e->graph_ctl = c;
-      e->graph_checks = e->checks_on;
+      e->graph_key = key;
     }
     for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(e->graph, st));
   } else {
@@ -3120,7 +3324,7 @@ extern "C" int cells_rates(cells_ensemble* e, void* stream, const double* y,
     hipLaunchKernelGGL(k_edges<false>, dim3(blocks(e->nedge)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
                        e->nedge, c, rates_out, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_cells<false>, dim3(blocks(e->ncell)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
-                     e->ncell, c, rates_out, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+                     e->ncell, c, rates_out, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, e->valid);
   hipLaunchKernelGGL(k_world_rates, dim3(e->W), dim3(kBlock), 0, st, P, e->contrib1, rates_out);
   if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * e->W, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipGetLastError());
@@ -3172,6 +3376,23 @@ extern "C" int cells_step_checks(cells_ensemble* e, int32_t enabled, double mate
   e->checks_on = enabled != 0;
   e->check_atol = material_atol;
   e->check_steps = steps_before;
+  return CELLS_OK;
+}
+
+extern "C" int cells_step_bounds(const cells_ensemble* e, double* rates_out) {
+  if (!e || !rates_out) return fail(CELLS_EINVAL, "NULL argument");
+  rates_out[0] = e->bound_explicit;
+  rates_out[1] = e->bound_gate;
+  rates_out[2] = e->bound_calcium;
+  return CELLS_OK;
+}
+
+extern "C" int cells_set_stage_tolerance(cells_ensemble* e, double atol, double rtol) {
+  if (!e) return fail(CELLS_EINVAL, "NULL argument");
+  if (!(atol >= 0) || !(rtol >= 0) || !std::isfinite(atol) || !std::isfinite(rtol))
+    return fail(CELLS_EINVAL, "tolerances must be finite and >= 0");
+  e->res_atol = atol;
+  e->res_rtol = rtol;
   return CELLS_OK;
 }
 
@@ -3305,19 +3526,20 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
     hipLaunchKernelGGL(k_edges<true>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, ne,
                        c, e->K1, e->gj, e->Gj, e->valid);
   hipLaunchKernelGGL(k_cells<true>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
-                     e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, e->valid);
+                     e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, 0.0, e->valid);
   launch_factor(e, st, 0, W, coef);
   solve_stage(e, st, W, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds, solve_args(e), part);
   });
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
-                     ne, dt, coef, e->gm, e->gj, e->d1, e->K1, e->contrib1, e->Y1);
+                     ne, dt, coef, e->gm, e->gj, e->Gj, e->d1, e->res_atol, e->res_rtol, e->K1, e->contrib1, e->Y1,
+                     e->valid);
   if (ne)
     hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, ne, c,
                        e->K2, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
-                     e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+                     e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, e->valid);
   solve_stage(e, st, W, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds, solve_args(e), part);

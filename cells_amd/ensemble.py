@@ -188,8 +188,20 @@ class Ensemble:
         _lib.check(_lib.load().cells_step_checks(self._h, 1 if enabled else 0, float(material_atol),
                                                  int(steps_before)))
 
+    def step_bounds(self):
+        """dict(explicit, gate, calcium): the definition-time rate bounds (1/s) that every step
+        checks its dt against (dt x explicit <= 1, dt x gate <= 0.125, dt x calcium <= 1)."""
+        out = np.zeros(3)
+        _lib.check(_lib.load().cells_step_bounds(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return dict(explicit=float(out[0]), gate=float(out[1]), calcium=float(out[2]))
+
+    def stage_tolerance(self, atol=3e-25, rtol=2e-12):
+        """The stage solves' residual bound (defaults: the reference's charge tolerances)."""
+        _lib.check(_lib.load().cells_set_stage_tolerance(self._h, float(atol), float(rtol)))
+
     def check_bits(self):
-        """Failure bits per world of the last step call with checks on (device int32 [W])."""
+        """Failure bits per world of the last step call (device int32 [W]): the ledger checks'
+        bits when they are on, and the refusal reasons (lib.REFUSE_*) always."""
         t = self.torch
         bits = t.empty(self.packed.W, dtype=t.int32, device=self.device)
         _lib.check(_lib.load().cells_step_check_bits(self._h, self._stream(), ctypes.c_void_p(bits.data_ptr())))

@@ -37,8 +37,12 @@ EXPORTS = (
     "cells_field_state_size", "cells_field_step", "cells_field_refresh", "cells_field_plan_stats",
     "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials",
     "cells_world_factor_positions", "cells_relaxed_padding", "cells_set_plan_options",
-    "cells_get_plan_options",
+    "cells_get_plan_options", "cells_step_bounds", "cells_set_stage_tolerance",
 )
+
+# refusal reasons in a world's failure bits (cells_step_check_bits)
+REFUSE_RESIDUAL = 1 << 16
+REFUSE_EXPLICIT_RATE = 1 << 17
 
 
 class StationaryOptions(ctypes.Structure):
@@ -134,6 +138,8 @@ def load():
     lib.cells_tape_reserve.argtypes = [vp, ctypes.c_int32, _i64p]
     lib.cells_step_checks.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int64]
     lib.cells_step_check_bits.argtypes = [vp, vp, vp]
+    lib.cells_step_bounds.argtypes = [vp, _f64p]
+    lib.cells_set_stage_tolerance.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     lib.cells_stationary.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ControlStruct),
                                      ctypes.POINTER(StationaryOptions), vp, vp, vp,
                                      ctypes.POINTER(StationaryReport)]

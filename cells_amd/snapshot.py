@@ -11,8 +11,15 @@ batch writes one such directory per world (execution/batch_io's cohort).
 
 Here a world's leaves are cut out of the packed field-major state and named as the reference
 names them, with its axes ([species, slot] or [slot]); the held bath is the one extracellular
-slot (no faces, no channels: those leaves are empty). Reading refuses what the reference's loader
-refuses: a changed manifest, state file, clock fields or leaf set.
+slot (no faces, no channels: those leaves are empty). Beside them, as the reference stores its
+layout's leaves in the same file (records/io.py:107), go the world's bound layout: its per-cell
+and per-junction parameters and its stage matrix's symbolic analysis (fill-reducing order, rank,
+column starts, entry rows and columns, elimination tree, levels), under `layout_*` names. Reading
+refuses what the reference's loader refuses (records/io.py:138-230): a changed manifest, state
+file, clock fields or leaf set, a definition other than the target world's, another source or
+runtime identity (unless the caller admits it with `continuity=True`, the counterpart of the
+reference's continuity request, which relaxes those two comparisons only), and a layout that
+differs from the target world's bound one (validate_checkpoint's comparison).
 """
 
 from __future__ import annotations
@@ -50,6 +57,50 @@ def source_hashes():
     """SHA-256 of the library's sources (the reference's source identity, records/io.py)."""
     here = Path(__file__).resolve().parent / "csrc"
     return {p.name: _sha256(p) for p in sorted(here.iterdir()) if p.suffix in (".hip", ".inc", ".cpp", ".h")}
+
+
+RUNTIME = {"backend": "rocm", "library": "libcells_hip", "precision": "float64"}
+
+
+def runtime_identity():
+    """The runtime a checkpoint records (the reference's runtime_identity, records/io.py)."""
+    return dict(RUNTIME)
+
+
+def layout_leaves(packed: Packed, world: int) -> dict:
+    """World `world`'s bound layout: its parameters (world-local junction ends) and its stage
+    matrix's symbolic analysis (numerics/sparse_cholesky.py:69-85's Symbolic arrays, from the
+    library's host bind, bit for bit the reference's)."""
+    from cells_amd import symbolic
+
+    c0, c1 = int(packed.cell_off[world]), int(packed.cell_off[world + 1])
+    e0, e1 = int(packed.edge_off[world]), int(packed.edge_off[world + 1])
+    snd = packed.snd[e0:e1] - c0
+    rcv = packed.rcv[e0:e1] - c0
+    out = {
+        "layout_volume_pL": packed.vol0[c0:c1].copy(),
+        "layout_membrane_capacitance_F": packed.cap[c0:c1].copy(),
+        "layout_membrane_area_permeability": packed.pam[:, c0:c1].copy(),
+        "layout_pump_capacity": packed.nak_cap[c0:c1].copy(),
+        "layout_calcium_pump_capacity": packed.ca_cap[c0:c1].copy(),
+        "layout_amount_reference": packed.n0[:, c0:c1].copy(),
+        "layout_initial_cell_charge_C": packed.q0[c0:c1].copy(),
+        "layout_water_conductance": packed.lpa[c0:c1].copy(),
+        "layout_impermeant_fmol": packed.imperm[c0:c1].copy(),
+        "layout_junction_sender": snd.astype(np.int64),
+        "layout_junction_receiver": rcv.astype(np.int64),
+        "layout_junction_area_permeability": packed.paj[:, e0:e1].copy(),
+    }
+    key = (c1 - c0, snd.astype(np.int64).tobytes(), rcv.astype(np.int64).tobytes())
+    sym = _SYMBOLICS.get(key)
+    if sym is None:
+        sym = _SYMBOLICS[key] = symbolic.bind(c1 - c0, snd, rcv)  # worlds of one lattice share it
+    for k in ("order", "rank", "column_start", "row", "column", "parent", "level"):
+        out["layout_operator_" + k] = np.asarray(getattr(sym, k), dtype=np.int64)
+    return out
+
+
+_SYMBOLICS: dict = {}
 
 
 def leaves(packed: Packed, y, world: int) -> dict:
@@ -151,7 +202,7 @@ def write_snapshot(directory, packed: Packed, y, world: int, time: float, step_i
     """One world's complete checkpoint directory (records/io.py:86-135's layout)."""
     path = Path(directory)
     path.mkdir(parents=True, exist_ok=False)
-    np.savez(path / "state.npz", **leaves(packed, y, world))
+    np.savez(path / "state.npz", **leaves(packed, y, world), **layout_leaves(packed, world))
     spec = world_spec(packed, world, solver)
     manifest = {
         "schema_version": SCHEMA_VERSION,
@@ -160,7 +211,7 @@ def write_snapshot(directory, packed: Packed, y, world: int, time: float, step_i
         "spec_sha256": digest(spec),
         "clock": {"time": float(time), "step_index": int(step_index), "event_index": 0},
         "random_state": None,
-        "runtime": {"backend": "rocm", "library": "libcells_hip", "precision": "float64"},
+        "runtime": runtime_identity(),
         "source_hashes": source_hashes(),
         "state_sha256": _sha256(path / "state.npz"),
     }
@@ -171,9 +222,11 @@ def write_snapshot(directory, packed: Packed, y, world: int, time: float, step_i
     return path
 
 
-def read_snapshot(directory, packed: Packed, y, world: int):
-    """Restore one world's checkpoint into y (in place); returns its clock. Refuses a changed
-    manifest or state file, unknown clock fields, another world's size or another leaf set."""
+def read_snapshot(directory, packed: Packed, y, world: int, solver: str = "ros2", continuity: bool = False):
+    """Restore one world's checkpoint into y (in place); returns its clock. Refuses (ValueError,
+    the reference's ConfigError messages) a changed manifest or state file, unknown clock fields,
+    a definition other than world_spec(packed, world, solver), another source or runtime identity
+    (continuity=True admits those two), another leaf set, and a layout other than the world's."""
     path = Path(directory)
     with open(path / "manifest.json") as f:
         manifest = json.load(f)
@@ -184,27 +237,43 @@ def read_snapshot(directory, packed: Packed, y, world: int):
     fingerprint = manifest.pop("manifest_sha256")
     if digest(manifest) != fingerprint:
         raise ValueError("Checkpoint manifest checksum mismatch")
-    if manifest["schema_version"] != SCHEMA_VERSION or manifest["kind"] != KIND:
+    if type(manifest["schema_version"]) is not int or manifest["schema_version"] != SCHEMA_VERSION:
+        raise ValueError("Unsupported checkpoint schema")
+    if manifest["kind"] != KIND:
         raise ValueError("Not a complete Cells numerical checkpoint")
     if manifest["random_state"] is not None:
         raise ValueError("These deterministic models have no random state")
-    if digest(manifest["spec"]) != manifest["spec_sha256"]:
+    if not continuity:
+        if manifest["source_hashes"] != source_hashes():
+            raise ValueError("Checkpoint model/engine source differs from this installation")
+        if manifest["runtime"] != runtime_identity():
+            raise ValueError("Checkpoint runtime differs; cross-runtime migration is not supported")
+    spec = manifest["spec"]
+    if digest(spec) != manifest["spec_sha256"]:
         raise ValueError("Checkpoint specification checksum mismatch")
+    n = int(packed.cell_off[world + 1] - packed.cell_off[world])
+    E = int(packed.edge_off[world + 1] - packed.edge_off[world])
+    if not isinstance(spec, dict) or spec.get("cells") != n or spec.get("edges") != E:
+        raise ValueError("Checkpoint world size differs from the target world")
+    if spec != world_spec(packed, world, solver):
+        raise ValueError("Checkpoint definition differs from the target world")
     if _sha256(path / "state.npz") != manifest["state_sha256"]:
         raise ValueError("Checkpoint state checksum mismatch")
     clock = manifest["clock"]
     if not isinstance(clock, dict) or set(clock) != {"time", "step_index", "event_index"}:
         raise ValueError("Invalid checkpoint clock fields")
-    spec = manifest["spec"]
-    n = int(packed.cell_off[world + 1] - packed.cell_off[world])
-    E = int(packed.edge_off[world + 1] - packed.edge_off[world])
-    if spec["cells"] != n or spec["edges"] != E:
-        raise ValueError("Checkpoint world size differs from the target world")
-    want = set(leaves(packed, np.zeros(StateLayout.of(packed).total), world))
+    if isinstance(clock["time"], bool) or not isinstance(clock["time"], (int, float)):
+        raise ValueError("Invalid checkpoint time")
+    state_names = set(leaves(packed, np.zeros(StateLayout.of(packed).total), world))
+    layout = layout_leaves(packed, world)
     with np.load(path / "state.npz", allow_pickle=False) as archive:
-        if set(archive.files) != want:
+        if set(archive.files) != state_names | set(layout):
             raise ValueError("Invalid checkpoint state fields")
-        place(packed, y, world, {k: archive[k] for k in archive.files})
+        for k, v in layout.items():
+            got = archive[k]
+            if got.shape != v.shape or got.dtype != v.dtype or not np.array_equal(got, v):
+                raise ValueError(f"Checkpoint layout differs from the target world ({k})")
+        place(packed, y, world, {k: archive[k] for k in state_names})
     return clock
 
 
@@ -221,7 +290,7 @@ def write_cohort(directory, packed: Packed, y, time, step_index, solver: str = "
     return root
 
 
-def read_cohort(directory, packed: Packed):
+def read_cohort(directory, packed: Packed, solver: str = "ros2", continuity: bool = False):
     """(state, times [W], step counts [W]) restored from write_cohort's directory."""
     root = Path(directory)
     y = np.zeros(StateLayout.of(packed).total)
@@ -231,6 +300,6 @@ def read_cohort(directory, packed: Packed):
     if names != [f"world_{k:05d}" for k in range(packed.W)]:
         raise ValueError("Cohort directory does not hold one checkpoint per world")
     for k in range(packed.W):
-        clock = read_snapshot(root / names[k], packed, y, k)
+        clock = read_snapshot(root / names[k], packed, y, k, solver, continuity)
         times[k], steps[k] = clock["time"], clock["step_index"]
     return y, times, steps

@@ -174,8 +174,31 @@ int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const doubl
  * nsteps with each call. enabled = 0 turns the epilogue off. */
 int cells_step_checks(cells_ensemble* e, int32_t enabled, double material_atol, int64_t steps_before);
 
-/* The failure bits per world (int32, device) of the last cells_step call with checks on. */
+/* The failure bits per world (int32, device) of the last cells_step call: the ledger bits of
+ * cells_device_checks when the epilogue is on, and the step's own refusal reasons always:
+ * CELLS_REFUSE_RESIDUAL (a stage solve's charge residual above its bound,
+ * numerics/surface_imex.py:269-278 as models/surface_pump.py:325-330 checks it) and
+ * CELLS_REFUSE_EXPLICIT_RATE (an explicit stage's membrane rate times dt above 1,
+ * models/surface_pump_v3.py:3640-3642 with membrane_rate :4588). */
+#define CELLS_REFUSE_RESIDUAL (1 << 16)
+#define CELLS_REFUSE_EXPLICIT_RATE (1 << 17)
 int cells_step_check_bits(cells_ensemble* e, void* stream, int32_t* bits_out);
+
+/* The definition-time rate bounds (1/s) of the bound worlds at their declared state, which every
+ * cells_step checks against its dt before it launches anything (the reference's step checks,
+ * models/surface_pump.py:402-555 StepChecks with explicit_rate_bound and junction_gate_rate_bound,
+ * surface_pump_v2.py:583-617 calcium_rate_bound, surface_pump_v3.py:4981-5017
+ * implicit_conductance_rate_bound for the method whose route slopes are implicit): rates_out[0]
+ * the explicit-rate bound (dt x it <= 1), [1] the junction gates' (dt x it <= 0.125; IMEX Euler
+ * only, 0 otherwise), [2] the Ca-ATPase's (dt x it <= 1). A dt beyond any of them is
+ * CELLS_EINVAL with the reference's ConfigError message. */
+int cells_step_bounds(const cells_ensemble* e, double* rates_out);
+
+/* The stage solves' residual bound (ROS2): a cell's |(C + gamma dt G) dphi - q| must stay within
+ * atol + rtol x its row's magnitude, else the world refuses the step with
+ * CELLS_REFUSE_RESIDUAL. Defaults: the reference's CHARGE_ATOL_C 3e-25 and CHARGE_RTOL 2e-12
+ * (numerics/surface_field.py:47-48). */
+int cells_set_stage_tolerance(cells_ensemble* e, double atol, double rtol);
 
 /* Step tape for the adjoint (no counterpart in the reference, which has no adjoint). Reserves
  * `slots` step records in device memory (0 frees them); each record keeps what the reverse of

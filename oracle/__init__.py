@@ -66,6 +66,9 @@ def lib():
             getattr(_lib, name).restype = ctypes.c_double
         for name in ("ref_selective_flux", "ref_route_slope", "ref_gate_rhs"):
             getattr(_lib, name).restype = ctypes.c_double
+        _lib.ref_membrane_rate.restype = ctypes.c_double
+        _lib.ref_membrane_rate.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
+                                           ctypes.c_void_p]
     return _lib
 
 

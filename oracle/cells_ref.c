@@ -534,6 +534,26 @@ static int water_half(const cells_world* w, const cells_state_layout* L, int32_t
 
 /* ---------------------------------------------------------------- IMEX Euler */
 
+/* The explicit stage's membrane charge relaxation rate of one cell (1/s): models/surface_pump_v3.py
+ * :4588-4638 membrane_rate with one patch per cell and no kinds (its patch slopes :4534-4585):
+ * F^2/RT sum_s z_s^2 (A P)_s max(c_in, c_out) times the membrane availability, over the cell's
+ * capacitance. The step refuses where dt times it exceeds EXPLICIT_STEP_BOUND = 1
+ * (surface_pump_v3.py:3640-3642). */
+double ref_membrane_rate(const cells_world* w, const double* y, int64_t i, const cells_control* c) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const double vol = cell_volume(w, &L, y, i);
+    const double RT = R_CONST * w->temperature;
+    double t = 0.0;
+    for (int s = 0; s < w->S; ++s) {
+        const double ci = (w->n0[(int64_t)s * w->ncell + i] + CF(&L, w, y, L.dn + s, i)) / vol;
+        const double term = w->valence[s] * w->valence[s] * w->pam[(int64_t)s * w->ncell + i] *
+                            (ci > w->bath_c[s] ? ci : w->bath_c[s]);
+        t = s == 0 ? term : t + term;
+    }
+    return F_CONST * F_CONST / RT * (t * c->membrane) / w->cap[i];
+}
+
 int ref_step_euler(const cells_world* w, const double* y, double* out, double dt,
                    const cells_control* c, int32_t* valid_out) {
     cells_state_layout L;
@@ -549,6 +569,8 @@ int ref_step_euler(const cells_world* w, const double* y, double* out, double dt
         /* surface_pump.py:292-362 with no faces: the stage is y + dt r(start potentials) */
         ref_potentials(w, k, start, phi);
         valid = ref_rates(w, k, start, phi, c, r, NULL, NULL) && valid;
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+            valid = valid && dt * ref_membrane_rate(w, start, i, c) <= 1.0;
         copy_world(w, &L, k, start, out);
         combine(w, &L, k, start, 1.0 * dt, r, out);
         double applied[4];
@@ -730,6 +752,35 @@ static void correct(const cells_world* w, const cells_state_layout* L, int32_t k
     for (int s = 0; s < S; ++s) WF(L, w, kv, L->sup + s, k) += sup[s];
 }
 
+/* The stage solve's charge residual (numerics/surface_imex.py:269-278 solve_operator's
+ * (C + a L) phi - q, checked as models/surface_pump.py:325-330 checks it, CHARGE_ATOL_C 3e-25 and
+ * CHARGE_RTOL 2e-12 of numerics/surface_field.py:47-48) of world k's stage x = dphi with
+ * right-hand side q (world-local), the matrix's unscaled diagonal diag and junction slopes Gj.
+ * The stage equation is in rate form, where q can vanish at a cell whose neighbours' terms do
+ * not, so each row's scale is the largest of |q|, |M x| and the sum of the product's absolute
+ * terms. */
+static int stage_residual(const cells_world* w, int32_t k, double coef, const double* diag,
+                          const double* Gj, const double* dphi, const double* q) {
+    const int64_t c0 = w->cell_off[k], e0 = w->edge_off[k];
+    int ok = 1;
+    for (int64_t i = c0; i < w->cell_off[k + 1]; ++i) {
+        double off = 0.0, offmag = 0.0;
+        for (int64_t p = w->inc_ptr[i]; p < w->inc_ptr[i + 1]; ++p) {
+            const int64_t e = w->inc_edge[p];
+            const int64_t other = w->snd[e] == i ? w->rcv[e] : w->snd[e];
+            const double t = Gj[e - e0] * dphi[other];
+            off += t;
+            offmag += fabs(t);
+        }
+        const double mx = diag[i - c0] * dphi[i] - coef * off;
+        const double mag = fabs(diag[i - c0] * dphi[i]) + fabs(coef) * offmag;
+        double scale = fabs(q[i - c0]) > fabs(mx) ? fabs(q[i - c0]) : fabs(mx);
+        scale = scale > mag ? scale : mag;
+        ok = ok && isfinite(mx) && fabs(mx - q[i - c0]) <= 3e-25 + 2e-12 * scale;
+    }
+    return ok;
+}
+
 static void charge_rate(const cells_world* w, const cells_state_layout* L, int32_t k,
                         const double* kv, double* q) {
     for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
@@ -789,21 +840,31 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym, const double*
             double v = diag[i];
             fv[s->diag_pos[i]] = (v * sc[i] * sc[i] + v * sc[i] * sc[i]) / 2;
         }
+        /* each off-diagonal entry is the sum of the conductances of every junction between its
+         * pair of cells, in junction order (numerics/surface_field.py:413-418), before the
+         * scaling (surface_imex.py:248-250) */
+        double* off = (double*)calloc((size_t)s->nnz, sizeof(double));
+        for (int64_t e = e0; e < w->edge_off[k + 1]; ++e)
+            off[s->edge_pos[e - e0]] += -(coef * Gj[e - e0]);
         for (int64_t e = e0; e < w->edge_off[k + 1]; ++e) {
-            double v = -(coef * Gj[e - e0]);
+            double v = off[s->edge_pos[e - e0]];
             int64_t a = w->snd[e] - c0, b = w->rcv[e] - c0;
             fv[s->edge_pos[e - e0]] = (v * sc[a] * sc[b] + v * sc[b] * sc[a]) / 2;
         }
+        free(off);
         ref_factor(s, fv);
         for (int64_t p = 0; p < s->nnz; ++p) valid = valid && isfinite(fv[p]);
         double* q = (double*)malloc(sizeof(double) * (size_t)n);
+        double* q0 = (double*)malloc(sizeof(double) * (size_t)n);
         double* x = (double*)malloc(sizeof(double) * (size_t)n);
         double* dphi = (double*)calloc((size_t)w->ncell, sizeof(double));
         /* stage 1 */
         charge_rate(w, &L, k, k1, q);
+        memcpy(q0, q, sizeof(double) * (size_t)n);
         for (int64_t i = 0; i < n; ++i) q[i] = q[i] * sc[i];
         ref_solve(s, fv, q, x);
         for (int64_t i = 0; i < n; ++i) dphi[c0 + i] = x[i] * sc[i];
+        valid = valid && stage_residual(w, k, coef, diag, Gj, dphi, q0);
         correct(w, &L, k, coef, gm, gj, dphi, k1);
         combine(w, &L, k, start, dt, k1, y1);
         /* stage 2: f(y + dt k1) - 2 k1 */
@@ -811,9 +872,11 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym, const double*
         valid = ref_rates(w, k, y1, phi, c, k2, NULL, NULL) && valid;
         combine(w, &L, k, k2, -2.0, k1, k2);
         charge_rate(w, &L, k, k2, q);
+        memcpy(q0, q, sizeof(double) * (size_t)n);
         for (int64_t i = 0; i < n; ++i) q[i] = q[i] * sc[i];
         ref_solve(s, fv, q, x);
         for (int64_t i = 0; i < n; ++i) dphi[c0 + i] = x[i] * sc[i];
+        valid = valid && stage_residual(w, k, coef, diag, Gj, dphi, q0);
         correct(w, &L, k, coef, gm, gj, dphi, k2);
         /* y + (3/2) dt k1 + (1/2) dt k2 */
         copy_world(w, &L, k, start, out);
@@ -830,7 +893,7 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym, const double*
         if (!valid) copy_world(w, &L, k, y, out);
         valid_out[k] = valid;
         all = all && valid;
-        free(Gm); free(Gj); free(diag); free(sc); free(fv); free(q); free(x); free(dphi);
+        free(Gm); free(Gj); free(diag); free(sc); free(fv); free(q); free(q0); free(x); free(dphi);
     }
     free(start); free(k1); free(k2); free(y1); free(phi); free(gm); free(gj);
     return all;

@@ -86,6 +86,10 @@ void ref_potentials(const cells_world* w, int32_t world, const double* y, double
 int ref_step_euler(const cells_world* w, const double* y, double* out, double dt,
                    const cells_control* c, int32_t* valid);
 
+/* An explicit stage's membrane charge relaxation rate (1/s) of cell i at state y; IMEX Euler
+ * refuses a world where dt times it exceeds 1 at any cell. */
+double ref_membrane_rate(const cells_world* w, const double* y, int64_t i, const cells_control* c);
+
 /* Sparse symbolic structure of one world's stage matrix (the product of oracle/symbolic.py's
  * bind): column_start [n+1] and each entry's row rank [nnz]; order [n]; plus, per lattice entry,
  * its factor position: diag_pos [n] (node order), edge_pos [E] (world edge order). */

@@ -182,6 +182,32 @@ def test_ros2_trajectory_matches_oracle(torch):
     e.close()
 
 
+def front_orders(n):
+    """Front orders of the default multifrontal plan of the n-cell lattice: the relaxed pattern's
+    fundamental supernodes (column j joins j + 1 when j + 1 is its parent with no other child and
+    holds its rows but itself), each front the order of its first column."""
+    from cells_amd import lib as hip
+    from cells_amd import symbolic
+    from cells_amd.world import lattice
+
+    relax_rows, relax_zeros = (np.zeros(1, np.int32) for _ in range(2))
+    hip.check(hip.load().cells_get_plan_options(relax_rows.ctypes.data_as(hip._i32p),
+                                                relax_zeros.ctypes.data_as(hip._i32p), None, None))
+    snd, rcv = lattice(n)
+    order, pr, pc = symbolic.relaxed_padding(n, snd, rcv, int(relax_rows[0]), int(relax_zeros[0]))
+    s = symbolic.bind(n, np.concatenate((snd, pr)), np.concatenate((rcv, pc)), order)
+    count = np.diff(s.column_start)
+    kids = np.bincount(s.parent[s.parent >= 0], minlength=n)
+    orders, k = [], 0
+    while k < n:
+        j = k
+        while j + 1 < n and s.parent[j] == j + 1 and kids[j + 1] == 1 and count[j] == count[j + 1] + 1:
+            j += 1
+        orders.append(int(count[k]))
+        k = j + 1
+    return orders
+
+
 @pytest.mark.parametrize("mode,cells,seed", [("pairwise", (64, 300), 8), ("supernodal", (64, 300), 8),
                                              ("supernodal", (1500, 4096), 3), ("supernodal", (4096, 4096), 5)])
 def test_factor_and_solve_against_oracle(torch, mode, cells, seed):
@@ -195,6 +221,10 @@ def test_factor_and_solve_against_oracle(torch, mode, cells, seed):
 
     p = build(config1(worlds=3, cells=cells, seed=seed))
     syms, sym = oracle.world_symbolics(p)
+    if cells == (4096, 4096):
+        # the plan of these worlds holds fronts of both workgroup kinds (checked on the host)
+        orders = front_orders(4096)
+        assert any(91 <= o <= 100 for o in orders) and any(101 <= o <= 128 for o in orders), max(orders)
     e = Ensemble(p, "ros2", factor=mode)
     rng = np.random.default_rng(9)
     values = np.zeros(e.factor_size)
