@@ -2302,10 +2302,8 @@ struct cells_ensemble {
   double *t_y = nullptr, *t_y1 = nullptr, *t_fv = nullptr, *t_msc = nullptr, *t_gm = nullptr;
   double *t_gj = nullptr, *t_d1 = nullptr, *t_d2 = nullptr;
   int* t_valid = nullptr;
-  // adjoint workspace (allocated at the first cells_step_vjp)
-  double *d1 = nullptr, *d2 = nullptr, *k1bar = nullptr, *k2bar = nullptr, *k1gbar = nullptr;
-  double *g2bar = nullptr, *gmbar = nullptr, *gjbar = nullptr, *dbar = nullptr, *wv = nullptr;
-  double *ca_bar = nullptr, *cb_bar = nullptr, *ev_bar = nullptr, *loss_w = nullptr;
+  // the adjoint workspace (allocated at the first cells_step_vjp; laid out by adjoint_layout)
+  double* adj_base = nullptr;
   hipStream_t capture_stream = nullptr;
   hipGraphExec_t graph = nullptr;
   // what the captured step graph bakes in: it is replayed only while all of it is unchanged
@@ -3406,16 +3404,48 @@ extern "C" int cells_step_check_bits(cells_ensemble* e, void* stream, int32_t* b
 
 namespace {
 
-int adjoint_workspace(cells_ensemble* e) {
-  if (e->d1) return CELLS_OK;
-  size_t nc = e->ncell, ne = std::max(e->nedge, 1), S = e->S;
-  int rc;
-  if ((rc = alloc(e, &e->d1, nc)) || (rc = alloc(e, &e->d2, nc)) || (rc = alloc(e, &e->k1bar, S * nc)) ||
-      (rc = alloc(e, &e->k2bar, S * nc)) || (rc = alloc(e, &e->k1gbar, ne)) || (rc = alloc(e, &e->g2bar, ne)) ||
-      (rc = alloc(e, &e->gmbar, S * nc)) || (rc = alloc(e, &e->gjbar, S * ne)) || (rc = alloc(e, &e->dbar, nc)) ||
-      (rc = alloc(e, &e->wv, nc)) || (rc = alloc(e, &e->ca_bar, S * ne)) || (rc = alloc(e, &e->cb_bar, S * ne)) ||
-      (rc = alloc(e, &e->ev_bar, ne)) || (rc = alloc(e, &e->loss_w, e->W)))
-    return rc;
+// The adjoint's workspace: the recomputed stage increments and the reverse sweep's cotangents.
+// Every call writes each of these doubles before it reads it, except gmbar and gjbar, which the
+// reverse sweep clears itself (k_fill), so no call depends on what an earlier one left.
+struct AdjointWork {
+  double *d1, *d2, *k1bar, *k2bar, *k1gbar, *g2bar, *gmbar, *gjbar, *dbar, *wv, *ca_bar, *cb_bar, *ev_bar;
+};
+
+// Lays the workspace of an ensemble out from `base` (each buffer on a 256-byte boundary); returns
+// the doubles it spans. base may be NULL to ask for the size alone.
+size_t adjoint_layout(const cells_ensemble* e, double* base, AdjointWork* w) {
+  const size_t nc = e->ncell, ne = std::max(e->nedge, 1), S = e->S;
+  size_t off = 0;
+  auto take = [&](size_t count) {
+    double* q = base ? base + off : nullptr;
+    off += (count + 31) & ~(size_t)31;
+    return q;
+  };
+  AdjointWork a;
+  a.d1 = take(nc);
+  a.d2 = take(nc);
+  a.k1bar = take(S * nc);
+  a.k2bar = take(S * nc);
+  a.k1gbar = take(ne);
+  a.g2bar = take(ne);
+  a.gmbar = take(S * nc);
+  a.gjbar = take(S * ne);
+  a.dbar = take(nc);
+  a.wv = take(nc);
+  a.ca_bar = take(S * ne);
+  a.cb_bar = take(S * ne);
+  a.ev_bar = take(ne);
+  if (w) *w = a;
+  return off;
+}
+
+// The ensemble's own workspace, allocated at its first adjoint call.
+int adjoint_workspace(cells_ensemble* e, AdjointWork* w) {
+  if (!e->adj_base) {
+    int rc = alloc(e, &e->adj_base, adjoint_layout(e, nullptr, nullptr));
+    if (rc) return rc;
+  }
+  adjoint_layout(e, e->adj_base, w);
   return CELLS_OK;
 }
 
@@ -3435,50 +3465,50 @@ int check_adjoint(const cells_ensemble* e, double dt) {
 
 // ybar_in (+)= the pullback of ybar_out through the step recorded in r; parameter cotangents
 // accumulate into pam_bar, cap_bar, paj_bar.
-int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const double* ybar_out,
-                  double dt, const Ctl& c, double* ybar_in, double* pam_bar, double* cap_bar,
-                  double* paj_bar) {
+int reverse_sweep(cells_ensemble* e, hipStream_t st, const StepRecord& r, const AdjointWork& a,
+                  const double* ybar_out, double dt, const Ctl& c, double* ybar_in, double* pam_bar,
+                  double* cap_bar, double* paj_bar) {
   const Params& P = e->P;
   const double coef = (1.0 + 1.0 / std::sqrt(2.0)) * dt;
   const int nc = e->ncell, ne = e->nedge, W = e->W;
   long S = e->S;
   double* d2 = const_cast<double*>(r.d2);
   double* d1 = const_cast<double*>(r.d1);
-  hipLaunchKernelGGL(k_adj_seed, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, ybar_out, dt, ybar_in, e->k1bar,
-                     e->k2bar, e->k1gbar, e->g2bar);
-  hipLaunchKernelGGL(k_fill, dim3(blocks(S * nc)), dim3(kBlock), 0, st, S * nc, e->gmbar, 0.0);
-  if (ne) hipLaunchKernelGGL(k_fill, dim3(blocks(S * ne)), dim3(kBlock), 0, st, S * ne, e->gjbar, 0.0);
+  hipLaunchKernelGGL(k_adj_seed, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, ybar_out, dt, ybar_in, a.k1bar,
+                     a.k2bar, a.k1gbar, a.g2bar);
+  hipLaunchKernelGGL(k_fill, dim3(blocks(S * nc)), dim3(kBlock), 0, st, S * nc, a.gmbar, 0.0);
+  if (ne) hipLaunchKernelGGL(k_fill, dim3(blocks(S * ne)), dim3(kBlock), 0, st, S * ne, a.gjbar, 0.0);
   // k2 = r2 + coef D delta2;  delta2 = M^-1 q(r2)
-  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k2bar,
-                     r.d2, e->dbar, e->gmbar, e->gjbar);
+  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, a.k2bar,
+                     r.d2, a.dbar, a.gmbar, a.gjbar);
   solve_stage(e, st, W, r.fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_adj_solve<false> : k_adj_solve<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, coef, r.msc, r.fvals, e->dbar, d2, e->wv, e->k2bar, e->gmbar, e->gjbar, e->ysc,
+                     e->foff, coef, r.msc, r.fvals, a.dbar, d2, a.wv, a.k2bar, a.gmbar, a.gjbar, e->ysc,
                      e->y_in_lds, solve_args(e), part);
   });
   // r2 = f(Y1) - 2 k1
-  hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, e->k1bar, e->k2bar,
-                     (long)ne, e->k1gbar, e->g2bar);
+  hipLaunchKernelGGL(k_adj_axpy, dim3(blocks(S * nc + ne)), dim3(kBlock), 0, st, S * nc, a.k1bar, a.k2bar,
+                     (long)ne, a.k1gbar, a.g2bar);
   double* gate_in = ybar_in + P.edge_base + (long)P.f_gate * P.nedge;
   if (ne)
-    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, r.y1, c, e->k2bar, e->g2bar,
-                       nullptr, e->ca_bar, e->cb_bar, e->ev_bar, gate_in, e->k1gbar, dt, paj_bar);
-  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, r.y1, c, e->k2bar, nullptr,
-                     e->ca_bar, e->cb_bar, e->ev_bar, ybar_in, e->k1bar, dt, pam_bar, cap_bar);
+    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, r.y1, c, a.k2bar, a.g2bar,
+                       nullptr, a.ca_bar, a.cb_bar, a.ev_bar, gate_in, a.k1gbar, dt, paj_bar);
+  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, r.y1, c, a.k2bar, nullptr,
+                     a.ca_bar, a.cb_bar, a.ev_bar, ybar_in, a.k1bar, dt, pam_bar, cap_bar);
   // k1 = f(y) + coef D delta1;  delta1 = M^-1 q(f(y))
-  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, e->k1bar,
-                     r.d1, e->dbar, e->gmbar, e->gjbar);
+  hipLaunchKernelGGL(k_adj_dtrans, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, coef, r.gm, r.gj, a.k1bar,
+                     r.d1, a.dbar, a.gmbar, a.gjbar);
   solve_stage(e, st, W, r.fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_adj_solve<false> : k_adj_solve<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, coef, r.msc, r.fvals, e->dbar, d1, e->wv, e->k1bar, e->gmbar, e->gjbar, e->ysc,
+                     e->foff, coef, r.msc, r.fvals, a.dbar, d1, a.wv, a.k1bar, a.gmbar, a.gjbar, e->ysc,
                      e->y_in_lds, solve_args(e), part);
   });
   // f(y) and the slopes at y
   if (ne)
-    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, r.y, c, e->k1bar, e->k1gbar,
-                       e->gjbar, e->ca_bar, e->cb_bar, e->ev_bar, gate_in, nullptr, 0.0, paj_bar);
-  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, r.y, c, e->k1bar, e->gmbar,
-                     e->ca_bar, e->cb_bar, e->ev_bar, ybar_in, nullptr, 0.0, pam_bar, cap_bar);
+    hipLaunchKernelGGL(k_edges_vjp, dim3(blocks(ne)), dim3(kBlock), 0, st, P, r.y, c, a.k1bar, a.k1gbar,
+                       a.gjbar, a.ca_bar, a.cb_bar, a.ev_bar, gate_in, nullptr, 0.0, paj_bar);
+  hipLaunchKernelGGL(k_cells_vjp, dim3(blocks(nc)), dim3(kBlock), 0, st, P, r.y, c, a.k1bar, a.gmbar,
+                     a.ca_bar, a.cb_bar, a.ev_bar, ybar_in, nullptr, 0.0, pam_bar, cap_bar);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }
@@ -3503,17 +3533,11 @@ int tape_alloc(cells_ensemble* e, T** dst, size_t count) {
   return CELLS_OK;
 }
 
-}  // namespace
-
-extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const double* ybar_out,
-                              double dt, const cells_control* control, double* ybar_in,
-                              double* pam_bar, double* cap_bar, double* paj_bar, int32_t* valid_out) {
-  if (!e || !y || !ybar_out || !control || !ybar_in || !pam_bar || !cap_bar || !paj_bar)
-    return fail(CELLS_EINVAL, "NULL argument");
-  int rc = check_adjoint(e, dt);
-  if (rc) return rc;
-  HIP_TRY(hipSetDevice(e->device));
-  if ((rc = adjoint_workspace(e))) return rc;
+// The reverse of one ROS2 step at y with its forward internals recomputed, on workspace a.
+int step_vjp(cells_ensemble* e, void* stream, const double* y, const double* ybar_out, double dt,
+             const cells_control* control, double* ybar_in, double* pam_bar, double* cap_bar,
+             double* paj_bar, int32_t* valid_out, const AdjointWork& a) {
+  int rc = CELLS_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
   const Params& P = e->P;
@@ -3530,10 +3554,10 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   launch_factor(e, st, 0, W, coef);
   solve_stage(e, st, W, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, e->d1, e->ysc, e->y_in_lds, solve_args(e), part);
+                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, a.d1, e->ysc, e->y_in_lds, solve_args(e), part);
   });
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
-                     ne, dt, coef, e->gm, e->gj, e->Gj, e->d1, e->res_atol, e->res_rtol, e->K1, e->contrib1, e->Y1,
+                     ne, dt, coef, e->gm, e->gj, e->Gj, a.d1, e->res_atol, e->res_rtol, e->K1, e->contrib1, e->Y1,
                      e->valid);
   if (ne)
     hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, ne, c,
@@ -3542,13 +3566,28 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
                      e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, e->valid);
   solve_stage(e, st, W, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, e->d2, e->ysc, e->y_in_lds, solve_args(e), part);
+                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, a.d2, e->ysc, e->y_in_lds, solve_args(e), part);
   });
-  StepRecord r{e->A, e->Y1, e->fvals, e->msc, e->gm, e->gj, e->d1, e->d2};
-  if ((rc = reverse_sweep(e, st, r, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;
+  StepRecord r{e->A, e->Y1, e->fvals, e->msc, e->gm, e->gj, a.d1, a.d2};
+  if ((rc = reverse_sweep(e, st, r, a, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;
   if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * W, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
+}
+
+}  // namespace
+
+extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const double* ybar_out,
+                              double dt, const cells_control* control, double* ybar_in,
+                              double* pam_bar, double* cap_bar, double* paj_bar, int32_t* valid_out) {
+  if (!e || !y || !ybar_out || !control || !ybar_in || !pam_bar || !cap_bar || !paj_bar)
+    return fail(CELLS_EINVAL, "NULL argument");
+  int rc = check_adjoint(e, dt);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(e->device));
+  AdjointWork a;
+  if ((rc = adjoint_workspace(e, &a))) return rc;
+  return step_vjp(e, stream, y, ybar_out, dt, control, ybar_in, pam_bar, cap_bar, paj_bar, valid_out, a);
 }
 
 extern "C" int cells_tape_reserve(cells_ensemble* e, int32_t slots, int64_t* bytes_out) {
@@ -3613,14 +3652,15 @@ extern "C" int cells_step_vjp_recorded(cells_ensemble* e, void* stream, int32_t 
   if (rc) return rc;
   if (slot < 0 || slot >= e->tape_slots) return fail(CELLS_EINVAL, "tape slot out of range");
   HIP_TRY(hipSetDevice(e->device));
-  if ((rc = adjoint_workspace(e))) return rc;
+  AdjointWork a;
+  if ((rc = adjoint_workspace(e, &a))) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
   const Params& P = e->P;
   size_t q = slot, nc = e->ncell, ne = std::max(e->nedge, 1), S = e->S, tot = P.total;
   StepRecord r{e->t_y + q * tot, e->t_y1 + q * tot, e->t_fv + q * e->factor_total, e->t_msc + q * nc,
                e->t_gm + q * S * nc, e->t_gj + q * S * ne, e->t_d1 + q * nc, e->t_d2 + q * nc};
-  if ((rc = reverse_sweep(e, st, r, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;
+  if ((rc = reverse_sweep(e, st, r, a, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;
   if (valid_out)
     HIP_TRY(hipMemcpyAsync(valid_out, e->t_valid + q * e->W, sizeof(int) * e->W, hipMemcpyDeviceToDevice, st));
   return CELLS_OK;

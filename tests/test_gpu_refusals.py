@@ -90,10 +90,11 @@ def test_definition_bounds_admit_the_configurations_and_refuse_beyond(torch):
             e.step(y0, 1.01 / b["explicit"], nsteps=1)
         assert err.value.code == hip.CELLS_EINVAL
         e.close()
-    # the plain configuration's junctions set its bound: dt x rate 0.84 at 1 ms
+    # the plain configuration's junctions set its bound: dt x rate 0.84 for a cell with the mean
+    # junctions, 0.885 for the worst cell of the seeded lognormal spread, at 1 ms
     p = build(config0())
     e = ensemble(p, "euler")
-    assert 0.80 <= 1e-3 * e.step_bounds()["explicit"] <= 0.88
+    assert 0.84 <= 1e-3 * e.step_bounds()["explicit"] <= 0.95
     e.close()
 
 

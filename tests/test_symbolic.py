@@ -348,3 +348,53 @@ def test_relaxed_supernodes_keep_the_tree_and_pad_only_structural_zeros(n):
     rest = np.ones(f1.size, bool)
     rest[at] = False
     assert np.all(f1[rest] == 0.0)
+
+
+def lattice_pattern(n):
+    snd, rcv = lattice(n)
+    p = np.eye(n, dtype=bool)
+    p[snd, rcv] = p[rcv, snd] = True
+    return p
+
+
+@pytest.mark.parametrize("pattern", [
+    pytest.param(lambda: random_pattern(60, 0.06, 11), id="random_60"),
+    pytest.param(lambda: random_pattern(90, 0.03, 12), id="random_90"),
+    pytest.param(lambda: grid(9), id="grid_9x9"),
+    pytest.param(lambda: lattice_pattern(150), id="lattice_150"),
+    pytest.param(lambda: lattice_pattern(300), id="lattice_300"),
+])
+def test_bound_pattern_is_exactly_the_dense_factor_nonzeros(pattern):
+    """bind's pattern is the fill, no more and no less: the nonzeros of the dense Cholesky factor
+    of a generic SPD matrix on the pattern (random values, so no entry cancels by accident) in
+    bind's order are exactly its entries (L L^T = P A P^T alone would admit extra entries; fill
+    entries decay with distance on these diagonally dominant matrices, so the comparison is with
+    the exact zeros, not a threshold). The product's bind gives the same pattern."""
+    pat = pattern()
+    n, rows, cols = coordinates(pat)
+    s = ref.bind(n, rows, cols)
+    a = spd(pat, seed=n)
+    lower = np.linalg.cholesky(a[np.ix_(s.order, s.order)])
+    # a structural zero of the dense factor is an exact zero (every product in its sum holds one)
+    nonzero = set(zip(*np.nonzero(lower)))
+    pattern_ranks = set(zip(s.rank[s.row].tolist(), s.rank[s.column].tolist()))
+    assert pattern_ranks == {(int(i), int(j)) for i, j in nonzero}
+    assert len(pattern_ranks) == len(s.row)  # no entry twice
+    got = product.bind(n, rows, cols)
+    assert set(zip(got.rank[got.row].tolist(), got.rank[got.column].tolist())) == pattern_ranks
+
+
+def test_a_pattern_with_a_dropped_fill_entry_misses_the_dense_nonzeros():
+    """The planted mistake the check above must catch: a bind that loses one fill entry (the
+    union rule skipping a child) no longer covers the dense factor's nonzeros."""
+    pat = grid(6)
+    n, rows, cols = coordinates(pat)
+    s = ref.bind(n, rows, cols)
+    lower = np.linalg.cholesky(spd(pat, seed=5)[np.ix_(s.order, s.order)])
+    nonzero = {(int(i), int(j)) for i, j in zip(*np.nonzero(lower))}
+    original = {(int(min(a, b)), int(max(a, b))) for a, b in zip(s.rank[rows], s.rank[cols])}
+    fill = [(int(r), int(c)) for r, c in zip(s.rank[s.row], s.rank[s.column])
+            if r != c and (int(c), int(r)) not in original]
+    assert fill, "the grid's elimination creates fill"
+    dropped = set(zip(s.rank[s.row].tolist(), s.rank[s.column].tolist())) - {fill[0]}
+    assert dropped != nonzero
