@@ -35,6 +35,13 @@
 #include "../../include/cells_hip.h"
 #include "symbolic.h"
 
+// The workspace of cells_step_vjp_noalloc: a fixed device array in the code object (64 MiB, each
+// GPU the library runs on holds its own copy until the process ends), so that entry point never
+// allocates. Host code takes its address once per device with hipGetSymbolAddress and uses only
+// that address; the name itself is a host-side stand-in, not a device address.
+constexpr size_t kScratchDoubles = (size_t)8 << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
 thread_local std::string g_error;
@@ -3587,6 +3594,49 @@ extern "C" int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, 
   HIP_TRY(hipSetDevice(e->device));
   AdjointWork a;
   if ((rc = adjoint_workspace(e, &a))) return rc;
+  return step_vjp(e, stream, y, ybar_out, dt, control, ybar_in, pam_bar, cap_bar, paj_bar, valid_out, a);
+}
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kScratchDevices] = {};  // per device: the scratch array's address on it
+
+// The scratch array's device address on `device` (current), found at the first call there.
+int scratch_address(int device, double** out) {
+  if (device < 0 || device >= kScratchDevices) return fail(CELLS_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr)
+      return fail(CELLS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return CELLS_OK;
+}
+
+}  // namespace
+
+extern "C" int cells_step_vjp_noalloc(cells_ensemble* e, void* stream, const double* y, const double* ybar_out,
+                                      double dt, const cells_control* control, double* ybar_in,
+                                      double* pam_bar, double* cap_bar, double* paj_bar, int32_t* valid_out) {
+  if (!e || !y || !ybar_out || !control || !ybar_in || !pam_bar || !cap_bar || !paj_bar)
+    return fail(CELLS_EINVAL, "NULL argument");
+  int rc = check_adjoint(e, dt);
+  if (rc) return rc;
+  const size_t need = adjoint_layout(e, nullptr, nullptr);
+  if (need > kScratchDoubles)
+    return fail(CELLS_EINVAL, "cells_step_vjp_noalloc: the ensemble's adjoint workspace needs " +
+                                  std::to_string(need) + " doubles, above the " + std::to_string(kScratchDoubles) +
+                                  "-double (64 MiB) scratch; use cells_step_vjp");
+  HIP_TRY(hipSetDevice(e->device));
+  double* base = nullptr;
+  if ((rc = scratch_address(e->device, &base))) return rc;
+  AdjointWork a;
+  adjoint_layout(e, base, &a);
   return step_vjp(e, stream, y, ybar_out, dt, control, ybar_in, pam_bar, cap_bar, paj_bar, valid_out, a);
 }
 

@@ -164,9 +164,10 @@ class Ensemble:
             ctypes.c_void_p(ratios.data_ptr())))
         return failed, ratios
 
-    def step_vjp(self, y, ybar, dt, ctl=None, pam_bar=None, cap_bar=None, paj_bar=None):
+    def step_vjp(self, y, ybar, dt, ctl=None, pam_bar=None, cap_bar=None, paj_bar=None, noalloc=False):
         """Reverse of one ROS2 step at y: (cotangent on y, pam_bar, cap_bar, paj_bar, valid).
-        Parameter cotangents are added to the given tensors (zeros when None)."""
+        Parameter cotangents are added to the given tensors (zeros when None). noalloc: through
+        cells_step_vjp_noalloc (the library's fixed workspace instead of one it allocates)."""
         t = self.torch
         p = self.packed
         y, ybar = self.to_device(y), self.to_device(ybar)
@@ -178,9 +179,9 @@ class Ensemble:
         valid = t.empty(p.W, dtype=t.int32, device=dev)
         ctl = ctl or control()
         v = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
-        _lib.check(_lib.load().cells_step_vjp(self._h, self._stream(), v(y), v(ybar), float(dt),
-                                              ctypes.byref(ctl), v(yin), v(pam_bar), v(cap_bar),
-                                              v(paj_bar), v(valid)))
+        entry = _lib.load().cells_step_vjp_noalloc if noalloc else _lib.load().cells_step_vjp
+        _lib.check(entry(self._h, self._stream(), v(y), v(ybar), float(dt), ctypes.byref(ctl), v(yin),
+                         v(pam_bar), v(cap_bar), v(paj_bar), v(valid)))
         return yin, pam_bar, cap_bar, paj_bar, valid
 
     def step_checks(self, enabled=True, material_atol=1e-12, steps_before=0):

@@ -37,7 +37,7 @@ EXPORTS = (
     "cells_field_state_size", "cells_field_step", "cells_field_refresh", "cells_field_plan_stats",
     "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials",
     "cells_world_factor_positions", "cells_relaxed_padding", "cells_set_plan_options",
-    "cells_get_plan_options", "cells_step_bounds", "cells_set_stage_tolerance",
+    "cells_get_plan_options", "cells_step_bounds", "cells_set_stage_tolerance", "cells_step_vjp_noalloc",
 )
 
 # refusal reasons in a world's failure bits (cells_step_check_bits)
@@ -134,6 +134,7 @@ def load():
     vp = ctypes.c_void_p
     lib.cells_step_vjp.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
                                    vp, vp, vp, vp, vp]
+    lib.cells_step_vjp_noalloc.argtypes = lib.cells_step_vjp.argtypes
     lib.cells_vmem_loss.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, vp]
     lib.cells_tape_reserve.argtypes = [vp, ctypes.c_int32, _i64p]
     lib.cells_step_checks.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int64]

@@ -165,6 +165,18 @@ int cells_step_vjp(cells_ensemble* e, void* stream, const double* y, const doubl
                    double dt, const cells_control* control, double* ybar_in, double* pam_bar,
                    double* cap_bar, double* paj_bar, int32_t* valid_out);
 
+/* cells_step_vjp for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); no counterpart in the reference. The
+ * same kernels and results as cells_step_vjp, whose first call allocates the adjoint workspace;
+ * this one takes it from a fixed 64 MiB device array of the library (per GPU, held until the
+ * process ends) and calls no allocating routine. An ensemble whose workspace exceeds it
+ * (4 + 3 S doubles per cell plus 3 + 3 S per junction, each buffer 256-byte aligned) is
+ * CELLS_EINVAL. Every call on one GPU shares that array: calls that overlap on a GPU (other
+ * streams, other threads) race; calls ordered on one stream do not. */
+int cells_step_vjp_noalloc(cells_ensemble* e, void* stream, const double* y, const double* ybar_out,
+                           double dt, const cells_control* control, double* ybar_in, double* pam_bar,
+                           double* cap_bar, double* paj_bar, int32_t* valid_out);
+
 /* The ledger checks of cells_device_checks as each step's epilogue (execution/vector_windows.py:
  * 374-410 with models/surface_pump_checks.py:112): for the cells_step calls that follow, every
  * live world's candidate state is checked against its current one before the commit, and a world
