@@ -46,28 +46,86 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, seconds, dt):
-    """The oracle (FP64 C restatement, one core) on the first worlds of the same ensemble."""
-    import oracle
-    from cells_amd.world import head, initial_state
+def vmem(p, y):
+    """Each cell's membrane potential (V) in a state: bath + (q0 + F' sum_s z_s dN_s) / C."""
+    from cells_amd.world import F, StateLayout
 
-    k = 4
-    sub = head(p, k)
-    _, sym = oracle.world_symbolics(sub)
-    w = oracle.world_struct(sub)
-    y = initial_state(sub)
+    L = StateLayout.of(p)
+    q = p.q0 + 1e-15 * F * sum(p.valence[s] * L.cell(y, L.dn + s) for s in range(p.S))
+    return p.config.bath_V + q / p.cap
+
+
+def conc(p, y):
+    """Concentrations [S, ncell] (mM) in a state."""
+    from cells_amd.world import StateLayout
+
+    L = StateLayout.of(p)
+    v = p.vol0 + (L.cell(y, L.dvol) if p.water else 0.0)
+    return np.stack([(p.n0[s] + L.cell(y, L.dn + s)) / v for s in range(p.S)])
+
+
+def relerr(a, b):
+    """Largest |a - b| / |b| (0 where both vanish)."""
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        e = np.where(np.abs(b) > 0, np.abs(a - b) / np.abs(b), np.abs(a - b))
+    return float(np.max(e)) if e.size else 0.0
+
+
+def host_cores():
+    """(usable core ids, CPU model) of this host."""
+    cores = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline(p, seconds, dt, max_threads=16):
+    """The oracle (FP64 C restatement) on the host's cores, one world per thread (the oracle's
+    calls release the interpreter), on the ensemble's first worlds, for about `seconds`."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+    from cells_amd.world import initial_state, worlds
+
+    cores, model = host_cores()
+    threads = max(1, min(len(cores), max_threads, p.W))
+    subs = [worlds(p, k, k + 1) for k in range(threads)]
+    prepared = []
+    for sub in subs:
+        _, sym = oracle.world_symbolics(sub)
+        prepared.append((sub, sym, oracle.world_struct(sub)))
+    deadline = [0.0]
+
+    def run(i):
+        sub, sym, w = prepared[i]
+        y = initial_state(sub)
+        steps = 0
+        while True:
+            y, ok = oracle.step_ros2(sub, y, dt, sym, world=w)
+            steps += 1
+            if time.perf_counter() >= deadline[0]:
+                return steps
+
     t0 = time.perf_counter()
-    steps = 0
-    while True:
-        y, ok = oracle.step_ros2(sub, y, dt, sym, world=w)
-        steps += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
+    deadline[0] = t0 + seconds
+    with ThreadPoolExecutor(threads) as ex:
+        counts = list(ex.map(run, range(threads)))
     el = time.perf_counter() - t0
+    cells = sum(sub.ncell * c for (sub, _, _), c in zip(prepared, counts))
     return {
-        "value": sub.ncell * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-        "sample": f"oracle ROS2 steps of the ensemble's first {k} worlds ({sub.ncell} cells), "
-                  f"{steps} steps in {el:.1f} s on one core",
+        "value": cells / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+        "sample": f"oracle ROS2 steps of the ensemble's first {threads} worlds "
+                  f"({sum(s.ncell for s in subs)} cells), one world per thread on {threads} of the host's "
+                  f"{len(cores)} usable cores, {sum(counts)} world-steps in {el:.1f} s",
+        "host": {"nproc": os.cpu_count(), "usable_cores": len(cores), "core_list": cores,
+                 "cpu_model": model, "threads": threads},
     }
 
 
@@ -121,7 +179,62 @@ def copy_probe(mib):
     del src, dst
 
 
-def spatial_leg(steps, refresh, starts=("solve", "carried"), tune_plan=True):
+def spatial_oracle_operator():
+    """The oracle's operator of the spatial configuration's grid (its symbolic analysis takes about
+    a minute on the host), built on a thread of its own while the device legs run."""
+    import threading
+
+    box = {}
+
+    def build():
+        import oracle
+        from cells_amd.field import FieldConfig, build_field
+
+        fp = build_field(FieldConfig())
+        box["operator"] = oracle.FieldOperator(fp)
+
+    t = threading.Thread(target=build, daemon=True)
+    t.start()
+    return t, box
+
+
+def spatial_checks(fp, y0, y, phi, oracle_op):
+    """The spatial run's ledger closure and its field potentials against the oracle: material per
+    species and world against the ring supplies, relative to the amounts; every interior node's
+    charge with its cell's, relative to the cells' charge; world 0's node potentials from the
+    device field solve against the oracle's damped Newton at the same state."""
+    import oracle
+    from cells_amd.field import FieldLayout, amounts, charges
+
+    L = FieldLayout.of(fp)
+    ring = fp.ring()
+    c0, n0 = amounts(fp, y0)
+    c1, n1 = amounts(fp, y)
+    material = 0.0
+    for s in range(fp.S):
+        for k in range(fp.W):
+            cs = slice(int(fp.cells.cell_off[k]), int(fp.cells.cell_off[k + 1]))
+            before = c0[s, cs].sum() + n0[s, k, ~ring].sum()
+            after = c1[s, cs].sum() + n1[s, k, ~ring].sum()
+            material = max(material, abs(after - before - L.world(y, L.sup + s)[k]) / before)
+    cq, nq = charges(fp, y)
+    neutral = 0.0
+    for k in range(fp.W):
+        cs = slice(int(fp.cells.cell_off[k]), int(fp.cells.cell_off[k + 1]))
+        q = nq[k].copy()
+        q[fp.cell_node[cs]] += cq[cs]
+        neutral = max(neutral, float(np.max(np.abs(q[~ring])) / np.max(np.abs(cq[cs]))))
+    out = {"material_closure_rel": material, "charge_neutrality_rel": neutral}
+    if oracle_op is not None:
+        cells, nodes, _, conv = oracle.field_start(fp, y, oracle_op, 0)
+        scale = float(np.max(np.abs(nodes)))
+        out["potential_error_vs_oracle"] = float(np.max(np.abs(phi[0] - nodes))) / scale
+        out["potential_error_world"] = 0
+        out["oracle_field_solve_converged"] = bool(conv)
+    return out
+
+
+def spatial_leg(steps, refresh, starts=("solve", "carried"), tune_plan=True, oracle_job=None):
     """The spatial configuration: 64 worlds of 2048 cells (a 45 x 46 lattice) in a 256 x 256
     extracellular grid with a held ring, Na/K/Cl, IMEX Euler dt = 1 ms with the implicit face drift
     through the nested-dissection operator, rebuilt every `refresh` steps; each step starts from
@@ -159,12 +272,20 @@ def spatial_leg(steps, refresh, starts=("solve", "carried"), tune_plan=True):
         log(f"spatial leg ({start}): {ms:.3f} ms/step")
         out[start] = (kt, ms, el, bool(ok.all()), plan)
         if start == "solve":
-            _, chord, _ = e.potentials(y, 1e-3)
+            phi, chord, _ = e.potentials(y, 1e-3)
             chord = chord.cpu().numpy()
+            op = None
+            if oracle_job is not None:
+                log("spatial leg: waiting for the oracle's operator")
+                oracle_job[0].join()
+                op = oracle_job[1].get("operator")
+            checks = spatial_checks(fp, initial_field_state(fp), y.cpu().numpy(), phi.cpu().numpy(), op)
+            log(f"spatial leg checks: {checks}")
         e.close()
     kt, ms, el, accepted, plan = out[starts[0]]
     if starts[0] != "solve":
         chord = np.zeros(1)
+        checks = None
     solve_ms = sum(v for k, v in kt if k.startswith("k_fd_solve") and "field" not in k)
     field_ms = sum(v for k, v in kt if k.startswith("k_fd_k") or k.startswith("k_fd_solve field"))
     factor_ms = sum(v for k, v in kt if k.startswith("k_fd_factor") or k == "k_fd_assemble")
@@ -177,7 +298,7 @@ def spatial_leg(steps, refresh, starts=("solve", "carried"), tune_plan=True):
             "steps": steps, "refresh_every": refresh, "autotune": tune.summary() if tune else None,
             "start_potentials": "field solve" if starts[0] == "solve" else "carried",
             "ms_per_step": ms, "cell_steps_per_s": fp.W * fp.config.cells * steps / el,
-            "all_worlds_accepted": all(v[3] for v in out.values()), "plan": plan,
+            "all_worlds_accepted": all(v[3] for v in out.values()), "plan": plan, "checks": checks,
             "kernel_ms": {k: round(v, 4) for k, v in kt},
             "field_solve": {"ms_per_step": field_ms, "chord_steps_mean": float(np.mean(chord)),
                             "chord_steps_max": int(np.max(chord))},
@@ -190,20 +311,17 @@ def spatial_leg(steps, refresh, starts=("solve", "carried"), tune_plan=True):
                        "frac": tflops / 78.6, "ms_per_refresh": factor_ms}}
 
 
-def adjoint_leg(steps, snapshots, tape):
+def adjoint_leg(steps, snapshots, tape, fd_directions=2):
     """Calibration gradient on the adjoint configuration's worlds (256 worlds of 32x32 cells,
     Na/K/Cl, ROS2 dt = 5 ms, 6 log-parameters per cell, Vmem targets every 20 steps): one loss
     (forward sweep) and one loss-and-gradient with binomial checkpointing, timed."""
     import torch
 
+    import threading
+
     from cells_amd.adjoint import Calibration
     from cells_amd.ensemble import Ensemble
-    from cells_amd.world import F, StateLayout, build, config1, initial_state
-
-    def vmem(p, y):
-        L = StateLayout.of(p)
-        q = p.q0 + 1e-15 * F * sum(p.valence[s] * L.cell(y, L.dn + s) for s in range(p.S))
-        return p.config.bath_V + q / p.cap
+    from cells_amd.world import build, config1, initial_state
 
     dt, every = 5e-3, 20
     p = build(config1(worlds=256, cells=(1024, 1024), species=("Na", "K", "Cl"),
@@ -232,14 +350,38 @@ def adjoint_leg(steps, snapshots, tape):
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
     cal.forward_steps = cal.recorded_steps = cal.reverse_steps = 0
+    # peak device use while the gradient runs: the least free memory, sampled every 5 ms
+    free0, total = torch.cuda.mem_get_info()
+    least = [free0]
+    stop = threading.Event()
+
+    def sample():
+        while not stop.is_set():
+            least[0] = min(least[0], torch.cuda.mem_get_info()[0])
+            stop.wait(0.005)
+
+    sampler = threading.Thread(target=sample, daemon=True)
+    sampler.start()
     t0 = time.perf_counter()
     loss2, grad = cal.loss_and_gradient(theta, snapshots=snapshots, tape=tape)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
-    free, total = torch.cuda.mem_get_info()
+    stop.set()
+    sampler.join()
+    least[0] = min(least[0], torch.cuda.mem_get_info()[0])
     log(f"adjoint leg: forward {t_fwd:.3f} s, loss+gradient {t_grad:.3f} s "
         f"({cal.forward_steps} forward + {cal.recorded_steps} recorded + {cal.reverse_steps} reverse "
-        f"steps)")
+        f"steps), peak device use {(total - least[0]) / 2**30:.2f} GiB")
+    # the gradient against central differences of the HIP loss along random directions
+    fd = []
+    for trial in range(fd_directions):
+        d = rng.normal(size=theta.shape)
+        h = 1e-6
+        diff = (cal.loss(theta + h * d) - cal.loss(theta - h * d)) / (2 * h)
+        an = float(np.sum(grad * d))
+        fd.append(abs(diff - an) / abs(diff))
+    if fd:
+        log(f"adjoint leg: central differences along {len(fd)} directions, worst relative error {max(fd):.2e}")
     e.close()
     return {"worlds": p.W, "cells_per_world": p.ncell // p.W, "steps": steps,
             "snapshots": snapshots, "tape_steps": tape, "tape_bytes": tape_bytes,
@@ -247,8 +389,9 @@ def adjoint_leg(steps, snapshots, tape):
             "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
             "forward_steps": cal.forward_steps, "recorded_steps": cal.recorded_steps,
             "reverse_steps": cal.reverse_steps, "loss": loss, "loss_from_gradient_sweep": loss2,
-            "snapshot_bytes": snapshots * 8 * int(y.numel()), "device_used_bytes": int(total - free),
-            "grad_norm": float(np.linalg.norm(grad))}
+            "snapshot_bytes": snapshots * 8 * int(y.numel()), "peak_device_bytes": int(total - least[0]),
+            "grad_norm": float(np.linalg.norm(grad)),
+            "fd_check": {"directions": len(fd), "h": 1e-6, "max_rel_err": max(fd) if fd else None}}
 
 
 def main():
@@ -288,13 +431,13 @@ def main():
         skip = set(filter(None, args.skip.split(",")))
         print(json.dumps({"spatial": spatial_leg(args.spatial_steps, 50, starts, "tune" not in skip)}), flush=True)
         return
+    oracle_job = spatial_oracle_operator() if "spatial" not in skip and "parity" not in skip else None
 
     import torch
 
     import oracle
     from cells_amd.ensemble import Ensemble
     from cells_amd.world import build, config1, head, initial_state, take_state
-    from tests.helpers import conc, relerr, vmem
 
     dt = 5e-3
     log("building worlds")
@@ -395,7 +538,7 @@ def main():
                     "launch_ms": avg_ms}
     step_gbs = BYTES_PER_CELL_STEP * p.ncell / (ms_per_step * 1e-3) / 1e9
 
-    plain = None
+    plain = sub_batched = None
     if "plain" not in skip:
         k = min(args.plain_worlds, p.W)
         sub = head(p, k)
@@ -408,10 +551,13 @@ def main():
         es.step(ysw, dt, nsteps=args.plain_steps, batched=False)
         torch.cuda.synchronize()
         plain = sub.ncell * args.plain_steps / (time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        es.step(ysw, dt, nsteps=args.plain_steps, batched=True)
+        es.step(ysw, dt, nsteps=1, batched=True)  # captures the sub-ensemble's step graph
         torch.cuda.synchronize()
-        sub_batched = sub.ncell * args.plain_steps / (time.perf_counter() - t0)
+        nb = max(args.plain_steps, 10)
+        t0 = time.perf_counter()
+        es.step(ysw, dt, nsteps=nb, batched=True)
+        torch.cuda.synchronize()
+        sub_batched = sub.ncell * nb / (time.perf_counter() - t0)
         log(f"plain: {plain:.4g} cell-steps/s; batched on the same worlds {sub_batched:.4g}")
         es.close()
 
@@ -435,7 +581,7 @@ def main():
 
     spatial = None
     if "spatial" not in skip:
-        spatial = spatial_leg(args.spatial_steps, 50, tune_plan="tune" not in skip)
+        spatial = spatial_leg(args.spatial_steps, 50, tune_plan="tune" not in skip, oracle_job=oracle_job)
 
     adj = None
     if "adjoint" not in skip:
@@ -467,7 +613,11 @@ def main():
         "plain_worlds": min(args.plain_worlds, p.W) if plain else None,
         "ms_per_step_with_checks": checked_ms,
         "check_failures": checked_bits,
-        "batched_over_plain": (cell_steps / plain) if plain else None,
+        "batched_over_plain": (sub_batched / plain) if plain else None,
+        "batched_over_plain_note": ("like for like: the same first plain_worlds worlds, captured graph "
+                                    "replays against one world per launch; batched_full_over_plain is the "
+                                    "whole ensemble's value over the same plain figure") if plain else None,
+        "batched_full_over_plain": (cell_steps / plain) if plain else None,
         "all_worlds_accepted": all_ok,
         "parity_max_rel_err": parity,
         "parity_worlds": args.parity_worlds if parity is not None else None,

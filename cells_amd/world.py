@@ -350,6 +350,23 @@ def head(p: Packed, k: int) -> Packed:
     )
 
 
+def worlds(p: Packed, a: int, b: int) -> Packed:
+    """Worlds [a, b) of a packed ensemble as an ensemble of their own (the same parameters)."""
+    if not 0 <= a < b <= p.W:
+        raise ValueError("need 0 <= a < b <= W")
+    c0, c1 = int(p.cell_off[a]), int(p.cell_off[b])
+    e0, e1 = int(p.edge_off[a]), int(p.edge_off[b])
+    return dataclasses.replace(
+        p, W=b - a, cell_off=p.cell_off[a:b + 1] - c0, edge_off=p.edge_off[a:b + 1] - e0,
+        vol0=p.vol0[c0:c1], cap=p.cap[c0:c1], pam=np.ascontiguousarray(p.pam[:, c0:c1]),
+        nak_cap=p.nak_cap[c0:c1], ca_cap=p.ca_cap[c0:c1], n0=np.ascontiguousarray(p.n0[:, c0:c1]),
+        q0=p.q0[c0:c1], lpa=p.lpa[c0:c1], imperm=p.imperm[c0:c1], snd=p.snd[e0:e1] - c0,
+        rcv=p.rcv[e0:e1] - c0, paj=np.ascontiguousarray(p.paj[:, e0:e1]),
+        inc_ptr=p.inc_ptr[c0:c1 + 1] - p.inc_ptr[c0], inc_edge=p.inc_edge[2 * e0:2 * e1] - e0,
+        inc_out=p.inc_out[c0:c1].copy(),
+    )
+
+
 def take_state(p: Packed, y: np.ndarray, k: int) -> np.ndarray:
     """The state of the first k worlds (head(p, k)'s layout) out of a state of p."""
     L = StateLayout.of(p)
