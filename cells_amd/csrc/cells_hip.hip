@@ -1764,22 +1764,26 @@ __global__ void k_commit(int w0, int nw, int* sel, int* alive, int* valid) {
   valid[w] = 1;
 }
 
-// out[field, item] = the selected buffer's value; threads over cells, edges, worlds.
+// out[field, item] = the selected buffer's value; threads over cells, edges, worlds. Items whose
+// selected buffer is out itself are left as they are.
 __global__ void k_gather(Params P, const double* A, const double* B, const int* sel, double* out,
                          const int* alive, int* ok_out) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < P.ncell) {
     int i = (int)t;
     const double* y = current(A, B, sel[P.cell_world[i]]);
-    for (int f = 0; f < P.cf; ++f) CF(P, out, f, i) = CFc(P, y, f, i);
+    if (y != out)
+      for (int f = 0; f < P.cf; ++f) CF(P, out, f, i) = CFc(P, y, f, i);
   } else if (t < (long)P.ncell + P.nedge) {
     int e = (int)(t - P.ncell);
     const double* y = current(A, B, sel[P.edge_world[e]]);
-    for (int f = 0; f < P.ef; ++f) EF(P, out, f, e) = EFc(P, y, f, e);
+    if (y != out)
+      for (int f = 0; f < P.ef; ++f) EF(P, out, f, e) = EFc(P, y, f, e);
   } else if (t < (long)P.ncell + P.nedge + P.W) {
     int w = (int)(t - P.ncell - P.nedge);
     const double* y = current(A, B, sel[w]);
-    for (int f = 0; f < P.wf; ++f) WF(P, out, f, w) = WFc(P, y, f, w);
+    if (y != out)
+      for (int f = 0; f < P.wf; ++f) WF(P, out, f, w) = WFc(P, y, f, w);
     if (ok_out) ok_out[w] = alive[w];
   }
 }
@@ -2312,22 +2316,26 @@ struct cells_ensemble {
   // the adjoint workspace (allocated at the first cells_step_vjp; laid out by adjoint_layout)
   double* adj_base = nullptr;
   hipStream_t capture_stream = nullptr;
-  hipGraphExec_t graph = nullptr;
-  // what the captured step graph bakes in: it is replayed only while all of it is unchanged
+  // what a captured step graph bakes in: one is replayed only while all of it is unchanged
   struct GraphKey {
     double dt = 0.0;
     Ctl ctl{};
     bool checks = false;
     double check_atol = 0.0, res_atol = 0.0, res_rtol = 0.0;
+    const double* A = nullptr;  // the first state buffer (the caller's output buffer)
     bool operator==(const GraphKey& o) const {
-      return dt == o.dt && ctl.membrane == o.ctl.membrane && ctl.junction == o.ctl.junction &&
+      return dt == o.dt && A == o.A && ctl.membrane == o.ctl.membrane && ctl.junction == o.ctl.junction &&
              ctl.pump == o.ctl.pump && ctl.calcium_pump == o.ctl.calcium_pump && ctl.water == o.ctl.water &&
              checks == o.checks && check_atol == o.check_atol && res_atol == o.res_atol &&
              res_rtol == o.res_rtol;
     }
-  } graph_key;
+  };
+  // the captured step graphs, most recently used first (a stepping loop whose output buffers
+  // alternate between a few addresses replays one graph per address)
+  static constexpr size_t kGraphs = 4;
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   ~cells_ensemble() {
-    if (graph) (void)hipGraphExecDestroy(graph);
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
     for (void* p : allocations) (void)hipFree(p);
     for (void* p : tape_allocations) (void)hipFree(p);
@@ -3124,7 +3132,9 @@ void launch_factor(cells_ensemble* e, hipStream_t st, int w0, int nw, double coe
                        e->ubuf, e->bigbuf, e->boffw, e->valid);
 }
 
-int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int world,
+// A, B: the state buffer pair (a world's current state is A or B by sel[w]; the step writes every
+// field of the other).
+int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int world, double* A, double* B,
                 KernelTimer* tm = nullptr) {
   auto mark = [&](const char* name) {
     if (tm) tm->mark(name);
@@ -3137,23 +3147,23 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
   int ed0 = (int)e->edge_off[w0], ed1 = (int)e->edge_off[w0 + nw];
   int nc = c1 - c0, ne = ed1 - ed0;
   if (P.water) {
-    hipLaunchKernelGGL(k_water, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, 0, 0.5 * dt, c, e->valid);
+    hipLaunchKernelGGL(k_water, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, w0, 0, 0.5 * dt, c, e->valid);
     mark("k_water");
   }
   int vol_next = P.water ? 1 : 0;
   if (e->method == CELLS_METHOD_EULER) {
     if (ne) {
-      hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+      hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, nullptr, vol_next,
                          ed0, ed1, c, e->K1, nullptr, nullptr, e->valid);
       mark("k_edges");
     }
-    hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+    hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, nullptr, vol_next,
                        c0, c1, c, e->K1, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, dt, e->valid);
     mark("k_cells");
-    hipLaunchKernelGGL(k_euler_update, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
+    hipLaunchKernelGGL(k_euler_update, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
                        ed0, ne, dt, e->K1, e->valid);
     mark("k_euler_update");
-    hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, e->contrib1,
+    hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, w0, e->contrib1,
                        1.0 * dt, nullptr, 0.0, e->valid);
     mark("k_world_ionic");
   } else {
@@ -3161,11 +3171,11 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     const double coef = gamma * dt;
     size_t lds = sizeof(double) * (size_t)std::max(e->max_n, 1);
     if (ne) {
-      hipLaunchKernelGGL(k_edges<true>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+      hipLaunchKernelGGL(k_edges<true>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, nullptr, vol_next,
                          ed0, ed1, c, e->K1, e->gj, e->Gj, e->valid);
       mark("k_edges");
     }
-    hipLaunchKernelGGL(k_cells<true>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, vol_next,
+    hipLaunchKernelGGL(k_cells<true>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, nullptr, vol_next,
                        c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, 0.0, e->valid);
     mark("k_cells");
     launch_factor(e, st, w0, nw, coef);
@@ -3175,16 +3185,16 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
                        e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds, solve_args(e), part);
   });
     mark("k_solve_stage");
-    hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
+    hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->Gj, e->dphi, e->res_atol, e->res_rtol, e->K1,
                        e->contrib1, e->Y1, e->valid);
     mark("k_ros2_stage1");
     if (ne) {
-      hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, ed0,
+      hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, e->Y1, 0, ed0,
                          ed1, c, e->K2, nullptr, nullptr, e->valid);
       mark("k_edges");
     }
-    hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, c0, c1,
+    hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, e->Y1, 0, c0, c1,
                        c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, e->valid);
     mark("k_cells");
     solve_stage(e, st, nw, e->fvals, [&](int part) {
@@ -3192,20 +3202,20 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
                        e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds, solve_args(e), part);
   });
     mark("k_solve_stage");
-    hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, c0, nc,
+    hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->Gj, e->dphi2, e->res_atol, e->res_rtol, e->K1, e->K2,
                        e->contrib1, e->contrib2, e->valid);
     mark("k_ros2_final");
-    hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, e->contrib1,
+    hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, w0, e->contrib1,
                        1.5 * dt, e->contrib2, 0.5 * dt, e->valid);
     mark("k_world_ionic");
   }
   if (P.water) {
-    hipLaunchKernelGGL(k_water, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, w0, 1, 0.5 * dt, c, e->valid);
+    hipLaunchKernelGGL(k_water, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, w0, 1, 0.5 * dt, c, e->valid);
     mark("k_water");
   }
   if (e->checks_on) {
-    hipLaunchKernelGGL(k_step_checks, dim3(nw), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->alive, e->valid,
+    hipLaunchKernelGGL(k_step_checks, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, e->alive, e->valid,
                        e->step_counter, e->check_atol, w0, e->check_bits);
     mark("k_step_checks");
   }
@@ -3227,10 +3237,13 @@ extern "C" int cells_step(cells_ensemble* e, void* stream, const double* y_in, d
   hipStream_t st = static_cast<hipStream_t>(stream);
   Ctl c{control->membrane, control->junction, control->pump, control->calcium_pump, control->water};
   const Params& P = e->P;
-  HIP_TRY(hipMemcpyAsync(e->A, y_in, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
-  // The second buffer starts as a copy too: fields a step leaves alone (water fields outside a
-  // water world, world fields) must hold the current values in both.
-  HIP_TRY(hipMemcpyAsync(e->B, y_in, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
+  // The caller's output buffer is the pair's first state buffer (in place when y_out == y_in),
+  // the ensemble's own the second: every step writes every field of a world's next state, so the
+  // second needs no copy, and at the end only the worlds whose current state is the second one
+  // are copied out.
+  double* A = y_out;
+  double* B = e->B;
+  if (y_out != y_in) HIP_TRY(hipMemcpyAsync(A, y_in, sizeof(double) * P.total, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_init, dim3(blocks(e->W)), dim3(kBlock), 0, st, e->W, active, e->sel, e->alive, e->valid);
   HIP_TRY(hipMemsetAsync(e->check_bits, 0, sizeof(int) * e->W, st));
   if (e->checks_on) {
@@ -3247,30 +3260,41 @@ extern "C" int cells_step(cells_ensemble* e, void* stream, const double* y_in, d
     key.check_atol = e->checks_on ? e->check_atol : 0.0;
     key.res_atol = e->res_atol;
     key.res_rtol = e->res_rtol;
-    bool same = e->graph && e->graph_key == key;
-    if (!same && nsteps > 0) {
-      if (e->graph) {
-        HIP_TRY(hipGraphExecDestroy(e->graph));
-        e->graph = nullptr;
-      }
+    key.A = A;
+    auto& cache = e->graphs;
+    size_t hit = cache.size();
+    for (size_t q = 0; q < cache.size(); ++q)
+      if (cache[q].first == key) hit = q;
+    if (hit == cache.size() && nsteps > 0) {
       hipGraph_t g = nullptr;
+      hipGraphExec_t exec = nullptr;
       HIP_TRY(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal));
-      launch_step(e, e->capture_stream, dt, c, -1);
+      launch_step(e, e->capture_stream, dt, c, -1, A, B);
       hipError_t cap = hipStreamEndCapture(e->capture_stream, &g);
       if (cap != hipSuccess) return fail(CELLS_EHIP, std::string("step capture: ") + hipGetErrorString(cap));
-      hipError_t inst = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+      hipError_t inst = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (inst != hipSuccess) return fail(CELLS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
-      e->graph_key = key;
+      if (cache.size() == cells_ensemble::kGraphs) {  // the least recently used one goes
+        HIP_TRY(hipDeviceSynchronize());  // it may still run on some stream
+        HIP_TRY(hipGraphExecDestroy(cache.back().second));
+        cache.pop_back();
+      }
+      cache.insert(cache.begin(), {key, exec});
+      hit = 0;
+    } else if (hit < cache.size() && hit > 0) {
+      std::rotate(cache.begin(), cache.begin() + hit, cache.begin() + hit + 1);
+      hit = 0;
     }
-    for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(e->graph, st));
+    if (nsteps > 0)
+      for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(cache[hit].second, st));
   } else {
     for (int k = 0; k < nsteps; ++k)
-      for (int w = 0; w < e->W; ++w) launch_step(e, st, dt, c, w);
+      for (int w = 0; w < e->W; ++w) launch_step(e, st, dt, c, w, A, B);
   }
   HIP_TRY(hipGetLastError());
   long items = (long)P.ncell + P.nedge + P.W;
-  hipLaunchKernelGGL(k_gather, dim3(blocks(items)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, y_out, e->alive,
+  hipLaunchKernelGGL(k_gather, dim3(blocks(items)), dim3(kBlock), 0, st, P, A, B, e->sel, y_out, e->alive,
                      ok_out);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
@@ -3294,7 +3318,7 @@ extern "C" int cells_kernel_times(cells_ensemble* e, void* stream, const double*
     KernelTimer tm;
     tm.stream = st;
     tm.mark("start");
-    launch_step(e, st, dt, c, -1, &tm);
+    launch_step(e, st, dt, c, -1, e->A, e->B, &tm);
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipGetLastError());
     if (r == 0) {
@@ -3673,15 +3697,18 @@ extern "C" int cells_step_record(cells_ensemble* e, void* stream, const double* 
   int rc = check_adjoint(e, dt);
   if (rc) return rc;
   if (slot < 0 || slot >= e->tape_slots) return fail(CELLS_EINVAL, "tape slot out of range");
-  rc = cells_step(e, stream, y_in, y_out, dt, 1, control, nullptr, ok_out, 1);
-  if (rc) return rc;
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(CELLS_EINVAL, "dt must be positive and finite");
+  if ((rc = check_step_size(e, dt))) return rc;
+  HIP_TRY(hipSetDevice(e->device));
   hipStream_t st = static_cast<hipStream_t>(stream);
   const Params& P = e->P;
   size_t q = slot, nc = e->ncell, ne = std::max(e->nedge, 1), S = e->S, tot = P.total;
   auto copy = [&](double* dst, const double* src, size_t count) {
     return hipMemcpyAsync(dst + q * count, src, sizeof(double) * count, hipMemcpyDeviceToDevice, st);
   };
-  HIP_TRY(copy(e->t_y, y_in, tot));
+  HIP_TRY(copy(e->t_y, y_in, tot));  // before the step, which may run in place (y_out == y_in)
+  rc = cells_step(e, stream, y_in, y_out, dt, 1, control, nullptr, ok_out, 1);
+  if (rc) return rc;
   HIP_TRY(copy(e->t_y1, e->Y1, tot));
   HIP_TRY(copy(e->t_fv, e->fvals, e->factor_total));
   HIP_TRY(copy(e->t_msc, e->msc, nc));

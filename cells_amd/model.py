@@ -11,7 +11,11 @@ a device ensemble, with the reference's meanings:
   not (models/surface_pump_v3.py:3934-3935), with ok per world, never raising inside the step;
 - the runner's error on a refused step (execution/runner.py:505-510) is `run(..., refusals=
   "raise")`, and the vector windows' "world" policy (execution/vector_windows.py:20-26), where a
-  failing world stops and the others continue, is `refusals="world"`.
+  failing world stops and the others continue, is `refusals="world"`;
+- `pack_states` / `unpack_states` map the reference's per-world state pytrees (the leaves of
+  STATE_TYPE by name: models/surface_tissue.py:31-44, surface_kinetic.py:36-44,
+  surface_pump_v2.py:44-50, water's operators/osmotic_water.py:76-83) to and from the packed
+  vector, world by world, with the names and axes the checkpoints use.
 """
 
 from __future__ import annotations
@@ -71,3 +75,43 @@ def potentials(state, p: Layout):
     y = state.cpu().numpy() if hasattr(state, "cpu") else np.asarray(state)
     q = pk.q0 + 1e-15 * F * sum(pk.valence[s] * L.cell(y, L.dn + s) for s in range(pk.S))
     return pk.config.bath_V + q / pk.cap
+
+
+def unpack_states(packed: Packed, y) -> list:
+    """Each world's state pytree (a dict of the reference's leaf names to arrays, with its axes)
+    cut out of the packed state y."""
+    from cells_amd.snapshot import leaves
+
+    yh = y.cpu().numpy() if hasattr(y, "cpu") else np.asarray(y)
+    return [leaves(packed, yh, k) for k in range(packed.W)]
+
+
+def pack_states(packed: Packed, states) -> np.ndarray:
+    """The packed state of per-world pytrees (one dict of leaf names per world, in world order, as
+    unpack_states gives them). Every leaf the packed layout holds must be present with its shape;
+    the leaves the lattice worlds have no slots for (faces, channels, finite outside nodes) must
+    be empty or zero."""
+    from cells_amd.snapshot import leaves, place
+
+    if len(states) != packed.W:
+        raise ValueError(f"need one state per world ({packed.W}), got {len(states)}")
+    L = StateLayout.of(packed)
+    y = np.zeros(L.total)
+    for k, st in enumerate(states):
+        want = leaves(packed, y, k)
+        missing = sorted(set(want) - set(st))
+        if missing:
+            raise ValueError(f"world {k}: missing state leaves {missing}")
+        arrays = {}
+        for name, template in want.items():
+            a = np.asarray(st[name], dtype=np.float64)
+            if a.shape != template.shape:
+                raise ValueError(f"world {k}: leaf {name} has shape {a.shape}, expected {template.shape}")
+            arrays[name] = a
+        for name in ("extracellular_amount_increment", "face_transferred", "face_dissipated_energy_pJ",
+                     "channel_gate", "channel_transferred", "channel_dissipated_energy_pJ",
+                     "extracellular_volume_increment_pL"):
+            if name in arrays and np.any(arrays[name] != 0):
+                raise ValueError(f"world {k}: leaf {name} has no slots in a lattice on a held bath")
+        place(packed, y, k, arrays)
+    return y

@@ -126,7 +126,10 @@ int64_t cells_state_size(const cells_ensemble* e);
  * lax.map when batched = 0: one world per launch). y_in, y_out: device [state size].
  * active, ok_out: device int32 [W]. A world stops at its first refused step and keeps its last
  * accepted state (refusals="world"); ok_out[w] = 0 then. batched = 1 replays the captured
- * step graph. Stream-ordered on `stream` (a hipStream_t, NULL for the null stream). */
+ * step graph. Stream-ordered on `stream` (a hipStream_t, NULL for the null stream). The steps run
+ * on y_out itself (with one ensemble-owned buffer beside it): y_out may equal y_in (in place, no
+ * copy of the state) or must not overlap it (one copy in). A new y_out pointer captures the step
+ * graph again. */
 int cells_step(cells_ensemble* e, void* stream, const double* y_in, double* y_out, double dt,
                int32_t nsteps, const cells_control* control, const int32_t* active,
                int32_t* ok_out, int32_t batched);

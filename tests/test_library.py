@@ -68,3 +68,12 @@ def test_spatial_worlds_without_a_gpu_are_an_error():
     fp.cell_node[1] = 0  # a cell on the held ring
     g = lib.field_grid_struct(fp)
     assert L.cells_field_create(ctypes.byref(w), ctypes.byref(g), 50, 0, ctypes.byref(h)) == lib.CELLS_EINVAL
+
+
+def test_the_packed_fp32_switch_reaches_the_device_pass():
+    """The build's guard: the probe kernel has packed FP32 instructions without the switch and
+    none with it (gfx950 assembly of code built here)."""
+    from cells_amd import build
+
+    assert build.packed_fp32_count([]) > 0
+    assert build.packed_fp32_count(build.NO_PACKED_FP32) == 0
