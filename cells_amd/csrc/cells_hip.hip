@@ -119,6 +119,8 @@ struct Params {
   const int* world_order;
   // per world: refusal reasons of the current cells_step call (kRefuse* bits), or-ed atomically
   int* refuse_bits;
+  // per cell: its position in its world's solve vector (the world's first cell plus its rank)
+  const int* cell_rank;
   // state layout
   int cf, ef, wf;
   int f_dn, f_mtr, f_mdiss, f_pcyc, f_pdiss, f_pwork, f_ccyc, f_cdiss, f_cwork, f_dvol, f_wtr, f_wdiss;
@@ -171,6 +173,7 @@ struct PlanOptions {
   int relax_zeros = kRelaxZeros; // lattice plans: explicit zeros per joined column
   int grid_relax_zeros = 3;      // grid plans: explicit zeros per joined column
   int grid_subtrees = 32;        // grid plans: fronts at the depth where the tree is cut (at least)
+  int widen_groups = 1;          // lattice plans: a level of few fronts takes wider lane groups
 };
 PlanOptions g_plan;
 
@@ -182,7 +185,15 @@ constexpr int max_order(int tri) {
 }
 constexpr int kMfBatch = 8;                                // global loads in flight per lane
 constexpr int kRecInts = 16;                               // ints per front record in the plan pool
-constexpr int kLevelBounds = 6;                            // per level: first front of each kind, end
+constexpr int kLevelBounds = 6;                            // grid plans, per level: first front of each kind, end
+
+// The lattice plans' lane groups: a front is held by the smallest group whose share of the
+// factor's LDS holds its packed triangle (1, 4, 8, 16, 32 or 64 lanes, then the workgroup with the
+// front in LDS or in HBM scratch), so that a level's many small fronts spread over many groups
+// instead of queueing on a few wide ones.
+constexpr int kLatClasses = 8;
+constexpr int kLatBounds = kLatClasses + 1;  // lattice plans, per level: first front of each class, end
+constexpr int kLatGroupTri[6] = {16, 64, 128, 256, 512, kMfBlockWaveTri};  // factor LDS per group (doubles)
 
 // The world a workgroup of a per-world launch takes: by size, largest first, when the launch
 // covers every world.
@@ -337,7 +348,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
                                                   int vol_next, int c0, int c1, Ctl c, double* R,
                                                   double* contrib, double* gm, const double* Gj,
                                                   double coef, double* mdiag, double* msc,
-                                                  double rate_dt, int* valid) {
+                                                  double rate_dt, double* yrhs, const double* K1,
+                                                  int* valid) {
   int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c1) return;
   int w = P.cell_world[i];
@@ -419,6 +431,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
       else jin[s] += t;
     }
   }
+  double qrate = 0.0;  // the charge rate of the amounts' rates (less 2 K1's with K1: stage 2)
   for (int s = 0; s < P.S; ++s) {
     double cell = -Jm[s];
     cell += jout[s];
@@ -428,6 +441,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     CF(P, R, P.f_mtr + s, i) = Jm[s];
     contrib[(long)s * P.ncell + i] = -(Jm[s] + extra[s]);
     ok = ok && isfinite(r);
+    if (yrhs) {
+      const double v = K1 ? r + -2.0 * CFc(P, K1, P.f_dn + s, i) : r;
+      qrate = s == 0 ? P.z[0] * v : qrate + P.z[s] * v;
+    }
   }
   double mdiss = 1e-3 * diss;
   CF(P, R, P.f_mdiss, i) = mdiss;
@@ -446,13 +463,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     CF(P, R, P.f_wtr, i) = 0.0;
     CF(P, R, P.f_wdiss, i) = 0.0;
   }
+  double sc_i = 0.0;
   if (slopes) {
     double d = P.cap[i] + coef * g;
     ok = ok && d > 0;
     double sc = 1.0 / sqrt(d > 0 ? d : 1.0);
     mdiag[i] = (d * sc * sc + d * sc * sc) / 2;
     msc[i] = sc;
+    sc_i = sc;
   }
+  // the stage solve's right-hand side, scaled, at the cell's place in its world's solve vector
+  // (the solve then starts at once instead of gathering it in rank order)
+  // (yrhs set: msc is the stage's Jacobi scales, written here with slopes, read here without)
+  if (yrhs) yrhs[P.cell_rank[i]] = (kChargePerFmol * qrate) * (slopes ? sc_i : msc[i]);
   if (rate_dt > 0.0) {
     // the explicit stage's membrane charge relaxation (models/surface_pump_v3.py:4588-4638
     // membrane_rate with one patch per cell and no kinds): F^2/RT sum_s z^2 (A P)_s max(c_in, c_out)
@@ -1027,7 +1050,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(8, 8
                                                             const double* fvals, const double* K,
                                                             const double* Kminus2, double* dphi,
                                                             double* ysc, int y_in_lds, SolveArgs sa,
-                                                            int part) {
+                                                            int part, int rhs_ready) {
   extern __shared__ double lds[];
   int w = launch_world(P, w0);
   const ShapeDev s = shapes[shape_of[w]];
@@ -1035,20 +1058,35 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(8, 8
   const double* fv = fvals + foff[w];
   int c0 = P.cell_off[w];
   double* y = y_in_lds ? lds : ysc + c0;
-  if (part != 2) {
-    for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
-      int i = c0 + v.order[k];
-      double q;
-      if (Kminus2) {
-        // the charge of K - 2 K2' field by field (the stage-2 right-hand side F1 - 2 k1)
-        double t2 = P.z[0] * (CFc(P, K, P.f_dn, i) + -2.0 * CFc(P, Kminus2, P.f_dn, i));
-        for (int sp = 1; sp < P.S; ++sp)
-          t2 = t2 + P.z[sp] * (CFc(P, K, P.f_dn + sp, i) + -2.0 * CFc(P, Kminus2, P.f_dn + sp, i));
-        q = kChargePerFmol * t2;
-      } else {
-        q = charge_increment(P, K, i);
+  if (part != 2 && !rhs_ready) {
+    // the right-hand side by rank: kRhsBatch ranks per thread at once, so that their scattered
+    // loads are in flight together (the stores come after every load of the batch)
+    constexpr int kRhsBatch = 2;
+    for (int k0 = threadIdx.x; k0 < s.n; k0 += kRhsBatch * (int)blockDim.x) {
+      int idx[kRhsBatch];
+#pragma unroll
+      for (int u = 0; u < kRhsBatch; ++u) {
+        const int k = k0 + u * (int)blockDim.x;
+        idx[u] = k < s.n ? c0 + v.order[k] : -1;
       }
-      y[k] = q * msc[i];
+      double q[kRhsBatch], sc[kRhsBatch];
+#pragma unroll
+      for (int u = 0; u < kRhsBatch; ++u) {
+        const int i = idx[u] < 0 ? c0 : idx[u];
+        if (Kminus2) {
+          // the charge of K - 2 K2' field by field (the stage-2 right-hand side F1 - 2 k1)
+          double t2 = P.z[0] * (CFc(P, K, P.f_dn, i) + -2.0 * CFc(P, Kminus2, P.f_dn, i));
+          for (int sp = 1; sp < P.S; ++sp)
+            t2 = t2 + P.z[sp] * (CFc(P, K, P.f_dn + sp, i) + -2.0 * CFc(P, Kminus2, P.f_dn + sp, i));
+          q[u] = kChargePerFmol * t2;
+        } else {
+          q[u] = charge_increment(P, K, i);
+        }
+        sc[u] = msc[i];
+      }
+#pragma unroll
+      for (int u = 0; u < kRhsBatch; ++u)
+        if (idx[u] >= 0) y[k0 + u * (int)blockDim.x] = q[u] * sc[u];
     }
   }
   if (part == 0) {
@@ -1148,8 +1186,10 @@ template <int G>
 struct FrontLanes {
   static constexpr int RW = G < 64 ? G : 64;
   static constexpr int CG = G / RW;
-  static constexpr int rows = G == 1 ? max_order(kMfThreadTri) : G == kMfOcta ? max_order(kMfOctaTri)
-                              : G == 64 ? max_order(kMfWaveTri) : kMfMaxRows;
+  static constexpr int rows = G == 1 ? max_order(kMfThreadTri) : G == 4 ? max_order(kLatGroupTri[1])
+                              : G == kMfOcta ? max_order(kMfOctaTri) : G == 16 ? max_order(kLatGroupTri[3])
+                              : G == 32 ? max_order(kLatGroupTri[4]) : G == 64 ? max_order(kMfWaveTri)
+                              : kMfMaxRows;
   static constexpr int NT = (rows + RW - 1) / RW;  // rows per lane
 };
 
@@ -1337,31 +1377,50 @@ __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double
   group_sync<G>();
 }
 
+// One class of a level's fronts in the factor: fronts [q0, q1) of the plan, a front per group of G
+// lanes, each group's front at base + group * TRI (G = kMfThreads: one front at a time, at base).
+// The thread index is made opaque here for the reason sv_class gives.
+template <int G, int TRI, int MR, bool ASM>
+__device__ __forceinline__ void mf_class(const int* pool, int qa0, int q0, int q1, double* fv, double* ub,
+                                         double* base, int* valid_w, const MfAssembly* am) {
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  if constexpr (G == 1) {
+    for (int q = q0 + tid; q < q1; q += kMfThreads)
+      mf_front<1, kMfThreads, MR, ASM>(pool, qa0 + q, fv, ub, base + tid, 0, valid_w, am);
+  } else if constexpr (G == kMfThreads) {
+    for (int q = q0; q < q1; ++q) mf_front<kMfThreads, 1, MR, ASM>(pool, qa0 + q, fv, ub, base, tid, valid_w, am);
+  } else {
+    for (int q = q0 + tid / G; q < q1; q += kMfThreads / G)
+      mf_front<G, 1, MR, ASM>(pool, qa0 + q, fv, ub, base + (tid / G) * TRI, tid % G, valid_w, am);
+  }
+}
+
 // The factor of one world; fronts in HBM scratch hold at most BIGROWS rows.
 template <int BIGROWS = kMfMaxRows, bool ASM = false>
 __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub, double* lds,
                                 double* big, int* valid_w, const MfAssembly* am = nullptr) {
   const int* lev = pool + s.o_snlev;
   const int qa0 = (int)(s.o_sn / kRecInts);
-  const int tid = threadIdx.x, wave = tid >> 6, octa = tid / kMfOcta;
   for (int L = 0; L < s.sn_levels; ++L) {
-    // thread, 8-lane, wavefront, workgroup (LDS) and workgroup (HBM scratch) fronts of level L
-    const int* bd = lev + kLevelBounds * L;
-    for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
-      mf_front<1, kMfThreads, FrontLanes<1>::rows, ASM>(pool, qa0 + q, fv, ub, lds + tid, 0, valid_w, am);
+    // the fronts of level L by class: groups of 1, 4, 8, 16, 32 and 64 lanes (each group's share of
+    // the LDS region), the workgroup with its front in LDS, the workgroup in HBM scratch
+    const int* bd = lev + kLatBounds * L;
+    mf_class<1, kMfThreadTri, FrontLanes<1>::rows, ASM>(pool, qa0, bd[0], bd[1], fv, ub, lds, valid_w, am);
     __syncthreads();
-    for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      mf_front<kMfOcta, 1, FrontLanes<kMfOcta>::rows, ASM>(pool, qa0 + q, fv, ub, lds + octa * kMfOctaTri,
-                                                           tid % kMfOcta, valid_w, am);
+    mf_class<4, kLatGroupTri[1], FrontLanes<4>::rows, ASM>(pool, qa0, bd[1], bd[2], fv, ub, lds, valid_w, am);
     __syncthreads();
-    for (int q = bd[2] + wave; q < bd[3]; q += kMfThreads / 64)
-      mf_front<64, 1, max_order(kMfBlockWaveTri), ASM>(pool, qa0 + q, fv, ub, lds + wave * kMfBlockWaveTri, tid & 63, valid_w,
-                                                 am);
+    mf_class<kMfOcta, kMfOctaTri, FrontLanes<kMfOcta>::rows, ASM>(pool, qa0, bd[2], bd[3], fv, ub, lds, valid_w, am);
     __syncthreads();
-    for (int q = bd[3]; q < bd[4]; ++q)
-      mf_front<kMfThreads, 1, FrontLanes<kMfThreads>::rows, ASM>(pool, qa0 + q, fv, ub, lds, tid, valid_w, am);
-    for (int q = bd[4]; q < bd[5]; ++q)
-      mf_front<kMfThreads, 1, BIGROWS, ASM>(pool, qa0 + q, fv, ub, big, tid, valid_w, am);
+    mf_class<16, kLatGroupTri[3], FrontLanes<16>::rows, ASM>(pool, qa0, bd[3], bd[4], fv, ub, lds, valid_w, am);
+    __syncthreads();
+    mf_class<32, kLatGroupTri[4], FrontLanes<32>::rows, ASM>(pool, qa0, bd[4], bd[5], fv, ub, lds, valid_w, am);
+    __syncthreads();
+    mf_class<64, kMfBlockWaveTri, max_order(kMfBlockWaveTri), ASM>(pool, qa0, bd[5], bd[6], fv, ub, lds, valid_w,
+                                                                   am);
+    __syncthreads();
+    mf_class<kMfThreads, 0, FrontLanes<kMfThreads>::rows, ASM>(pool, qa0, bd[6], bd[7], fv, ub, lds, valid_w, am);
+    mf_class<kMfThreads, 0, BIGROWS, ASM>(pool, qa0, bd[7], bd[8], fv, ub, big, valid_w, am);
   }
   __syncthreads();
 }
@@ -1379,8 +1438,9 @@ __device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, 
 // workgroup fronts alike, order <= kMfMaxRows).
 template <int G>
 struct SolveLanes {
-  static constexpr int rows = G == 1 ? max_order(kMfThreadTri) : G == kMfOcta ? max_order(kMfOctaTri)
-                              : kMfMaxRows;
+  static constexpr int rows = G == 1 ? max_order(kMfThreadTri) : G == 4 ? max_order(kLatGroupTri[1])
+                              : G == kMfOcta ? max_order(kMfOctaTri) : G == 16 ? max_order(kLatGroupTri[3])
+                              : G == 32 ? max_order(kLatGroupTri[4]) : kMfMaxRows;
   static constexpr int NT = (rows + G - 1) / G;
 };
 static_assert(FrontLanes<1>::NT * 1 >= max_order(kMfThreadTri) && FrontLanes<kMfThreads>::NT * 64 >= kMfMaxRows,
@@ -1391,6 +1451,15 @@ constexpr int kSvThreadLds = kMfThreads * SolveLanes<1>::NT;
 constexpr int kSvOctaLds = (kMfThreads / kMfOcta) * kMfOcta * SolveLanes<kMfOcta>::NT;
 constexpr int kSvWaveLds = (kMfThreads / 64) * 64 * SolveLanes<64>::NT;
 constexpr int kSvLds = kSvThreadLds + kSvOctaLds + kSvWaveLds;
+// The lattice solve: each wavefront runs its groups' fronts of a level class after class (the
+// wavefront's lanes reconverge between classes), so every class of one wavefront shares one
+// region, sized for the largest: 64 thread fronts, 16 four-lane, 8 eight-lane, 4 sixteen-lane,
+// 2 thirty-two-lane fronts or one wavefront front.
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kLatSvWave = cmax(cmax(64 * SolveLanes<1>::NT, 64 * SolveLanes<4>::NT),
+                                cmax(cmax(64 * SolveLanes<kMfOcta>::NT, 64 * SolveLanes<16>::NT),
+                                     cmax(64 * SolveLanes<32>::NT, 64 * SolveLanes<64>::NT)));
+constexpr int kLatSvLds = (kMfThreads / 64) * kLatSvWave;
 
 template <int RW>
 __device__ __forceinline__ double group_shfl(double v, int src) {
@@ -1501,32 +1570,58 @@ __device__ void sv_backward_front(const int* pool, int qa, const double* __restr
   group_sync<G>();
 }
 
-// The solve of one world's y (by rank) on its supernodal factor; blockDim.x == kMfThreads.
+// One class of a level's fronts in the solve (fronts [q0, q1) of the plan, a front per group of G
+// lanes; G = 64 also takes the workgroup classes' fronts). The thread index is made opaque here,
+// so that no value of one class is computed once above the level loop and held in registers
+// through every other class (which made the kernel spill).
+template <int G, bool FORWARD, int UNROLL>
+__device__ __forceinline__ void sv_class(const int* pool, int qa0, int q0, int q1, const double* __restrict__ fv,
+                                         double* __restrict__ y, double* __restrict__ vec, double* R) {
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63;
+  if constexpr (G == 1) {
+    for (int q = q0 + tid; q < q1; q += kMfThreads) {
+      if constexpr (FORWARD) sv_forward_front<1, 64>(pool, qa0 + q, fv, y, vec, R + lane, 0);
+      else sv_backward_front<1, 64>(pool, qa0 + q, fv, y, R + lane, 0);
+    }
+    wave_sync();
+  } else {
+    double* F = R + (lane / G) * (G * SolveLanes<G>::NT);
+    for (int q = q0 + tid / G; q < q1; q += kMfThreads / G) {
+      if constexpr (FORWARD) sv_forward_front<G, 1>(pool, qa0 + q, fv, y, vec, F, tid % G);
+      else sv_backward_front<G, 1, UNROLL>(pool, qa0 + q, fv, y, F, tid % G);
+    }
+  }
+}
+
+// The solve of one world's y (by rank) on its supernodal factor; blockDim.x == kMfThreads, LDS
+// kLatSvLds doubles. A wavefront takes its share of each class of a level in turn (one front per
+// group of 1, 4, 8, 16, 32 or 64 of its lanes; the workgroup classes' fronts go to wavefronts too,
+// up to kMfMaxRows rows), in one region of its own.
 __device__ void sv_solve_world(const ShapeDev& s, const int* pool, const double* fv, double* y, double* vec,
                                double* lds) {
   const int* lev = pool + s.o_snlev;
   const int qa0 = (int)(s.o_sn / kRecInts);
-  const int tid = threadIdx.x, wave = tid >> 6, octa = tid / kMfOcta;
-  double* Fq = lds + kSvThreadLds + octa * (kMfOcta * SolveLanes<kMfOcta>::NT);
-  double* Fw = lds + kSvThreadLds + kSvOctaLds + wave * (64 * SolveLanes<64>::NT);
+  double* R = lds + (threadIdx.x >> 6) * kLatSvWave;
   for (int L = 0; L < s.sn_levels; ++L) {
-    const int* bd = lev + kLevelBounds * L;
-    for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
-      sv_forward_front<1, kMfThreads>(pool, qa0 + q, fv, y, vec, lds + tid, 0);
-    for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      sv_forward_front<kMfOcta, 1>(pool, qa0 + q, fv, y, vec, Fq, tid % kMfOcta);
-    for (int q = bd[2] + wave; q < bd[5]; q += kMfThreads / 64)
-      sv_forward_front<64, 1>(pool, qa0 + q, fv, y, vec, Fw, tid & 63);
+    const int* bd = lev + kLatBounds * L;
+    sv_class<1, true, 1>(pool, qa0, bd[0], bd[1], fv, y, vec, R);
+    sv_class<4, true, 1>(pool, qa0, bd[1], bd[2], fv, y, vec, R);
+    sv_class<kMfOcta, true, 1>(pool, qa0, bd[2], bd[3], fv, y, vec, R);
+    sv_class<16, true, 1>(pool, qa0, bd[3], bd[4], fv, y, vec, R);
+    sv_class<32, true, 1>(pool, qa0, bd[4], bd[5], fv, y, vec, R);
+    sv_class<64, true, 1>(pool, qa0, bd[5], bd[8], fv, y, vec, R);
     __syncthreads();
   }
   for (int L = s.sn_levels - 1; L >= 0; --L) {
-    const int* bd = lev + kLevelBounds * L;
-    for (int q = bd[0] + tid; q < bd[1]; q += kMfThreads)
-      sv_backward_front<1, kMfThreads>(pool, qa0 + q, fv, y, lds + tid, 0);
-    for (int q = bd[1] + octa; q < bd[2]; q += kMfThreads / kMfOcta)
-      sv_backward_front<kMfOcta, 1, 4>(pool, qa0 + q, fv, y, Fq, tid % kMfOcta);
-    for (int q = bd[2] + wave; q < bd[5]; q += kMfThreads / 64)
-      sv_backward_front<64, 1, 8>(pool, qa0 + q, fv, y, Fw, tid & 63);
+    const int* bd = lev + kLatBounds * L;
+    sv_class<1, false, 1>(pool, qa0, bd[0], bd[1], fv, y, vec, R);
+    sv_class<4, false, 4>(pool, qa0, bd[1], bd[2], fv, y, vec, R);
+    sv_class<kMfOcta, false, 4>(pool, qa0, bd[2], bd[3], fv, y, vec, R);
+    sv_class<16, false, 8>(pool, qa0, bd[3], bd[4], fv, y, vec, R);
+    sv_class<32, false, 8>(pool, qa0, bd[4], bd[5], fv, y, vec, R);
+    sv_class<64, false, 8>(pool, qa0, bd[5], bd[8], fv, y, vec, R);
     __syncthreads();
   }
 }
@@ -2380,12 +2475,26 @@ double host_logaddexp0(double t) {
 // wavefront holds at most kMfMaxRows rows).
 int32_t front_class(int32_t rows) {
   int64_t tri = (int64_t)rows * (rows + 1) / 2;
-  if (tri <= kMfThreadTri) return 0;
-  if (tri <= kMfOctaTri) return 1;
-  if (tri <= kMfBlockWaveTri) return 2;
-  if (tri <= kMfBlockLds) return 3;
-  if (rows <= kMfMaxRows) return 4;
-  return 5;
+  for (int c = 0; c < 6; ++c)
+    if (tri <= kLatGroupTri[c]) return c;
+  if (tri <= kMfBlockLds) return 6;
+  if (rows <= kMfMaxRows) return 7;
+  return 8;
+}
+
+// A level's fronts fill the workgroup's lanes: a lane-group front of a level with F fronts takes at
+// least the widest group of which the workgroup holds F (so a level of few fronts runs them on
+// wide groups, one front per group, and only crowded levels use the narrow ones).
+int32_t widen_class(int32_t cls, int32_t level_fronts) {
+  if (cls > 5 || level_fronts < 1 || !g_plan.widen_groups) return cls;
+  static const int32_t lanes[6] = {1, 4, 8, 16, 32, 64};
+  int32_t wide = 0;
+  for (int c = 5; c >= 0; --c)
+    if ((int64_t)level_fronts * lanes[c] <= kMfThreads) {
+      wide = c;
+      break;
+    }
+  return std::max(cls, wide);
 }
 
 int check_world(const cells_world* w) {
@@ -2850,7 +2959,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
               }
             }
             e->shapes_host[q] = std::move(s);
-            e->sn_host[q] = cells::supernodes(e->shapes_host[q], front_class);
+            e->sn_host[q] = cells::supernodes(e->shapes_host[q], front_class, widen_class);
             cells::share_update_storage(e->sn_host[q]);
           } catch (const std::exception& ex) {
             bad_msg[q] = ex.what();
@@ -2945,7 +3054,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       while (pool.size() % kRecInts) pool.push_back(0);  // records on 64-byte boundaries
       d.o_sn = (long)pool.size();
       const long o_lev = d.o_sn + (long)kRecInts * ns;
-      long o_child = o_lev + (long)kLevelBounds * sn.levels;
+      long o_child = o_lev + (long)kLatBounds * sn.levels;
       o_child += (4 - o_child % 4) % 4;
       const long o_rel = o_child + 4L * (long)sn.child.size();
       // each front's junction entries: (front-local packed position, world-local junction)
@@ -2965,8 +3074,8 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       for (int t = 0; t < ns; ++t) fstart[t + 1] = fstart[t] + (long)fedges[t].size();
       if (o_fedge + fstart[ns] > 0x7fffffffL) return fail(CELLS_EINVAL, "plans exceed 2^31 entries");
       for (int t = 0; t < ns; ++t) {
-        if (front_class(sn.rows[t]) > 4) return fail(CELLS_EINVAL, "a front of order above 128");
-        if (front_class(sn.rows[t]) == 4)
+        if (sn.cls[t] > 7) return fail(CELLS_EINVAL, "a front of order above 128");
+        if (sn.cls[t] == 7)
           d.big_tri = std::max(d.big_tri, sn.rows[t] * (sn.rows[t] + 1) / 2);
         const int64_t last = sn.first[t] + sn.width[t] - 1;
         int32_t r16[kRecInts] = {(int32_t)s.column_start[sn.first[t]], sn.width[t], sn.rows[t],
@@ -2979,14 +3088,14 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       d.sn_levels = sn.levels;
       d.o_snlev = (long)pool.size();
       {
-        std::vector<int> bounds(kLevelBounds * (size_t)sn.levels, 0);
+        std::vector<int> bounds(kLatBounds * (size_t)sn.levels, 0);
         int t = 0;
         for (int L = 0; L < sn.levels; ++L) {
-          for (int c = 0; c < kLevelBounds - 1; ++c) {
-            bounds[kLevelBounds * L + c] = t;
-            while (t < ns && sn.level[t] == L && front_class(sn.rows[t]) == c) ++t;
+          for (int c = 0; c < kLatBounds - 1; ++c) {
+            bounds[kLatBounds * L + c] = t;
+            while (t < ns && sn.level[t] == L && sn.cls[t] == c) ++t;
           }
-          bounds[kLevelBounds * L + kLevelBounds - 1] = t;
+          bounds[kLatBounds * L + kLatBounds - 1] = t;
           if (t < ns && sn.level[t] == L) return fail(CELLS_EINVAL, "front schedule out of order");
         }
         pool.insert(pool.end(), bounds.begin(), bounds.end());
@@ -3015,6 +3124,18 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       e->factor_lds = sizeof(double) * (size_t)std::max<int64_t>(most, 1);
       e->y_in_lds = e->pairwise && (size_t)e->max_n * sizeof(double) <= kSolveLdsBytes ? 1 : 0;
       e->solve_lds = e->y_in_lds ? sizeof(double) * (size_t)std::max(e->max_n, 1) : 0;
+    }
+    {
+      // each cell's place in its world's solve vector: the world's first cell plus the cell's rank
+      std::vector<int> cell_rank(e->ncell);
+      for (int k = 0; k < e->W; ++k) {
+        const cells::Symbolic& sy = e->shapes_host[e->shape_of_host[k]];
+        const int c0 = (int)w->cell_off[k];
+        for (int i = 0; i < sy.n; ++i) cell_rank[c0 + i] = c0 + sy.rank[i];
+      }
+      int* d_rank = nullptr;
+      if ((rc = upload(e.get(), &d_rank, cell_rank.data(), cell_rank.size()))) return rc;
+      P.cell_rank = d_rank;
     }
     e->foff_host.resize(e->W + 1);
     e->foff_host[0] = 0;
@@ -3110,7 +3231,7 @@ struct KernelTimer {
 // The solve kernels' workgroup, LDS and mode.
 dim3 solve_block(const cells_ensemble* e) { return dim3(e->pairwise ? kFactorBlock : kMfThreads); }
 size_t solve_lds_bytes(const cells_ensemble* e) {
-  return e->pairwise ? e->solve_lds : sizeof(double) * (size_t)kSvLds;
+  return e->pairwise ? e->solve_lds : sizeof(double) * (size_t)kLatSvLds;
 }
 SolveArgs solve_args(const cells_ensemble* e) { return SolveArgs{e->pairwise ? 0 : 1, e->voffw, e->vbuf}; }
 
@@ -3158,7 +3279,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
       mark("k_edges");
     }
     hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, nullptr, vol_next,
-                       c0, c1, c, e->K1, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, dt, e->valid);
+                       c0, c1, c, e->K1, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, dt, nullptr, nullptr,
+                       e->valid);
     mark("k_cells");
     hipLaunchKernelGGL(k_euler_update, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
                        ed0, ne, dt, e->K1, e->valid);
@@ -3175,14 +3297,19 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
                          ed0, ed1, c, e->K1, e->gj, e->Gj, e->valid);
       mark("k_edges");
     }
+    // the stage solves' right-hand sides written by the rate kernels into the solve vector (with
+    // the vector in HBM; the reference-order mode's small worlds keep it in LDS and gather it)
+    double* yrhs = e->y_in_lds ? nullptr : e->ysc;
     hipLaunchKernelGGL(k_cells<true>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, nullptr, vol_next,
-                       c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, 0.0, e->valid);
+                       c0, c1, c, e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, 0.0, yrhs, nullptr,
+                       e->valid);
     mark("k_cells");
     launch_factor(e, st, w0, nw, coef);
     mark("k_factor");
     solve_stage(e, st, nw, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds, solve_args(e), part);
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K1, nullptr, e->dphi, e->ysc, e->y_in_lds, solve_args(e), part,
+                       yrhs ? 1 : 0);
   });
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
@@ -3194,12 +3321,15 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
                          ed1, c, e->K2, nullptr, nullptr, e->valid);
       mark("k_edges");
     }
+    // stage 2 reads the stage-1 Jacobi scales (msc) for its right-hand side; it writes neither msc
+    // nor mdiag (no slopes)
     hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, e->Y1, 0, c0, c1,
-                       c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, e->valid);
+                       c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, e->msc, 0.0, yrhs, e->K1, e->valid);
     mark("k_cells");
     solve_stage(e, st, nw, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
-                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds, solve_args(e), part);
+                       e->pool, e->foff, w0, e->msc, e->fvals, e->K2, e->K1, e->dphi2, e->ysc, e->y_in_lds, solve_args(e), part,
+                       yrhs ? 1 : 0);
   });
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
@@ -3353,7 +3483,8 @@ extern "C" int cells_rates(cells_ensemble* e, void* stream, const double* y,
     hipLaunchKernelGGL(k_edges<false>, dim3(blocks(e->nedge)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
                        e->nedge, c, rates_out, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_cells<false>, dim3(blocks(e->ncell)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0,
-                     e->ncell, c, rates_out, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, e->valid);
+                     e->ncell, c, rates_out, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr,
+                     nullptr, e->valid);
   hipLaunchKernelGGL(k_world_rates, dim3(e->W), dim3(kBlock), 0, st, P, e->contrib1, rates_out);
   if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * e->W, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipGetLastError());
@@ -3581,11 +3712,11 @@ int step_vjp(cells_ensemble* e, void* stream, const double* y, const double* yba
     hipLaunchKernelGGL(k_edges<true>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, ne,
                        c, e->K1, e->gj, e->Gj, e->valid);
   hipLaunchKernelGGL(k_cells<true>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, nullptr, 0, 0, nc, c,
-                     e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, 0.0, e->valid);
+                     e->K1, e->contrib1, e->gm, e->Gj, coef, e->mdiag, e->msc, 0.0, nullptr, nullptr, e->valid);
   launch_factor(e, st, 0, W, coef);
   solve_stage(e, st, W, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, a.d1, e->ysc, e->y_in_lds, solve_args(e), part);
+                     e->foff, 0, e->msc, e->fvals, e->K1, nullptr, a.d1, e->ysc, e->y_in_lds, solve_args(e), part, 0);
   });
   hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
                      ne, dt, coef, e->gm, e->gj, e->Gj, a.d1, e->res_atol, e->res_rtol, e->K1, e->contrib1, e->Y1,
@@ -3594,10 +3725,10 @@ int step_vjp(cells_ensemble* e, void* stream, const double* y, const double* yba
     hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, ne, c,
                        e->K2, nullptr, nullptr, e->valid);
   hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, e->Y1, 0, 0, nc, c,
-                     e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, e->valid);
+                     e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
   solve_stage(e, st, W, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
-                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, a.d2, e->ysc, e->y_in_lds, solve_args(e), part);
+                     e->foff, 0, e->msc, e->fvals, e->K2, e->K1, a.d2, e->ysc, e->y_in_lds, solve_args(e), part, 0);
   });
   StepRecord r{e->A, e->Y1, e->fvals, e->msc, e->gm, e->gj, a.d1, a.d2};
   if ((rc = reverse_sweep(e, st, r, a, ybar_out, dt, c, ybar_in, pam_bar, cap_bar, paj_bar))) return rc;

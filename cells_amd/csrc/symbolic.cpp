@@ -181,7 +181,8 @@ int64_t position(const Symbolic& s, int32_t i, int32_t j) {
   return it - s.row_rank.begin();
 }
 
-Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows)) {
+Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows),
+                      int32_t (*widen)(int32_t cls, int32_t level_fronts)) {
   const int32_t n = s.n;
   auto col_len = [&](int32_t j) { return static_cast<int32_t>(s.column_start[j + 1] - s.column_start[j]); };
   std::vector<int32_t> nchild(n, 0);
@@ -208,8 +209,15 @@ Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows)) {
   // schedule: by level, then class, then first column
   std::vector<int32_t> sched(ns);
   for (int32_t t = 0; t < ns; ++t) sched[t] = t;
-  std::vector<int32_t> fcls(ns);
-  for (int32_t t = 0; t < ns; ++t) fcls[t] = cls(col_len(f0[t]));
+  std::vector<int32_t> fcls(ns), per_level;
+  for (int32_t t = 0; t < ns; ++t) {
+    if (flev[t] >= (int32_t)per_level.size()) per_level.resize(flev[t] + 1, 0);
+    per_level[flev[t]]++;
+  }
+  for (int32_t t = 0; t < ns; ++t) {
+    fcls[t] = cls(col_len(f0[t]));
+    if (widen) fcls[t] = widen(fcls[t], per_level[flev[t]]);
+  }
   std::sort(sched.begin(), sched.end(), [&](int32_t a, int32_t b) {
     if (flev[a] != flev[b]) return flev[a] < flev[b];
     if (fcls[a] != fcls[b]) return fcls[a] < fcls[b];
@@ -224,8 +232,10 @@ Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows)) {
   out.level.resize(ns);
   out.parent.resize(ns);
   out.update_offset.resize(ns);
+  out.cls.resize(ns);
   for (int32_t q = 0; q < ns; ++q) {
     int32_t t = sched[q];
+    out.cls[q] = fcls[t];
     out.first[q] = f0[t];
     out.width[q] = fw[t];
     out.rows[q] = col_len(f0[t]);

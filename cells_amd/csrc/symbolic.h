@@ -34,6 +34,7 @@ struct Supernodes {
   std::vector<int32_t> level;               // supernodal tree level
   std::vector<int32_t> parent;              // schedule index of the parent, -1 at a root
   std::vector<int64_t> update_offset;       // start of the packed |S| x |S| update matrix
+  std::vector<int32_t> cls;                 // the front's class (the device's lane group)
   std::vector<int32_t> child_ptr, child;    // children (schedule indices, ascending first column)
   std::vector<int32_t> rel_ptr, rel;        // per supernode: its rows S as positions in the
                                             // parent's front rows
@@ -41,9 +42,11 @@ struct Supernodes {
   int32_t levels = 0;
 };
 
-// Supernodes of s, scheduled by level; within a level the fronts are ordered by `cls`, a class
-// of the front order (the device groups fronts of one class).
-Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows));
+// Supernodes of s, scheduled by level; within a level the fronts are ordered by their class, `cls`
+// of the front order (the device groups fronts of one class), or, with `widen`, widen(cls(rows),
+// the number of fronts at the front's level): a class may depend on how many fronts share its level.
+Supernodes supernodes(const Symbolic& s, int32_t (*cls)(int32_t rows),
+                      int32_t (*widen)(int32_t cls, int32_t level_fronts) = nullptr);
 
 // Update matrices sharing storage: a supernode's update is written when its level runs and read
 // when its parent's level runs, so it lives over the levels [level, parent's level]; updates whose

@@ -12,7 +12,20 @@
 
 static int32_t front_class(int32_t rows) {
   const int64_t tri = (int64_t)rows * (rows + 1) / 2;
-  return tri <= 16 ? 0 : tri <= 128 ? 1 : tri <= 1280 ? 2 : tri <= 5120 ? 3 : 4;
+  return tri <= 16 ? 0 : tri <= 64 ? 1 : tri <= 128 ? 2 : tri <= 256 ? 3 : tri <= 512 ? 4 : tri <= 1280 ? 5
+         : tri <= 5120 ? 6 : 7;
+}
+
+static int32_t widen_class(int32_t cls, int32_t level_fronts) {
+  if (cls > 5 || level_fronts < 1) return cls;
+  static const int32_t lanes[6] = {1, 4, 8, 16, 32, 64};
+  int32_t wide = 0;
+  for (int c = 5; c >= 0; --c)
+    if ((int64_t)level_fronts * lanes[c] <= 256) {
+      wide = c;
+      break;
+    }
+  return cls > wide ? cls : wide;
 }
 
 int main(int argc, char** argv) {
@@ -30,7 +43,7 @@ int main(int argc, char** argv) {
       return 2;
     }
     const cells::Symbolic s = cells::bind(n, m, rows.data(), cols.data(), order.data());
-    cells::Supernodes sn = cells::supernodes(s, front_class);
+    cells::Supernodes sn = cells::supernodes(s, front_class, widen_class);
     const int64_t total = sn.update_total;
     cells::share_update_storage(sn);
     const int32_t ns = static_cast<int32_t>(sn.first.size());
