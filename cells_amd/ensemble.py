@@ -153,11 +153,12 @@ class Ensemble:
         return x
 
     def device_checks(self, previous, y, steps, material_atol=1e-12):
-        """(failed bits [W], ratios [W, 4]) of a state against its predecessor."""
+        """(failed bits [W], ratios [W, len(lib.LEDGER_RATIOS)]) of a state against its predecessor:
+        the bits of lib.CHECKS, and per ledger of lib.LEDGER_RATIOS its largest |residual| / bound."""
         t = self.torch
         a, b = self.to_device(previous), self.to_device(y)
         failed = t.empty(self.packed.W, dtype=t.int32, device=self.device)
-        ratios = t.empty((self.packed.W, 4), dtype=t.float64, device=self.device)
+        ratios = t.empty((self.packed.W, len(_lib.LEDGER_RATIOS)), dtype=t.float64, device=self.device)
         _lib.check(_lib.load().cells_device_checks(
             self._h, self._stream(), ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
             float(steps), float(material_atol), ctypes.c_void_p(failed.data_ptr()),

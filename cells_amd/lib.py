@@ -44,6 +44,16 @@ EXPORTS = (
 REFUSE_RESIDUAL = 1 << 16
 REFUSE_EXPLICIT_RATE = 1 << 17
 
+# the device row checks' failure bits (cells_device_checks), the reference's check names
+CHECKS = {
+    "domain": 1, "local": 2, "total": 4, "global_charge": 8, "dissipation": 16, "water": 32,
+    "outside": 64, "charge": 128, "electrode": 256, "energy": 512, "reservoir_work": 1024,
+    "electrode_work": 2048, "metabolic_work": 4096, "bath_water_work": 8192,
+}
+# the ledgers of the bound ratios, in their column order (CELLS_RATIO_*)
+LEDGER_RATIOS = ("local", "total", "global_charge", "water", "outside", "charge", "electrode",
+                 "energy", "reservoir_work", "electrode_work", "metabolic_work", "bath_water_work")
+
 
 class StationaryOptions(ctypes.Structure):
     _fields_ = [("tolerance", ctypes.c_double), ("max_newton", ctypes.c_int32),

@@ -23,6 +23,7 @@ import time
 
 import numpy as np
 
+from cells_amd import lib as _lib
 from cells_amd.world import F, StateLayout
 
 
@@ -180,7 +181,7 @@ def run_windowed(ensemble, y0, dt, steps, policy: WindowPolicy, ctl=None, checkp
     observations["final"] = cell_potentials(packed, y)
     _, ratios = ensemble.device_checks(first_state, y, float(max(at - start, 1)), material_atol)
     r = ratios.cpu().numpy()
-    accounting = {k: float(r[:, i].max()) for i, k in enumerate(("material", "total", "charge", "water"))}
+    accounting = {k: float(r[:, i].max()) for i, k in enumerate(_lib.LEDGER_RATIOS)}
     timings["windows"] = window
     timings["elapsed_seconds"] = time.perf_counter() - began
     return RunResult(Checkpoint(y, counts, dt), refused, observations, accounting, timings)

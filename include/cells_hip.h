@@ -149,11 +149,33 @@ int cells_rates(cells_ensemble* e, void* stream, const double* y, const cells_co
                 double* rates_out, int32_t* valid_out);
 
 /* Device row checks of a state against its predecessor (models/surface_pump_checks.py:112
- * row_checks; bounds models/surface_pump.py:752-800): failed_out[w] bits 1 domain, 2 material
- * ledger per cell and species, 4 material ledger per world (cells vs bath supply), 8 global charge
- * (electrode vs cells), 16 dissipation negative or decreasing, 32 water ledger;
- * ratios_out[4 w + k] = largest |residual| / bound of ledger k (material, total, charge, water).
- * steps: accepted steps behind y (the rounding allowance 2.5 eps steps x magnitudes). */
+ * row_checks over models/surface_pump.py:685-696's LEDGERS with the water operator's,
+ * operators/osmotic_water.py:240; bounds models/surface_pump.py:752-800 ledger_limits): the bits
+ * of failed_out[w] name the failed checks (the reference's check names in parentheses), and
+ * ratios_out[CELLS_LEDGER_RATIOS w + k] is the largest |residual| / bound of ledger k (CELLS_RATIO_*).
+ * steps: accepted steps behind y (the rounding allowance 2.5 eps steps x magnitudes); the energy
+ * ledger's relative bound is IMEX Euler's 2e-2 (surface_pump.py:63) for both methods. */
+#define CELLS_CHECK_DOMAIN 1               /* domain */
+#define CELLS_CHECK_LOCAL 2                /* local: material per cell and species */
+#define CELLS_CHECK_TOTAL 4                /* total: cells against the bath supplies */
+#define CELLS_CHECK_GLOBAL_CHARGE 8        /* global_charge: electrode and supply charge */
+#define CELLS_CHECK_DISSIPATION 16         /* dissipation negative or decreasing */
+#define CELLS_CHECK_WATER 32               /* water_local, water_outside, water_total */
+#define CELLS_CHECK_OUTSIDE 64             /* outside: the held bath's material */
+#define CELLS_CHECK_CAPACITOR 128          /* charge: each capacitor's node charge */
+#define CELLS_CHECK_ELECTRODE 256          /* electrode: electrode charge against plate charge */
+#define CELLS_CHECK_ENERGY 512             /* energy */
+#define CELLS_CHECK_RESERVOIR_WORK 1024    /* reservoir_work */
+#define CELLS_CHECK_ELECTRODE_WORK 2048    /* electrode_work */
+#define CELLS_CHECK_METABOLIC_WORK 4096    /* metabolic_work (Na/K, then Ca) */
+#define CELLS_CHECK_BATH_WATER_WORK 8192   /* bath_water_work */
+#define CELLS_LEDGER_RATIOS 12
+enum {
+    CELLS_RATIO_LOCAL, CELLS_RATIO_TOTAL, CELLS_RATIO_GLOBAL_CHARGE, CELLS_RATIO_WATER,
+    CELLS_RATIO_OUTSIDE, CELLS_RATIO_CAPACITOR, CELLS_RATIO_ELECTRODE, CELLS_RATIO_ENERGY,
+    CELLS_RATIO_RESERVOIR_WORK, CELLS_RATIO_ELECTRODE_WORK, CELLS_RATIO_METABOLIC_WORK,
+    CELLS_RATIO_BATH_WATER_WORK
+};
 int cells_device_checks(cells_ensemble* e, void* stream, const double* previous, const double* y,
                         double steps, double material_atol, int32_t* failed_out,
                         double* ratios_out);

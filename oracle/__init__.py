@@ -192,6 +192,33 @@ def rates(p, y, ctl=None, world=None):
     return r, valid
 
 
+# IMEX Euler's energy bound (models/surface_pump.py:63); ROS2, which the reference lacks, takes it
+# too (its stiff start books energy errors above ARS(2,2,2)'s 2e-3 in the first steps)
+ENERGY_RTOL = {"euler": 2e-2, "ros2": 2e-2}
+LEDGER_RATIOS = ("local", "total", "global_charge", "water", "outside", "charge", "electrode",
+                 "energy", "reservoir_work", "electrode_work", "metabolic_work", "bath_water_work")
+ENERGY_TERMS = ("mixing", "capacitor", "heat", "work", "water", "residual", "flows")
+
+
+def ledgers(p, prev, y, steps, method, material_atol=1e-12, world=None):
+    """The row checks of every world's state y against prev: (bits [W], ratios [W, 12] in
+    LEDGER_RATIOS order, energy terms [W, 7] in ENERGY_TERMS order)."""
+    w = world if world is not None else world_struct(p)
+    prev = np.ascontiguousarray(prev, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    bits = np.zeros(p.W, dtype=np.int64)
+    ratios = np.zeros((p.W, len(LEDGER_RATIOS)))
+    terms = np.zeros((p.W, len(ENERGY_TERMS)))
+    for k in range(p.W):
+        r = np.zeros(len(LEDGER_RATIOS))
+        t = np.zeros(len(ENERGY_TERMS))
+        bits[k] = lib().ref_ledgers(ctypes.byref(w), k, _p(prev, _f64p), _p(y, _f64p),
+                                    ctypes.c_double(steps), ctypes.c_double(material_atol),
+                                    ctypes.c_double(ENERGY_RTOL[method]), _p(r, _f64p), _p(t, _f64p))
+        ratios[k], terms[k] = r, t
+    return bits, ratios, terms
+
+
 def world_symbolics(p):
     """oracle.symbolic.bind of every world's lattice pattern, and their structs."""
     from oracle.symbolic import bind

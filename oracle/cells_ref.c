@@ -539,6 +539,201 @@ static int water_half(const cells_world* w, const cells_state_layout* L, int32_t
  * F^2/RT sum_s z_s^2 (A P)_s max(c_in, c_out) times the membrane availability, over the cell's
  * capacitance. The step refuses where dt times it exceeds EXPLICIT_STEP_BOUND = 1
  * (surface_pump_v3.py:3640-3642). */
+/* ---------------------------------------------------------------- ledgers */
+
+#define LEDGER_STEP_ROUNDING 2.5        /* surface_pump.py:113 */
+#define COMPONENT_STEP_ROUNDING 2.5     /* surface_field.py:70 */
+#define GLOBAL_CHARGE_ATOL_C 3e-25      /* surface_pump.py:96 */
+#define GLOBAL_CHARGE_ROUNDING (256.0 * 2.220446049250313e-16) /* surface_pump.py:97 */
+#define CHARGE_ATOL_C 3e-25             /* surface_field.py:47 */
+#define CHARGE_RTOL 2e-12               /* surface_field.py:48 */
+#define ENERGY_ATOL_PJ 2e-8             /* surface_tissue.py:29 */
+#define WORK_FLOOR_PJ 3e-12             /* surface_pump.py:789-791 */
+#define VOLUME_ATOL_PL 2e-12            /* osmotic_water.py:70 */
+#define WATER_WORK_ATOL_PJ 3e-12        /* osmotic_water.py:73 */
+#define DISSIPATION_ATOL 1e-12          /* surface_pump.py:865-866 */
+
+static double ledger_ratio(double residual, double bound) {
+    double m = fabs(residual);
+    if (m == 0.0) return 0.0;
+    double r = m / bound;
+    return isnan(r) ? INFINITY : r;
+}
+
+int ref_ledgers(const cells_world* w, int32_t k, const double* prev, const double* y, double steps,
+                double material_atol, double energy_rtol, double* ratios, double* terms) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const int S = w->S;
+    const double RT = R_CONST * w->temperature, eps = 2.220446049250313e-16;
+    const double rounding = LEDGER_STEP_ROUNDING * eps * steps;
+    const double drive = nak_drive(w);
+    const double ca_drive = w->ca_dg0 + RT * (log(w->adp) + log(w->pi) - log(w->atp) -
+                                              log(STANDARD_STATE_mM));
+    double r_local = 0, r_water = 0, r_cap = 0, r_met = 0;
+    double tot[4] = {0}, totm[4] = {0}, hm[4] = {0}, hmm[4] = {0};
+    double qsum = 0, qmag = 0, plate_sum = 0;
+    double chem = 0, elec = 0, heat = 0, work = 0, water = 0, flows = 0;
+    double dvs = 0, dvm = 0, trs = 0, trm = 0;
+    int dissipation_ok = 1;
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        const double cyc = CF(&L, w, y, L.pcyc, i);
+        const double ccyc = w->calcium ? CF(&L, w, y, L.ccyc, i) : 0.0;
+        double zq = 0, zm = 0;
+        for (int s = 0; s < S; ++s) {
+            /* balances' local ledger (surface_pump_checks.py:42-47): dn + membrane total + the
+             * junction transfers out - in, the membrane total with the pumps' stoichiometry */
+            double net = 0, netm = 0;
+            for (int64_t q = w->inc_ptr[i]; q < w->inc_ptr[i + 1]; ++q) {
+                int64_t e = w->inc_edge[q];
+                double j = EF(&L, w, y, L.jtr + s, e);
+                if (q < w->inc_ptr[i] + w->inc_out[i]) net -= j;
+                else net += j;
+                netm += fabs(j);
+            }
+            const double dn = CF(&L, w, y, L.dn + s, i), mtr = CF(&L, w, y, L.mtr + s, i);
+            double pump = s == 0 ? 3.0 * cyc : s == 1 ? -2.0 * cyc : 0.0;
+            if (s == 3 && w->calcium) pump += ccyc;
+            const double h = mtr + pump, hmag = fabs(mtr) + fabs(pump);
+            double r = ledger_ratio(dn + h - net, material_atol + rounding * (fabs(dn) + hmag + netm));
+            if (!(r <= r_local)) r_local = r;
+            tot[s] += dn;
+            totm[s] += fabs(dn);
+            hm[s] += h;
+            hmm[s] += hmag;
+            zq += w->valence[s] * dn;
+            zm += fabs(w->valence[s] * dn);
+            /* the mixing free energy at the reference volume (surface_pump_checks.py:68-78) */
+            const double a = w->n0[(int64_t)s * w->ncell + i], v = w->vol0[i];
+            chem += 1e-3 * RT * (dn * (log(a / v) - 1.0) + (a + dn) * log1p(dn / a));
+        }
+        const double q = CHARGE_PER_FMOL * zq, C = w->cap[i], q0 = w->q0[i];
+        qsum += q;
+        qmag += CHARGE_PER_FMOL * zm;
+        /* the diagonal capacitor on the held bath (surface_tissue.py:358-393, balances' charge
+         * ledger, surface_pump_checks.py:56-63): dp = q / C, plate charge C dp */
+        const double dp = q / C, plate = C * dp;
+        const double image = C * ((w->bath_V + q0 / C) - w->bath_V);
+        double r = ledger_ratio(image - q0 + plate - q, CHARGE_ATOL_C + CHARGE_RTOL * fabs(q0 + q));
+        if (!(r <= r_cap)) r_cap = r;
+        plate_sum += plate;
+        /* the capacitor energy 0.5 C dv (2 V0 + dv) (surface_pump_checks.py:79-81) */
+        elec += 0.5e12 * (C * dp * (2.0 * (q0 / C) + dp));
+        const double md = CF(&L, w, y, L.mdiss, i), pd = CF(&L, w, y, L.pdiss, i);
+        const double pw = CF(&L, w, y, L.pwork, i);
+        heat += md + pd;
+        work += pw;
+        flows += fabs(md) + fabs(pd) + fabs(pw);
+        /* metabolic work against cycles x drive (surface_pump_v2.py:700-707) */
+        r = ledger_ratio(pw + 1e-3 * drive * cyc,
+                         WORK_FLOOR_PJ + rounding * (fabs(pw) + fabs(1e-3 * drive * cyc)));
+        if (!(r <= r_met)) r_met = r;
+        int heat_fields[4] = {L.mdiss, L.pdiss, L.cdiss, L.wdiss};
+        if (w->calcium) {
+            const double cd = CF(&L, w, y, L.cdiss, i), cw = CF(&L, w, y, L.cwork, i);
+            heat += cd;
+            work += cw;
+            flows += fabs(cd) + fabs(cw);
+            r = ledger_ratio(cw + 1e-3 * ca_drive * ccyc,
+                             WORK_FLOOR_PJ + rounding * (fabs(cw) + fabs(1e-3 * ca_drive * ccyc)));
+            if (!(r <= r_met)) r_met = r;
+        }
+        for (int f = 0; f < 4; ++f) {
+            const int fi = heat_fields[f];
+            if (fi < 0) continue;
+            const double v = CF(&L, w, y, fi, i), u = CF(&L, w, prev, fi, i);
+            if (v < -DISSIPATION_ATOL || v - u < -DISSIPATION_ATOL) dissipation_ok = 0;
+        }
+        if (w->water) {
+            /* the water ledgers and water's energy terms (osmotic_water.py:254-295) */
+            const double dv = CF(&L, w, y, L.dvol, i), tr = CF(&L, w, y, L.wtr, i);
+            const double wd = CF(&L, w, y, L.wdiss, i);
+            r = ledger_ratio(dv + tr, VOLUME_ATOL_PL + rounding * (fabs(dv) + fabs(tr)));
+            if (!(r <= r_water)) r_water = r;
+            dvs += dv;
+            dvm += fabs(dv);
+            trs += tr;
+            trm += fabs(tr);
+            double total = 0;
+            for (int s = 0; s < S; ++s)
+                total += w->n0[(int64_t)s * w->ncell + i] + CF(&L, w, y, L.dn + s, i);
+            total += w->imperm[i];
+            const double vr = w->vol0[i], stiff = w->water_K / vr;
+            const double pressure = stiff * (vr - vr) + w->water_P0;
+            water += -1e-3 * RT * (total * log1p(dv / vr)) + 1e-3 * (pressure * dv + 0.5 * stiff * dv * dv);
+            heat += wd;
+            flows += fabs(wd);
+        }
+    }
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        const double v = EF(&L, w, y, L.jdiss, e), u = EF(&L, w, prev, L.jdiss, e);
+        if (v < -DISSIPATION_ATOL || v - u < -DISSIPATION_ATOL) dissipation_ok = 0;
+        heat += v;
+        flows += fabs(v);
+    }
+    double r_tot = 0, r_out = 0, supq = 0, supm = 0, rw_expect = 0, rw_mag = 0;
+    for (int s = 0; s < S; ++s) {
+        const double sup = WF(&L, w, y, L.sup + s, k);
+        double r = ledger_ratio(tot[s] - sup, material_atol + rounding * (totm[s] + fabs(sup)));
+        if (!(r <= r_tot)) r_tot = r;
+        /* the held bath: its increment 0, less the membranes' totals, less its supply */
+        r = ledger_ratio(0.0 - hm[s] - sup, material_atol + rounding * (hmm[s] + fabs(sup)));
+        if (!(r <= r_out)) r_out = r;
+        supq += w->valence[s] * sup;
+        supm += fabs(w->valence[s] * sup);
+        const double mu = RT * log(w->bath_c[s]) + w->valence[s] * F_CONST * w->bath_V;
+        rw_expect += mu * sup;
+        rw_mag += fabs(mu) * fabs(sup);
+    }
+    const double echg = WF(&L, w, y, L.echg, k), rwork = WF(&L, w, y, L.rwork, k);
+    const double ework = WF(&L, w, y, L.ework, k);
+    const double moved = CHARGE_PER_FMOL * supm + fabs(echg);
+    const double grounded = COMPONENT_STEP_ROUNDING * eps * steps * qmag;
+    const double base = GLOBAL_CHARGE_ATOL_C + GLOBAL_CHARGE_ROUNDING * moved;
+    const double gbound = fmax(base + rounding * moved, base + grounded);
+    const double r_gq = ledger_ratio(echg + CHARGE_PER_FMOL * supq, gbound);
+    /* the electrode: its charge less the bath's plate charge, -sum C dp */
+    const double r_el = ledger_ratio(echg - -plate_sum, CHARGE_ATOL_C + CHARGE_RTOL * fabs(echg));
+    work += rwork + ework;
+    flows += fabs(rwork) + fabs(ework);
+    const double r_rw = ledger_ratio(rwork - 1e-3 * rw_expect,
+                                     WORK_FLOOR_PJ + rounding * (fabs(rwork) + 1e-3 * rw_mag));
+    const double ew_expect = 1e12 * w->bath_V * echg;
+    const double r_ew = ledger_ratio(ework - ew_expect, WORK_FLOOR_PJ + rounding * (fabs(ework) + fabs(ew_expect)));
+    double r_bw = 0;
+    if (w->water) {
+        const double ws = WF(&L, w, y, L.wsup, k), ww = WF(&L, w, y, L.wwork, k);
+        double r = ledger_ratio(dvs - ws, VOLUME_ATOL_PL + rounding * (dvm + fabs(ws)));
+        if (!(r <= r_water)) r_water = r;
+        r = ledger_ratio(ws + trs, VOLUME_ATOL_PL + rounding * (trm + fabs(ws)));
+        if (!(r <= r_water)) r_water = r;
+        const double bw = 1e-3 * water_psi_bath(w) * ws;
+        r_bw = ledger_ratio(ww - bw, WATER_WORK_ATOL_PJ + rounding * (fabs(ww) + fabs(bw)));
+        work += ww;
+        flows += fabs(ww);
+    }
+    const double energy = chem + elec + heat - work + water;
+    const double r_en = ledger_ratio(energy, ENERGY_ATOL_PJ + energy_rtol * flows);
+    const double rr[12] = {r_local, r_tot, r_gq, r_water, r_out, r_cap, r_el, r_en, r_rw, r_ew, r_met, r_bw};
+    const int bit[12] = {2, 4, 8, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192};
+    int bits = domain_valid(w, &L, k, y) ? 0 : 1;
+    if (!dissipation_ok) bits |= 16;
+    for (int j = 0; j < 12; ++j) {
+        ratios[j] = rr[j];
+        if (!(rr[j] <= 1.0)) bits |= bit[j];
+    }
+    if (terms) {
+        terms[0] = chem;
+        terms[1] = elec;
+        terms[2] = heat;
+        terms[3] = work;
+        terms[4] = water;
+        terms[5] = energy;
+        terms[6] = flows;
+    }
+    return bits;
+}
+
 double ref_membrane_rate(const cells_world* w, const double* y, int64_t i, const cells_control* c) {
     cells_state_layout L;
     ref_state_layout(w, &L);

@@ -90,6 +90,19 @@ int ref_step_euler(const cells_world* w, const double* y, double* out, double dt
  * refuses a world where dt times it exceeds 1 at any cell. */
 double ref_membrane_rate(const cells_world* w, const double* y, int64_t i, const cells_control* c);
 
+/* The device row checks of world k's state y against its predecessor prev
+ * (models/surface_pump_checks.py:112-192 row_checks; balances surface_pump_checks.py:34-109 with
+ * surface_pump_v2.py:686-709's pump terms and operators/osmotic_water.py:254-295's water terms;
+ * bounds surface_pump.py:752-800 ledger_limits and osmotic_water.py:341-344) for cells on one
+ * held bath. Returns the failed checks' bits (the values of include/cells_hip.h's CELLS_CHECK_*);
+ * ratios[12] receives each ledger's largest |residual| / bound in CELLS_RATIO_* order. energy_rtol:
+ * the energy ledger's relative bound (2e-2, IMEX Euler's). terms[7] (may be NULL) receives the energy ledger's parts,
+ * pJ: mixing free energy, capacitor energy, heat, work (metabolic, reservoir, electrode, bath
+ * water), water's dilution and elastic energy, the residual (mixing + capacitor + heat - work +
+ * water) and the flows its bound scales. */
+int ref_ledgers(const cells_world* w, int32_t k, const double* prev, const double* y, double steps,
+                double material_atol, double energy_rtol, double* ratios, double* terms);
+
 /* Sparse symbolic structure of one world's stage matrix (the product of oracle/symbolic.py's
  * bind): column_start [n+1] and each entry's row rank [nnz]; order [n]; plus, per lattice entry,
  * its factor position: diag_pos [n] (node order), edge_pos [E] (world edge order). */

@@ -12,6 +12,7 @@ cells_step with the ledger checks as each step's epilogue).
 import numpy as np
 import pytest
 
+from cells_amd import lib as hip
 from cells_amd.windows import NumericalError, WindowPolicy, cell_potentials, run_windowed
 from cells_amd.world import StateLayout, build, config1, initial_state
 
@@ -53,7 +54,8 @@ def test_windows_equal_one_call_and_observe_the_plain_run(torch, ens):
     s = res.summary(p)
     assert s["integrated_world_steps"] == 30 * p.W and abs(s["through_time_s"] - 0.15) < 1e-12
     assert len(s["final_voltage_V"]) == p.W
-    assert s["ionic_accounting"]["material"] <= 1.0
+    acc = s["ionic_accounting"]
+    assert set(acc) == set(hip.LEDGER_RATIOS) and all(v <= 1.0 for v in acc.values()), acc
 
 
 def test_resume_from_a_window_checkpoint(torch, ens):
