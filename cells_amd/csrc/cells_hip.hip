@@ -196,6 +196,16 @@ constexpr int kLevelBounds = 6;                            // grid plans, per le
 constexpr int kLatClasses = 8;
 constexpr int kLatBounds = kLatClasses + 1;  // lattice plans, per level: first front of each class, end
 constexpr int kLatGroupTri[6] = {16, 64, 128, 256, 512, kMfBlockWaveTri};  // factor LDS per group (doubles)
+constexpr int kLatGroupLanes[6] = {1, 4, 8, 16, 32, 64};
+// every class's groups together fit the workgroup's LDS region, which the classes share in turn
+constexpr bool lattice_classes_fit() {
+  for (int c = 0; c < 6; ++c)
+    if ((kMfThreads / kLatGroupLanes[c]) * kLatGroupTri[c] > kMfBlockLds) return false;
+  return max_order(kMfBlockLds) <= kMfMaxRows;
+}
+static_assert(lattice_classes_fit(), "a lattice class's lane groups outgrow the factor's LDS region");
+static_assert(kLatGroupTri[0] == kMfThreadTri && kLatGroupTri[2] == kMfOctaTri && kLatGroupLanes[2] == kMfOcta,
+              "the factor's thread and 8-lane classes take the plan's shares");
 
 // The world a workgroup of a per-world launch takes: by size, largest first, when the launch
 // covers every world.
@@ -1323,43 +1333,54 @@ __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double
   }
   // partial factor over the supernode's columns: lanes own rows i = jj + 1 + lr + t RW. Groups of
   // whole wavefronts work in blocks of kMfKB columns: rank-1 steps inside the block, then one
-  // rank-kMfKB update of everything to its right on the matrix cores.
+  // rank-kMfKB update of everything to its right on the matrix cores. In a workgroup's front the
+  // first wavefront alone takes the block's rank-1 steps (wavefront barriers only), so the
+  // workgroup meets twice per block instead of three times per column.
   constexpr int KB = G >= 64 ? kMfKB : 1 << 20;
+  constexpr bool WAVE_PANEL = G > 64;
+  constexpr int PCG = WAVE_PANEL ? 1 : CG;  // column groups of the rank-1 steps
   for (int jb = 0; jb < J; jb += KB) {
     const int je = J < jb + KB ? J : jb + KB;
     const int kend = G >= 64 ? je : m;  // columns the rank-1 steps update
-    for (int jj = jb; jj < je; ++jj) {
-      const int cb = colbase(jj, m);
-      const double d = sqrt(F[(cb + jj) * STRIDE]);
-      const double rinv = 1.0 / d;
-      double av[NT];
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int i = jj + 1 + lr + t * RW;
-        av[t] = i < m ? F[(cb + i) * STRIDE] * rinv : 0.0;
-      }
-      group_sync<G>();  // every lane has read the pivot and its rows
-      if (lc == 0) {
+    if (!WAVE_PANEL || lc == 0) {
+      for (int jj = jb; jj < je; ++jj) {
+        const int cb = colbase(jj, m);
+        const double d = sqrt(F[(cb + jj) * STRIDE]);
+        const double rinv = 1.0 / d;
+        double av[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           const int i = jj + 1 + lr + t * RW;
-          if (i < m) F[(cb + i) * STRIDE] = av[t];
+          av[t] = i < m ? F[(cb + i) * STRIDE] * rinv : 0.0;
         }
-      }
-      if (lane == 0) F[(cb + jj) * STRIDE] = d;
-      group_sync<G>();  // the scaled column is in place
+        if constexpr (WAVE_PANEL) wave_sync();
+        else group_sync<G>();  // every lane has read the pivot and its rows
+        if (lc == 0) {
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int i = jj + 1 + lr + t * RW;
+            if (i < m) F[(cb + i) * STRIDE] = av[t];
+          }
+        }
+        if (lane == 0) F[(cb + jj) * STRIDE] = d;
+        if constexpr (WAVE_PANEL) wave_sync();
+        else group_sync<G>();  // the scaled column is in place
+        const int pc = WAVE_PANEL ? 0 : lc;
 #pragma unroll 4
-      for (int k = jj + 1 + lc; k < kend; k += CG) {
-        const double bk = F[(cb + k) * STRIDE];
-        const int ck = colbase(k, m);
+        for (int k = jj + 1 + pc; k < kend; k += PCG) {
+          const double bk = F[(cb + k) * STRIDE];
+          const int ck = colbase(k, m);
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const int i = jj + 1 + lr + t * RW;
-          if (i >= k && i < m) F[(ck + i) * STRIDE] -= av[t] * bk;
+          for (int t = 0; t < NT; ++t) {
+            const int i = jj + 1 + lr + t * RW;
+            if (i >= k && i < m) F[(ck + i) * STRIDE] -= av[t] * bk;
+          }
         }
+        if constexpr (WAVE_PANEL) wave_sync();
+        else group_sync<G>();
       }
-      group_sync<G>();
     }
+    if constexpr (WAVE_PANEL) group_sync<G>();  // the block's columns are factored
     if constexpr (G >= 64) {
       if (je < m) {
         mf_trailing_update<G>(F, m, jb, je, lane);
@@ -1400,27 +1421,44 @@ __device__ __forceinline__ void mf_class(const int* pool, int qa0, int q0, int q
 
 // The factor of one world; fronts in HBM scratch hold at most BIGROWS rows.
 template <int BIGROWS = kMfMaxRows, bool ASM = false>
-__device__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub, double* lds,
-                                double* big, int* valid_w, const MfAssembly* am = nullptr) {
+__device__ __forceinline__ void mf_factor_world(const ShapeDev& s, const int* pool, double* fv, double* ub,
+                                                double* lds, double* big, int* valid_w,
+                                                const MfAssembly* am = nullptr) {
   const int* lev = pool + s.o_snlev;
   const int qa0 = (int)(s.o_sn / kRecInts);
   for (int L = 0; L < s.sn_levels; ++L) {
     // the fronts of level L by class: groups of 1, 4, 8, 16, 32 and 64 lanes (each group's share of
-    // the LDS region), the workgroup with its front in LDS, the workgroup in HBM scratch
+    // the LDS region), the workgroup with its front in LDS, the workgroup in HBM scratch. The bounds
+    // are the workgroup's alike, so an empty class skips its barrier (the LDS region and the
+    // update matrices are handed on at the barrier of the last class that ran).
     const int* bd = lev + kLatBounds * L;
-    mf_class<1, kMfThreadTri, FrontLanes<1>::rows, ASM>(pool, qa0, bd[0], bd[1], fv, ub, lds, valid_w, am);
-    __syncthreads();
-    mf_class<4, kLatGroupTri[1], FrontLanes<4>::rows, ASM>(pool, qa0, bd[1], bd[2], fv, ub, lds, valid_w, am);
-    __syncthreads();
-    mf_class<kMfOcta, kMfOctaTri, FrontLanes<kMfOcta>::rows, ASM>(pool, qa0, bd[2], bd[3], fv, ub, lds, valid_w, am);
-    __syncthreads();
-    mf_class<16, kLatGroupTri[3], FrontLanes<16>::rows, ASM>(pool, qa0, bd[3], bd[4], fv, ub, lds, valid_w, am);
-    __syncthreads();
-    mf_class<32, kLatGroupTri[4], FrontLanes<32>::rows, ASM>(pool, qa0, bd[4], bd[5], fv, ub, lds, valid_w, am);
-    __syncthreads();
-    mf_class<64, kMfBlockWaveTri, max_order(kMfBlockWaveTri), ASM>(pool, qa0, bd[5], bd[6], fv, ub, lds, valid_w,
-                                                                   am);
-    __syncthreads();
+    if (bd[0] < bd[1]) {
+      mf_class<1, kMfThreadTri, FrontLanes<1>::rows, ASM>(pool, qa0, bd[0], bd[1], fv, ub, lds, valid_w, am);
+      __syncthreads();
+    }
+    if (bd[1] < bd[2]) {
+      mf_class<4, kLatGroupTri[1], FrontLanes<4>::rows, ASM>(pool, qa0, bd[1], bd[2], fv, ub, lds, valid_w, am);
+      __syncthreads();
+    }
+    if (bd[2] < bd[3]) {
+      mf_class<kMfOcta, kMfOctaTri, FrontLanes<kMfOcta>::rows, ASM>(pool, qa0, bd[2], bd[3], fv, ub, lds, valid_w,
+                                                                    am);
+      __syncthreads();
+    }
+    if (bd[3] < bd[4]) {
+      mf_class<16, kLatGroupTri[3], FrontLanes<16>::rows, ASM>(pool, qa0, bd[3], bd[4], fv, ub, lds, valid_w, am);
+      __syncthreads();
+    }
+    if (bd[4] < bd[5]) {
+      mf_class<32, kLatGroupTri[4], FrontLanes<32>::rows, ASM>(pool, qa0, bd[4], bd[5], fv, ub, lds, valid_w, am);
+      __syncthreads();
+    }
+    if (bd[5] < bd[6]) {
+      mf_class<64, kMfBlockWaveTri, max_order(kMfBlockWaveTri), ASM>(pool, qa0, bd[5], bd[6], fv, ub, lds,
+                                                                     valid_w, am);
+      __syncthreads();
+    }
+    // the workgroup classes end each front at a barrier of their own
     mf_class<kMfThreads, 0, FrontLanes<kMfThreads>::rows, ASM>(pool, qa0, bd[6], bd[7], fv, ub, lds, valid_w, am);
     mf_class<kMfThreads, 0, BIGROWS, ASM>(pool, qa0, bd[7], bd[8], fv, ub, big, valid_w, am);
   }
