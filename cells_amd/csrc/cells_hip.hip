@@ -123,6 +123,9 @@ struct Params {
   int* refuse_bits;
   // per cell: its position in its world's solve vector (the world's first cell plus its rank)
   const int* cell_rank;
+  // the rate kernels' cell potentials when set (N2's stage rates at the operator's potentials),
+  // else each cell's from its state's charge
+  const double* phi;
   // state layout
   int cf, ef, wf;
   int f_dn, f_mtr, f_mdiss, f_pcyc, f_pdiss, f_pwork, f_ccyc, f_cdiss, f_cwork, f_dvol, f_wtr, f_wdiss;
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, con
   double ca[4], cb[4], va, vb;
   cell_concentrations(P, y, yv, a, ca, va);
   cell_concentrations(P, y, yv, b, cb, vb);
-  double dv = potential(P, y, a) - potential(P, y, b);
+  double dv = P.phi ? P.phi[a] - P.phi[b] : potential(P, y, a) - potential(P, y, b);
   double g = EFc(P, y, P.f_gate, e);
   double avail = c.junction * g;
   bool ok = true;
@@ -370,7 +373,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
   const double* yv = vol_next ? next(const_cast<double*>(A), const_cast<double*>(B), sw) : y;
   double ci[4], vol;
   cell_concentrations(P, y, yv, i, ci, vol);
-  double dv = potential(P, y, i) - P.Vb;
+  double dv = (P.phi ? P.phi[i] : potential(P, y, i)) - P.Vb;
   bool ok = true;
   constexpr bool slopes = SLOPES;  // gm, Gj, mdiag, msc non-null exactly when SLOPES
   double Jm[4], mu[4];
@@ -2543,6 +2546,8 @@ struct cells_symbolic {
   cells::Symbolic s;
 };
 
+#include "n2.inc"
+
 struct cells_ensemble {
   int device = 0;
   int method = 0;
@@ -2599,6 +2604,8 @@ struct cells_ensemble {
   int* t_valid = nullptr;
   // the adjoint workspace (allocated at the first cells_step_vjp; laid out by adjoint_layout)
   double* adj_base = nullptr;
+  // N2's stage buffers (CELLS_METHOD_N2)
+  N2Buffers n2;
   hipStream_t capture_stream = nullptr;
   // what a captured step graph bakes in: one is replayed only while all of it is unchanged
   struct GraphKey {
@@ -2958,7 +2965,7 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
   if (rc) return rc;
   const bool pairwise = (method & CELLS_FACTOR_PAIRWISE) != 0;
   method &= ~CELLS_FACTOR_PAIRWISE;
-  if (method != CELLS_METHOD_EULER && method != CELLS_METHOD_ROS2)
+  if (method != CELLS_METHOD_EULER && method != CELLS_METHOD_ROS2 && method != CELLS_METHOD_N2)
     return fail(CELLS_EINVAL, "unknown method");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
@@ -3015,10 +3022,10 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
     for (int s = 0; s < w->S; ++s) total += w->bath_c[s];
     P.psib = w->bath_pressure - kR * T * total;
   }
-  // IMEX Euler's value (surface_pump.py:63-64 ENERGY_RTOL). ROS2, which the reference lacks, takes
-  // it too: its frozen-slope stages carry the stiff start's energy error at first order (at 5 ms
-  // the first steps of the calcium worlds book 6e-3 of their flows, above ARS(2,2,2)'s 2e-3)
-  P.energy_rtol = 2e-2;
+  // surface_pump.py:63-64 ENERGY_RTOL: IMEX Euler's 2e-2, N2's (ARS(2,2,2)'s) 2e-3. ROS2, which
+  // the reference lacks, takes Euler's: its frozen-slope stages carry the stiff start's energy
+  // error at first order (at 5 ms the first steps of the calcium worlds book 6e-3 of their flows)
+  P.energy_rtol = method == CELLS_METHOD_N2 ? 2e-3 : 2e-2;
   fill_layout(P, w);
   if (!(T >= 200.0 && T <= 400.0)) return fail(CELLS_EINVAL, "temperature outside 200-400 K");
   if (!(w->gate_threshold >= 0 && w->gate_threshold <= kGateMaxVoltage && w->gate_minimum >= 0 &&
@@ -3080,7 +3087,18 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       (rc = alloc(e.get(), &e->check_bits, e->W)))
     return rc;
   P.refuse_bits = e->check_bits;
-  if (method == CELLS_METHOD_ROS2) {
+  if (method == CELLS_METHOD_N2) {
+    N2Buffers& n = e->n2;
+    if ((rc = alloc(e.get(), &n.E0, total)) || (rc = alloc(e.get(), &n.E1, total)) ||
+        (rc = alloc(e.get(), &n.D1, total)) || (rc = alloc(e.get(), &n.D2, total)) ||
+        (rc = alloc(e.get(), &n.known, total)) || (rc = alloc(e.get(), &n.stage, total)) ||
+        (rc = alloc(e.get(), &n.diag, nc)) || (rc = alloc(e.get(), &n.q, nc)) ||
+        (rc = alloc(e.get(), &n.cD1, S * nc)) || (rc = alloc(e.get(), &n.cD2, S * nc)) ||
+        (rc = alloc(e.get(), &n.phi0, nc)) || (rc = alloc(e.get(), &n.target0, std::max<size_t>(ne, 1))) ||
+        (rc = alloc(e.get(), &n.relax0, std::max<size_t>(ne, 1))))
+      return rc;
+  }
+  if (method == CELLS_METHOD_ROS2 || method == CELLS_METHOD_N2) {
     if ((rc = alloc(e.get(), &e->Y1, total)) || (rc = alloc(e.get(), &e->K2, total)) ||
         (rc = alloc(e.get(), &e->contrib2, S * nc)) || (rc = alloc(e.get(), &e->gm, S * nc)) ||
         (rc = alloc(e.get(), &e->gj, std::max<size_t>(S * ne, 1))) ||
@@ -3315,7 +3333,9 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
       int64_t most = 0;
       for (const ShapeDev& d : sd) most = std::max<int64_t>(most, d.nnz - d.lds_pt);
       e->factor_lds = sizeof(double) * (size_t)std::max<int64_t>(most, 1);
-      e->y_in_lds = e->pairwise && (size_t)e->max_n * sizeof(double) <= kSolveLdsBytes ? 1 : 0;
+      // (N2's stage solves take their right-hand sides in HBM, at each cell's rank)
+      e->y_in_lds = e->pairwise && e->method != CELLS_METHOD_N2 &&
+                    (size_t)e->max_n * sizeof(double) <= kSolveLdsBytes ? 1 : 0;
       e->solve_lds = e->y_in_lds ? sizeof(double) * (size_t)std::max(e->max_n, 1) : 0;
     }
     {
@@ -3446,6 +3466,112 @@ void launch_factor(cells_ensemble* e, hipStream_t st, int w0, int nw, double coe
                        e->ubuf, e->bigbuf, e->boffw, e->valid);
 }
 
+// N2's step of worlds [w0, w0 + nw) between the water half-steps (models/surface_pump_v3.py:
+// 3547-3955 imex_step under cells.solver.imex_ars222_implicit_conductances@1): the start's rates,
+// the operator-time prediction and its slopes, the operator's factor, then the tableau's two
+// stages (each: its known part and right-hand side, the operator solve, the linear parts and the
+// stage state with its gates; the first's rates at the stage potentials less the linear parts)
+// and the ledgers' tail.
+template <typename Mark>
+void launch_n2(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int w0, int nw, double* A, double* B,
+               Mark& mark) {
+  const Params& P = e->P;
+  const N2Buffers& n = e->n2;
+  const int c0 = (int)e->cell_off[w0], c1 = (int)e->cell_off[w0 + nw];
+  const int ed0 = (int)e->edge_off[w0], ed1 = (int)e->edge_off[w0 + nw];
+  const int nc = c1 - c0, ne = ed1 - ed0;
+  const int vol_next = P.water ? 1 : 0;
+  const double gamma = kArsGamma, delta = kArsDelta;
+  const double coef = gamma * dt, h = gamma * dt, theta = 0.5 * dt;
+  // each stage's explicit fraction of the step, its tableau row summed in order
+  const double span1 = ((gamma + 0.0) + 0.0) * dt, span2 = ((delta + (1.0 - delta)) + 0.0) * dt;
+  // the start's rates
+  if (ne) {
+    hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, nullptr, vol_next,
+                       ed0, ed1, c, n.E0, nullptr, nullptr, e->valid);
+    mark("k_edges");
+  }
+  hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, nullptr, vol_next,
+                     c0, c1, c, n.E0, e->contrib1, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr,
+                     e->valid);
+  mark("k_cells");
+  // the operator-time prediction (its state in n.stage until the first stage) and its slopes
+  hipLaunchKernelGGL(k_n2_predict, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, vol_next, c0, nc,
+                     ed0, ne, theta, n.E0, n.stage, n.phi0, n.target0, n.relax0, e->valid);
+  if (ne)
+    hipLaunchKernelGGL(k_n2_slopes_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, n.stage, c, ed0, ne, e->gj,
+                       e->Gj, e->valid);
+  hipLaunchKernelGGL(k_n2_slopes_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, n.stage, c, coef, c0, nc, e->Gj,
+                     e->gm, e->mdiag, e->msc, n.diag, e->valid);
+  mark("k_n2_predict");
+  launch_factor(e, st, w0, nw, coef);
+  mark("k_factor");
+  // stage 0's explicit rates: the start's less the linear parts at the start's potentials
+  const N2Src start{nullptr, vol_next}, at_stage{n.stage, 0};
+  if (ne)
+    hipLaunchKernelGGL(k_n2_remainder_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, start, n.phi0,
+                       e->gj, ed0, ne, n.E0);
+  hipLaunchKernelGGL(k_n2_remainder_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, start, n.phi0,
+                     e->gm, e->gj, c0, nc, n.E0, e->contrib1);
+  hipLaunchKernelGGL(k_n2_world_rates, dim3(nw), dim3(kBlock), 0, st, P, w0, e->contrib1, n.E0);
+  mark("k_n2_remainder");
+  auto solve = [&](double* phi) {
+    hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(nw), solve_block(e),
+                       solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool, e->foff, w0, e->msc, e->fvals,
+                       n.E0, nullptr, phi, e->ysc, e->y_in_lds, solve_args(e), 0, 1);
+    mark("k_solve_stage");
+  };
+  // stage 1: known = y + gamma dt E0; the operator's potentials; D1 and the stage state
+  hipLaunchKernelGGL(k_n2_known, dim3(blocks(nc + ne + nw)), dim3(kBlock), 0, st, P, A, B, e->sel, vol_next, c0, nc,
+                     ed0, ne, w0, nw, gamma * dt, n.E0, 0.0, nullptr, 0.0, nullptr, coef, e->gm, e->msc, n.known,
+                     n.q, e->ysc);
+  mark("k_n2_known");
+  solve(e->dphi);
+  hipLaunchKernelGGL(k_n2_stage_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, n.known, e->dphi, e->gm, e->gj,
+                     e->Gj, n.diag, n.q, coef, h, c0, nc, n.D1, n.cD1, e->sel, A, B, n.stage, e->res_atol,
+                     e->res_rtol, e->valid);
+  if (ne)
+    hipLaunchKernelGGL(k_n2_stage_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, n.known, e->dphi, e->gj,
+                       n.target0, n.relax0, c, h, span1, ed0, ne, n.D1, e->sel, A, B, n.stage, e->valid);
+  hipLaunchKernelGGL(k_n2_world_rates, dim3(nw), dim3(kBlock), 0, st, P, w0, n.cD1, n.D1);
+  mark("k_n2_stage");
+  // stage 1's explicit rates at the stage state and the operator's potentials, less the linear parts
+  Params P1 = P;
+  P1.phi = e->dphi;
+  if (ne) {
+    hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P1, A, B, e->sel, n.stage, 0, ed0,
+                       ed1, c, n.E1, nullptr, nullptr, e->valid);
+    mark("k_edges");
+  }
+  hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P1, A, B, e->sel, n.stage, 0, c0, c1, c,
+                     n.E1, e->contrib2, nullptr, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, e->valid);
+  mark("k_cells");
+  if (ne)
+    hipLaunchKernelGGL(k_n2_remainder_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, at_stage,
+                       e->dphi, e->gj, ed0, ne, n.E1);
+  hipLaunchKernelGGL(k_n2_remainder_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, at_stage,
+                     e->dphi, e->gm, e->gj, c0, nc, n.E1, e->contrib2);
+  hipLaunchKernelGGL(k_n2_world_rates, dim3(nw), dim3(kBlock), 0, st, P, w0, e->contrib2, n.E1);
+  mark("k_n2_remainder");
+  // stage 2: known = ((y + delta dt E0) + (1 - delta) dt E1) + (1 - gamma) dt D1; the new state
+  hipLaunchKernelGGL(k_n2_known, dim3(blocks(nc + ne + nw)), dim3(kBlock), 0, st, P, A, B, e->sel, vol_next, c0, nc,
+                     ed0, ne, w0, nw, delta * dt, n.E0, (1.0 - delta) * dt, n.E1, (1.0 - gamma) * dt, n.D1, coef,
+                     e->gm, e->msc, n.known, n.q, e->ysc);
+  mark("k_n2_known");
+  solve(e->dphi2);
+  hipLaunchKernelGGL(k_n2_stage_cells, dim3(blocks(nc)), dim3(kBlock), 0, st, P, n.known, e->dphi2, e->gm, e->gj,
+                     e->Gj, n.diag, n.q, coef, h, c0, nc, n.D2, n.cD2, e->sel, A, B, nullptr, e->res_atol,
+                     e->res_rtol, e->valid);
+  if (ne)
+    hipLaunchKernelGGL(k_n2_stage_edges, dim3(blocks(ne)), dim3(kBlock), 0, st, P, n.known, e->dphi2, e->gj,
+                       n.target0, n.relax0, c, h, span2, ed0, ne, n.D2, e->sel, A, B, nullptr, e->valid);
+  hipLaunchKernelGGL(k_n2_world_rates, dim3(nw), dim3(kBlock), 0, st, P, w0, n.cD2, n.D2);
+  mark("k_n2_stage");
+  hipLaunchKernelGGL(k_n2_tail, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, w0, dt, h, n.known, n.E0, n.E1,
+                     n.D1, n.D2, e->valid);
+  mark("k_n2_tail");
+}
+
 // A, B: the state buffer pair (a world's current state is A or B by sel[w]; the step writes every
 // field of the other).
 int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int world, double* A, double* B,
@@ -3481,6 +3607,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, w0, e->contrib1,
                        1.0 * dt, nullptr, 0.0, e->valid);
     mark("k_world_ionic");
+  } else if (e->method == CELLS_METHOD_N2) {
+    launch_n2(e, st, dt, c, w0, nw, A, B, mark);
   } else {
     const double gamma = 1.0 + 1.0 / std::sqrt(2.0);
     const double coef = gamma * dt;

@@ -19,6 +19,7 @@ CELLS_ENOMEM = 3
 CELLS_ENODEV = 4
 METHOD_EULER = 0
 METHOD_ROS2 = 1
+METHOD_N2 = 2  # the reference's ARS(2,2,2) with implicit slope conductances (N2)
 FACTOR_PAIRWISE = 0x100  # or-ed into the method: the reference-order factor
 
 _i64p = ctypes.POINTER(ctypes.c_int64)

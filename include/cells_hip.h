@@ -23,6 +23,10 @@ extern "C" {
 
 #define CELLS_METHOD_EULER 0 /* numerics/surface_imex.py:36 imex_euler_face_electromigration@1 */
 #define CELLS_METHOD_ROS2 1  /* ROS2 (gamma = 1 + 1/sqrt 2), stage matrix C + gamma dt G */
+/* numerics/surface_imex.py:38-45 imex_ars222_implicit_conductances@1 (N2): ARS(2,2,2) with the
+ * routes' slope conductances, frozen at half the step, in the operator C + gamma dt G, the junction
+ * gates by their exponential update (models/surface_pump_v3.py:3547-3955 imex_step) */
+#define CELLS_METHOD_N2 2
 /* Or-ed into the method: factor each stage matrix with the reference's summation order (every
  * entry's sources folded in adjacent pairs, numerics/sparse_cholesky.py:474-490), bit for bit
  * the reference's factor, instead of the default multifrontal supernodal factor (same pattern and

@@ -66,6 +66,10 @@ def lib():
             getattr(_lib, name).restype = ctypes.c_double
         for name in ("ref_selective_flux", "ref_route_slope", "ref_gate_rhs"):
             getattr(_lib, name).restype = ctypes.c_double
+        for name in ("ref_lag_weight", "ref_gate_advance"):
+            getattr(_lib, name).restype = ctypes.c_double
+        _lib.ref_lag_weight.argtypes = [ctypes.c_double]
+        _lib.ref_gate_advance.argtypes = [ctypes.c_double] * 6
         _lib.ref_membrane_rate.restype = ctypes.c_double
         _lib.ref_membrane_rate.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
                                            ctypes.c_void_p]
@@ -163,6 +167,18 @@ def step_ros2(p, y, dt, sym, ctl=None, world=None):
     return out, valid.astype(bool)
 
 
+def step_n2(p, y, dt, sym, ctl=None, world=None):
+    """One N2 step (the reference's ARS(2,2,2) with implicit slope conductances) of every world."""
+    w = world if world is not None else world_struct(p)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(y)
+    valid = np.zeros(p.W, dtype=np.int32)
+    lib().ref_step_n2(ctypes.byref(w), sym, _p(y, _f64p), _p(out, _f64p), ctypes.c_double(dt),
+                      ctypes.byref(ctl), _p(valid, _i32p))
+    return out, valid.astype(bool)
+
+
 def factor(sym_one, values):
     v = np.ascontiguousarray(values, dtype=np.float64).copy()
     lib().ref_factor(ctypes.byref(sym_one), _p(v, _f64p))
@@ -194,7 +210,7 @@ def rates(p, y, ctl=None, world=None):
 
 # IMEX Euler's energy bound (models/surface_pump.py:63); ROS2, which the reference lacks, takes it
 # too (its stiff start books energy errors above ARS(2,2,2)'s 2e-3 in the first steps)
-ENERGY_RTOL = {"euler": 2e-2, "ros2": 2e-2}
+ENERGY_RTOL = {"euler": 2e-2, "ros2": 2e-2, "n2": 2e-3}  # N2: models/surface_pump.py:64
 LEDGER_RATIOS = ("local", "total", "global_charge", "water", "outside", "charge", "electrode",
                  "energy", "reservoir_work", "electrode_work", "metabolic_work", "bath_water_work")
 ENERGY_TERMS = ("mixing", "capacitor", "heat", "work", "water", "residual", "flows")

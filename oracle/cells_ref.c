@@ -1094,6 +1094,345 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym, const double*
     return all;
 }
 
+/* ================================================================ N2
+ *
+ * cells.solver.imex_ars222_implicit_conductances@1 (numerics/surface_imex.py:38-79: ARS(2,2,2)'s
+ * arrays with the routes' operator time 0.5), as models/surface_pump_v3.py:3547-3955 imex_step runs
+ * it for cells on one held bath (no faces, channels or kinds): the routes' slope conductances W,
+ * frozen at the operator time (operator_time_conductances :2961-3028), make the implicit operator
+ * C + gamma dt G (conductance_laplacian :2816-2837); the explicit stages carry the remainder
+ * J - W dv (explicit_remainder :2905-2927), the implicit ones the linear parts W dv
+ * (implicit_rates :2659-2700, linear_route_fluxes :2843-2857, route_amount_rates :2860-2872) with
+ * their heat at the stage (implicit_route_heat :2930-2950); the junction gates take their
+ * exponential update at each stage (stage_junction_gates :3127-3156, exponential_gates.advance
+ * numerics/exponential_gates.py:44-61). */
+
+#define ARS_GAMMA (1.0 - 1.0 / 1.4142135623730951)
+#define ARS_DELTA (1.0 - 1.0 / (2.0 * ARS_GAMMA))
+#define GATE_SERIES_LIMIT 1e-3
+
+/* junction_voltage_relaxation.py:35-48 kinetics: the target and the relaxation rate */
+static void junction_kinetics(double v, double thr, double mn, double* target, double* relax) {
+    double shift = fabs(v) - thr;
+    double alpha = 1.3 * exp(-77.0 * shift);
+    double log_beta = log(0.065) + 140.0 * shift;
+    double beta = 20.0 * exp(-ref_logaddexp(0.0, -log_beta));
+    double r = alpha + beta;
+    *relax = r;
+    *target = (alpha + beta * mn) / r;
+}
+
+/* exponential_gates.py:37-42 lag_weight: 1 - (1 - e^-z) / z, its series below 1e-3 */
+double ref_lag_weight(double z) {
+    if (z < GATE_SERIES_LIMIT) return z * (0.5 - z * (1.0 / 6 - z * (1.0 / 24 - z / 120)));
+    return (expm1(-z) + z) / z;
+}
+
+/* exponential_gates.py:44-51 advance */
+double ref_gate_advance(double gate, double target_start, double tau_start, double target_end,
+                        double tau_end, double dt) {
+    double z = 0.5 * dt * (1 / tau_start + 1 / tau_end);
+    return gate + (target_start - gate) * -expm1(-z) + (target_end - target_start) * ref_lag_weight(z);
+}
+
+/* The routes' slopes W (conductance_weights :2758-2813) of world k at state y and potentials phi:
+ * gm [S][ncell] (membranes, at the control's membrane availability) and gj [S][nedge] (junctions,
+ * at the control's junction availability times the state's gate). Returns their finiteness. */
+static int route_slopes(const cells_world* w, const cells_state_layout* L, int32_t k, const double* y,
+                        const double* phi, const cells_control* c, double* gm, double* gj) {
+    const int S = w->S;
+    const double T = w->temperature;
+    int ok = 1;
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double vol = cell_volume(w, L, y, i);
+        for (int s = 0; s < S; ++s) {
+            double ci = (w->n0[(int64_t)s * w->ncell + i] + CF(L, w, y, L->dn + s, i)) / vol;
+            double g = ref_route_slope(ci, w->bath_c[s], phi[i] - w->bath_V, w->valence[s], T,
+                                       w->pam[(int64_t)s * w->ncell + i], c->membrane);
+            gm[(int64_t)s * w->ncell + i] = g;
+            ok = ok && isfinite(g);
+        }
+    }
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        int64_t a = w->snd[e], b = w->rcv[e];
+        double va = cell_volume(w, L, y, a), vb = cell_volume(w, L, y, b);
+        double avail = c->junction * EF(L, w, y, L->gate, e);
+        for (int s = 0; s < S; ++s) {
+            double ca = (w->n0[(int64_t)s * w->ncell + a] + CF(L, w, y, L->dn + s, a)) / va;
+            double cb = (w->n0[(int64_t)s * w->ncell + b] + CF(L, w, y, L->dn + s, b)) / vb;
+            double g = ref_route_slope(ca, cb, phi[a] - phi[b], w->valence[s], T,
+                                       w->paj[(int64_t)s * w->nedge + e], avail);
+            gj[(int64_t)s * w->nedge + e] = g;
+            ok = ok && isfinite(g);
+        }
+    }
+    return ok;
+}
+
+/* The linear parts W dv at potentials phi of world k, as rates r (every other field zero): amounts
+ * (route_amount_rates: the junctions into a cell, less those out of it, less its membrane's),
+ * membrane and junction transfers, the bath supply (minus the membranes' inflow) and, from the
+ * state `at` (its concentrations) with phi, their heat (route_heat_rates :2890-2902). */
+static void linear_parts(const cells_world* w, const cells_state_layout* L, int32_t k, const double* gm,
+                         const double* gj, const double* phi, const double* at, double* r) {
+    const int S = w->S;
+    const double RT = R_CONST * w->temperature;
+    for (int f = 0; f < L->cell_fields; ++f)
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) CF(L, w, r, f, i) = 0.0;
+    for (int f = 0; f < L->edge_fields; ++f)
+        for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) EF(L, w, r, f, e) = 0.0;
+    for (int f = 0; f < L->world_fields; ++f) WF(L, w, r, f, k) = 0.0;
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        int64_t a = w->snd[e], b = w->rcv[e];
+        double dj = phi[a] - phi[b];
+        double va = cell_volume(w, L, at, a), vb = cell_volume(w, L, at, b);
+        double heat = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double j = gj[(int64_t)s * w->nedge + e] * dj;
+            EF(L, w, r, L->jtr + s, e) = j;
+            double ca = (w->n0[(int64_t)s * w->ncell + a] + CF(L, w, at, L->dn + s, a)) / va;
+            double cb = (w->n0[(int64_t)s * w->ncell + b] + CF(L, w, at, L->dn + s, b)) / vb;
+            double mu = RT * (log(ca) - log(cb)) + w->valence[s] * F_CONST * dj;
+            heat = s == 0 ? j * mu : heat + j * mu;
+        }
+        EF(L, w, r, L->jdiss, e) = 1e-3 * heat;
+    }
+    double inflow[4] = {0, 0, 0, 0};
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        int64_t p0 = w->inc_ptr[i], p1 = w->inc_ptr[i + 1], pm = p0 + w->inc_out[i];
+        double dv = phi[i] - w->bath_V;
+        double vol = cell_volume(w, L, at, i);
+        double heat = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double m = gm[(int64_t)s * w->ncell + i] * dv;
+            double in = 0.0, out = 0.0;
+            for (int64_t p = pm; p < p1; ++p) in += EF(L, w, r, L->jtr + s, w->inc_edge[p]);
+            for (int64_t p = p0; p < pm; ++p) out += EF(L, w, r, L->jtr + s, w->inc_edge[p]);
+            CF(L, w, r, L->dn + s, i) = (in - out) - m;
+            CF(L, w, r, L->mtr + s, i) = m;
+            inflow[s] += m;
+            double ci = (w->n0[(int64_t)s * w->ncell + i] + CF(L, w, at, L->dn + s, i)) / vol;
+            double mu = RT * (log(ci) - log(w->bath_c[s])) + w->valence[s] * F_CONST * dv;
+            heat = s == 0 ? m * mu : heat + m * mu;
+        }
+        CF(L, w, r, L->mdiss, i) = 1e-3 * heat;
+    }
+    for (int s = 0; s < S; ++s) WF(L, w, r, L->sup + s, k) = -inflow[s];
+}
+
+/* rates - lin on the fields the linear parts reach (explicit_remainder) */
+static void explicit_remainder(const cells_world* w, const cells_state_layout* L, int32_t k, const double* lin,
+                      double* rates) {
+    const int S = w->S;
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        for (int s = 0; s < S; ++s) {
+            CF(L, w, rates, L->dn + s, i) = CF(L, w, rates, L->dn + s, i) - CF(L, w, lin, L->dn + s, i);
+            CF(L, w, rates, L->mtr + s, i) = CF(L, w, rates, L->mtr + s, i) - CF(L, w, lin, L->mtr + s, i);
+        }
+        CF(L, w, rates, L->mdiss, i) = CF(L, w, rates, L->mdiss, i) - CF(L, w, lin, L->mdiss, i);
+    }
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) {
+        for (int s = 0; s < S; ++s)
+            EF(L, w, rates, L->jtr + s, e) = EF(L, w, rates, L->jtr + s, e) - EF(L, w, lin, L->jtr + s, e);
+        EF(L, w, rates, L->jdiss, e) = EF(L, w, rates, L->jdiss, e) - EF(L, w, lin, L->jdiss, e);
+    }
+    for (int s = 0; s < S; ++s) WF(L, w, rates, L->sup + s, k) = WF(L, w, rates, L->sup + s, k) - WF(L, w, lin, L->sup + s, k);
+}
+
+/* The operator's stage: potentials phi (cells of world k) of the node charges of `known`, solving
+ * (C + coef G) phi = q on the cells with the bath held at V_bath (solve_operator, surface_imex.py:
+ * 269-278), and the residual check |(C + coef G) phi - q| <= 3e-25 + 2e-12 max(|q|, |M phi|). */
+static int operator_stage(const cells_world* w, const cells_state_layout* L, int32_t k, const cells_symbolic* s,
+                          double coef, const double* Gm, const double* Gj, const double* diag, const double* sc,
+                          const double* fv, const double* known, double* phi) {
+    const int64_t c0 = w->cell_off[k], n = w->cell_off[k + 1] - c0, e0 = w->edge_off[k];
+    double* q = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    double* x = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        q[i] = w->q0[c0 + i] + cell_charge_increment(w, L, known, c0 + i);
+        double held = -(w->cap[c0 + i] + coef * Gm[i]) * w->bath_V; /* M's bath column at V_bath */
+        rhs[i] = (q[i] - held) * sc[i];
+    }
+    ref_solve(s, fv, rhs, x);
+    for (int64_t i = 0; i < n; ++i) phi[c0 + i] = x[i] * sc[i];
+    int ok = 1;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t gi = c0 + i;
+        double off = 0.0;
+        for (int64_t p = w->inc_ptr[gi]; p < w->inc_ptr[gi + 1]; ++p) {
+            int64_t e = w->inc_edge[p];
+            int64_t other = w->snd[e] == gi ? w->rcv[e] : w->snd[e];
+            off += Gj[e - e0] * phi[other];
+        }
+        double held = -(w->cap[gi] + coef * Gm[i]) * w->bath_V;
+        double mx = (diag[i] * phi[gi] - coef * off) + held;
+        double scale = fabs(q[i]) > fabs(mx) ? fabs(q[i]) : fabs(mx);
+        ok = ok && isfinite(phi[gi]) && isfinite(mx) && fabs(mx - q[i]) <= 3e-25 + 2e-12 * scale;
+    }
+    free(q); free(rhs); free(x);
+    return ok;
+}
+
+/* y_out = y + (h r) on every cell, edge and supply field (the reference's _combine, one term) */
+static void combine_into(const cells_world* w, const cells_state_layout* L, int32_t k, const double* y,
+                         double h, const double* r, double* out) {
+    combine(w, L, k, y, h, r, out);
+}
+
+int ref_step_n2(const cells_world* w, const cells_symbolic* sym, const double* y, double* out,
+                double dt, const cells_control* c, int32_t* valid_out) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const int S = w->S;
+    const double gamma = ARS_GAMMA, delta = ARS_DELTA, coef = gamma * dt, theta = 0.5 * dt;
+    /* the tableau's rows (surface_imex.py:47-57) and each row's explicit fraction, summed in order */
+    const double ae[3][3] = {{0, 0, 0}, {gamma, 0, 0}, {delta, 1.0 - delta, 0}};
+    const double ai[3][3] = {{0, 0, 0}, {0, gamma, 0}, {0, 1.0 - gamma, gamma}};
+    const size_t T = (size_t)L.total;
+    double* start = (double*)malloc(sizeof(double) * T);
+    double* ahead = (double*)malloc(sizeof(double) * T);
+    double* E0 = (double*)calloc(T, sizeof(double));
+    double* E1 = (double*)calloc(T, sizeof(double));
+    double* D1 = (double*)calloc(T, sizeof(double));
+    double* D2 = (double*)calloc(T, sizeof(double));
+    double* lin = (double*)calloc(T, sizeof(double));
+    double* known = (double*)malloc(sizeof(double) * T);
+    double* stage = (double*)malloc(sizeof(double) * T);
+    const size_t nc = (size_t)(w->ncell ? w->ncell : 1), ne = (size_t)(w->nedge ? w->nedge : 1);
+    double* phi0 = (double*)malloc(sizeof(double) * nc);
+    double* phi = (double*)malloc(sizeof(double) * nc);
+    double* gm = (double*)malloc(sizeof(double) * (size_t)S * nc);
+    double* gj = (double*)malloc(sizeof(double) * (size_t)S * ne);
+    double* target0 = (double*)malloc(sizeof(double) * ne);
+    double* relax0 = (double*)malloc(sizeof(double) * ne);
+    int all = 1;
+    for (int32_t k = 0; k < w->W; ++k) {
+        const cells_symbolic* s = &sym[k];
+        const int64_t c0 = w->cell_off[k], n = w->cell_off[k + 1] - c0, e0 = w->edge_off[k];
+        const int64_t e1 = w->edge_off[k + 1];
+        int valid = 1;
+        copy_world(w, &L, k, y, start);
+        if (w->water) valid = water_half(w, &L, k, 0.5 * dt, c, start) && valid;
+        /* the step's start: its potentials and rates */
+        ref_potentials(w, k, start, phi0);
+        valid = ref_rates(w, k, start, phi0, c, E0, NULL, NULL) && valid;
+        /* the operator-time prediction (operator_time_conductances): amounts ahead by theta times
+         * the start's rates, each junction gate by its exponential relaxation at the start's
+         * junction voltage, the potentials of the predicted charges */
+        copy_world(w, &L, k, start, ahead);
+        for (int s2 = 0; s2 < S; ++s2)
+            for (int64_t i = c0; i < c0 + n; ++i)
+                CF(&L, w, ahead, L.dn + s2, i) = CF(&L, w, start, L.dn + s2, i) + theta * CF(&L, w, E0, L.dn + s2, i);
+        for (int64_t e = e0; e < e1; ++e) {
+            junction_kinetics(phi0[w->snd[e]] - phi0[w->rcv[e]], w->gate_threshold, w->gate_minimum,
+                              &target0[e], &relax0[e]);
+            double g = EF(&L, w, start, L.gate, e);
+            EF(&L, w, ahead, L.gate, e) = g + (target0[e] - g) * -expm1(-relax0[e] * theta);
+        }
+        for (int64_t i = c0; i < c0 + n; ++i)
+            for (int s2 = 0; s2 < S; ++s2) {
+                double ci = w->n0[(int64_t)s2 * w->ncell + i] + CF(&L, w, ahead, L.dn + s2, i);
+                valid = valid && isfinite(ci) && ci > 0;
+            }
+        ref_potentials(w, k, ahead, phi);
+        for (int64_t i = c0; i < c0 + n; ++i) valid = valid && isfinite(phi[i]);
+        valid = route_slopes(w, &L, k, ahead, phi, c, gm, gj) && valid;
+        /* the operator C + gamma dt G, Jacobi-scaled and symmetrised (surface_imex.py:228-259) */
+        double* Gm = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+        double* Gj = (double*)malloc(sizeof(double) * (size_t)(e1 - e0 + 1));
+        for (int64_t i = 0; i < n; ++i) {
+            double t = w->valence[0] * gm[c0 + i];
+            for (int q = 1; q < S; ++q) t = t + w->valence[q] * gm[(int64_t)q * w->ncell + c0 + i];
+            Gm[i] = CHARGE_PER_FMOL * t;
+        }
+        for (int64_t e = e0; e < e1; ++e) {
+            double t = w->valence[0] * gj[e];
+            for (int q = 1; q < S; ++q) t = t + w->valence[q] * gj[(int64_t)q * w->nedge + e];
+            Gj[e - e0] = CHARGE_PER_FMOL * t;
+        }
+        double* diag = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+        double* sc = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t gi = c0 + i;
+            double g = Gm[i];
+            for (int64_t p = w->inc_ptr[gi]; p < w->inc_ptr[gi + 1]; ++p) g += Gj[w->inc_edge[p] - e0];
+            diag[i] = w->cap[gi] + coef * g;
+            valid = valid && diag[i] > 0;
+            sc[i] = 1.0 / sqrt(diag[i] > 0 ? diag[i] : 1.0);
+        }
+        double* fv = (double*)calloc((size_t)s->nnz, sizeof(double));
+        for (int64_t i = 0; i < n; ++i) {
+            double v = diag[i];
+            fv[s->diag_pos[i]] = (v * sc[i] * sc[i] + v * sc[i] * sc[i]) / 2;
+        }
+        for (int64_t e = e0; e < e1; ++e) {
+            double v = -(coef * Gj[e - e0]);
+            int64_t a = w->snd[e] - c0, b = w->rcv[e] - c0;
+            fv[s->edge_pos[e - e0]] = (v * sc[a] * sc[b] + v * sc[b] * sc[a]) / 2;
+        }
+        ref_factor(s, fv);
+        for (int64_t p = 0; p < s->nnz; ++p) valid = valid && isfinite(fv[p]);
+        /* stage 0's explicit rates: the start's less the linear parts at the start */
+        linear_parts(w, &L, k, gm, gj, phi0, start, lin);
+        explicit_remainder(w, &L, k, lin, E0);
+        double* Es[2] = {E0, E1};
+        double* Ds[3] = {NULL, D1, D2};
+        for (int i = 1; i < 3; ++i) {
+            /* known = state + dt (explicit terms, then implicit terms), one term at a time */
+            copy_world(w, &L, k, start, known);
+            for (int j = 0; j < i; ++j) combine_into(w, &L, k, known, ae[i][j] * dt, Es[j], known);
+            for (int j = 1; j < i; ++j) combine_into(w, &L, k, known, ai[i][j] * dt, Ds[j], known);
+            valid = operator_stage(w, &L, k, s, coef, Gm, Gj, diag, sc, fv, known, phi) && valid;
+            /* the linear parts at the stage potentials, their heat at the stage they reach */
+            linear_parts(w, &L, k, gm, gj, phi, known, Ds[i]);
+            copy_world(w, &L, k, known, stage);
+            combine_into(w, &L, k, known, ai[i][i] * dt, Ds[i], stage);
+            linear_parts(w, &L, k, gm, gj, phi, stage, lin);
+            for (int64_t i2 = c0; i2 < c0 + n; ++i2) CF(&L, w, Ds[i], L.mdiss, i2) = CF(&L, w, lin, L.mdiss, i2);
+            for (int64_t e = e0; e < e1; ++e) EF(&L, w, Ds[i], L.jdiss, e) = EF(&L, w, lin, L.jdiss, e);
+            copy_world(w, &L, k, known, stage);
+            combine_into(w, &L, k, known, ai[i][i] * dt, Ds[i], stage);
+            /* the junction gates' exponential update from the start to the stage */
+            const double fraction = (ae[i][0] + ae[i][1]) + ae[i][2];
+            for (int64_t e = e0; e < e1; ++e) {
+                double v = phi[w->snd[e]] - phi[w->rcv[e]], t1, r1;
+                junction_kinetics(v, w->gate_threshold, w->gate_minimum, &t1, &r1);
+                double g = ref_gate_advance(EF(&L, w, start, L.gate, e), target0[e], 1.0 / relax0[e], t1,
+                                            1.0 / r1, fraction * dt);
+                EF(&L, w, stage, L.gate, e) = g;
+                valid = valid && gate_domain(g, v, w->gate_threshold, w->gate_minimum, c->junction);
+            }
+            if (i < 2) {
+                valid = ref_rates(w, k, stage, phi, c, E1, NULL, NULL) && valid;
+                linear_parts(w, &L, k, gm, gj, phi, stage, lin);
+                explicit_remainder(w, &L, k, lin, E1);
+            }
+        }
+        /* the tail: the stage is the new state; electrode and reservoir ledgers from the applied
+         * supply (the last row's weights) */
+        copy_world(w, &L, k, stage, out);
+        for (int f = 0; f < L.world_fields; ++f)
+            if (f < L.sup || f >= L.sup + S) WF(&L, w, out, f, k) = WF(&L, w, start, f, k);
+        double applied[4];
+        for (int q2 = 0; q2 < S; ++q2) {
+            double ex = (0.0 + ae[2][0] * WF(&L, w, E0, L.sup + q2, k)) + ae[2][1] * WF(&L, w, E1, L.sup + q2, k);
+            double im = (0.0 + ai[2][1] * WF(&L, w, D1, L.sup + q2, k)) + ai[2][2] * WF(&L, w, D2, L.sup + q2, k);
+            applied[q2] = dt * (ex + im);
+        }
+        ionic_tail(w, &L, k, start, out, applied);
+        valid = valid && domain_valid(w, &L, k, out);
+        if (w->water) valid = water_half(w, &L, k, 0.5 * dt, c, out) && valid;
+        if (!valid) copy_world(w, &L, k, y, out);
+        valid_out[k] = valid;
+        all = all && valid;
+        free(Gm); free(Gj); free(diag); free(sc); free(fv);
+    }
+    free(start); free(ahead); free(E0); free(E1); free(D1); free(D2); free(lin); free(known); free(stage);
+    free(phi0); free(phi); free(gm); free(gj); free(target0); free(relax0);
+    return all;
+}
+
 /* ================================================================ adjoint of the ROS2 step
  *
  * The reference has no adjoint (it differentiates only forward, analysis/stationary.py:505-517);

@@ -118,6 +118,19 @@ typedef struct {
 int ref_step_ros2(const cells_world* w, const cells_symbolic* sym /* [W] */, const double* y,
                   double* out, double dt, const cells_control* c, int32_t* valid);
 
+/* One checked step of every world under N2, the reference's ARS(2,2,2) with implicit slope
+ * conductances (numerics/surface_imex.py:38-79 cells.solver.imex_ars222_implicit_conductances@1, as
+ * models/surface_pump_v3.py:3547-3955 imex_step runs it for cells on one held bath); sym as for
+ * ref_step_ros2 (the operator C + gamma dt G has the same pattern). */
+int ref_step_n2(const cells_world* w, const cells_symbolic* sym, const double* y, double* out, double dt,
+                const cells_control* c, int32_t* valid_out);
+
+/* numerics/exponential_gates.py:37-51: the lag weight 1 - (1 - e^-z) / z and the gate advance from
+ * the start and end kinetics (targets, time constants in s) over dt. */
+double ref_lag_weight(double z);
+double ref_gate_advance(double gate, double target_start, double tau_start, double target_end,
+                        double tau_end, double dt);
+
 /* Numeric factor in the reference's order (left-looking, pairwise folds in ascending column
  * order, below entries times the pivot's reciprocal) and the solve (forward then back, each pivot
  * applied as a multiplication by its reciprocal, the device path of the reference). */
