@@ -394,6 +394,52 @@ def adjoint_leg(steps, snapshots, tape, fd_directions=2):
             "fd_check": {"directions": len(fd), "h": 1e-6, "max_rel_err": max(fd) if fd else None}}
 
 
+def n2_leg(p, dt, warmup, steps, parity_worlds, ros2_ms, parity):
+    """The reference's N2 (ARS(2,2,2) with implicit slope conductances) on the same worlds: its
+    batched step time beside ROS2's, its kernels, and its trajectory against the oracle's N2 on the
+    first worlds."""
+    import torch
+
+    from cells_amd.ensemble import Ensemble
+    from cells_amd.world import head, initial_state, take_state
+
+    log("N2 on the same worlds")
+    e = Ensemble(p, "n2")
+    y0 = e.to_device(initial_state(p))
+    out = torch.empty_like(y0)
+    ok = torch.empty(p.W, dtype=torch.int32, device=y0.device)
+    y, _ = e.step(y0, dt, nsteps=warmup, out=out, ok=ok)
+    yw = y.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y, ok = e.step(yw, dt, nsteps=steps, out=out, ok=ok)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    final = y.cpu().numpy()
+    accepted = bool(ok.cpu().numpy().all())
+    kernels = {}
+    for name, t in e.kernel_times(yw, dt, reps=2):
+        kernels[name] = round(kernels.get(name, 0.0) + t, 4)
+    e.close()
+    log(f"N2: {ms:.3f} ms/step ({p.ncell * 1e3 / ms:.4g} cell-steps/s), all accepted {accepted}; kernels {kernels}")
+    err = None
+    if parity:
+        import oracle  # the checker of the parity leg
+
+        sub = head(p, parity_worlds)
+        _, sym = oracle.world_symbolics(sub)
+        ws = oracle.world_struct(sub)
+        ref = initial_state(sub)
+        for _ in range(warmup + steps):
+            ref, okr = oracle.step_n2(sub, ref, dt, sym, world=ws)
+        got = take_state(p, final, parity_worlds)
+        err = max(relerr(vmem(sub, got), vmem(sub, ref)), relerr(conc(sub, got), conc(sub, ref)))
+        log(f"N2 parity max relative error {err:.3e}")
+    return {"ms_per_step": ms, "cell_steps_per_s": p.ncell * 1e3 / ms, "over_ros2_ms": ms / ros2_ms,
+            "all_worlds_accepted": accepted, "kernel_ms": kernels, "parity_max_rel_err": err,
+            "parity_worlds": parity_worlds if parity else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,7 +451,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--parity-worlds", type=int, default=3)
     ap.add_argument("--skip", default="",
-                    help="comma list of legs to skip: tune,plain,parity,cpu,adjoint,stationary,spatial")
+                    help="comma list of legs to skip: tune,plain,parity,cpu,adjoint,stationary,spatial,n2")
     ap.add_argument("--spatial-steps", type=int, default=100)
     ap.add_argument("--spatial-start", choices=("both", "solve", "carried"), default="both",
                     help="the spatial leg's step-start potentials: the field solve, the carried ones, or both")
@@ -538,6 +584,10 @@ def main():
                     "launch_ms": avg_ms}
     step_gbs = BYTES_PER_CELL_STEP * p.ncell / (ms_per_step * 1e-3) / 1e9
 
+    n2 = None
+    if "n2" not in skip:
+        n2 = n2_leg(p, dt, args.warmup, args.steps, args.parity_worlds, ms_per_step, "parity" not in skip)
+
     plain = sub_batched = None
     if "plain" not in skip:
         k = min(args.plain_worlds, p.W)
@@ -626,6 +676,7 @@ def main():
         "plans": stats,
         "roofline": roofline,
         "adjoint": adj,
+        "n2": n2,
         "stationary": sta,
         "spatial": spatial,
         "autotune": tune.summary() if tune is not None else None,

@@ -2547,6 +2547,7 @@ struct cells_symbolic {
 };
 
 #include "n2.inc"
+#include "rk.inc"
 
 struct cells_ensemble {
   int device = 0;
@@ -2579,7 +2580,7 @@ struct cells_ensemble {
   std::vector<cells::Supernodes> sn_host;
   size_t factor_lds = 0, solve_lds = 0;  // dynamic LDS bytes of the factor and solve kernels
   int y_in_lds = 1;
-  double *A = nullptr, *B = nullptr, *Y1 = nullptr, *K1 = nullptr, *K2 = nullptr;
+  double *A = nullptr, *B = nullptr, *Y1 = nullptr, *K1 = nullptr, *K2 = nullptr, *K3 = nullptr, *K4 = nullptr;
   double *contrib1 = nullptr, *contrib2 = nullptr, *gm = nullptr, *gj = nullptr, *Gj = nullptr;
   double *mdiag = nullptr, *msc = nullptr, *dphi = nullptr, *dphi2 = nullptr, *fvals = nullptr;
   double* ysc = nullptr;
@@ -2798,8 +2799,13 @@ void rate_bounds(const cells_world* w, int method, double* bexp, double* bgate, 
     return x / std::expm1(x);
   };
   auto rows = [](double g, double c) { return c > 0 ? g / c : (g == 0 ? 0.0 : INFINITY); };
+  *bexp = *bgate = *bca = 0.0;
+  if (method == CELLS_METHOD_RK4) return;  // RK4 has none of the IMEX step checks (surface_pump.py:434-437)
+  // the IMEX tableaux with explicit junctions and membranes (Euler, ARS(2,2,2)) against those whose
+  // routes' slopes are implicit (N2, and this build's ROS2)
+  const bool explicit_routes = method == CELLS_METHOD_EULER || method == CELLS_METHOD_ARS222;
   double best = 0.0;
-  if (method == CELLS_METHOD_EULER) {
+  if (explicit_routes) {
     std::vector<double> jrow(nc, 0.0);
     for (int64_t e = 0; e < ne; ++e) {
       const int64_t a = w->snd[e], b = w->rcv[e];
@@ -2829,17 +2835,17 @@ void rate_bounds(const cells_world* w, int method, double* bexp, double* bgate, 
     }
   double material = 0.0;
   for (int s = 0; s < S; ++s) {
-    if (method == CELLS_METHOD_EULER && w->valence[s] != 0.0) continue;
+    if (explicit_routes && w->valence[s] != 0.0) continue;
     for (int64_t i = 0; i < nc; ++i) {
       const double v0 = w->q0[i] / w->cap[i];
       const double x = w->valence[s] * kF * v0 / RT;
       const double pm = w->pam[(int64_t)s * nc + i];
-      const double route = method == CELLS_METHOD_EULER ? pm : pm * host_bernoulli(-x);
+      const double route = explicit_routes ? pm : pm * host_bernoulli(-x);
       material = std::max(material, (route + mat[(size_t)s * nc + i]) / (w->vol0[i] * 1e-15));
     }
   }
   *bexp = std::max(best, 2.0 * material);
-  *bgate = (method == CELLS_METHOD_EULER && ne > 0 && w->gate_minimum < 1.0)
+  *bgate = (explicit_routes && ne > 0 && w->gate_minimum < 1.0)
                ? 1.3 * std::exp(77.0 * w->gate_threshold) + 20.0 : 0.0;
   double ca = 0.0;
   if (w->calcium)
@@ -2965,8 +2971,12 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
   if (rc) return rc;
   const bool pairwise = (method & CELLS_FACTOR_PAIRWISE) != 0;
   method &= ~CELLS_FACTOR_PAIRWISE;
-  if (method != CELLS_METHOD_EULER && method != CELLS_METHOD_ROS2 && method != CELLS_METHOD_N2)
+  if (method != CELLS_METHOD_EULER && method != CELLS_METHOD_ROS2 && method != CELLS_METHOD_N2 &&
+      method != CELLS_METHOD_ARS222 && method != CELLS_METHOD_RK4)
     return fail(CELLS_EINVAL, "unknown method");
+  if (method == CELLS_METHOD_RK4 && w->water)
+    return fail(CELLS_EINVAL, "RK4 covers worlds without water (the reference's water RK4 couples the water "
+                              "rates into the right-hand side)");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
     return fail(CELLS_ENODEV, "no GPU visible: the HIP step has no CPU fallback");
@@ -3025,7 +3035,9 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
   // surface_pump.py:63-64 ENERGY_RTOL: IMEX Euler's 2e-2, N2's (ARS(2,2,2)'s) 2e-3. ROS2, which
   // the reference lacks, takes Euler's: its frozen-slope stages carry the stiff start's energy
   // error at first order (at 5 ms the first steps of the calcium worlds book 6e-3 of their flows)
-  P.energy_rtol = method == CELLS_METHOD_N2 ? 2e-3 : 2e-2;
+  P.energy_rtol = method == CELLS_METHOD_N2 || method == CELLS_METHOD_ARS222 ? 2e-3
+                  : method == CELLS_METHOD_RK4                             ? 0.0
+                                                                           : 2e-2;
   fill_layout(P, w);
   if (!(T >= 200.0 && T <= 400.0)) return fail(CELLS_EINVAL, "temperature outside 200-400 K");
   if (!(w->gate_threshold >= 0 && w->gate_threshold <= kGateMaxVoltage && w->gate_minimum >= 0 &&
@@ -3098,6 +3110,12 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
         (rc = alloc(e.get(), &n.relax0, std::max<size_t>(ne, 1))))
       return rc;
   }
+  if (method == CELLS_METHOD_RK4)
+    if ((rc = alloc(e.get(), &e->K3, total)) || (rc = alloc(e.get(), &e->K4, total))) return rc;
+  if (method == CELLS_METHOD_ARS222 || method == CELLS_METHOD_RK4)
+    if ((rc = alloc(e.get(), &e->Y1, total)) || (rc = alloc(e.get(), &e->K2, total)) ||
+        (rc = alloc(e.get(), &e->contrib2, S * nc)))
+      return rc;
   if (method == CELLS_METHOD_ROS2 || method == CELLS_METHOD_N2) {
     if ((rc = alloc(e.get(), &e->Y1, total)) || (rc = alloc(e.get(), &e->K2, total)) ||
         (rc = alloc(e.get(), &e->contrib2, S * nc)) || (rc = alloc(e.get(), &e->gm, S * nc)) ||
@@ -3572,6 +3590,66 @@ void launch_n2(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int w
   mark("k_n2_tail");
 }
 
+// The explicit tableaux' step of worlds [w0, w0 + nw) between the water half-steps: ARS(2,2,2)
+// (its stages' rates with the membrane-rate refusal, each stage's capacitor potentials checked, the
+// ledgers' tail) or RK4 (four slopes of every leaf, each trial's domain, the end's).
+template <typename Mark>
+void launch_explicit(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int w0, int nw, double* A, double* B,
+                     Mark& mark) {
+  const Params& P = e->P;
+  const int c0 = (int)e->cell_off[w0], c1 = (int)e->cell_off[w0 + nw];
+  const int ed0 = (int)e->edge_off[w0], ed1 = (int)e->edge_off[w0 + nw];
+  const int nc = c1 - c0, ne = ed1 - ed0;
+  const int vol_next = P.water ? 1 : 0;
+  const bool rk4 = e->method == CELLS_METHOD_RK4;
+  const double rate_dt = rk4 ? 0.0 : dt;  // the explicit stages' membrane-rate refusal (IMEX only)
+  const int all = nc + ne + nw;
+  auto rates = [&](const double* src, int vn, double* R, double* contrib) {
+    if (ne) {
+      hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, src, vn, ed0, ed1, c,
+                         R, nullptr, nullptr, e->valid);
+      mark("k_edges");
+    }
+    hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, src, vn, c0, c1, c, R,
+                       contrib, nullptr, nullptr, 0.0, nullptr, nullptr, rate_dt, nullptr, nullptr, e->valid);
+    mark("k_cells");
+    if (rk4)
+      hipLaunchKernelGGL(k_rk4_world_rates, dim3(nw), dim3(kBlock), 0, st, P, w0, contrib, R, e->valid);
+    else
+      hipLaunchKernelGGL(k_n2_world_rates, dim3(nw), dim3(kBlock), 0, st, P, w0, contrib, R);
+  };
+  if (!rk4) {
+    const double gamma = kArsGamma, delta = kArsDelta;
+    rates(nullptr, vol_next, e->K1, e->contrib1);
+    hipLaunchKernelGGL(k_rk_combine, dim3(blocks(all)), dim3(kBlock), 0, st, P, A, B, e->sel, vol_next, c0, nc, ed0,
+                       ne, w0, nw, gamma * dt, e->K1, 0.0, nullptr, e->Y1, 0, 1, 0, e->valid);
+    mark("k_rk_combine");
+    rates(e->Y1, 0, e->K2, e->contrib2);
+    hipLaunchKernelGGL(k_rk_combine, dim3(blocks(all)), dim3(kBlock), 0, st, P, A, B, e->sel, vol_next, c0, nc, ed0,
+                       ne, w0, nw, delta * dt, e->K1, (1.0 - delta) * dt, e->K2, nullptr, 0, 1, 1, e->valid);
+    mark("k_rk_combine");
+    hipLaunchKernelGGL(k_ars_tail, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, w0, dt, e->K1, e->K2, e->valid);
+    mark("k_ars_tail");
+    return;
+  }
+  double* k[4] = {e->K1, e->K2, e->K3, e->K4};
+  double* contrib[2] = {e->contrib1, e->contrib2};
+  const double factor[4] = {0.0, 0.5, 0.5, 1.0};
+  for (int j = 0; j < 4; ++j) {
+    if (j == 0) {
+      rates(nullptr, 0, k[0], contrib[0]);
+      continue;
+    }
+    hipLaunchKernelGGL(k_rk_combine, dim3(blocks(all)), dim3(kBlock), 0, st, P, A, B, e->sel, 0, c0, nc, ed0, ne,
+                       w0, nw, factor[j] * dt, k[j - 1], 0.0, nullptr, e->Y1, 1, 0, 1, e->valid);
+    mark("k_rk_combine");
+    rates(e->Y1, 0, k[j], contrib[j % 2]);
+  }
+  hipLaunchKernelGGL(k_rk4_final, dim3(blocks(all)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc, ed0, ne, w0, nw,
+                     dt, k[0], k[1], k[2], k[3], e->valid);
+  mark("k_rk4_final");
+}
+
 // A, B: the state buffer pair (a world's current state is A or B by sel[w]; the step writes every
 // field of the other).
 int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int world, double* A, double* B,
@@ -3609,6 +3687,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     mark("k_world_ionic");
   } else if (e->method == CELLS_METHOD_N2) {
     launch_n2(e, st, dt, c, w0, nw, A, B, mark);
+  } else if (e->method == CELLS_METHOD_ARS222 || e->method == CELLS_METHOD_RK4) {
+    launch_explicit(e, st, dt, c, w0, nw, A, B, mark);
   } else {
     const double gamma = 1.0 + 1.0 / std::sqrt(2.0);
     const double coef = gamma * dt;

@@ -47,7 +47,8 @@ class Ensemble:
         self.torch = torch
         self.packed = packed
         self.layout = StateLayout.of(packed)
-        self.method = {"euler": _lib.METHOD_EULER, "ros2": _lib.METHOD_ROS2, "n2": _lib.METHOD_N2}[method]
+        self.method = {"euler": _lib.METHOD_EULER, "ros2": _lib.METHOD_ROS2, "n2": _lib.METHOD_N2,
+                       "ars222": _lib.METHOD_ARS222, "rk4": _lib.METHOD_RK4}[method]
         flag = {"supernodal": 0, "pairwise": _lib.FACTOR_PAIRWISE}[factor]
         self.device = torch.device("cuda", device)
         L = _lib.load()

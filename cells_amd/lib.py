@@ -20,6 +20,8 @@ CELLS_ENODEV = 4
 METHOD_EULER = 0
 METHOD_ROS2 = 1
 METHOD_N2 = 2  # the reference's ARS(2,2,2) with implicit slope conductances (N2)
+METHOD_ARS222 = 3  # the reference's ARS(2,2,2) (its implicit part empty on a held bath)
+METHOD_RK4 = 4  # the reference's RK4 of an empty tableau (worlds without water)
 FACTOR_PAIRWISE = 0x100  # or-ed into the method: the reference-order factor
 
 _i64p = ctypes.POINTER(ctypes.c_int64)

@@ -27,7 +27,9 @@ from cells_amd.world import F, Config, Packed, StateLayout, build, initial_state
 MODEL_ID = "cells.model.resolved_pump_tissue@3"
 # the reference's solver ids (numerics/surface_imex.py:36-45) and this build's ROS2
 SOLVERS = {"imex_euler": "euler", "ros2": "ros2", "imex_ars222_implicit_conductances": "n2",
+           "imex_ars222": "ars222", "rk4": "rk4",
            "cells.solver.imex_euler_face_electromigration@1": "euler",
+           "cells.solver.imex_ars222_face_electromigration@1": "ars222",
            "cells.solver.imex_ars222_implicit_conductances@1": "n2"}
 
 

@@ -77,13 +77,19 @@ class RunResult:
     observations: dict  # absolute step -> [ncell] cell potentials (V)
     accounting: dict  # the largest ratio of each ledger's residual to its bound over the worlds
     timings: dict
+    resources: dict = dataclasses.field(default_factory=dict)
+    world_id_start: int = 0
 
     def summary(self, packed):
         """ensemble/ionic.py:182-205's summary fields that this run defines."""
         ncell = np.diff(packed.cell_off)
         final = self.observations.get("final")
         return {
+            "model": "cells.model.ionic_worlds@1",
+            "physical_model": "cells.model.resolved_pump_tissue@3",
             "worlds": int(packed.W),
+            "world_id_start": int(self.world_id_start),
+            "deterministic_replicas": True,
             "through_time_s": float(self.checkpoint.time.min()),
             "integrated_world_steps": int(self.checkpoint.step_index.sum()),
             "final_voltage_V": [final[int(a):int(a) + int(n)].tolist()
@@ -91,8 +97,15 @@ class RunResult:
             "final_voltage_axis": "membrane_patch",
             "ionic_accounting": self.accounting,
             "refused_worlds": sorted(self.refused),
+            "compile_seconds": float(self.timings.get("compile_seconds", 0.0)),
+            "execute_seconds": float(self.timings.get("execute_seconds", 0.0)),
             "phase_timings": self.timings,
+            "resources": self.resources,
+            "elapsed_seconds": float(self.timings.get("elapsed_seconds", 0.0)),
+            "backend": "gpu",
             "precision": "float64",
+            # every world in one captured launch sequence per step: the vectorised windows' form
+            "world_batching": "vmap",
         }
 
 
@@ -116,15 +129,17 @@ def _boundaries(start, steps, window, observe):
 
 
 def run_windowed(ensemble, y0, dt, steps, policy: WindowPolicy, ctl=None, checkpoint=None, sink=None,
-                 material_atol=1e-12):
+                 material_atol=1e-12, world_id_start=0):
     """`steps` steps of every world in windows; from `checkpoint` when given (its state and step
-    counts), else from y0 at step 0. sink(window_index, checkpoint) runs after each window."""
+    counts), else from y0 at step 0. sink(window_index, checkpoint) runs after each window.
+    world_id_start: the first world's id in a larger ensemble (reported in the summary)."""
     import torch
 
     packed = ensemble.packed
     W = packed.W
     began = time.perf_counter()
-    timings = {"execute_seconds": 0.0, "locate_seconds": 0.0, "observe_seconds": 0.0, "windows": 0}
+    timings = {"compile_seconds": 0.0, "execute_seconds": 0.0, "locate_seconds": 0.0, "observe_seconds": 0.0,
+               "windows": 0}
     if checkpoint is not None:
         y = ensemble.to_device(checkpoint.state)
         counts = np.array(checkpoint.step_index, dtype=np.int64)
@@ -151,7 +166,8 @@ def run_windowed(ensemble, y0, dt, steps, policy: WindowPolicy, ctl=None, checkp
         out, ok = ensemble.step(y, dt, n, ctl=ctl, active=live.astype(np.int32))
         okh = ok.cpu().numpy().astype(bool)
         bits = ensemble.check_bits().cpu().numpy()
-        timings["execute_seconds"] += time.perf_counter() - t0
+        # the first window's call captures the step graph (the compile of a jitted window)
+        timings["compile_seconds" if window == 0 else "execute_seconds"] += time.perf_counter() - t0
         failed = np.flatnonzero(live & ~okh)
         if failed.size:
             t1 = time.perf_counter()
@@ -184,7 +200,14 @@ def run_windowed(ensemble, y0, dt, steps, policy: WindowPolicy, ctl=None, checkp
     accounting = {k: float(r[:, i].max()) for i, k in enumerate(_lib.LEDGER_RATIOS)}
     timings["windows"] = window
     timings["elapsed_seconds"] = time.perf_counter() - began
-    return RunResult(Checkpoint(y, counts, dt), refused, observations, accounting, timings)
+    free, total = torch.cuda.mem_get_info(ensemble.device)
+    resources = {
+        "executed_world_steps": int(counts.sum() - (0 if checkpoint is None else np.sum(checkpoint.step_index))),
+        "state_bytes": int(8 * ensemble.state_size),
+        "device_memory_in_use_bytes": int(total - free),
+    }
+    return RunResult(Checkpoint(y, counts, dt), refused, observations, accounting, timings, resources,
+                     world_id_start)
 
 
 def _locate(ensemble, y, dt, ctl, world, at, n, material_atol):

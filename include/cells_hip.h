@@ -27,6 +27,13 @@ extern "C" {
  * routes' slope conductances, frozen at half the step, in the operator C + gamma dt G, the junction
  * gates by their exponential update (models/surface_pump_v3.py:3547-3955 imex_step) */
 #define CELLS_METHOD_N2 2
+/* numerics/surface_imex.py:53-57 imex_ars222_face_electromigration@1: on cells over a held bath
+ * its implicit part is empty (no faces), a two-stage explicit tableau with weights
+ * (delta, 1 - delta) */
+#define CELLS_METHOD_ARS222 3
+/* numerics/fixed_step.py:25-71 rk4_checked_step, the step of an empty tableau
+ * (models/surface_pump_v3.py:3989-3998); worlds without water */
+#define CELLS_METHOD_RK4 4
 /* Or-ed into the method: factor each stage matrix with the reference's summation order (every
  * entry's sources folded in adjacent pairs, numerics/sparse_cholesky.py:474-490), bit for bit
  * the reference's factor, instead of the default multifrontal supernodal factor (same pattern and

@@ -127,6 +127,27 @@ def step_euler(p, y, dt, ctl=None, world=None):
     return out, valid.astype(bool)
 
 
+def step_ars222(p, y, dt, ctl=None, world=None):
+    """One ARS(2,2,2) step of every world (its implicit part empty on a held bath)."""
+    return _explicit_step("ref_step_ars222", p, y, dt, ctl, world)
+
+
+def step_rk4(p, y, dt, ctl=None, world=None):
+    """One classical RK4 step of every world (the reference's empty tableau)."""
+    return _explicit_step("ref_step_rk4", p, y, dt, ctl, world)
+
+
+def _explicit_step(name, p, y, dt, ctl, world):
+    w = world if world is not None else world_struct(p)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(y)
+    valid = np.zeros(p.W, dtype=np.int32)
+    getattr(lib(), name)(ctypes.byref(w), _p(y, _f64p), _p(out, _f64p), ctypes.c_double(dt), ctypes.byref(ctl),
+                         _p(valid, _i32p))
+    return out, valid.astype(bool)
+
+
 def symbolic_structs(p, symbolics):
     """Symbolic structs per world from oracle.symbolic.bind results (one per world, in order),
     with each lattice entry's factor position."""
@@ -210,7 +231,7 @@ def rates(p, y, ctl=None, world=None):
 
 # IMEX Euler's energy bound (models/surface_pump.py:63); ROS2, which the reference lacks, takes it
 # too (its stiff start books energy errors above ARS(2,2,2)'s 2e-3 in the first steps)
-ENERGY_RTOL = {"euler": 2e-2, "ros2": 2e-2, "n2": 2e-3}  # N2: models/surface_pump.py:64
+ENERGY_RTOL = {"euler": 2e-2, "ros2": 2e-2, "n2": 2e-3, "ars222": 2e-3, "rk4": 0.0}  # surface_pump.py:63-64
 LEDGER_RATIOS = ("local", "total", "global_charge", "water", "outside", "charge", "electrode",
                  "energy", "reservoir_work", "electrode_work", "metabolic_work", "bath_water_work")
 ENERGY_TERMS = ("mixing", "capacitor", "heat", "work", "water", "residual", "flows")

@@ -45,6 +45,8 @@
 #define STANDARD_STATE_mM 1000.0
 #define WATER_MAX_RELAXATION 0.05
 #define ROS2_GAMMA (1.0 + 1.0 / 1.4142135623730951)
+#define ARS_GAMMA (1.0 - 1.0 / 1.4142135623730951)
+#define ARS_DELTA (1.0 - 1.0 / (2.0 * ARS_GAMMA))
 
 /* ---------------------------------------------------------------- point laws */
 
@@ -791,6 +793,164 @@ int ref_step_euler(const cells_world* w, const double* y, double* out, double dt
     return all;
 }
 
+/* The stage potentials' residual of world k's state (the diagonal operator solve of a lattice on
+ * its held bath, surface_imex.py:269-278 with no faces): each cell's C (phi - V_bath) against its
+ * node charge, 3e-25 + 2e-12 of the larger. */
+static int stage_potentials_ok(const cells_world* w, const cells_state_layout* L, int32_t k, const double* y,
+                               double* phi) {
+    int ok = 1;
+    ref_potentials(w, k, y, phi);
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double q = w->q0[i] + cell_charge_increment(w, L, y, i);
+        double cphi = w->cap[i] * (phi[i] - w->bath_V);
+        double scale = fabs(q) > fabs(cphi) ? fabs(q) : fabs(cphi);
+        ok = ok && isfinite(phi[i]) && fabs(cphi - q) <= 3e-25 + 2e-12 * scale;
+    }
+    return ok;
+}
+
+/* ARS(2,2,2) (numerics/surface_imex.py:53-57, cells.solver.imex_ars222_face_electromigration@1) as
+ * models/surface_pump_v3.py:3547-3955 imex_step runs it for cells on one held bath: with no faces
+ * the implicit part is empty and the operator is the capacitors alone, so each stage is its known
+ * part (stage 1: y + gamma dt E0; stage 2: (y + delta dt E0) + (1 - delta) dt E1), the stage
+ * potentials the capacitor solve's, every explicit stage checked against the membrane rate
+ * (:3640-3642, :3717-3737), the gates moved by the tableau. */
+int ref_step_ars222(const cells_world* w, const double* y, double* out, double dt, const cells_control* c,
+                    int32_t* valid_out) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const double gamma = ARS_GAMMA, delta = ARS_DELTA;
+    const size_t T = (size_t)L.total;
+    double* E0 = (double*)calloc(T, sizeof(double));
+    double* E1 = (double*)calloc(T, sizeof(double));
+    double* stage = (double*)malloc(sizeof(double) * T);
+    double* start = (double*)malloc(sizeof(double) * T);
+    double* phi = (double*)malloc(sizeof(double) * (size_t)(w->ncell ? w->ncell : 1));
+    int all = 1;
+    for (int32_t k = 0; k < w->W; ++k) {
+        int valid = 1;
+        copy_world(w, &L, k, y, start);
+        if (w->water) valid = water_half(w, &L, k, 0.5 * dt, c, start) && valid;
+        ref_potentials(w, k, start, phi);
+        valid = ref_rates(w, k, start, phi, c, E0, NULL, NULL) && valid;
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+            valid = valid && dt * ref_membrane_rate(w, start, i, c) <= 1.0;
+        /* stage 1 */
+        copy_world(w, &L, k, start, stage);
+        combine(w, &L, k, start, gamma * dt, E0, stage);
+        valid = stage_potentials_ok(w, &L, k, stage, phi) && valid;
+        valid = ref_rates(w, k, stage, phi, c, E1, NULL, NULL) && valid;
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+            valid = valid && dt * ref_membrane_rate(w, stage, i, c) <= 1.0;
+        /* stage 2: the new state */
+        copy_world(w, &L, k, start, out);
+        combine(w, &L, k, start, delta * dt, E0, out);
+        combine(w, &L, k, out, (1.0 - delta) * dt, E1, out);
+        valid = stage_potentials_ok(w, &L, k, out, phi) && valid;
+        double applied[4];
+        for (int s2 = 0; s2 < w->S; ++s2) {
+            double ex = (0.0 + delta * WF(&L, w, E0, L.sup + s2, k)) + (1.0 - delta) * WF(&L, w, E1, L.sup + s2, k);
+            applied[s2] = dt * (ex + 0.0);
+        }
+        for (int f = 0; f < L.world_fields; ++f)
+            if (f < L.sup || f >= L.sup + w->S) WF(&L, w, out, f, k) = WF(&L, w, start, f, k);
+        ionic_tail(w, &L, k, start, out, applied);
+        valid = valid && domain_valid(w, &L, k, out);
+        if (w->water) valid = water_half(w, &L, k, 0.5 * dt, c, out) && valid;
+        if (!valid) copy_world(w, &L, k, y, out);
+        valid_out[k] = valid;
+        all = all && valid;
+    }
+    free(E0); free(E1); free(stage); free(start); free(phi);
+    return all;
+}
+
+/* The reference's right-hand side of every leaf (models/surface_tissue.py:328-347
+ * derivative_from_field, with the pumps): the explicit rates, the electrode current (the held
+ * bath's plate-charge rate, minus the cells' charge rate), the reservoir work rate
+ * 1e-3 sum_s mu_s supply_s and the electrode work rate 1e12 V_bath current. */
+static int full_rhs(const cells_world* w, const cells_state_layout* L, int32_t k, const double* y,
+                    const cells_control* c, double* phi, double* r) {
+    ref_potentials(w, k, y, phi);
+    ref_rates(w, k, y, phi, c, r, NULL, NULL);
+    double current = 0.0;
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) current += cell_charge_increment(w, L, r, i);
+    current = -current;
+    const double RT = R_CONST * w->temperature;
+    double work = 0.0;
+    for (int s = 0; s < w->S; ++s) {
+        double mu = RT * log(w->bath_c[s]) + w->valence[s] * F_CONST * w->bath_V;
+        work += mu * WF(L, w, r, L->sup + s, k);
+    }
+    WF(L, w, r, L->echg, k) = current;
+    WF(L, w, r, L->rwork, k) = 1e-3 * work;
+    WF(L, w, r, L->ework, k) = 1e12 * w->bath_V * current;
+    int ok = 1;
+    for (int f = 0; f < L->cell_fields; ++f)
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) ok = ok && isfinite(CF(L, w, r, f, i));
+    for (int f = 0; f < L->edge_fields; ++f)
+        for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) ok = ok && isfinite(EF(L, w, r, f, e));
+    for (int f = 0; f < L->world_fields; ++f) ok = ok && isfinite(WF(L, w, r, f, k));
+    return ok;
+}
+
+/* trial = y + (f dt) r on every field of world k (numerics/fixed_step.py's increment) */
+static void rk4_trial(const cells_world* w, const cells_state_layout* L, int32_t k, const double* y, double h,
+                      const double* r, double* trial) {
+    for (int f = 0; f < L->cell_fields; ++f)
+        for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+            CF(L, w, trial, f, i) = CF(L, w, y, f, i) + h * CF(L, w, r, f, i);
+    for (int f = 0; f < L->edge_fields; ++f)
+        for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e)
+            EF(L, w, trial, f, e) = EF(L, w, y, f, e) + h * EF(L, w, r, f, e);
+    for (int f = 0; f < L->world_fields; ++f) WF(L, w, trial, f, k) = WF(L, w, y, f, k) + h * WF(L, w, r, f, k);
+}
+
+/* Classical RK4 with refusal at every trial and at the endpoint (numerics/fixed_step.py:25-71
+ * rk4_checked_step, the step of an empty tableau, models/surface_pump_v3.py:3989-3998): each
+ * trial's domain, each slope finite, the end's domain. Worlds without water (the reference's
+ * water runs RK4 on a coupled right-hand side). */
+int ref_step_rk4(const cells_world* w, const double* y, double* out, double dt, const cells_control* c,
+                 int32_t* valid_out) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const size_t T = (size_t)L.total;
+    double* kk[4];
+    for (int j = 0; j < 4; ++j) kk[j] = (double*)calloc(T, sizeof(double));
+    double* trial = (double*)malloc(sizeof(double) * T);
+    double* phi = (double*)malloc(sizeof(double) * (size_t)(w->ncell ? w->ncell : 1));
+    const double factor[4] = {0.0, 0.5, 0.5, 1.0};
+    int all = 1;
+    for (int32_t k = 0; k < w->W; ++k) {
+        int valid = !w->water;
+        copy_world(w, &L, k, y, trial);
+        for (int j = 0; j < 4; ++j) {
+            if (j > 0) rk4_trial(w, &L, k, y, factor[j] * dt, kk[j - 1], trial);
+            int ok = domain_valid(w, &L, k, trial);
+            if (ok) ok = full_rhs(w, &L, k, trial, c, phi, kk[j]);
+            valid = valid && ok;
+        }
+        for (int f = 0; f < L.cell_fields; ++f)
+            for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i)
+                CF(&L, w, out, f, i) = CF(&L, w, y, f, i) + (dt / 6.0) * (((CF(&L, w, kk[0], f, i) +
+                    2.0 * CF(&L, w, kk[1], f, i)) + 2.0 * CF(&L, w, kk[2], f, i)) + CF(&L, w, kk[3], f, i));
+        for (int f = 0; f < L.edge_fields; ++f)
+            for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e)
+                EF(&L, w, out, f, e) = EF(&L, w, y, f, e) + (dt / 6.0) * (((EF(&L, w, kk[0], f, e) +
+                    2.0 * EF(&L, w, kk[1], f, e)) + 2.0 * EF(&L, w, kk[2], f, e)) + EF(&L, w, kk[3], f, e));
+        for (int f = 0; f < L.world_fields; ++f)
+            WF(&L, w, out, f, k) = WF(&L, w, y, f, k) + (dt / 6.0) * (((WF(&L, w, kk[0], f, k) +
+                2.0 * WF(&L, w, kk[1], f, k)) + 2.0 * WF(&L, w, kk[2], f, k)) + WF(&L, w, kk[3], f, k));
+        valid = valid && domain_valid(w, &L, k, out);
+        if (!valid) copy_world(w, &L, k, y, out);
+        valid_out[k] = valid;
+        all = all && valid;
+    }
+    for (int j = 0; j < 4; ++j) free(kk[j]);
+    free(trial); free(phi);
+    return all;
+}
+
 /* ---------------------------------------------------------------- sparse factor and solve */
 
 typedef struct {
@@ -1107,8 +1267,6 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym, const double*
  * exponential update at each stage (stage_junction_gates :3127-3156, exponential_gates.advance
  * numerics/exponential_gates.py:44-61). */
 
-#define ARS_GAMMA (1.0 - 1.0 / 1.4142135623730951)
-#define ARS_DELTA (1.0 - 1.0 / (2.0 * ARS_GAMMA))
 #define GATE_SERIES_LIMIT 1e-3
 
 /* junction_voltage_relaxation.py:35-48 kinetics: the target and the relaxation rate */

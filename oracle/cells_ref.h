@@ -125,6 +125,14 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym /* [W] */, con
 int ref_step_n2(const cells_world* w, const cells_symbolic* sym, const double* y, double* out, double dt,
                 const cells_control* c, int32_t* valid_out);
 
+/* ARS(2,2,2) (numerics/surface_imex.py:53-57) for cells on a held bath, where its implicit part is
+ * empty; and the classical RK4 of an empty tableau (numerics/fixed_step.py:25-71, worlds without
+ * water), both as ref_step_euler is called. */
+int ref_step_ars222(const cells_world* w, const double* y, double* out, double dt, const cells_control* c,
+                    int32_t* valid_out);
+int ref_step_rk4(const cells_world* w, const double* y, double* out, double dt, const cells_control* c,
+                 int32_t* valid_out);
+
 /* numerics/exponential_gates.py:37-51: the lag weight 1 - (1 - e^-z) / z and the gate advance from
  * the start and end kinetics (targets, time constants in s) over dt. */
 double ref_lag_weight(double z);

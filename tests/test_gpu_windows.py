@@ -54,6 +54,10 @@ def test_windows_equal_one_call_and_observe_the_plain_run(torch, ens):
     s = res.summary(p)
     assert s["integrated_world_steps"] == 30 * p.W and abs(s["through_time_s"] - 0.15) < 1e-12
     assert len(s["final_voltage_V"]) == p.W
+    for k in ("model", "physical_model", "world_id_start", "deterministic_replicas", "compile_seconds",
+              "execute_seconds", "resources", "elapsed_seconds", "backend", "precision", "world_batching"):
+        assert k in s, k  # ensemble/ionic.py:182-205's fields
+    assert s["resources"]["executed_world_steps"] == 30 * p.W
     acc = s["ionic_accounting"]
     assert set(acc) == set(hip.LEDGER_RATIOS) and all(v <= 1.0 for v in acc.values()), acc
 
