@@ -2207,6 +2207,63 @@ __device__ int ledger_checks(const Params& P, const double* prev, const double* 
   return bits;
 }
 
+// The readouts' energy rates of each world at its state (models/surface_pump_v3.py:2221-2262
+// readouts, for cells on one held bath without water): from the rates R at the state's potentials,
+// out[kEnergyRates w + k] = the chemical rate 1e-3 RT sum ln(c) dN/dt, the capacitor rate
+// 1e12 sum C V dV/dt (dV = the capacitor solve of the charge rate), the dissipation (membrane,
+// pump and junction heat), the reservoir power 1e-3 sum mu_s supply_s, the electrode power
+// 1e12 V_bath I, the metabolic work rate, the energy balance chemical + capacitor + dissipation -
+// reservoir - electrode - metabolic (zero in exact arithmetic), and the electrode current I
+// (minus the cells' plate-charge rate).
+constexpr int kEnergyRates = 8;
+
+__global__ __launch_bounds__(kBlock) void k_energy_rates(Params P, const double* y, const double* R, double* out) {
+  __shared__ double sh[kBlock];
+  const int w = blockIdx.x;
+  const int c0 = P.cell_off[w], c1 = P.cell_off[w + 1], e0 = P.edge_off[w], e1 = P.edge_off[w + 1];
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // chemical, capacitor, dissipation, metabolic, plate rate
+  for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    double ci[4], vol;
+    cell_concentrations(P, y, y, i, ci, vol);
+    double chem = 0.0, zq = 0.0;
+    for (int s = 0; s < P.S; ++s) {
+      const double r = CFc(P, R, P.f_dn + s, i);
+      chem = s == 0 ? log(ci[s]) * r : chem + log(ci[s]) * r;
+      zq = s == 0 ? P.z[s] * r : zq + P.z[s] * r;
+    }
+    const double qdot = kChargePerFmol * zq, C = P.cap[i];
+    const double dvm = qdot / C;
+    v[0] += chem;
+    v[1] += C * (potential(P, y, i) - P.Vb) * dvm;
+    double heat = CFc(P, R, P.f_mdiss, i) + CFc(P, R, P.f_pdiss, i);
+    double work = CFc(P, R, P.f_pwork, i);
+    if (P.calcium) {
+      heat += CFc(P, R, P.f_cdiss, i);
+      work += CFc(P, R, P.f_cwork, i);
+    }
+    v[2] += heat;
+    v[3] += work;
+    v[4] += C * dvm;
+  }
+  for (int e = e0 + threadIdx.x; e < e1; e += kBlock) v[2] += EFc(P, R, P.f_jdiss, e);
+  block_fold<5, false>(v, sh);
+  if (threadIdx.x != 0) return;
+  double reservoir = 0.0;
+  for (int s = 0; s < P.S; ++s) reservoir += P.mu_b[s] * WFc(P, R, P.f_sup + s, w);
+  reservoir *= 1e-3;
+  const double current = -v[4];
+  const double chemical = 1e-3 * P.RT * v[0], capacitor = 1e12 * v[1], electrode = 1e12 * P.Vb * current;
+  double* o = out + (long)kEnergyRates * w;
+  o[0] = chemical;
+  o[1] = capacitor;
+  o[2] = v[2];
+  o[3] = reservoir;
+  o[4] = electrode;
+  o[5] = v[3];
+  o[6] = chemical + capacitor + v[2] - (reservoir + electrode) - v[3];
+  o[7] = current;
+}
+
 __global__ __launch_bounds__(kBlock) void k_device_checks(Params P, const double* prev, const double* y,
                                                           double steps, double material_atol,
                                                           int* failed, double* ratios) {
@@ -3888,6 +3945,16 @@ extern "C" int cells_rates(cells_ensemble* e, void* stream, const double* y,
                      nullptr, e->valid);
   hipLaunchKernelGGL(k_world_rates, dim3(e->W), dim3(kBlock), 0, st, P, e->contrib1, rates_out);
   if (valid_out) HIP_TRY(hipMemcpyAsync(valid_out, e->valid, sizeof(int) * e->W, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipGetLastError());
+  return CELLS_OK;
+}
+
+extern "C" int cells_energy_rates(cells_ensemble* e, void* stream, const double* y, const cells_control* control,
+                                  double* out) {
+  if (!e || !y || !control || !out) return fail(CELLS_EINVAL, "NULL argument");
+  if (int rc = cells_rates(e, stream, y, control, e->K1, nullptr)) return rc;
+  hipLaunchKernelGGL(k_energy_rates, dim3(e->W), dim3(kBlock), 0, static_cast<hipStream_t>(stream), e->P, e->A,
+                     e->K1, out);
   HIP_TRY(hipGetLastError());
   return CELLS_OK;
 }

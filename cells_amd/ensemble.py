@@ -153,6 +153,19 @@ class Ensemble:
                                            ctypes.c_void_p(x.data_ptr())))
         return x
 
+    def energy_rates(self, y, ctl=None):
+        """The readouts' energy rates of every world at y: [W, len(lib.ENERGY_RATES)] on the device
+        (pJ/s; the electrode current in A)."""
+        t = self.torch
+        y = self.to_device(y)
+        out = t.empty((self.packed.W, len(_lib.ENERGY_RATES)), dtype=t.float64, device=self.device)
+        c = ctl or control()
+        if self.packed.water:
+            raise ValueError("the energy readouts cover worlds without water")
+        _lib.check(_lib.load().cells_energy_rates(self._h, self._stream(), ctypes.c_void_p(y.data_ptr()),
+                                                  ctypes.byref(c), ctypes.c_void_p(out.data_ptr())))
+        return out
+
     def device_checks(self, previous, y, steps, material_atol=1e-12):
         """(failed bits [W], ratios [W, len(lib.LEDGER_RATIOS)]) of a state against its predecessor:
         the bits of lib.CHECKS, and per ledger of lib.LEDGER_RATIOS its largest |residual| / bound."""

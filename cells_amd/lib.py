@@ -40,7 +40,7 @@ EXPORTS = (
     "cells_field_state_size", "cells_field_step", "cells_field_refresh", "cells_field_plan_stats",
     "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials",
     "cells_world_factor_positions", "cells_relaxed_padding", "cells_set_plan_options",
-    "cells_get_plan_options", "cells_step_bounds", "cells_set_stage_tolerance", "cells_step_vjp_noalloc",
+    "cells_get_plan_options", "cells_step_bounds", "cells_set_stage_tolerance", "cells_step_vjp_noalloc", "cells_energy_rates",
 )
 
 # refusal reasons in a world's failure bits (cells_step_check_bits)
@@ -53,6 +53,9 @@ CHECKS = {
     "outside": 64, "charge": 128, "electrode": 256, "energy": 512, "reservoir_work": 1024,
     "electrode_work": 2048, "metabolic_work": 4096, "bath_water_work": 8192,
 }
+# the readouts' energy rates, in their column order (cells_energy_rates)
+ENERGY_RATES = ("chemical", "capacitor", "dissipation", "reservoir_power", "electrode_power", "metabolic",
+                "energy_balance", "electrode_current")
 # the ledgers of the bound ratios, in their column order (CELLS_RATIO_*)
 LEDGER_RATIOS = ("local", "total", "global_charge", "water", "outside", "charge", "electrode",
                  "energy", "reservoir_work", "electrode_work", "metabolic_work", "bath_water_work")
@@ -134,6 +137,8 @@ def load():
                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]
     lib.cells_rates.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.POINTER(ControlStruct), ctypes.c_void_p, ctypes.c_void_p]
+    lib.cells_energy_rates.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.POINTER(ControlStruct), ctypes.c_void_p]
     lib.cells_factor.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.cells_solve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_void_p, ctypes.c_void_p]

@@ -159,6 +159,14 @@ int cells_kernel_times(cells_ensemble* e, void* stream, const double* y, double 
 int cells_rates(cells_ensemble* e, void* stream, const double* y, const cells_control* control,
                 double* rates_out, int32_t* valid_out);
 
+/* The readouts' energy rates per world at y (models/surface_pump_v3.py:2221-2262 readouts'
+ * chemical_energy_rate_pJ_s, capacitor_energy_rate_pJ_s, dissipation_pJ_s, reservoir_power_pJ_s,
+ * electrode_power_pJ_s, the metabolic work rate, energy_balance_pJ_s and electrode_current_A), for
+ * worlds without water: out (device, double [W][CELLS_ENERGY_RATES]) in that order. */
+#define CELLS_ENERGY_RATES 8
+int cells_energy_rates(cells_ensemble* e, void* stream, const double* y, const cells_control* control,
+                       double* out);
+
 /* Device row checks of a state against its predecessor (models/surface_pump_checks.py:112
  * row_checks over models/surface_pump.py:685-696's LEDGERS with the water operator's,
  * operators/osmotic_water.py:240; bounds models/surface_pump.py:752-800 ledger_limits): the bits

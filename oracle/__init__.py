@@ -256,6 +256,23 @@ def ledgers(p, prev, y, steps, method, material_atol=1e-12, world=None):
     return bits, ratios, terms
 
 
+ENERGY_RATES = ("chemical", "capacitor", "dissipation", "reservoir_power", "electrode_power", "metabolic",
+                "energy_balance", "electrode_current")
+
+
+def energy_rates(p, y, ctl=None, world=None):
+    """The readouts' energy rates [W, 8] (ENERGY_RATES order) of every world at y."""
+    w = world if world is not None else world_struct(p)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros((p.W, len(ENERGY_RATES)))
+    for k in range(p.W):
+        o = np.zeros(len(ENERGY_RATES))
+        lib().ref_energy_rates(ctypes.byref(w), k, _p(y, _f64p), ctypes.byref(ctl), _p(o, _f64p))
+        out[k] = o
+    return out
+
+
 def world_symbolics(p):
     """oracle.symbolic.bind of every world's lattice pattern, and their structs."""
     from oracle.symbolic import bind

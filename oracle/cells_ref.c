@@ -951,6 +951,58 @@ int ref_step_rk4(const cells_world* w, const double* y, double* out, double dt, 
     return all;
 }
 
+/* The readouts' energy rates of world k at y (models/surface_pump_v3.py:2221-2262 readouts, cells
+ * on one held bath without water): out[8] = chemical, capacitor, dissipation, reservoir power,
+ * electrode power, metabolic work rate, energy balance, electrode current. */
+int ref_energy_rates(const cells_world* w, int32_t k, const double* y, const cells_control* c, double* out) {
+    cells_state_layout L;
+    ref_state_layout(w, &L);
+    const int S = w->S;
+    const double RT = R_CONST * w->temperature;
+    double* r = (double*)calloc((size_t)L.total, sizeof(double));
+    double* phi = (double*)malloc(sizeof(double) * (size_t)(w->ncell ? w->ncell : 1));
+    ref_potentials(w, k, y, phi);
+    int valid = ref_rates(w, k, y, phi, c, r, NULL, NULL);
+    double chem = 0, cap = 0, heat = 0, work = 0, plate = 0;
+    for (int64_t i = w->cell_off[k]; i < w->cell_off[k + 1]; ++i) {
+        double vol = cell_volume(w, &L, y, i), ch = 0.0;
+        for (int s = 0; s < S; ++s) {
+            double ci = (w->n0[(int64_t)s * w->ncell + i] + CF(&L, w, y, L.dn + s, i)) / vol;
+            ch = s == 0 ? log(ci) * CF(&L, w, r, L.dn + s, i) : ch + log(ci) * CF(&L, w, r, L.dn + s, i);
+        }
+        chem += ch;
+        double qdot = cell_charge_increment(w, &L, r, i), C = w->cap[i];
+        double dvm = qdot / C;
+        cap += C * (phi[i] - w->bath_V) * dvm;
+        plate += C * dvm;
+        heat += CF(&L, w, r, L.mdiss, i) + CF(&L, w, r, L.pdiss, i);
+        work += CF(&L, w, r, L.pwork, i);
+        if (w->calcium) {
+            heat += CF(&L, w, r, L.cdiss, i);
+            work += CF(&L, w, r, L.cwork, i);
+        }
+    }
+    for (int64_t e = w->edge_off[k]; e < w->edge_off[k + 1]; ++e) heat += EF(&L, w, r, L.jdiss, e);
+    double reservoir = 0.0;
+    for (int s = 0; s < S; ++s) {
+        double mu = RT * log(w->bath_c[s]) + w->valence[s] * F_CONST * w->bath_V;
+        reservoir += mu * WF(&L, w, r, L.sup + s, k);
+    }
+    reservoir *= 1e-3;
+    double current = -plate;
+    out[0] = 1e-3 * RT * chem;
+    out[1] = 1e12 * cap;
+    out[2] = heat;
+    out[3] = reservoir;
+    out[4] = 1e12 * w->bath_V * current;
+    out[5] = work;
+    out[6] = out[0] + out[1] + out[2] - (out[3] + out[4]) - out[5];
+    out[7] = current;
+    free(r);
+    free(phi);
+    return valid;
+}
+
 /* ---------------------------------------------------------------- sparse factor and solve */
 
 typedef struct {

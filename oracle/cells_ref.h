@@ -125,6 +125,11 @@ int ref_step_ros2(const cells_world* w, const cells_symbolic* sym /* [W] */, con
 int ref_step_n2(const cells_world* w, const cells_symbolic* sym, const double* y, double* out, double dt,
                 const cells_control* c, int32_t* valid_out);
 
+/* The readouts' energy rates of world k at y (models/surface_pump_v3.py:2221-2262): out[8] =
+ * chemical, capacitor, dissipation, reservoir power, electrode power, metabolic work rate, energy
+ * balance, electrode current. Worlds without water. Returns the rates' validity. */
+int ref_energy_rates(const cells_world* w, int32_t k, const double* y, const cells_control* c, double* out);
+
 /* ARS(2,2,2) (numerics/surface_imex.py:53-57) for cells on a held bath, where its implicit part is
  * empty; and the classical RK4 of an empty tableau (numerics/fixed_step.py:25-71, worlds without
  * water), both as ref_step_euler is called. */

@@ -140,3 +140,22 @@ def test_the_fused_epilogue_refuses_heat_that_never_flowed(torch):
     assert np.array_equal(out.cpu().numpy()[L.world_base:][2::p.W], bad[L.world_base:][2::p.W])
     e.step_checks(False)
     e.close()
+
+
+def test_energy_readouts_match_the_oracle_and_balance(torch):
+    """cells_energy_rates (the readouts' energy rates, models/surface_pump_v3.py:2221-2262) against
+    the oracle's at trajectory states: every term within 1e-12 of the dissipation, the balance
+    within 1e-12 of it (zero in exact arithmetic)."""
+    p = build(config1(worlds=3, cells=(40, 260), seed=9))
+    _, y = trajectory(p, "ros2", 5e-3, 30)
+    e = ensemble(p, "ros2")
+    got = e.energy_rates(y).cpu().numpy()
+    want = oracle.energy_rates(p, y)
+    d = want[:, hip.ENERGY_RATES.index("dissipation")]
+    say(f"energy readouts: balance / dissipation {(got[:, 6] / d).tolist()}")
+    for k in range(len(hip.ENERGY_RATES) - 1):
+        assert np.all(np.abs(got[:, k] - want[:, k]) <= 1e-12 * d), (hip.ENERGY_RATES[k], got[:, k], want[:, k])
+    assert np.all(np.abs(got[:, 6]) <= 1e-12 * d)
+    cur = hip.ENERGY_RATES.index("electrode_current")
+    assert np.allclose(got[:, cur], want[:, cur], rtol=1e-10, atol=1e-25)
+    e.close()

@@ -168,3 +168,42 @@ def test_water_ledgers_trip_on_their_own_fields(plain):
     L.cell(bad, L.wdiss)[2] = L.cell(prev, L.wdiss)[2] - 1.0
     bits, _, _ = oracle.ledgers(p, prev, bad, 400, "euler")
     assert bits[0] & hip_bit("dissipation")
+
+
+def test_the_readouts_energy_balance_vanishes_and_each_slip_breaks_it(calcium):
+    """The readouts' energy rates (models/surface_pump_v3.py:2221-2262) at any state: the free
+    energy's rate (chemical + capacitor) plus the dissipation less the work is zero in exact
+    arithmetic, since the chemical rate sum ln(c) dN/dt is the ledger's mixing term's derivative
+    and the capacitor rate its capacitor term's. The oracle's balance is below 1e-13 of the
+    dissipation; a capacitor term with the wrong sign, the mixing rate without its logarithm (the
+    concentration itself) or the pump heat left out each leave an imbalance."""
+    p, (prev, y) = calcium
+    L = StateLayout.of(p)
+    rng = np.random.default_rng(5)
+    states = [y, prev]
+    for _ in range(3):  # states off the trajectory: neutral salt moved, a little charge, gates
+        z = y.copy()
+        salt = rng.normal(scale=0.02, size=p.ncell) * p.n0[0]
+        L.cell(z, L.dn + 0)[:] += salt
+        L.cell(z, L.dn + 2)[:] += salt
+        L.cell(z, L.dn + 1)[:] += rng.normal(scale=1e-7, size=p.ncell) * p.n0[1]
+        L.edge(z, L.gate)[:] = rng.uniform(0.2, 1.0, size=p.nedge)
+        states.append(z)
+    names = oracle.ENERGY_RATES
+    for z in states:
+        er = oracle.energy_rates(p, z)
+        d = er[:, names.index("dissipation")]
+        assert np.all(d > 0)
+        assert np.all(np.abs(er[:, names.index("energy_balance")]) <= 1e-13 * d), er
+        cap = er[:, names.index("capacitor")]
+        assert np.all(np.abs(er[:, 6] - 2 * cap) > 1e3 * 1e-13 * d)  # the capacitor's sign
+        r, _ = oracle.rates(p, z)
+        for k in range(p.W):
+            c0, c1 = int(p.cell_off[k]), int(p.cell_off[k + 1])
+            vol = p.vol0[c0:c1]
+            lin = sum(np.sum((p.n0[s][c0:c1] + L.cell(z, L.dn + s)[c0:c1]) / vol * L.cell(r, L.dn + s)[c0:c1])
+                      for s in range(p.S))
+            slipped = er[k, 6] - er[k, 0] + 1e-3 * R * p.config.temperature * lin
+            assert abs(slipped) > 1e-6 * d[k]
+            pump_heat = np.sum(L.cell(r, L.pdiss)[c0:c1])
+            assert abs(er[k, 6] - pump_heat) > 1e-6 * d[k]
