@@ -494,7 +494,7 @@ def main():
 
         tune = autotune_ensemble(p, "ros2", dt, steps=3)
         log(f"autotuned plan options {tune.best} ({tune.ms_per_step:.2f} ms/step); candidates "
-            + ", ".join(f"{c['relax_zeros']}: {m:.2f}" for c, m in tune.table))
+            + "; ".join(f"{c}: {m:.2f}" for c, m in tune.table))
     log(f"{p.W} worlds, {p.ncell} cells, {p.nedge} edges; binding")
     t0 = time.perf_counter()
     e = Ensemble(p, "ros2")

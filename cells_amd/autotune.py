@@ -21,7 +21,7 @@ import time
 
 from cells_amd import lib as _lib
 
-_FIELDS = ("relax_rows", "relax_zeros", "grid_relax_zeros", "grid_subtrees")
+_FIELDS = ("relax_rows", "relax_zeros", "grid_relax_zeros", "grid_subtrees", "widen_groups", "largest_first")
 
 
 def plan_options() -> dict:
@@ -70,14 +70,17 @@ def _sweep(candidates, run, apply):
 
 def autotune_ensemble(packed, method="ros2", dt=5e-3, steps=3, candidates=None, apply=True) -> TuneResult:
     """Times `steps` captured steps of `packed` (after one warm step) per candidate set of lattice
-    plan options (default: explicit zeros per joined column 16, 64 or unbounded, fronts up to 128)."""
+    plan options (default: explicit zeros per joined column 16, 64 or unbounded with fronts up to
+    128; fronts up to 96; the level widening off; the worlds in their own order)."""
     import torch
 
     from cells_amd.ensemble import Ensemble
     from cells_amd.world import initial_state
 
     if candidates is None:
-        candidates = [{"relax_rows": 128, "relax_zeros": z} for z in (16, 64, 4096)]
+        base = {"relax_rows": 128, "relax_zeros": 64, "widen_groups": 1, "largest_first": 1}
+        candidates = [dict(base, relax_zeros=z) for z in (16, 64, 4096)]
+        candidates += [dict(base, relax_rows=96), dict(base, widen_groups=0), dict(base, largest_first=0)]
     y0 = initial_state(packed)
 
     def run():

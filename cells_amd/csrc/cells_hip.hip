@@ -179,6 +179,7 @@ struct PlanOptions {
   int grid_relax_zeros = 3;      // grid plans: explicit zeros per joined column
   int grid_subtrees = 32;        // grid plans: fronts at the depth where the tree is cut (at least)
   int widen_groups = 1;          // lattice plans: a level of few fronts takes wider lane groups
+  int largest_first = 1;         // per-world launches take the worlds largest first (else in order)
 };
 PlanOptions g_plan;
 
@@ -2941,22 +2942,27 @@ extern "C" {
 const char* cells_last_error(void) { return g_error.c_str(); }
 
 int cells_set_plan_options(int32_t relax_rows, int32_t relax_zeros, int32_t grid_relax_zeros,
-                           int32_t grid_subtrees) {
-  if (relax_rows < 1 || relax_rows > kMfMaxRows || relax_zeros < 0 || grid_relax_zeros < 0 || grid_subtrees < 1)
+                           int32_t grid_subtrees, int32_t widen_groups, int32_t largest_first) {
+  if (relax_rows < 1 || relax_rows > kMfMaxRows || relax_zeros < 0 || grid_relax_zeros < 0 || grid_subtrees < 1 ||
+      (widen_groups != 0 && widen_groups != 1) || (largest_first != 0 && largest_first != 1))
     return fail(CELLS_EINVAL, "plan options out of range");
   g_plan.relax_rows = relax_rows;
   g_plan.relax_zeros = relax_zeros;
   g_plan.grid_relax_zeros = grid_relax_zeros;
   g_plan.grid_subtrees = grid_subtrees;
+  g_plan.widen_groups = widen_groups;
+  g_plan.largest_first = largest_first;
   return CELLS_OK;
 }
 
 int cells_get_plan_options(int32_t* relax_rows, int32_t* relax_zeros, int32_t* grid_relax_zeros,
-                           int32_t* grid_subtrees) {
+                           int32_t* grid_subtrees, int32_t* widen_groups, int32_t* largest_first) {
   if (relax_rows) *relax_rows = g_plan.relax_rows;
   if (relax_zeros) *relax_zeros = g_plan.relax_zeros;
   if (grid_relax_zeros) *grid_relax_zeros = g_plan.grid_relax_zeros;
   if (grid_subtrees) *grid_subtrees = g_plan.grid_subtrees;
+  if (widen_groups) *widen_groups = g_plan.widen_groups;
+  if (largest_first) *largest_first = g_plan.largest_first;
   return CELLS_OK;
 }
 
@@ -3128,9 +3134,10 @@ int cells_ensemble_create(const cells_world* w, int32_t method, int32_t device,
   {
     std::vector<int> order(e->W);
     for (int k = 0; k < e->W; ++k) order[k] = k;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-      return w->cell_off[a + 1] - w->cell_off[a] > w->cell_off[b + 1] - w->cell_off[b];
-    });
+    if (g_plan.largest_first)
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return w->cell_off[a + 1] - w->cell_off[a] > w->cell_off[b + 1] - w->cell_off[b];
+      });
     UP_I(world_order, order);
   }
   UP_D(vol0, w->vol0, nc);

@@ -180,7 +180,8 @@ def load():
     lib.cells_symbolic_export.argtypes = [ctypes.c_void_p, _i32p, _i32p, _i64p, _i32p, _i32p,
                                           _i32p, _i32p]
     lib.cells_symbolic_free.argtypes = [ctypes.c_void_p]
-    lib.cells_set_plan_options.argtypes = [ctypes.c_int32] * 4
+    lib.cells_set_plan_options.argtypes = [ctypes.c_int32] * 6
+    lib.cells_get_plan_options.argtypes = [ctypes.c_void_p] * 6
     lib.cells_get_plan_options.argtypes = [_i32p] * 4
     lib.cells_relaxed_padding.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p, ctypes.c_int64,
                                           _i64p]

@@ -83,12 +83,14 @@ typedef struct cells_symbolic cells_symbolic;
  * reference; the schedule knobs the autotuner of cells_amd/autotune.py sweeps): the lattice
  * plans' relaxed supernodes (largest joined front, 1..128, default 128; explicit zeros per
  * joined column, default 64) and the grid plans' (explicit zeros per joined column, default 3;
- * the depth of the tree's cut: the first holding at least grid_subtrees fronts, default 32).
+ * the depth of the tree's cut: the first holding at least grid_subtrees fronts, default 32);
+ * widen_groups (0 or 1, default 1): a lattice level of few fronts takes wider lane groups;
+ * largest_first (0 or 1, default 1): the per-world launches take the worlds largest first.
  * Process-wide; set them between creates, not during one. */
 int cells_set_plan_options(int32_t relax_rows, int32_t relax_zeros, int32_t grid_relax_zeros,
-                           int32_t grid_subtrees);
+                           int32_t grid_subtrees, int32_t widen_groups, int32_t largest_first);
 int cells_get_plan_options(int32_t* relax_rows, int32_t* relax_zeros, int32_t* grid_relax_zeros,
-                           int32_t* grid_subtrees);
+                           int32_t* grid_subtrees, int32_t* widen_groups, int32_t* largest_first);
 typedef struct cells_ensemble cells_ensemble;
 
 const char* cells_last_error(void);

@@ -192,7 +192,7 @@ def front_orders(n):
 
     relax_rows, relax_zeros = (np.zeros(1, np.int32) for _ in range(2))
     hip.check(hip.load().cells_get_plan_options(relax_rows.ctypes.data_as(hip._i32p),
-                                                relax_zeros.ctypes.data_as(hip._i32p), None, None))
+                                                relax_zeros.ctypes.data_as(hip._i32p), None, None, None, None))
     snd, rcv = lattice(n)
     order, pr, pc = symbolic.relaxed_padding(n, snd, rcv, int(relax_rows[0]), int(relax_zeros[0]))
     s = symbolic.bind(n, np.concatenate((snd, pr)), np.concatenate((rcv, pc)), order)
