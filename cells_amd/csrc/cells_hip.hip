@@ -1249,6 +1249,79 @@ __device__ void mf_trailing_update(double* F, int m, int jb, int je, int lane) {
   }
 }
 
+// A double broadcast from lane `src` (uniform) of the wavefront.
+__device__ __forceinline__ double readlane_d(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)(unsigned)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Columns [jb, je) (at most kMfKB) of a packed lower front of order m <= 64 NT, factored by one
+// wavefront in its registers: lane lr holds rows lr + 64 t of the block's columns. Column j: pivot
+// d = sqrt(F(j, j)), the rows below times 1/d, then F(i, k) -= F(i, j) F(k, j) for the block's
+// columns k > j, each F(k, j) broadcast from the lane that holds row k. These are the operations
+// of the rank-1 steps in LDS, in the same order, so the block's values are the same bits.
+template <int NT, int STRIDE>
+__device__ __forceinline__ void mf_panel_registers(double* F, int m, int jb, int je, int lr) {
+  double P[NT][kMfKB];
+  const int kb = je - jb;
+#pragma unroll
+  for (int c = 0; c < kMfKB; ++c) {
+    const int j = jb + c;
+    const int cb = colbase(j, m);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int i = lr + 64 * t;
+      P[t][c] = (c < kb && i >= j && i < m) ? F[(cb + i) * STRIDE] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kMfKB; ++c) {
+    if (c < kb) {
+      const int j = jb + c;
+      double pv = P[0][c];
+#pragma unroll
+      for (int t = 1; t < NT; ++t)
+        if ((j >> 6) == t) pv = P[t][c];
+      const double d = sqrt(readlane_d(pv, j & 63));
+      const double rinv = 1.0 / d;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int i = lr + 64 * t;
+        if (i > j && i < m) P[t][c] = P[t][c] * rinv;
+        else if (i == j) P[t][c] = d;
+      }
+#pragma unroll
+      for (int c2 = c + 1; c2 < kMfKB; ++c2) {
+        if (c2 < kb) {
+          const int k = jb + c2;
+          double bv = P[0][c];
+#pragma unroll
+          for (int t = 1; t < NT; ++t)
+            if ((k >> 6) == t) bv = P[t][c];
+          const double bk = readlane_d(bv, k & 63);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int i = lr + 64 * t;
+            if (i >= k && i < m) P[t][c2] -= P[t][c] * bk;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kMfKB; ++c) {
+    const int j = jb + c;
+    const int cb = colbase(j, m);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int i = lr + 64 * t;
+      if (c < kb && i >= j && i < m) F[(cb + i) * STRIDE] = P[t][c];
+    }
+  }
+}
+
 // One front, by G lanes (this lane = lane) with its packed triangle at F[e * STRIDE]; fronts of at
 // most MR rows.
 // The stage matrix's entries scattered straight into the fronts (the lattice factor): each front
@@ -1266,7 +1339,9 @@ struct MfAssembly {
   int c0, e0;           // the world's first cell and junction
 };
 
-template <int G, int STRIDE, int MR = FrontLanes<G>::rows, bool ASM = false>
+// REG_PANEL: wavefront and workgroup fronts of at most 128 rows take each column block's rank-1
+// steps in one wavefront's registers (mf_panel_registers) instead of in LDS.
+template <int G, int STRIDE, int MR = FrontLanes<G>::rows, bool ASM = false, bool REG_PANEL = true>
 __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double* F, int lane, int* valid_w,
                          const MfAssembly* am = nullptr) {
   using FL = FrontLanes<G>;
@@ -1335,56 +1410,66 @@ __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double
     }
     group_sync<G>();
   }
-  // partial factor over the supernode's columns: lanes own rows i = jj + 1 + lr + t RW. Groups of
-  // whole wavefronts work in blocks of kMfKB columns: rank-1 steps inside the block, then one
-  // rank-kMfKB update of everything to its right on the matrix cores. In a workgroup's front the
-  // first wavefront alone takes the block's rank-1 steps (wavefront barriers only), so the
-  // workgroup meets twice per block instead of three times per column.
+  // partial factor over the supernode's columns. Groups of whole wavefronts work in blocks of
+  // kMfKB columns: the block's rank-1 steps in the registers of one wavefront (in a workgroup's
+  // front the first; in LDS for fronts above 128 rows), then one rank-kMfKB update of everything
+  // to its right on the matrix cores, so the group meets twice per block. Narrower groups take
+  // rank-1 steps over the whole front in LDS (lanes own rows i = jj + 1 + lr + t RW).
   constexpr int KB = G >= 64 ? kMfKB : 1 << 20;
-  constexpr bool WAVE_PANEL = G > 64;
-  constexpr int PCG = WAVE_PANEL ? 1 : CG;  // column groups of the rank-1 steps
   for (int jb = 0; jb < J; jb += KB) {
     const int je = J < jb + KB ? J : jb + KB;
     const int kend = G >= 64 ? je : m;  // columns the rank-1 steps update
-    if (!WAVE_PANEL || lc == 0) {
-      for (int jj = jb; jj < je; ++jj) {
-        const int cb = colbase(jj, m);
-        const double d = sqrt(F[(cb + jj) * STRIDE]);
-        const double rinv = 1.0 / d;
-        double av[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const int i = jj + 1 + lr + t * RW;
-          av[t] = i < m ? F[(cb + i) * STRIDE] * rinv : 0.0;
-        }
-        if constexpr (WAVE_PANEL) wave_sync();
-        else group_sync<G>();  // every lane has read the pivot and its rows
-        if (lc == 0) {
+    if constexpr (REG_PANEL && G >= 64 && NT <= 2) {
+      // the block's columns in one wavefront's registers (lane lr holds rows lr + 64 t), their
+      // rank-1 steps with the pivots and the block's rows broadcast from the owning lanes: the
+      // same operations in the same order as the steps in LDS below, with no barrier inside
+      if (lc == 0) mf_panel_registers<NT, STRIDE>(F, m, jb, je, lr);
+      group_sync<G>();  // the block's columns are factored
+    } else {
+      // rank-1 steps in LDS: over the whole front for narrower groups; for wavefront groups (fronts
+      // above 128 rows) over the block, by the first wavefront alone in a workgroup's front
+      constexpr bool WAVE_PANEL = G > 64;
+      constexpr int PCG = WAVE_PANEL ? 1 : CG;  // column groups of the rank-1 steps
+      if (!WAVE_PANEL || lc == 0) {
+        for (int jj = jb; jj < je; ++jj) {
+          const int cb = colbase(jj, m);
+          const double d = sqrt(F[(cb + jj) * STRIDE]);
+          const double rinv = 1.0 / d;
+          double av[NT];
 #pragma unroll
           for (int t = 0; t < NT; ++t) {
             const int i = jj + 1 + lr + t * RW;
-            if (i < m) F[(cb + i) * STRIDE] = av[t];
+            av[t] = i < m ? F[(cb + i) * STRIDE] * rinv : 0.0;
           }
-        }
-        if (lane == 0) F[(cb + jj) * STRIDE] = d;
-        if constexpr (WAVE_PANEL) wave_sync();
-        else group_sync<G>();  // the scaled column is in place
-        const int pc = WAVE_PANEL ? 0 : lc;
+          if constexpr (WAVE_PANEL) wave_sync();
+          else group_sync<G>();  // every lane has read the pivot and its rows
+          if (lc == 0) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              const int i = jj + 1 + lr + t * RW;
+              if (i < m) F[(cb + i) * STRIDE] = av[t];
+            }
+          }
+          if (lane == 0) F[(cb + jj) * STRIDE] = d;
+          if constexpr (WAVE_PANEL) wave_sync();
+          else group_sync<G>();  // the scaled column is in place
+          const int pc = WAVE_PANEL ? 0 : lc;
 #pragma unroll 4
-        for (int k = jj + 1 + pc; k < kend; k += PCG) {
-          const double bk = F[(cb + k) * STRIDE];
-          const int ck = colbase(k, m);
+          for (int k = jj + 1 + pc; k < kend; k += PCG) {
+            const double bk = F[(cb + k) * STRIDE];
+            const int ck = colbase(k, m);
 #pragma unroll
-          for (int t = 0; t < NT; ++t) {
-            const int i = jj + 1 + lr + t * RW;
-            if (i >= k && i < m) F[(ck + i) * STRIDE] -= av[t] * bk;
+            for (int t = 0; t < NT; ++t) {
+              const int i = jj + 1 + lr + t * RW;
+              if (i >= k && i < m) F[(ck + i) * STRIDE] -= av[t] * bk;
+            }
           }
+          if constexpr (WAVE_PANEL) wave_sync();
+          else group_sync<G>();
         }
-        if constexpr (WAVE_PANEL) wave_sync();
-        else group_sync<G>();
       }
+      if constexpr (WAVE_PANEL) group_sync<G>();  // the block's columns are factored
     }
-    if constexpr (WAVE_PANEL) group_sync<G>();  // the block's columns are factored
     if constexpr (G >= 64) {
       if (je < m) {
         mf_trailing_update<G>(F, m, jb, je, lane);
