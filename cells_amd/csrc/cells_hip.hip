@@ -190,6 +190,7 @@ constexpr int max_order(int tri) {
   return m;
 }
 constexpr int kMfBatch = 8;                                // global loads in flight per lane
+constexpr int kAsmBatch = 4;                               // a front's assembled entries per lane per pass
 constexpr int kRecInts = 16;                               // ints per front record in the plan pool
 constexpr int kLevelBounds = 6;                            // grid plans, per level: first front of each kind, end
 
@@ -767,8 +768,8 @@ struct PlanView {
 // when bit j of n is set. Adding a term merges it, oldest block on the left, into every block the
 // increment of n carries through; the total folds the blocks from the newest up. The loops are
 // unrolled over a fixed depth so every partial stays in registers.
-struct PairSum {
-  static constexpr int kDepth = 13;  // up to 8191 terms
+template <int kDepth>
+struct PairSumT {  // up to 2^kDepth - 1 terms
   double s[kDepth];
   unsigned n;
   __device__ __forceinline__ void reset() { n = 0u; }
@@ -797,6 +798,27 @@ struct PairSum {
     return r;
   }
 };
+using PairSum = PairSumT<13>;  // up to 8191 terms
+
+// y = A x on a product plan (numerics/sparse_cholesky.py:467-471 `product`): thread per row, the
+// row's products value x x[column] formed and rounded one by one in the plan's (ascending column)
+// order and summed in adjacent pairs, step by step (the reference's fold), by the binary-counter
+// stack; an empty row gives +0.0. blockIdx.y is the system: its values, x and y at the strides.
+__global__ __launch_bounds__(256) void k_product(int n, const long long* row_start, const int* column,
+                                                 const double* values, const double* x, double* y,
+                                                 long long values_stride, long long vector_stride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* v = values + (long long)blockIdx.y * values_stride;
+  const double* xb = x + (long long)blockIdx.y * vector_stride;
+  PairSumT<31> pw;
+  pw.reset();
+  for (long long p = row_start[i]; p < row_start[i + 1]; ++p) {
+    const double t = v[p] * xb[column[p]];
+    pw.push(t);
+  }
+  y[(long long)blockIdx.y * vector_stride + i] = pw.sum();
+}
 
 __device__ PlanView plan_view(const ShapeDev& s, const int* pool,
                               const unsigned long long* masks = nullptr) {
@@ -1354,16 +1376,54 @@ __device__ void mf_front(const int* pool, int qa, double* fv, double* ub, double
   const int S = m - J, tri = m * (m + 1) / 2, Pn = colbase(J, m) + J;
   if constexpr (ASM) {
     const int2 ec = reinterpret_cast<const int2*>(pool)[(kRecInts * qa + 10) / 2];  // junction list
-    for (int e = lane; e < tri; e += G) F[e * STRIDE] = 0.0;
-    group_sync<G>();
-    for (int jj = lane; jj < J; jj += G) F[(colbase(jj, m) + jj) * STRIDE] = am->mdiag[am->c0 + am->order[rb4.w + jj]];
     const int2* el = reinterpret_cast<const int2*>(pool + ec.x);
-    for (int x = lane; x < ec.y; x += G) {
-      const int2 le = el[x];  // (front-local packed position, world-local junction)
-      const int e = am->e0 + le.y;
-      const double val = -(am->coef * am->Gj[e]);
-      const double sa = am->msc[am->snd[e]], sb = am->msc[am->rcv[e]];
-      F[le.x * STRIDE] = (val * sa * sb + val * sb * sa) / 2;
+    // kAsmBatch diagonal entries and kAsmBatch junction entries per lane at a time, each level of
+    // their index chains (column -> cell -> diagonal; entry -> junction -> ends -> scales) loaded
+    // for the whole batch before the next; the first batch's first level goes out before the
+    // front is cleared
+    for (int b0 = 0; b0 < J || b0 < ec.y; b0 += kAsmBatch * G) {
+      int cell[kAsmBatch];
+      int2 le[kAsmBatch];  // (front-local packed position, world-local junction)
+#pragma unroll
+      for (int u = 0; u < kAsmBatch; ++u) {
+        const int jj = b0 + lane + u * G;
+        cell[u] = jj < J ? am->order[rb4.w + jj] : -1;
+        le[u] = jj < ec.y ? el[jj] : make_int2(-1, 0);
+      }
+      if (b0 == 0) {
+        for (int e = lane; e < tri; e += G) F[e * STRIDE] = 0.0;
+        group_sync<G>();
+      }
+      double dv[kAsmBatch], gv[kAsmBatch];
+      int ea[kAsmBatch], eb[kAsmBatch];
+#pragma unroll
+      for (int u = 0; u < kAsmBatch; ++u) {
+        dv[u] = cell[u] >= 0 ? am->mdiag[am->c0 + cell[u]] : 0.0;
+        const int e = am->e0 + le[u].y;
+        const bool j_in = le[u].x >= 0;
+        gv[u] = j_in ? am->Gj[e] : 0.0;
+        ea[u] = j_in ? am->snd[e] : 0;
+        eb[u] = j_in ? am->rcv[e] : 0;
+      }
+      double sa[kAsmBatch], sb[kAsmBatch];
+#pragma unroll
+      for (int u = 0; u < kAsmBatch; ++u) {
+        const bool j_in = le[u].x >= 0;
+        sa[u] = j_in ? am->msc[ea[u]] : 0.0;
+        sb[u] = j_in ? am->msc[eb[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kAsmBatch; ++u) {
+        const int jj = b0 + lane + u * G;
+        if (cell[u] >= 0) F[(colbase(jj, m) + jj) * STRIDE] = dv[u];
+        if (le[u].x >= 0) {
+          const double val = -(am->coef * gv[u]);
+          F[le[u].x * STRIDE] = (val * sa[u] * sb[u] + val * sb[u] * sa[u]) / 2;
+        }
+      }
+    }
+    if (J == 0 && ec.y == 0) {
+      for (int e = lane; e < tri; e += G) F[e * STRIDE] = 0.0;
     }
   } else {
     // the supernode's columns are contiguous in the factor and are the front's first J columns;
@@ -3048,6 +3108,46 @@ int cells_get_plan_options(int32_t* relax_rows, int32_t* relax_zeros, int32_t* g
   if (grid_subtrees) *grid_subtrees = g_plan.grid_subtrees;
   if (widen_groups) *widen_groups = g_plan.widen_groups;
   if (largest_first) *largest_first = g_plan.largest_first;
+  return CELLS_OK;
+}
+
+int cells_product_plan_size(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, int64_t* nnz) {
+  if (!nnz || (m > 0 && (!rows || !cols))) return fail(CELLS_EINVAL, "NULL argument");
+  if (n < 1) return fail(CELLS_EINVAL, "a sparse pattern needs a positive size");
+  try {
+    *nnz = (int64_t)cells::product_plan(n, m, rows, cols).column.size();
+    return CELLS_OK;
+  } catch (const std::exception& ex) {
+    return fail(CELLS_EINVAL, ex.what());
+  }
+}
+
+int cells_product_plan(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, int64_t* row_start,
+                       int32_t* column, int64_t* diagonal) {
+  if (!row_start || !column || !diagonal || (m > 0 && (!rows || !cols))) return fail(CELLS_EINVAL, "NULL argument");
+  if (n < 1) return fail(CELLS_EINVAL, "a sparse pattern needs a positive size");
+  try {
+    const cells::ProductPlan p = cells::product_plan(n, m, rows, cols);
+    std::copy(p.row_start.begin(), p.row_start.end(), row_start);
+    std::copy(p.column.begin(), p.column.end(), column);
+    std::copy(p.diagonal.begin(), p.diagonal.end(), diagonal);
+    return CELLS_OK;
+  } catch (const std::exception& ex) {
+    return fail(CELLS_EINVAL, ex.what());
+  }
+}
+
+int cells_product(void* stream, int32_t n, const int64_t* row_start, const int32_t* column, const double* values,
+                  const double* x, double* y, int32_t count, int64_t values_stride, int64_t vector_stride) {
+  if (n < 0 || count < 0 || values_stride < 0 || vector_stride < 0) return fail(CELLS_EINVAL, "negative size or stride");
+  if (n == 0 || count == 0) return CELLS_OK;
+  if (!row_start || !column || !values || !x || !y) return fail(CELLS_EINVAL, "NULL argument");
+  if (count > 65535) return fail(CELLS_EINVAL, "at most 65535 systems per call");
+  if (x == y) return fail(CELLS_EINVAL, "y must not be x");
+  hipLaunchKernelGGL(k_product, dim3((n + 255) / 256, count), dim3(256), 0, (hipStream_t)stream, n,
+                     reinterpret_cast<const long long*>(row_start), column, values, x, y, (long long)values_stride,
+                     (long long)vector_stride);
+  if (hipGetLastError() != hipSuccess) return fail(CELLS_EHIP, "k_product launch failed");
   return CELLS_OK;
 }
 
