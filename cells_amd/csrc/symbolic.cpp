@@ -32,7 +32,7 @@ Adjacency neighbours(int32_t n, int64_t m, const int32_t* rows, const int32_t* c
   Adjacency adj(n);
   for (int64_t q = 0; q < m; ++q) {
     int32_t i = rows[q], j = cols[q];
-    if (i < 0 || j < 0 || i >= n || j >= n) throw std::invalid_argument("pattern outside its size");
+    if (i < 0 || j < 0 || i >= n || j >= n) throw std::invalid_argument("A sparse pattern's coordinates must lie within its size");
     if (i == j) continue;
     adj[i].push_back(j);
     adj[j].push_back(i);
@@ -170,6 +170,24 @@ Symbolic bind(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, co
   std::vector<int32_t> lf(s.level_ptr.begin(), s.level_ptr.end() - 1);
   for (int32_t k = 0; k < n; ++k) s.level_col[lf[s.level[k]]++] = k;
   return s;
+}
+
+ProductPlan product_plan(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols) {
+  // numerics/sparse_cholesky.py:444-464: row i's entries are its neighbours and itself, ascending
+  Adjacency adj = neighbours(n, m, rows, cols);
+  ProductPlan p;
+  p.row_start.assign(1, 0);
+  p.diagonal.resize(n);
+  for (int32_t i = 0; i < n; ++i) {
+    const auto& a = adj[i];
+    auto at = std::lower_bound(a.begin(), a.end(), i);
+    p.column.insert(p.column.end(), a.begin(), at);
+    p.diagonal[i] = static_cast<int64_t>(p.column.size());
+    p.column.push_back(i);
+    p.column.insert(p.column.end(), at, a.end());
+    p.row_start.push_back(static_cast<int64_t>(p.column.size()));
+  }
+  return p;
 }
 
 int64_t position(const Symbolic& s, int32_t i, int32_t j) {

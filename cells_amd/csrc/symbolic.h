@@ -63,6 +63,16 @@ void share_update_storage(Supernodes& sn);
 // the joined fronts.
 std::vector<int32_t> relaxed_padding(const Symbolic& s, int32_t max_rows, int32_t max_zeros);
 
+// A matrix-vector product's plan on a symmetric pattern with its whole diagonal: each row's
+// entries (its neighbours and itself) in ascending column order, rows in order (CSR), and each
+// row's diagonal entry.
+struct ProductPlan {
+  std::vector<int64_t> row_start;  // [n+1]
+  std::vector<int32_t> column;     // [nnz]
+  std::vector<int64_t> diagonal;   // [n]
+};
+ProductPlan product_plan(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols);
+
 std::vector<int32_t> minimum_degree(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols);
 Symbolic bind(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, const int32_t* order);
 int64_t position(const Symbolic& s, int32_t i, int32_t j);

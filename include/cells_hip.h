@@ -123,6 +123,23 @@ void cells_symbolic_free(cells_symbolic* s);
 int cells_relaxed_padding(const cells_symbolic* s, int32_t max_rows, int32_t max_zeros, int32_t* pairs,
                           int64_t capacity, int64_t* count);
 
+/* product_plan (numerics/sparse_cholesky.py:444-464): the matrix-vector product's plan on the
+ * symmetric pattern of n nodes with off-diagonal coordinates (rows, cols) in either or both
+ * triangles and the whole diagonal. Row i's entries are its neighbours and itself in ascending
+ * column order, rows in order (a value array on the pattern is row-major): row_start [n+1],
+ * column [nnz], each row's diagonal entry [n]. The size call gives nnz. Malformed patterns are
+ * CELLS_EINVAL, as _neighbours (:122-135) refuses them. */
+int cells_product_plan_size(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, int64_t* nnz);
+int cells_product_plan(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, int64_t* row_start,
+                       int32_t* column, int64_t* diagonal);
+/* product (numerics/sparse_cholesky.py:467-471): y = A x on the device for `count` systems on one
+ * plan (system b: values + b values_stride, x and y + b vector_stride), each row's products formed
+ * one by one and summed in adjacent pairs, step by step (the reference's fold, :311-322), so the
+ * result is the reference's bit for bit. row_start, column, values, x and y are device pointers;
+ * y must not overlap x or values; stream-ordered. */
+int cells_product(void* stream, int32_t n, const int64_t* row_start, const int32_t* column, const double* values,
+                  const double* x, double* y, int32_t count, int64_t values_stride, int64_t vector_stride);
+
 /* ---- device ensemble: checked_step under the execution layer's loops ---- */
 
 /* Binds the worlds once (models/surface_pump_v3.py:6851 model_for; numerics/surface_field.py:371

@@ -15,6 +15,13 @@ Restates src/cells/numerics/sparse_cholesky.py of the reference:
 - `row_structures` follows `_row_structures` (sparse_cholesky.py:229-235).
 - `pairwise_tree_sum` is the fold's order (`_pairwise_steps`/`fold`, sparse_cholesky.py:244-322):
   adjacent pairs, step by step, an odd tail passing through.
+- `product_plan`  follows `product_plan`     (sparse_cholesky.py:444-464): each row's entries, its
+  neighbours and itself in ascending column order, rows in order (so the value array on the
+  pattern is row-major), each row's diagonal position; the row sums' pairwise plan is the fold
+  over each row's entries in that order.
+- `product`       follows `product`          (sparse_cholesky.py:467-471): each row's products
+  value x x[column], summed by `pairwise_tree_sum` (the products are formed and rounded before the
+  sum, as the reference's optimization barrier keeps them).
 
 Pinned by: the O(n^2) definition of minimum degree (tests/unit/test_sparse_cholesky.py:198-211 of
 the reference, restated in tests/test_oracle_symbolic.py), the arrowhead special cases
@@ -133,3 +140,42 @@ def pairwise_tree_sum(values):
         values = [values[i] + values[i + 1] if i + 1 < len(values) else values[i]
                   for i in range(0, len(values), 2)]
     return values[0]
+
+
+class ProductPlan:
+    """A product's plan on a symmetric pattern with its whole diagonal, as row starts (CSR)."""
+
+    def __init__(self, row_start, column, diagonal):
+        self.row_start = row_start  # [n + 1] int64
+        self.column = column  # [nnz] int64, ascending within each row
+        self.diagonal = diagonal  # [n] int64: each row's diagonal entry
+        self.row = np.repeat(np.arange(row_start.size - 1, dtype=np.int64), np.diff(row_start))
+
+
+def product_plan(size, rows, columns):
+    """The ProductPlan of the symmetric pattern of `size` nodes with off-diagonal entries at (rows,
+    columns), either or both triangles, and its whole diagonal."""
+    adjacency = neighbours(size, rows, columns)
+    row_start, column, diagonal = [0], [], []
+    for i in range(size):
+        entries = sorted(adjacency[i] | {i})
+        diagonal.append(row_start[-1] + entries.index(i))
+        column.extend(entries)
+        row_start.append(len(column))
+    return ProductPlan(np.asarray(row_start, dtype=np.int64), np.asarray(column, dtype=np.int64),
+                       np.asarray(diagonal, dtype=np.int64))
+
+
+def product(values, x, plan):
+    """A x for the matrix A whose entries on the plan's pattern are `values` (x [n] or [n, k]; its
+    trailing axes are independent): each row's products summed in adjacent pairs, step by step."""
+    values = np.asarray(values, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    flat = x.reshape(x.shape[0], -1)
+    y = np.empty_like(flat)
+    for i in range(plan.row_start.size - 1):
+        a, b = int(plan.row_start[i]), int(plan.row_start[i + 1])
+        terms = values[a:b, None] * flat[plan.column[a:b]]
+        for k in range(flat.shape[1]):
+            y[i, k] = pairwise_tree_sum(terms[:, k])
+    return y.reshape(x.shape)
