@@ -3111,6 +3111,33 @@ int cells_get_plan_options(int32_t* relax_rows, int32_t* relax_zeros, int32_t* g
   return CELLS_OK;
 }
 
+int cells_capacitor_topology(int32_t n, int32_t o, int64_t m, const int32_t* membrane_cell,
+                             const int32_t* membrane_outside, const uint8_t* membrane_active, const uint8_t* cell_active,
+                             const uint8_t* outside_active, const uint8_t* reservoir, const uint8_t* gauge, int64_t f,
+                             const int32_t* face_sender, const int32_t* face_receiver, const uint8_t* face_conductive,
+                             int64_t r, const int32_t* return_paths, int32_t* component, int32_t* floating,
+                             uint8_t* fixed, uint8_t* offset_free, int32_t* counts) {
+  if (n < 0 || o < 1 || m < 0 || f < 0 || r < 0) return fail(CELLS_EINVAL, "negative layout size");
+  if ((m > 0 && (!membrane_cell || !membrane_outside || !membrane_active)) || (n > 0 && !cell_active) ||
+      !outside_active || !reservoir || !gauge || (f > 0 && (!face_sender || !face_receiver || !face_conductive)) ||
+      (r > 0 && !return_paths) || !counts)
+    return fail(CELLS_EINVAL, "NULL argument");
+  try {
+    const cells::CapacitorTopology t =
+        cells::capacitor_topology(n, o, m, membrane_cell, membrane_outside, membrane_active, cell_active, outside_active,
+                                  reservoir, gauge, f, face_sender, face_receiver, face_conductive, r, return_paths);
+    counts[0] = t.components;
+    counts[1] = (int32_t)t.offset_free.size();
+    if (component) std::copy(t.component.begin(), t.component.end(), component);
+    if (floating) std::copy(t.floating.begin(), t.floating.end(), floating);
+    if (fixed) std::copy(t.fixed.begin(), t.fixed.end(), fixed);
+    if (offset_free) std::copy(t.offset_free.begin(), t.offset_free.end(), offset_free);
+    return CELLS_OK;
+  } catch (const std::exception& ex) {
+    return fail(CELLS_EINVAL, ex.what());
+  }
+}
+
 int cells_product_plan_size(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, int64_t* nnz) {
   if (!nnz || (m > 0 && (!rows || !cols))) return fail(CELLS_EINVAL, "NULL argument");
   if (n < 1) return fail(CELLS_EINVAL, "a sparse pattern needs a positive size");

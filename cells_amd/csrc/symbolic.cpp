@@ -19,6 +19,7 @@
 #include <map>
 #include <queue>
 #include <stdexcept>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -188,6 +189,120 @@ ProductPlan product_plan(int32_t n, int64_t m, const int32_t* rows, const int32_
     p.row_start.push_back(static_cast<int64_t>(p.column.size()));
   }
   return p;
+}
+
+CapacitorTopology capacitor_topology(int32_t n, int32_t o, int64_t m, const int32_t* membrane_cell,
+                                     const int32_t* membrane_outside, const uint8_t* membrane_active,
+                                     const uint8_t* cell_active, const uint8_t* outside_active,
+                                     const uint8_t* reservoir, const uint8_t* gauge, int64_t f,
+                                     const int32_t* face_sender, const int32_t* face_receiver,
+                                     const uint8_t* face_conductive, int64_t r, const int32_t* return_paths) {
+  // numerics/surface_field.py:253-340 (bind) with representative (:238-250); the layout checks of
+  // numerics/surface_transport.py:194-195 that bind relies on
+  if (n < 0 || o < 1) throw std::invalid_argument("Surface geometry requires active cell and outside owners");
+  const int32_t size = n + o;
+  bool any_reservoir = false;
+  int32_t gauges = 0;
+  for (int32_t k = 0; k < o; ++k) {
+    if (reservoir[k] && !outside_active[k]) throw std::invalid_argument("Reservoir must be an active outside owner");
+    any_reservoir = any_reservoir || reservoir[k];
+    gauges += gauge[k] ? 1 : 0;
+    if (gauge[k] && !outside_active[k])
+      throw std::invalid_argument("Use reservoir voltage clamps or exactly one active outside gauge");
+  }
+  if (gauges != (any_reservoir ? 0 : 1))
+    throw std::invalid_argument("Use reservoir voltage clamps or exactly one active outside gauge");
+  std::vector<int32_t> parent(size);
+  for (int32_t i = 0; i < size; ++i) parent[i] = i;
+  auto root = [&](int32_t i) {
+    while (parent[i] != i) i = parent[i] = parent[parent[i]];
+    return i;
+  };
+  auto join = [&](int32_t i, int32_t j) { parent[root(j)] = root(i); };
+  auto node_of = [&](int64_t k, const int32_t* idx, int32_t count, int32_t base, const char* key) {
+    if (idx[k] < 0 || idx[k] >= count) throw std::invalid_argument(std::string("Invalid resolved-surface owner: ") + key);
+    return base + idx[k];
+  };
+  auto active = [&](int32_t i) { return i < n ? cell_active[i] != 0 : outside_active[i - n] != 0; };
+  for (int64_t k = 0; k < m; ++k) {
+    const int32_t i = node_of(k, membrane_cell, n, 0, "membrane_cell");
+    const int32_t j = node_of(k, membrane_outside, o, n, "membrane_outside");
+    if (!membrane_active[k]) continue;
+    if (!active(i) || !active(j)) throw std::invalid_argument("Every route must join active owners");
+    join(i, j);
+  }
+  CapacitorTopology t;
+  t.component.assign(size, -1);
+  t.floating.assign(size, -1);
+  t.fixed.assign(size, 0);
+  // components in order of their lowest active node (bind's groups, in its insertion order)
+  std::vector<int32_t> comp_of_root(size, -1), first_member;
+  std::vector<std::vector<int32_t>> members;
+  for (int32_t i = 0; i < size; ++i) {
+    if (!active(i)) {
+      t.fixed[i] = 1;
+      continue;
+    }
+    const int32_t rt = root(i);
+    if (comp_of_root[rt] < 0) {
+      comp_of_root[rt] = (int32_t)members.size();
+      members.emplace_back();
+    }
+    t.component[i] = comp_of_root[rt];
+    members[comp_of_root[rt]].push_back(i);
+  }
+  for (int32_t k = 0; k < o; ++k)
+    if (reservoir[k]) t.fixed[n + k] = 1;
+  t.components = (int32_t)members.size();
+  std::vector<int32_t> floating_of(members.size(), -1);
+  for (size_t c = 0; c < members.size(); ++c) {
+    int32_t chosen = -1, lowest_outside = -1;
+    bool held = false;
+    for (int32_t i : members[c]) {
+      if (i >= n && reservoir[i - n]) held = true;
+      if (i >= n && gauge[i - n]) chosen = i;
+      if (i >= n && lowest_outside < 0) lowest_outside = i;
+    }
+    if (held) continue;  // the held bath's imposed potential fixes the component
+    if (chosen < 0) chosen = lowest_outside;
+    if (chosen < 0)  // a component of cells alone: no outside node to fix (bind's min() has none)
+      throw std::invalid_argument("Floating capacitor components require an outside return path to a clamp/gauge");
+    t.fixed[chosen] = 1;
+    floating_of[c] = (int32_t)t.offset_free.size();
+    t.offset_free.push_back(gauge[chosen - n] ? 0 : 1);
+    for (int32_t i : members[c]) t.floating[i] = floating_of[c];
+  }
+  // connectivity of the reduced offsets through guaranteed outside conductance: a component is
+  // reached when it holds a clamp or the gauge, or shares a conductive face or a return path with a
+  // reached one (to the fixed point)
+  std::vector<uint8_t> reached(members.size(), 0);
+  for (int32_t k = 0; k < o; ++k)
+    if ((reservoir[k] || gauge[k]) && t.component[n + k] >= 0) reached[t.component[n + k]] = 1;
+  std::vector<std::pair<int32_t, int32_t>> links;
+  auto comp_at = [&](int32_t node) { return t.component[node]; };
+  for (int64_t k = 0; k < f; ++k) {
+    const int32_t a = node_of(k, face_sender, o, n, "face_sender"), b = node_of(k, face_receiver, o, n, "face_receiver");
+    if (!face_conductive[k]) continue;
+    if (!active(a) || !active(b)) throw std::invalid_argument("Every route must join active owners");
+    links.emplace_back(comp_at(a), comp_at(b));
+  }
+  for (int64_t k = 0; k < r; ++k)
+    links.emplace_back(comp_at(node_of(2 * k, return_paths, o, n, "return path")),
+                       comp_at(node_of(2 * k + 1, return_paths, o, n, "return path")));
+  for (bool grew = true; grew;) {
+    grew = false;
+    for (const auto& [a, b] : links) {
+      if (a < 0 || b < 0) continue;
+      if (reached[a] != reached[b]) {
+        reached[a] = reached[b] = 1;
+        grew = true;
+      }
+    }
+  }
+  for (size_t c = 0; c < members.size(); ++c)
+    if (!reached[c])
+      throw std::invalid_argument("Floating capacitor components require an outside return path to a clamp/gauge");
+  return t;
 }
 
 int64_t position(const Symbolic& s, int32_t i, int32_t j) {

@@ -73,6 +73,26 @@ struct ProductPlan {
 };
 ProductPlan product_plan(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols);
 
+// The capacitance topology of a field layout (bind's host part): nodes are the n cells, then the o
+// outside nodes; each active membrane joins its cell and its outside node into one capacitor
+// component. A component holding a reservoir (a voltage clamp) is held; any other is floating and
+// fixes one representative node: the gauge where it holds it, else its lowest outside node. Every
+// floating component must reach a clamp or the gauge through conductive faces or return paths.
+struct CapacitorTopology {
+  std::vector<int32_t> component;  // [n+o] component index (first appearance by node), -1 inactive
+  std::vector<int32_t> floating;   // [n+o] floating component index, -1 where held or inactive
+  std::vector<uint8_t> fixed;      // [n+o] inactive, reservoir or representative
+  std::vector<uint8_t> offset_free;  // [floating components] its representative is not the gauge
+  int32_t components = 0;
+};
+// Throws std::invalid_argument with the reference's messages for the refused layouts.
+CapacitorTopology capacitor_topology(int32_t n, int32_t o, int64_t m, const int32_t* membrane_cell,
+                                     const int32_t* membrane_outside, const uint8_t* membrane_active,
+                                     const uint8_t* cell_active, const uint8_t* outside_active,
+                                     const uint8_t* reservoir, const uint8_t* gauge, int64_t f,
+                                     const int32_t* face_sender, const int32_t* face_receiver,
+                                     const uint8_t* face_conductive, int64_t r, const int32_t* return_paths);
+
 std::vector<int32_t> minimum_degree(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols);
 Symbolic bind(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, const int32_t* order);
 int64_t position(const Symbolic& s, int32_t i, int32_t j);

@@ -41,7 +41,7 @@ EXPORTS = (
     "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials",
     "cells_world_factor_positions", "cells_relaxed_padding", "cells_set_plan_options",
     "cells_get_plan_options", "cells_step_bounds", "cells_set_stage_tolerance", "cells_step_vjp_noalloc", "cells_energy_rates",
-    "cells_product_plan_size", "cells_product_plan", "cells_product",
+    "cells_product_plan_size", "cells_product_plan", "cells_product", "cells_capacitor_topology",
 )
 
 # refusal reasons in a world's failure bits (cells_step_check_bits)
@@ -175,6 +175,10 @@ def load():
     lib.cells_world_factor_positions.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p, ctypes.c_int64,
                                                  _i64p]
     lib.cells_minimum_degree.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p]
+    _u8p = ctypes.POINTER(ctypes.c_uint8)
+    lib.cells_capacitor_topology.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _u8p, _u8p, _u8p,
+                                             _u8p, _u8p, ctypes.c_int64, _i32p, _i32p, _u8p, ctypes.c_int64, _i32p,
+                                             _i32p, _i32p, _u8p, _u8p, _i32p]
     lib.cells_product_plan_size.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i64p]
     lib.cells_product_plan.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i64p, _i32p, _i64p]
     lib.cells_product.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

@@ -123,6 +123,26 @@ void cells_symbolic_free(cells_symbolic* s);
 int cells_relaxed_padding(const cells_symbolic* s, int32_t max_rows, int32_t max_zeros, int32_t* pairs,
                           int64_t capacity, int64_t* count);
 
+/* bind's capacitance topology (numerics/surface_field.py:253-340, representative :238-250; the
+ * owner checks of numerics/surface_transport.py:194-216 it relies on): nodes are the n cells then
+ * the o outside nodes; each active membrane joins its cell and its outside node into one capacitor
+ * component (union-find). A component holding a reservoir (a voltage clamp) is held; any other is
+ * floating and fixes its representative: the gauge where it holds it, else its lowest outside
+ * node. The gauge rule (no gauge with reservoirs, exactly one active gauge without), and every
+ * floating component reaching a clamp or the gauge through conductive faces (active, with a
+ * charged species of positive face factor) or return paths (outside-node pairs, the forms faces'),
+ * are refused otherwise, CELLS_EINVAL with the reference's messages. Outputs (any may be NULL but
+ * counts): component [n+o] (components numbered by their lowest active node, -1 inactive),
+ * floating [n+o] (floating component index, -1 where held or inactive), fixed [n+o] (inactive,
+ * reservoir or representative), offset_free [floating components] (its representative is not the
+ * gauge; capacity n+o suffices), counts[0] components, counts[1] floating components. */
+int cells_capacitor_topology(int32_t n, int32_t o, int64_t m, const int32_t* membrane_cell,
+                             const int32_t* membrane_outside, const uint8_t* membrane_active, const uint8_t* cell_active,
+                             const uint8_t* outside_active, const uint8_t* reservoir, const uint8_t* gauge, int64_t f,
+                             const int32_t* face_sender, const int32_t* face_receiver, const uint8_t* face_conductive,
+                             int64_t r, const int32_t* return_paths, int32_t* component, int32_t* floating,
+                             uint8_t* fixed, uint8_t* offset_free, int32_t* counts);
+
 /* product_plan (numerics/sparse_cholesky.py:444-464): the matrix-vector product's plan on the
  * symmetric pattern of n nodes with off-diagonal coordinates (rows, cols) in either or both
  * triangles and the whole diagonal. Row i's entries are its neighbours and itself in ascending
