@@ -282,12 +282,16 @@ __device__ __forceinline__ void cell_concentrations(const Params& P, const doubl
                                                     const double* yv, int i, double* c,
                                                     double& vol) {
   vol = P.water ? P.vol0[i] + CFc(P, yv, P.f_dvol, i) : P.vol0[i];
-  for (int s = 0; s < P.S; ++s) c[s] = (P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i)) / vol;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    if (s < P.S) c[s] = (P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i)) / vol;
 }
 
 __device__ __forceinline__ double charge_increment(const Params& P, const double* y, int i) {
   double t = P.z[0] * CFc(P, y, P.f_dn, i);
-  for (int s = 1; s < P.S; ++s) t = t + P.z[s] * CFc(P, y, P.f_dn + s, i);
+#pragma unroll
+  for (int s = 1; s < 4; ++s)
+    if (s < P.S) t = t + P.z[s] * CFc(P, y, P.f_dn + s, i);
   return kChargePerFmol * t;
 }
 
@@ -326,7 +330,9 @@ __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, con
   bool ok = true;
   double diss = 0.0, G = 0.0;
   constexpr bool slopes = SLOPES;  // gj, Gj non-null exactly when SLOPES
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {  // species, unrolled (S <= 4)
+    if (s >= P.S) break;
     double z = P.z[s];
     double pa = P.paj[(long)s * P.nedge + e];
     ok = ok && isfinite(ca[s]) && ca[s] > 0 && isfinite(cb[s]) && cb[s] > 0;
@@ -380,7 +386,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
   constexpr bool slopes = SLOPES;  // gm, Gj, mdiag, msc non-null exactly when SLOPES
   double Jm[4], mu[4];
   double diss = 0.0, Gm = 0.0;
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {  // species, unrolled (S <= 4)
+    if (s >= P.S) break;
     double z = P.z[s];
     ok = ok && isfinite(ci[s]) && ci[s] > 0;
     double pa = P.pam[(long)s * P.ncell + i];
@@ -449,7 +457,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     }
   }
   double qrate = 0.0;  // the charge rate of the amounts' rates (less 2 K1's with K1: stage 2)
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {  // species, unrolled (S <= 4)
+    if (s >= P.S) break;
     double cell = -Jm[s];
     cell += jout[s];
     cell += jin[s];
@@ -498,7 +508,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     // membrane_rate with one patch per cell and no kinds): F^2/RT sum_s z^2 (A P)_s max(c_in, c_out)
     // times the availability, over the cell's capacitance; dt x rate <= 1 (surface_pump.py:116)
     double t = 0.0;
-    for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {  // species, unrolled (S <= 4)
+      if (s >= P.S) break;
       const double z2 = P.z[s] * P.z[s];
       const double term = z2 * P.pam[(long)s * P.ncell + i] * fmax(ci[s], P.cb[s]);
       t = s == 0 ? term : t + term;
@@ -548,7 +560,9 @@ __device__ double block_max(double v, double* sh) {
 __device__ bool cell_domain(const Params& P, const double* y, const double* yv, int i) {
   bool ok = true;
   for (int f = 0; f < P.cf; ++f) ok = ok && isfinite(CFc(P, y, f, i));
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
     double n = P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i);
     ok = ok && isfinite(n) && n > 0;
   }
@@ -621,7 +635,9 @@ __global__ __launch_bounds__(kBlock) void k_world_ionic(Params P, double* A, dou
   int c0 = P.cell_off[w], c1 = P.cell_off[w + 1];
   double applied[4];
   bool ok = true;
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
     double a1 = 0.0, a2 = 0.0;
     for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
       a1 += contrib1[(long)s * P.ncell + i];
@@ -645,7 +661,9 @@ __global__ __launch_bounds__(kBlock) void k_world_ionic(Params P, double* A, dou
   if (threadIdx.x == 0) {
     double electrode = -qe - -qs;
     double work = 0.0;
-    for (int s = 0; s < P.S; ++s) work = s == 0 ? P.mu_b[s] * applied[s] : work + P.mu_b[s] * applied[s];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (s < P.S) work = s == 0 ? P.mu_b[s] * applied[s] : work + P.mu_b[s] * applied[s];
     double ec = WFc(P, y, P.f_echg, w) + electrode;
     double rw = WFc(P, y, P.f_rwork, w) + 1e-3 * work;
     double ew = WFc(P, y, P.f_ework, w) + 1e12 * P.Vb * electrode;
@@ -665,7 +683,9 @@ __global__ __launch_bounds__(kBlock) void k_world_ionic(Params P, double* A, dou
 __device__ __forceinline__ double water_psi(const Params& P, const double* y, int i, double vol,
                                             double& total) {
   double t = 0.0;
-  for (int s = 0; s < P.S; ++s) t += P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i);
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    if (s < P.S) t += P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i);
   t = t + P.imperm[i];
   total = t;
   double osm = P.RT * t / vol;
@@ -1114,7 +1134,9 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(8, 8
         if (Kminus2) {
           // the charge of K - 2 K2' field by field (the stage-2 right-hand side F1 - 2 k1)
           double t2 = P.z[0] * (CFc(P, K, P.f_dn, i) + -2.0 * CFc(P, Kminus2, P.f_dn, i));
-          for (int sp = 1; sp < P.S; ++sp)
+#pragma unroll
+          for (int sp = 1; sp < 4; ++sp)
+            if (sp < P.S)
             t2 = t2 + P.z[sp] * (CFc(P, K, P.f_dn + sp, i) + -2.0 * CFc(P, Kminus2, P.f_dn + sp, i));
           q[u] = kChargePerFmol * t2;
         } else {
@@ -1918,7 +1940,9 @@ __device__ bool correct_cell(const Params& P, int i, double coef, const double* 
                              double* K, double* contrib) {
   double dn_c[4], dm_c[4], mx, mag;
   cell_correction<true>(P, i, coef, gm, gj, Gj, dphi, dn_c, dm_c, &mx, &mag);
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
     CF(P, K, P.f_dn + s, i) += dn_c[s];
     CF(P, K, P.f_mtr + s, i) += dm_c[s];
     contrib[(long)s * P.ncell + i] = contrib[(long)s * P.ncell + i] - dm_c[s];
@@ -1929,7 +1953,9 @@ __device__ bool correct_cell(const Params& P, int i, double coef, const double* 
 __device__ void correct_edge(const Params& P, int e, double coef, const double* gj, const double* dphi,
                              double* K) {
   double d = dphi[P.snd[e]] - dphi[P.rcv[e]];
-  for (int s = 0; s < P.S; ++s) EF(P, K, P.f_jtr + s, e) += coef * (gj[(long)s * P.nedge + e] * d);
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    if (s < P.S) EF(P, K, P.f_jtr + s, e) += coef * (gj[(long)s * P.nedge + e] * d);
 }
 
 // Stage 1: the stage solve's residual (its right-hand side is K1's charge rate before the
@@ -1953,7 +1979,9 @@ __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* 
       valid[w] = 0;
       atomicOr(P.refuse_bits + w, kRefuseResidual);
     }
-    for (int s = 0; s < P.S; ++s)
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (s < P.S)
       CF(P, Y1, P.f_dn + s, i) = CFc(P, y, P.f_dn + s, i) + dt * CFc(P, K1, P.f_dn + s, i);
     if (P.water) {
       const double* yv = next(const_cast<double*>(A), const_cast<double*>(B), sw);
@@ -2000,7 +2028,9 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
       if (f >= P.f_mtr && f < P.f_mtr + P.S) k2 += dm_c[f - P.f_mtr];
       CF(P, o, f, i) = (CFc(P, y, f, i) + h1 * k1) + h2 * k2;
     }
-    for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s >= P.S) break;
       long q = (long)s * P.ncell + i;
       contrib2[q] = (contrib2[q] + -2.0 * contrib1[q]) - dm_c[s];
     }
@@ -2077,7 +2107,9 @@ __global__ __launch_bounds__(kBlock) void k_world_rates(Params P, const double* 
   __shared__ double sh[kBlock];
   int w = blockIdx.x;
   int c0 = P.cell_off[w], c1 = P.cell_off[w + 1];
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
     double a = 0.0;
     for (int i = c0 + threadIdx.x; i < c1; i += kBlock) a += contrib[(long)s * P.ncell + i];
     a = block_sum(a, sh);
@@ -2260,7 +2292,9 @@ __device__ int ledger_checks(const Params& P, const double* prev, const double* 
       sm[24] += fabs(tr);
       // dilution of every osmolyte to the current volume and the wall's elastic energy
       double total = 0.0;
-      for (int s = 0; s < P.S; ++s) total += P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (s < P.S) total += P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i);
       total = total + P.imperm[i];
       const double vr = P.vol0[i], stiff = P.wK / vr;
       const double pressure = stiff * (vr - vr) + P.wP0;
@@ -2372,7 +2406,9 @@ __global__ __launch_bounds__(kBlock) void k_energy_rates(Params P, const double*
     double ci[4], vol;
     cell_concentrations(P, y, y, i, ci, vol);
     double chem = 0.0, zq = 0.0;
-    for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s >= P.S) break;
       const double r = CFc(P, R, P.f_dn + s, i);
       chem = s == 0 ? log(ci[s]) * r : chem + log(ci[s]) * r;
       zq = s == 0 ? P.z[s] * r : zq + P.z[s] * r;
@@ -2395,7 +2431,9 @@ __global__ __launch_bounds__(kBlock) void k_energy_rates(Params P, const double*
   block_fold<5, false>(v, sh);
   if (threadIdx.x != 0) return;
   double reservoir = 0.0;
-  for (int s = 0; s < P.S; ++s) reservoir += P.mu_b[s] * WFc(P, R, P.f_sup + s, w);
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    if (s < P.S) reservoir += P.mu_b[s] * WFc(P, R, P.f_sup + s, w);
   reservoir *= 1e-3;
   const double current = -v[4];
   const double chemical = 1e-3 * P.RT * v[0], capacitor = 1e12 * v[1], electrode = 1e12 * P.Vb * current;
@@ -2488,7 +2526,9 @@ __global__ __launch_bounds__(kBlock) void k_edges_vjp(Params P, const double* y,
   double g = EFc(P, y, P.f_gate, e);
   double unit = c.junction / kMolPerFmol;
   double vsum = 0.0, gbar = 0.0;
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
     double z = P.z[s], kk = z * kF / P.RT, x = kk * dv;
     double pa = P.paj[(long)s * P.nedge + e];
     double Bm = bernoulli(-x), Bp = bernoulli(x);
@@ -2544,7 +2584,9 @@ __global__ __launch_bounds__(kBlock) void k_cells_vjp(Params P, const double* y,
   double unit = c.membrane / kMolPerFmol;
   double cbar[4] = {0, 0, 0, 0};
   double vbar = 0.0;
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
     double z = P.z[s], kk = z * kF / P.RT, x = kk * dv, co = P.cb[s];
     double pa = P.pam[(long)s * P.ncell + i];
     double A = pa * unit;
@@ -2587,15 +2629,21 @@ __global__ __launch_bounds__(kBlock) void k_cells_vjp(Params P, const double* y,
   int p0 = P.inc_ptr[i], p1 = P.inc_ptr[i + 1], pm = p0 + P.inc_out[i];
   for (int p = p0; p < pm; ++p) {
     int e = P.inc_edge[p];
-    for (int s = 0; s < P.S; ++s) cbar[s] += ca_bar[(long)s * P.nedge + e];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (s < P.S) cbar[s] += ca_bar[(long)s * P.nedge + e];
     vbar += ev_bar[e];
   }
   for (int p = pm; p < p1; ++p) {
     int e = P.inc_edge[p];
-    for (int s = 0; s < P.S; ++s) cbar[s] += cb_bar[(long)s * P.nedge + e];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (s < P.S) cbar[s] += cb_bar[(long)s * P.nedge + e];
     vbar -= ev_bar[e];
   }
-  for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
     double nb = cbar[s] / vol + vbar * kChargePerFmol * P.z[s] / P.cap[i];
     CF(P, out, P.f_dn + s, i) += nb;
     if (out2) out2[(long)s * P.ncell + i] += h2 * nb;
@@ -2610,7 +2658,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(Params P, const double* yba
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < P.ncell) {
     int i = (int)t;
-    for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s >= P.S) break;
       double v = CFc(P, ybar, P.f_dn + s, i);
       CF(P, ybar_in, P.f_dn + s, i) = v;
       k1bar[(long)s * P.ncell + i] = 1.5 * dt * v;
@@ -2634,7 +2684,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_dtrans(Params P, double coef, co
   if (t < P.ncell) {
     int i = (int)t;
     double acc = 0.0;
-    for (int s = 0; s < P.S; ++s) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s >= P.S) break;
       double us = u[(long)s * P.ncell + i];
       acc += -gm[(long)s * P.ncell + i] * us;
       gmbar[(long)s * P.ncell + i] += coef * (us * -delta[i]);
@@ -2644,7 +2696,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_dtrans(Params P, double coef, co
       int e = P.inc_edge[p];
       int a = P.snd[e], b = P.rcv[e];
       double tt = 0.0;
-      for (int s = 0; s < P.S; ++s)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (s < P.S)
         tt += gj[(long)s * P.nedge + e] * (u[(long)s * P.ncell + b] - u[(long)s * P.ncell + a]);
       acc += p < pm ? tt : -tt;
     }
@@ -2652,7 +2706,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_dtrans(Params P, double coef, co
   } else if (t < (long)P.ncell + P.nedge) {
     int e = (int)(t - P.ncell);
     int a = P.snd[e], b = P.rcv[e];
-    for (int s = 0; s < P.S; ++s)
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (s < P.S)
       gjbar[(long)s * P.nedge + e] +=
           coef * ((delta[a] - delta[b]) * (u[(long)s * P.ncell + b] - u[(long)s * P.ncell + a]));
   }
@@ -2693,14 +2749,18 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(8, 8
   }
   __syncthreads();
   for (int i = c0 + threadIdx.x; i < c0 + s.n; i += blockDim.x)
-    for (int q = 0; q < P.S; ++q) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q >= P.S) break;
       rate_bar[(long)q * P.ncell + i] += kChargePerFmol * P.z[q] * wv[i];
       gmbar[(long)q * P.ncell + i] += kChargePerFmol * P.z[q] * (-coef * wv[i] * delta[i]);
     }
   for (int e = e0 + threadIdx.x; e < e0 + s.E; e += blockDim.x) {
     int a = P.snd[e], b = P.rcv[e];
     double t = -coef * (wv[a] - wv[b]) * (delta[a] - delta[b]);
-    for (int q = 0; q < P.S; ++q) gjbar[(long)q * P.nedge + e] += kChargePerFmol * P.z[q] * t;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < P.S) gjbar[(long)q * P.nedge + e] += kChargePerFmol * P.z[q] * t;
   }
 }
 
@@ -2731,7 +2791,9 @@ __global__ __launch_bounds__(kBlock) void k_vmem_loss(Params P, const double* y,
     acc += wgt * d * d;
     if (ybar) {
       double g = 2.0 * wgt * d;
-      for (int s = 0; s < P.S; ++s) CF(P, ybar, P.f_dn + s, i) += g * kChargePerFmol * P.z[s] / P.cap[i];
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (s < P.S) CF(P, ybar, P.f_dn + s, i) += g * kChargePerFmol * P.z[s] / P.cap[i];
     }
   }
   acc = block_sum(acc, sh);
@@ -3013,7 +3075,9 @@ void rate_bounds(const cells_world* w, int method, double* bexp, double* bgate, 
     for (int64_t e = 0; e < ne; ++e) {
       const int64_t a = w->snd[e], b = w->rcv[e];
       double t = 0.0;
-      for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (s < S)
         t += w->valence[s] * w->valence[s] * w->paj[(int64_t)s * ne + e] * std::max(conc(s, a), conc(s, b));
       const double g = slope * t;
       jrow[a] += g;
@@ -3022,7 +3086,9 @@ void rate_bounds(const cells_world* w, int method, double* bexp, double* bgate, 
     double jmax = 0.0, mmax = 0.0;
     for (int64_t i = 0; i < nc; ++i) {
       double t = 0.0;
-      for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (s < S)
         t += w->valence[s] * w->valence[s] * w->pam[(int64_t)s * nc + i] * std::max(conc(s, i), w->bath_c[s]);
       jmax = std::max(jmax, rows(jrow[i], w->cap[i]));
       mmax = std::max(mmax, rows(slope * t, w->cap[i]));
@@ -3032,12 +3098,16 @@ void rate_bounds(const cells_world* w, int method, double* bexp, double* bgate, 
   // material relaxation through the routes: every species under ROS2, the neutral ones under Euler
   std::vector<double> mat((size_t)S * nc, 0.0);
   for (int64_t e = 0; e < ne; ++e)
-    for (int s = 0; s < S; ++s) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s >= S) break;
       mat[(size_t)s * nc + w->snd[e]] += w->paj[(int64_t)s * ne + e];
       mat[(size_t)s * nc + w->rcv[e]] += w->paj[(int64_t)s * ne + e];
     }
   double material = 0.0;
-  for (int s = 0; s < S; ++s) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= S) break;
     if (explicit_routes && w->valence[s] != 0.0) continue;
     for (int64_t i = 0; i < nc; ++i) {
       const double v0 = w->q0[i] / w->cap[i];
