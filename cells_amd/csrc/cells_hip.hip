@@ -1710,8 +1710,7 @@ __device__ void sv_forward_front(const int* pool, int qa, const double* __restri
     const int i = lane + t * G;
     fr[t] = i < m ? Fv[i * STRIDE] : 0.0;
   }
-#pragma unroll 4
-  for (int jj = 0; jj < J; ++jj) {
+  auto column = [&](int jj) {
     const long cbp = p0 + colbase(jj, m);
     const double d = fv[cbp + jj];
     double lv[NT];
@@ -1727,6 +1726,15 @@ __device__ void sv_forward_front(const int* pool, int qa, const double* __restri
       if (i > jj && i < m) fr[t] -= lv[t] * xj;
     }
     if (lane == jj % G) y[j0 + jj] = xj;
+  };
+  if constexpr (G <= kMfOcta) {
+    // a narrow group's front (J <= its rows): the columns unrolled, so its rows stay in registers
+#pragma unroll
+    for (int jj = 0; jj < SolveLanes<G>::rows; ++jj)
+      if (jj < J) column(jj);
+  } else {
+#pragma unroll 4
+    for (int jj = 0; jj < J; ++jj) column(jj);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -1762,7 +1770,7 @@ __device__ void sv_backward_front(const int* pool, int qa, const double* __restr
     }
     xr[t] = acc;
   }
-  for (int jj = J - 1; jj >= 0; --jj) {
+  auto column = [&](int jj) {
     const double d = fv[p0 + colbase(jj, m) + jj];
     double lv[NT];
 #pragma unroll
@@ -1777,6 +1785,13 @@ __device__ void sv_backward_front(const int* pool, int qa, const double* __restr
       const int k = lane + t * G;
       if (k < jj) xr[t] -= lv[t] * zj;
     }
+  };
+  if constexpr (G <= kMfOcta) {
+#pragma unroll
+    for (int jj = SolveLanes<G>::rows - 1; jj >= 0; --jj)
+      if (jj < J) column(jj);
+  } else {
+    for (int jj = J - 1; jj >= 0; --jj) column(jj);
   }
   group_sync<G>();
 }
