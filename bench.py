@@ -34,7 +34,7 @@ def measured_traffic(kernel):
     try:
         for line in open(PROFILE):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 6 and cells[0] == kernel and cells[3].endswith("GB"):
+            if len(cells) == 6 and cells[0].split("<")[0] == kernel and cells[3].endswith("GB"):
                 return float(cells[3].split()[0]) * 1e9
     except OSError:
         pass
