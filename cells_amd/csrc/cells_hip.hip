@@ -202,11 +202,12 @@ constexpr int kLatClasses = 8;
 constexpr int kLatBounds = kLatClasses + 1;  // lattice plans, per level: first front of each class, end
 constexpr int kLatGroupTri[6] = {16, 64, 128, 256, 512, kMfBlockWaveTri};  // factor LDS per group (doubles)
 constexpr int kLatGroupLanes[6] = {1, 4, 8, 16, 32, 64};
-// every class's groups together fit the workgroup's LDS region, which the classes share in turn
+// the groups of every class within one wavefront fit that wavefront's quarter of the LDS region, in which it
+// takes its share of each class in turn
 constexpr bool lattice_classes_fit() {
-  for (int c = 0; c < 6; ++c)
-    if ((kMfThreads / kLatGroupLanes[c]) * kLatGroupTri[c] > kMfBlockLds) return false;
-  return max_order(kMfBlockLds) <= kMfMaxRows;
+  for (int c = 0; c < 6; ++c)  // a wavefront's groups of each class within its quarter of the region
+    if ((64 / kLatGroupLanes[c]) * kLatGroupTri[c] > kMfBlockLds / (kMfThreads / 64)) return false;
+  return max_order(kMfBlockLds) <= kMfMaxRows && kMfBlockWaveTri == kMfBlockLds / (kMfThreads / 64);
 }
 static_assert(lattice_classes_fit(), "a lattice class's lane groups outgrow the factor's LDS region");
 static_assert(kLatGroupTri[0] == kMfThreadTri && kLatGroupTri[2] == kMfOctaTri && kLatGroupLanes[2] == kMfOcta,
@@ -1580,13 +1581,17 @@ __device__ __forceinline__ void mf_class(const int* pool, int qa0, int q0, int q
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   if constexpr (G == 1) {
+    // a wavefront's slice of the region (base is the wave's), a thread's front interleaved across it
+    const int lane = tid & 63;
     for (int q = q0 + tid; q < q1; q += kMfThreads)
-      mf_front<1, kMfThreads, MR, ASM>(pool, qa0 + q, fv, ub, base + tid, 0, valid_w, am);
+      mf_front<1, 64, MR, ASM>(pool, qa0 + q, fv, ub, base + lane, 0, valid_w, am);
   } else if constexpr (G == kMfThreads) {
     for (int q = q0; q < q1; ++q) mf_front<kMfThreads, 1, MR, ASM>(pool, qa0 + q, fv, ub, base, tid, valid_w, am);
   } else {
+    // groups of G lanes within one wavefront, each with its TRI doubles of the wave's slice
+    const int lane = tid & 63;
     for (int q = q0 + tid / G; q < q1; q += kMfThreads / G)
-      mf_front<G, 1, MR, ASM>(pool, qa0 + q, fv, ub, base + (tid / G) * TRI, tid % G, valid_w, am);
+      mf_front<G, 1, MR, ASM>(pool, qa0 + q, fv, ub, base + (lane / G) * TRI, tid % G, valid_w, am);
   }
 }
 
@@ -1598,35 +1603,39 @@ __device__ __forceinline__ void mf_factor_world(const ShapeDev& s, const int* po
   const int* lev = pool + s.o_snlev;
   const int qa0 = (int)(s.o_sn / kRecInts);
   for (int L = 0; L < s.sn_levels; ++L) {
-    // the fronts of level L by class: groups of 1, 4, 8, 16, 32 and 64 lanes (each group's share of
-    // the LDS region), the workgroup with its front in LDS, the workgroup in HBM scratch. The bounds
-    // are the workgroup's alike, so an empty class skips its barrier (the LDS region and the
-    // update matrices are handed on at the barrier of the last class that ran).
+    // the fronts of level L by class: groups of 1, 4, 8, 16, 32 and 64 lanes, the workgroup with
+    // its front in LDS, the workgroup in HBM scratch. Every group up to a wavefront lies within one
+    // wavefront and works in that wavefront's slice of the LDS region (a quarter), so each wavefront
+    // takes its share of every class in turn with no barrier between classes (its lanes reconverge
+    // between them); the workgroup meets once, when the level's small fronts are done (their update
+    // matrices handed on, the region free for the workgroup classes).
     const int* bd = lev + kLatBounds * L;
-    if (bd[0] < bd[1]) {
-      mf_class<1, kMfThreadTri, FrontLanes<1>::rows, ASM>(pool, qa0, bd[0], bd[1], fv, ub, lds, valid_w, am);
-      __syncthreads();
-    }
-    if (bd[1] < bd[2]) {
-      mf_class<4, kLatGroupTri[1], FrontLanes<4>::rows, ASM>(pool, qa0, bd[1], bd[2], fv, ub, lds, valid_w, am);
-      __syncthreads();
-    }
-    if (bd[2] < bd[3]) {
-      mf_class<kMfOcta, kMfOctaTri, FrontLanes<kMfOcta>::rows, ASM>(pool, qa0, bd[2], bd[3], fv, ub, lds, valid_w,
-                                                                    am);
-      __syncthreads();
-    }
-    if (bd[3] < bd[4]) {
-      mf_class<16, kLatGroupTri[3], FrontLanes<16>::rows, ASM>(pool, qa0, bd[3], bd[4], fv, ub, lds, valid_w, am);
-      __syncthreads();
-    }
-    if (bd[4] < bd[5]) {
-      mf_class<32, kLatGroupTri[4], FrontLanes<32>::rows, ASM>(pool, qa0, bd[4], bd[5], fv, ub, lds, valid_w, am);
-      __syncthreads();
-    }
-    if (bd[5] < bd[6]) {
-      mf_class<64, kMfBlockWaveTri, max_order(kMfBlockWaveTri), ASM>(pool, qa0, bd[5], bd[6], fv, ub, lds,
-                                                                     valid_w, am);
+    if (bd[0] < bd[6]) {
+      double* R = lds + (threadIdx.x >> 6) * kMfBlockWaveTri;
+      if (bd[0] < bd[1]) {
+        mf_class<1, kLatGroupTri[0], FrontLanes<1>::rows, ASM>(pool, qa0, bd[0], bd[1], fv, ub, R, valid_w, am);
+        wave_sync();
+      }
+      if (bd[1] < bd[2]) {
+        mf_class<4, kLatGroupTri[1], FrontLanes<4>::rows, ASM>(pool, qa0, bd[1], bd[2], fv, ub, R, valid_w, am);
+        wave_sync();
+      }
+      if (bd[2] < bd[3]) {
+        mf_class<kMfOcta, kLatGroupTri[2], FrontLanes<kMfOcta>::rows, ASM>(pool, qa0, bd[2], bd[3], fv, ub, R,
+                                                                         valid_w, am);
+        wave_sync();
+      }
+      if (bd[3] < bd[4]) {
+        mf_class<16, kLatGroupTri[3], FrontLanes<16>::rows, ASM>(pool, qa0, bd[3], bd[4], fv, ub, R, valid_w, am);
+        wave_sync();
+      }
+      if (bd[4] < bd[5]) {
+        mf_class<32, kLatGroupTri[4], FrontLanes<32>::rows, ASM>(pool, qa0, bd[4], bd[5], fv, ub, R, valid_w, am);
+        wave_sync();
+      }
+      if (bd[5] < bd[6])
+        mf_class<64, kMfBlockWaveTri, max_order(kMfBlockWaveTri), ASM>(pool, qa0, bd[5], bd[6], fv, ub, R,
+                                                                       valid_w, am);
       __syncthreads();
     }
     // the workgroup classes end each front at a barrier of their own
