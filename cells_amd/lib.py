@@ -38,7 +38,7 @@ EXPORTS = (
     "cells_step_vjp_recorded", "cells_step_checks", "cells_step_check_bits", "cells_stationary",
     "cells_stationary_residual", "cells_stationary_jvp", "cells_field_create", "cells_field_destroy",
     "cells_field_state_size", "cells_field_step", "cells_field_refresh", "cells_field_plan_stats",
-    "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials",
+    "cells_field_kernel_times", "cells_field_set_start", "cells_field_potentials", "cells_field_energy_rates",
     "cells_world_factor_positions", "cells_relaxed_padding", "cells_set_plan_options",
     "cells_get_plan_options", "cells_step_bounds", "cells_set_stage_tolerance", "cells_step_vjp_noalloc", "cells_energy_rates",
     "cells_product_plan_size", "cells_product_plan", "cells_product", "cells_capacitor_topology",
@@ -57,6 +57,10 @@ CHECKS = {
 # the readouts' energy rates, in their column order (cells_energy_rates)
 ENERGY_RATES = ("chemical", "capacitor", "dissipation", "reservoir_power", "electrode_power", "metabolic",
                 "energy_balance", "electrode_current")
+# the field solve's energy terms, in their column order (cells_field_energy_rates)
+FIELD_ENERGY_RATES = ("chemical", "capacitor", "dissipation", "reservoir_power", "electrode_power",
+                      "energy_balance", "electrode_current", "capacitor_energy", "membrane_heat", "junction_heat",
+                      "face_heat")
 # the ledgers of the bound ratios, in their column order (CELLS_RATIO_*)
 LEDGER_RATIOS = ("local", "total", "global_charge", "water", "outside", "charge", "electrode",
                  "energy", "reservoir_work", "electrode_work", "metabolic_work", "bath_water_work")
@@ -204,6 +208,7 @@ def load():
     lib.cells_field_refresh.argtypes = [vp]
     lib.cells_field_set_start.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
     lib.cells_field_potentials.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct), vp, vp, vp]
+    lib.cells_field_energy_rates.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct), vp]
     lib.cells_field_plan_stats.argtypes = [vp, _i64p, _f64p, _i32p, _i32p, _i32p]
     lib.cells_field_kernel_times.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
                                              ctypes.c_int32, ctypes.c_int32, _f64p, ctypes.c_char_p, _i32p]

@@ -441,6 +441,15 @@ int cells_field_set_start(cells_field* f, int32_t field_solve, int32_t max_itera
 int cells_field_potentials(cells_field* f, void* stream, const double* y, double dt,
                            const cells_control* control, double* phi_out, int32_t* iterations_out,
                            int32_t* ok_out);
+/* The field solve's energy terms at state y (device): the field solve as cells_field_potentials,
+ * then with those potentials transport alone (membranes into their nodes, junctions, faces by
+ * log-mean electrodiffusion; no pumps) priced as the reference's field readout does
+ * (numerics/surface_field.py:792-835 energy terms). out [W][11], per world: chemical, capacitor,
+ * dissipation, reservoir power, electrode power, energy balance, electrode current, capacitor
+ * energy, membrane heat, junction heat, face heat. The balance (chemical + capacitor +
+ * dissipation - reservoir - electrode) vanishes to rounding at the field solve's potentials. */
+int cells_field_energy_rates(cells_field* f, void* stream, const double* y, double dt,
+                             const cells_control* control, double* out);
 /* The operator plan: factor entries per world and flops per factor (sum of column counts squared)
  * of the bound pattern, and the supernodes, largest front and supernodal tree levels of the plan
  * (relaxed supernodes: the chains joined with their explicit zeros). */

@@ -483,3 +483,27 @@ def logmean(left, right):
     m, lg = ctypes.c_double(), ctypes.c_double()
     lib().ref_logmean(ctypes.c_double(left), ctypes.c_double(right), ctypes.byref(m), ctypes.byref(lg))
     return m.value, lg.value
+
+
+FIELD_ENERGY_RATES = ("chemical", "capacitor", "dissipation", "reservoir_power", "electrode_power",
+                      "energy_balance", "electrode_current", "capacitor_energy", "membrane_heat", "junction_heat",
+                      "face_heat")
+
+
+def field_energy_rates(fp, y, op, world, ctl=None, phi=None):
+    """The field solve's energy rates [11] (FIELD_ENERGY_RATES order) of one spatial world at y, with
+    the oracle's field solve for the potentials (or the given (cell, node) potentials); returns
+    (rates, converged)."""
+    w, g = world_struct(fp.cells), field_grid_struct(fp)
+    ctl = ctl or control()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    converged = True
+    if phi is None:
+        pc, pn, _, converged = field_start(fp, y, op, world, ctl)
+    else:
+        pc, pn = phi
+    both = np.ascontiguousarray(np.concatenate((pc, pn)), dtype=np.float64)
+    out = np.zeros(len(FIELD_ENERGY_RATES))
+    ok = lib().ref_field_energy_rates(ctypes.byref(w), ctypes.byref(g), int(world), _p(y, _f64p), ctypes.byref(ctl),
+                                      _p(both, _f64p), _p(out, _f64p))
+    return out, bool(converged and ok)

@@ -2683,3 +2683,139 @@ int ref_field_step(const cells_world* w, const cells_field_grid* g, const cells_
     free(drift);
     return all;
 }
+
+/* ---- the field solve's energy rates (numerics/surface_field.py:792-835)
+ *
+ * At spatial world k's state y with the field's potentials phi (its cells', then its nodes'):
+ * transport alone at phi (numerics/surface_transport.py:298-386: each cell's membrane into its
+ * node, the junctions, the faces by log-mean electrodiffusion), then the field's energy terms:
+ * chemical 1e-3 RT (sum ln ci cell_rate + sum ln co outside_rate), the outside rate being zero on
+ * the held ring (its supply the ring's net outflow); the patch charge C V_m, the capacitor energy
+ * 0.5e12 sum C V_m^2 and rate 1e12 sum C V_m dV_m/dt, dV_m/dt from the capacitor solve of the ionic
+ * charge rate (each cell with its node one component whose node is the representative, so
+ * dV_m/dt = the cell's charge rate over C); the reservoir power 1e-3 sum (RT ln co + z F phi)
+ * supply over the ring; the electrode power 1e12 sum phi I over the ring with I the ring's
+ * capacitor current (the ring holds no membrane: zero); the dissipation, every route's 1e-3 J mu;
+ * and the balance chemical + capacitor + dissipation - reservoir - electrode.
+ * out: chemical, capacitor, dissipation, reservoir power, electrode power, energy balance,
+ * electrode current, capacitor energy, membrane heat, junction heat, face heat. */
+int ref_field_energy_rates(const cells_world* w, const cells_field_grid* g, int32_t k, const double* y,
+                           const cells_control* c, const double* phi, double* out) {
+    cells_field_layout L;
+    ref_field_layout(w, g, &L);
+    const int S = w->S;
+    const double T = w->temperature, RT = R_CONST * T;
+    const int64_t c0 = w->cell_off[k], nc = w->cell_off[k + 1] - c0;
+    const int64_t e0 = w->edge_off[k], ne = w->edge_off[k + 1] - e0;
+    const int64_t NN = L.nodes, NF = L.faces;
+    double* ci = (double*)malloc(sizeof(double) * (size_t)(S * nc));
+    double* co = (double*)malloc(sizeof(double) * (size_t)(S * NN));
+    double* cr = (double*)calloc((size_t)(S * nc), sizeof(double));  /* cell rates */
+    double* orr = (double*)calloc((size_t)(S * NN), sizeof(double)); /* outside rates (all nodes) */
+    for (int s = 0; s < S; ++s) {
+        for (int64_t i = 0; i < nc; ++i)
+            ci[s * nc + i] = (w->n0[(int64_t)s * w->ncell + c0 + i] + FCF(&L, w, y, L.dn + s, c0 + i)) / w->vol0[c0 + i];
+        for (int64_t t = 0; t < NN; ++t)
+            co[s * NN + t] = (g->node_n0[(int64_t)s * NN + t] + FNF(&L, w, y, L.dno + s, k, t)) / g->node_volume;
+    }
+    const double* pc = phi;
+    const double* pn = phi + nc;
+    double mheat = 0.0, jheat = 0.0, fheat = 0.0;
+    int valid = 1;
+    /* membranes: cell -> its node */
+    double* vm = (double*)malloc(sizeof(double) * (size_t)(nc ? nc : 1));
+    for (int64_t i = 0; i < nc; ++i) {
+        const int64_t t = g->cell_node[c0 + i];
+        const double dv = pc[i] - pn[t];
+        vm[i] = dv;
+        double h = 0.0;
+        for (int s = 0; s < S; ++s) {
+            const double z = w->valence[s];
+            const double J = ref_selective_flux(ci[s * nc + i], co[s * NN + t], dv, z, T,
+                                                w->pam[(int64_t)s * w->ncell + c0 + i], c->membrane);
+            const double mu = RT * (log(ci[s * nc + i]) - log(co[s * NN + t])) + z * F_CONST * dv;
+            h += J * mu;
+            cr[s * nc + i] -= J;
+            orr[s * NN + t] += J;
+        }
+        mheat += 1e-3 * h;
+    }
+    /* junctions: sender -> receiver */
+    for (int64_t e = 0; e < ne; ++e) {
+        const int64_t a = w->snd[e0 + e] - c0, b = w->rcv[e0 + e] - c0;
+        const double dv = pc[a] - pc[b];
+        const double avail = c->junction * FEF(&L, w, y, L.gate, e0 + e);
+        double h = 0.0;
+        for (int s = 0; s < S; ++s) {
+            const double z = w->valence[s];
+            const double J = ref_selective_flux(ci[s * nc + a], ci[s * nc + b], dv, z, T,
+                                                w->paj[(int64_t)s * w->nedge + e0 + e], avail);
+            const double mu = RT * (log(ci[s * nc + a]) - log(ci[s * nc + b])) + z * F_CONST * dv;
+            h += J * mu;
+            cr[s * nc + a] -= J;
+            cr[s * nc + b] += J;
+        }
+        jheat += 1e-3 * h;
+    }
+    /* faces: sender -> receiver, log-mean electrodiffusion */
+    for (int64_t q = 0; q < NF; ++q) {
+        int64_t a, b;
+        face_nodes(g->side, q, &a, &b);
+        const double dv = pn[a] - pn[b];
+        double h = 0.0;
+        for (int s = 0; s < S; ++s) {
+            const double z = w->valence[s];
+            const double l = co[s * NN + a], r = co[s * NN + b];
+            const double mu = RT * (log(l) - log(r)) + z * F_CONST * dv;
+            double mean, lg;
+            ref_logmean(l, r, &mean, &lg);
+            const double J = g->face_factor[s] * mean * mu / (RT * MOL_PER_FMOL);
+            h += J * mu;
+            orr[s * NN + a] -= J;
+            orr[s * NN + b] += J;
+        }
+        fheat += 1e-3 * h;
+    }
+    double chem_c = 0.0, chem_o = 0.0, reservoir = 0.0, cap_e = 0.0, cap_r = 0.0;
+    for (int64_t i = 0; i < nc; ++i) {
+        double zq = 0.0;
+        for (int s = 0; s < S; ++s) {
+            chem_c += log(ci[s * nc + i]) * cr[s * nc + i];
+            zq += w->valence[s] * cr[s * nc + i];
+        }
+        const double C = w->cap[c0 + i];
+        const double patch = C * vm[i];
+        const double rate = CHARGE_PER_FMOL * zq / C; /* the capacitor solve: the node is fixed */
+        cap_e += patch * vm[i];
+        cap_r += patch * rate;
+    }
+    for (int64_t t = 0; t < NN; ++t) {
+        const int ring = on_ring(g->side, t);
+        for (int s = 0; s < S; ++s) {
+            if (ring) {
+                const double supply = -orr[s * NN + t];
+                reservoir += (RT * log(co[s * NN + t]) + w->valence[s] * F_CONST * pn[t]) * supply;
+            } else {
+                chem_o += log(co[s * NN + t]) * orr[s * NN + t];
+            }
+        }
+    }
+    const double chemical = 1e-3 * RT * (chem_c + chem_o);
+    const double capacitor = 1e12 * cap_r, capacitor_energy = 0.5e12 * cap_e;
+    const double reservoir_power = 1e-3 * reservoir, electrode_power = 0.0, electrode_current = 0.0;
+    const double dissipation = (mheat + jheat) + fheat;
+    out[0] = chemical;
+    out[1] = capacitor;
+    out[2] = dissipation;
+    out[3] = reservoir_power;
+    out[4] = electrode_power;
+    out[5] = chemical + capacitor + dissipation - reservoir_power - electrode_power;
+    out[6] = electrode_current;
+    out[7] = capacitor_energy;
+    out[8] = mheat;
+    out[9] = jheat;
+    out[10] = fheat;
+    for (int j = 0; j < 11; ++j) valid = valid && isfinite(out[j]);
+    free(ci); free(co); free(cr); free(orr); free(vm);
+    return valid;
+}

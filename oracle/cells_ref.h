@@ -202,6 +202,14 @@ int ref_field_step(const cells_world* w, const cells_field_grid* g, const cells_
                    const cells_symbolic* ksym, const double* y, double* out, double dt, const cells_control* c,
                    int32_t refresh, double* op_weight, double* op_scale, double* op_factor, int32_t* valid);
 
+/* The field solve's energy rates (numerics/surface_field.py:792-835) of spatial world k at state y
+ * with the field's potentials phi (its cells', then its nodes', as ref_field_start gives them):
+ * out[11] = chemical, capacitor, dissipation, reservoir power, electrode power, energy balance,
+ * electrode current, capacitor energy, membrane heat, junction heat, face heat. Returns whether
+ * every value is finite. */
+int ref_field_energy_rates(const cells_world* w, const cells_field_grid* g, int32_t k, const double* y,
+                           const cells_control* c, const double* phi, double* out);
+
 /* The reference's log-mean (numerics/extracellular_field.py:48-62): mean and log ratio. */
 void ref_logmean(double left, double right, double* mean, double* log_ratio);
 

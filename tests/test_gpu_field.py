@@ -8,6 +8,8 @@
   (above 100 rows) run in HBM scratch and through the workgroup solve.
 - The field solve alone (cells_field_potentials) against the oracle's Newton at states some steps
   into a run: node and cell potentials within 1e-9 of their scale.
+- The field solve's energy terms (cells_field_energy_rates) against the oracle's at the device's
+  potentials, and their balance, which vanishes at the field solve's potentials.
 - At the full 256 x 256 grid with 2048 cells: material closes per species against the ring
   supplies (1e-12 of the amounts), every interior node with its cell is neutral after each step
   (1e-12 of the cells' charge), and the refusal path keeps a world's state.
@@ -128,6 +130,54 @@ def test_field_solve_matches_the_oracle(torch, cfg, steps):
         c0 = int(fp.cells.cell_off[k])
         got_cells = phi[k][fp.cell_node[c0:c0 + n]] + cq[c0:c0 + n] / fp.cells.cap[c0:c0 + n]
         assert np.max(np.abs(got_cells - cells)) <= 1e-9 * np.max(np.abs(cells))
+    e.close()
+
+
+@pytest.mark.parametrize("cfg,steps", [
+    (small_field(worlds=2, seed=21), 8),
+    (FieldConfig(worlds=2, cells=36, row_width=6, side=40, spacing=5, seed=22), 6),
+], ids=["14-grid", "40-grid"])
+def test_field_energy_rates_match_the_oracle(torch, cfg, steps):
+    """The field solve's energy terms (cells_field_energy_rates; numerics/surface_field.py:792-835)
+    against the oracle's at the device's own potentials: every rate within 1e-10 of the world's
+    dissipation, the capacitor energy within 1e-12 of itself; the balance vanishes on the device
+    (below 1e-8 of the dissipation: the chord iteration meets the field solve's continuity target)
+    and the dissipation is the three routes' heat. Against the oracle's own field solve (a Newton
+    root met to the same target) the terms agree within 1e-6 of the dissipation."""
+    from cells_amd.field import FieldEnsemble
+
+    fp = build_field(cfg)
+    say(f"field energy rates: side {cfg.side}, {cfg.cells} cells, {steps} steps")
+    y, ok_ref = oracle.field_run(fp, initial_field_state(fp), 1e-3, steps, 3)
+    assert ok_ref.all()
+    op = oracle.FieldOperator(fp)
+    e = FieldEnsemble(fp, refresh_every=3)
+    got = e.energy_rates(y, 1e-3).cpu().numpy()
+    phi, _, ok = e.potentials(y, 1e-3)
+    assert bool(ok.all())
+    phi = phi.cpu().numpy()
+    names = oracle.FIELD_ENERGY_RATES
+    ix = {n: names.index(n) for n in names}
+    rates = [ix[n] for n in names if n != "capacitor_energy"]
+    n = fp.config.cells
+    cq, _ = charges(fp, y)
+    for k in range(fp.W):
+        c0 = int(fp.cells.cell_off[k])
+        cells = phi[k][fp.cell_node[c0:c0 + n]] + cq[c0:c0 + n] / fp.cells.cap[c0:c0 + n]
+        want, valid = oracle.field_energy_rates(fp, y, op, k, phi=(cells, phi[k]))
+        assert valid
+        d = want[ix["dissipation"]]
+        assert d > 0
+        err = np.abs(got[k] - want)
+        assert np.max(err[rates]) <= 1e-10 * d, (k, dict(zip(names, err / d)))
+        ce = ix["capacitor_energy"]
+        assert err[ce] <= 1e-12 * abs(want[ce])
+        assert abs(got[k][ix["energy_balance"]]) <= 1e-8 * d, (k, got[k][ix["energy_balance"]] / d)
+        heat = got[k][ix["membrane_heat"]] + got[k][ix["junction_heat"]] + got[k][ix["face_heat"]]
+        assert abs(got[k][ix["dissipation"]] - heat) <= 1e-12 * d
+        solved, conv = oracle.field_energy_rates(fp, y, op, k)
+        assert conv
+        assert np.max(np.abs(got[k] - solved)[rates]) <= 1e-6 * d
     e.close()
 
 

@@ -322,3 +322,62 @@ def test_field_solve_balances_faces_and_junctions():
     cq, _ = charges(fp, y)
     n = fp.config.cells
     assert np.allclose(cells, nodes[fp.cell_node[:n]] + cq[:n] / fp.cells.cap[:n], rtol=1e-14, atol=0)
+
+
+def _field_states(fp, op, steps=8):
+    """A few accepted oracle states of a small spatial world along its opening transient."""
+    w, g = oracle.world_struct(fp.cells), oracle.field_grid_struct(fp)
+    y = initial_field_state(fp)
+    out = []
+    for t in range(steps):
+        y, ok = oracle.field_step(fp, y, 1e-3, op, True, None, w, g)
+        assert ok.all()
+        if t % 2 == 1:
+            out.append(y.copy())
+    return out
+
+
+def test_field_energy_balance_vanishes_and_slips_break_it():
+    """The field solve's energy terms (numerics/surface_field.py:792-835): with current continuity
+    met at every floating component (the field solve's target), the free energy's rate (chemical +
+    capacitor) plus the transport's dissipation less the ring's reservoir power (the electrode's is
+    zero: the ring holds no capacitor) vanishes: every route's J mu telescopes into the nodes'
+    electrochemical potentials, and each cell-with-node component's potential terms reduce to its
+    capacitor's V_m times its charge rate. The balance stays below 1e-9 of the dissipation; a
+    capacitor rate of the wrong sign, or the face or membrane heat left out, leaves an imbalance
+    above 1e-6 of it (the faces' heat is the smallest, about 5e-5 of the total here)."""
+    fp = build_field(small_field(worlds=1, seed=17))
+    op = oracle.FieldOperator(fp)
+    names = oracle.FIELD_ENERGY_RATES
+    ix = {n: names.index(n) for n in names}
+    for y in _field_states(fp, op):
+        er, ok = oracle.field_energy_rates(fp, y, op, 0)
+        assert ok
+        d = er[ix["dissipation"]]
+        assert d > 0 and er[ix["membrane_heat"]] > 0 and er[ix["face_heat"]] > 0
+        assert er[ix["junction_heat"]] >= 0
+        assert abs(er[ix["energy_balance"]]) <= 1e-9 * d, (er[ix["energy_balance"]], d)
+        assert er[ix["electrode_power"]] == 0.0 and er[ix["electrode_current"]] == 0.0
+        assert er[ix["capacitor_energy"]] > 0
+        # the dissipation is the three routes' heat
+        assert abs(d - (er[ix["membrane_heat"]] + er[ix["junction_heat"]] + er[ix["face_heat"]])) <= 1e-12 * d
+        for slip in (er[ix["energy_balance"]] - 2 * er[ix["capacitor"]],
+                     er[ix["energy_balance"]] - er[ix["face_heat"]],
+                     er[ix["energy_balance"]] - er[ix["membrane_heat"]]):
+            assert abs(slip) > 1e-6 * d, (slip, d)
+
+
+def test_field_energy_rates_with_potentials_off_the_field_solve_do_not_balance():
+    """The balance rests on the field solve's current continuity: the same terms at the state's
+    carried node potentials (the last operator solve's, which lag the field solve) miss it."""
+    fp = build_field(small_field(worlds=1, seed=17))
+    op = oracle.FieldOperator(fp)
+    y = _field_states(fp, op, steps=4)[-1]
+    L = FieldLayout.of(fp)
+    carried = L.node(y, L.phi)[0].copy()
+    cq, _ = charges(fp, y)
+    n = fp.config.cells
+    cells = carried[fp.cell_node[:n]] + cq[:n] / fp.cells.cap[:n]
+    er, _ = oracle.field_energy_rates(fp, y, op, 0, phi=(cells, carried))
+    names = oracle.FIELD_ENERGY_RATES
+    assert abs(er[names.index("energy_balance")]) > 1e-6 * er[names.index("dissipation")]
