@@ -279,6 +279,50 @@ __device__ __forceinline__ void route(double cf, double ct, double dv, double z,
   }
 }
 
+// The Bernoulli values of a route at charge z and potential drop dv, as route() takes them:
+// bf = B(-x), bt = B(x) with x = z F dv / (R T), and with slopes sf = B'(-xs), st = B'(xs) with
+// xs = (z F / (R T)) dv. Every operation in x and xs is odd in z (a product or quotient whose
+// rounding is symmetric in sign), so a species whose charge equals another's has bitwise the same
+// values, and one whose charge is its negative has them swapped: the route's transcendentals are
+// evaluated once per distinct |z| and the fluxes keep their bits.
+struct RouteBasis {
+  double bf, bt, sf, st;
+};
+
+__device__ __forceinline__ RouteBasis route_basis(double z, double dv, double T, bool slope) {
+  RouteBasis r;
+  const double x = z * kF * dv / (kR * T);
+  r.bf = bernoulli(-x);
+  r.bt = bernoulli(x);
+  r.sf = 0.0;
+  r.st = 0.0;
+  if (slope) {
+    const double xs = z * kF / (kR * T) * dv;
+    r.sf = bernoulli_slope(-xs);
+    r.st = bernoulli_slope(xs);
+  }
+  return r;
+}
+
+// Species s's basis from species 0's (b0), reused when |z_s| = |z_0| (uniform branches).
+__device__ __forceinline__ RouteBasis species_basis(const double* z, int s, const RouteBasis& b0, double dv,
+                                                    double T, bool slope) {
+  if (s == 0 || z[s] == z[0]) return b0;
+  if (z[s] == -z[0]) return RouteBasis{b0.bt, b0.bf, b0.st, b0.sf};
+  return route_basis(z[s], dv, T, slope);
+}
+
+// route() on a precomputed basis: the same operations in the same order, so the same bits.
+__device__ __forceinline__ void route_on(const RouteBasis& b, double cf, double ct, double z, double T,
+                                         double pa, double avail, bool slope, double& J, double& g) {
+  J = pa * avail / kMolPerFmol * (cf * b.bf - ct * b.bt);
+  if (slope) {
+    double k = z * kF / (kR * T);
+    double pre = pa * avail / kMolPerFmol;
+    g = pre * k * (-cf * b.sf - ct * b.st);
+  }
+}
+
 __device__ __forceinline__ void cell_concentrations(const Params& P, const double* y,
                                                     const double* yv, int i, double* c,
                                                     double& vol) {
@@ -331,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, con
   bool ok = true;
   double diss = 0.0, G = 0.0;
   constexpr bool slopes = SLOPES;  // gj, Gj non-null exactly when SLOPES
+  const RouteBasis b0 = route_basis(P.z[0], dv, P.T, slopes);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {  // species, unrolled (S <= 4)
     if (s >= P.S) break;
@@ -338,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, con
     double pa = P.paj[(long)s * P.nedge + e];
     ok = ok && isfinite(ca[s]) && ca[s] > 0 && isfinite(cb[s]) && cb[s] > 0;
     double J, sl = 0.0;
-    route(ca[s], cb[s], dv, z, P.T, pa, avail, slopes, J, sl);
+    route_on(species_basis(P.z, s, b0, dv, P.T, slopes), ca[s], cb[s], z, P.T, pa, avail, slopes, J, sl);
     double mu = P.RT * (log(ca[s]) - log(cb[s])) + z * kF * dv;
     diss = s == 0 ? J * mu : diss + J * mu;
     ok = ok && fabs(z * kF * dv / P.RT) <= kMaxAbsX && isfinite(J);
@@ -387,6 +432,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
   constexpr bool slopes = SLOPES;  // gm, Gj, mdiag, msc non-null exactly when SLOPES
   double Jm[4], mu[4];
   double diss = 0.0, Gm = 0.0;
+  const RouteBasis b0 = route_basis(P.z[0], dv, P.T, slopes);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {  // species, unrolled (S <= 4)
     if (s >= P.S) break;
@@ -394,7 +440,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     ok = ok && isfinite(ci[s]) && ci[s] > 0;
     double pa = P.pam[(long)s * P.ncell + i];
     double sl = 0.0;
-    route(ci[s], P.cb[s], dv, z, P.T, pa, c.membrane, slopes, Jm[s], sl);
+    route_on(species_basis(P.z, s, b0, dv, P.T, slopes), ci[s], P.cb[s], z, P.T, pa, c.membrane, slopes, Jm[s],
+             sl);
     mu[s] = P.RT * (log(ci[s]) - P.log_cb[s]) + z * kF * dv;
     diss = s == 0 ? Jm[s] * mu[s] : diss + Jm[s] * mu[s];
     ok = ok && fabs(z * kF * dv / P.RT) <= kMaxAbsX && isfinite(Jm[s]);
