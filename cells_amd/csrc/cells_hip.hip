@@ -354,11 +354,13 @@ __device__ __forceinline__ double* next(double* A, double* B, int s) { return s 
 // Junction routes (numerics/surface_transport.py:310-367, junction branch; gate rhs
 // junction_voltage_relaxation.py:41-59; slopes models/surface_pump_v3.py:2747-2755).
 
-template <bool SLOPES>
-__global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, const double* B,
-                                                  const int* sel, const double* src_override,
-                                                  int vol_next, int e0, int e1, Ctl c, double* R,
-                                                  double* gj, double* Gj, int* valid) {
+// AHEAD (the ROS2 second stage): the stage state's gate is formed here, y's gate + gdt K1's (the
+// value the stage combination would have stored), and stored into the stage state Y1g as well.
+template <bool SLOPES, bool AHEAD>
+__device__ __forceinline__ void edges_body(const Params& P, const double* A, const double* B, const int* sel,
+                                           const double* src_override, int vol_next, int e0, int e1, const Ctl& c,
+                                           double* R, double* gj, double* Gj, int* valid, const double* K1g,
+                                           double gdt, double* Y1g) {
   int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= e1) return;
   int w = P.edge_world[e];
@@ -370,7 +372,13 @@ __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, con
   cell_concentrations(P, y, yv, a, ca, va);
   cell_concentrations(P, y, yv, b, cb, vb);
   double dv = P.phi ? P.phi[a] - P.phi[b] : potential(P, y, a) - potential(P, y, b);
-  double g = EFc(P, y, P.f_gate, e);
+  double g;
+  if constexpr (AHEAD) {
+    g = EFc(P, current(A, B, sw), P.f_gate, e) + gdt * EFc(P, K1g, P.f_gate, e);
+    EF(P, Y1g, P.f_gate, e) = g;
+  } else {
+    g = EFc(P, y, P.f_gate, e);
+  }
   double avail = c.junction * g;
   bool ok = true;
   double diss = 0.0, G = 0.0;
@@ -404,6 +412,23 @@ __global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, con
   ok = ok && isfinite(dv) && fabs(dv) <= kGateMaxVoltage && isfinite(g) && g >= P.gate_min &&
        g <= 1.0;
   if (!ok) valid[w] = 0;
+}
+
+template <bool SLOPES>
+__global__ __launch_bounds__(kBlock) void k_edges(Params P, const double* A, const double* B,
+                                                  const int* sel, const double* src_override,
+                                                  int vol_next, int e0, int e1, Ctl c, double* R,
+                                                  double* gj, double* Gj, int* valid) {
+  edges_body<SLOPES, false>(P, A, B, sel, src_override, vol_next, e0, e1, c, R, gj, Gj, valid, nullptr, 0.0,
+                            nullptr);
+}
+
+// The ROS2 second stage's junction rates at Y1 (its amounts from the stage-1 combination, its gates
+// formed here from y and K1 and stored into Y1).
+__global__ __launch_bounds__(kBlock) void k_edges_ahead(Params P, const double* A, const double* B, const int* sel,
+                                                        double* Y1, int e0, int e1, Ctl c, double* R,
+                                                        const double* K1, double dt, int* valid) {
+  edges_body<false, true>(P, A, B, sel, Y1, 0, e0, e1, c, R, nullptr, nullptr, valid, K1, dt, Y1);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2032,6 +2057,9 @@ __device__ void correct_edge(const Params& P, int e, double coef, const double* 
 // Stage 1: the stage solve's residual (its right-hand side is K1's charge rate before the
 // correction); K1 += correction(dphi1); Y1 = y + dt K1 on the fields the rates read (amounts,
 // gates, and the water volume, which the stage carries from the water half-step).
+// EDGES = false leaves the junctions to the kernels after it: the second stage's junction rates form
+// Y1's gates (k_edges_ahead) and the final combination adds K1's junction correction itself.
+template <bool EDGES>
 __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* A, const double* B,
                                                         const int* sel, int c0, int nc, int e0,
                                                         int ne, double dt, double coef,
@@ -2058,7 +2086,7 @@ __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* 
       const double* yv = next(const_cast<double*>(A), const_cast<double*>(B), sw);
       CF(P, Y1, P.f_dvol, i) = CFc(P, yv, P.f_dvol, i);
     }
-  } else if (t < nc + ne) {
+  } else if (EDGES && t < nc + ne) {
     int e = e0 + (t - nc);
     const double* y = current(A, B, sel[P.edge_world[e]]);
     correct_edge(P, e, coef, gj, dphi, K1);
@@ -2068,7 +2096,9 @@ __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* 
 
 // Final: k2 = (F1 - 2 k1) + correction(dphi2), held in registers; the stage-2 solve's residual
 // (its right-hand side is the charge rate of F1 - 2 k1); y_new = (y + 1.5 dt k1) + 0.5 dt k2; the
-// cell's bath-supply contribution of k2; domain. F1 is read from K2.
+// cell's bath-supply contribution of k2; domain. F1 is read from K2. With dphi1 (the stage-1
+// combination left the junctions alone) k1's junction transfers take their correction here, as
+// that combination would have stored it.
 __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, double* B,
                                                        const int* sel, int c0, int nc, int e0,
                                                        int ne, double dt, double coef,
@@ -2076,7 +2106,7 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
                                                        const double* Gj, const double* dphi,
                                                        double atol, double rtol, const double* K1,
                                                        const double* K2, const double* contrib1,
-                                                       double* contrib2, int* valid) {
+                                                       double* contrib2, const double* dphi1, int* valid) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   double h1 = 1.5 * dt, h2 = 0.5 * dt;
   if (t < nc) {
@@ -2117,8 +2147,10 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
     const double* y = current(A, B, sel[w]);
     double* o = next(A, B, sel[w]);
     double d = dphi[P.snd[e]] - dphi[P.rcv[e]];
+    const double d1 = dphi1 ? dphi1[P.snd[e]] - dphi1[P.rcv[e]] : 0.0;
     for (int f = 0; f < P.ef; ++f) {
       double k1 = EFc(P, K1, f, e);
+      if (dphi1 && f >= P.f_jtr && f < P.f_jtr + P.S) k1 += coef * (gj[(long)(f - P.f_jtr) * P.nedge + e] * d1);
       double k2 = EFc(P, K2, f, e) + -2.0 * k1;
       if (f >= P.f_jtr && f < P.f_jtr + P.S) k2 += coef * (gj[(long)(f - P.f_jtr) * P.nedge + e] * d);
       EF(P, o, f, e) = (EFc(P, y, f, e) + h1 * k1) + h2 * k2;
@@ -4130,13 +4162,13 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
                        yrhs ? 1 : 0);
   });
     mark("k_solve_stage");
-    hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
+    hipLaunchKernelGGL(k_ros2_stage1<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->Gj, e->dphi, e->res_atol, e->res_rtol, e->K1,
                        e->contrib1, e->Y1, e->valid);
     mark("k_ros2_stage1");
     if (ne) {
-      hipLaunchKernelGGL(k_edges<false>, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, e->Y1, 0, ed0,
-                         ed1, c, e->K2, nullptr, nullptr, e->valid);
+      hipLaunchKernelGGL(k_edges_ahead, dim3(blocks(ne)), dim3(kBlock), 0, st, P, A, B, e->sel, e->Y1, ed0, ed1,
+                         c, e->K2, e->K1, dt, e->valid);
       mark("k_edges");
     }
     // stage 2 reads the stage-1 Jacobi scales (msc) for its right-hand side; it writes neither msc
@@ -4152,7 +4184,7 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     mark("k_solve_stage");
     hipLaunchKernelGGL(k_ros2_final, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, A, B, e->sel, c0, nc,
                        ed0, ne, dt, coef, e->gm, e->gj, e->Gj, e->dphi2, e->res_atol, e->res_rtol, e->K1, e->K2,
-                       e->contrib1, e->contrib2, e->valid);
+                       e->contrib1, e->contrib2, e->dphi, e->valid);
     mark("k_ros2_final");
     hipLaunchKernelGGL(k_world_ionic, dim3(nw), dim3(kBlock), 0, st, P, A, B, e->sel, w0, e->contrib1,
                        1.5 * dt, e->contrib2, 0.5 * dt, e->valid);
@@ -4546,7 +4578,7 @@ int step_vjp(cells_ensemble* e, void* stream, const double* y, const double* yba
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(W), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of, e->pool,
                      e->foff, 0, e->msc, e->fvals, e->K1, nullptr, a.d1, e->ysc, e->y_in_lds, solve_args(e), part, 0);
   });
-  hipLaunchKernelGGL(k_ros2_stage1, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
+  hipLaunchKernelGGL(k_ros2_stage1<true>, dim3(blocks(nc + ne)), dim3(kBlock), 0, st, P, e->A, e->B, e->sel, 0, nc, 0,
                      ne, dt, coef, e->gm, e->gj, e->Gj, a.d1, e->res_atol, e->res_rtol, e->K1, e->contrib1, e->Y1,
                      e->valid);
   if (ne)
