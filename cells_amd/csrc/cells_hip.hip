@@ -354,8 +354,9 @@ __device__ __forceinline__ double* next(double* A, double* B, int s) { return s 
 // Junction routes (numerics/surface_transport.py:310-367, junction branch; gate rhs
 // junction_voltage_relaxation.py:41-59; slopes models/surface_pump_v3.py:2747-2755).
 
-// AHEAD (the ROS2 second stage): the stage state's gate is formed here, y's gate + gdt K1's (the
-// value the stage combination would have stored), and stored into the stage state Y1g as well.
+// AHEAD (the ROS2 second stage, gdt = dt): the stage state's gate is formed here, y's gate + gdt
+// K1's (the value the stage combination would have stored), and stored into the stage state Y1g as
+// well; the junction heat and gate fields of the step's result are finished here.
 template <bool SLOPES, bool AHEAD>
 __device__ __forceinline__ void edges_body(const Params& P, const double* A, const double* B, const int* sel,
                                            const double* src_override, int vol_next, int e0, int e1, const Ctl& c,
@@ -402,13 +403,25 @@ __device__ __forceinline__ void edges_body(const Params& P, const double* A, con
     }
   }
   if (slopes) Gj[e] = kChargePerFmol * G;
-  EF(P, R, P.f_jdiss, e) = 1e-3 * diss;
   ok = ok && isfinite(1e-3 * diss);
   double shift = fabs(dv) - P.gate_thr;
   double alpha = 1.3 * exp(-77.0 * shift);
   double log_beta = P.log_0065 + 140.0 * shift;
   double beta = 20.0 * exp(-logaddexp0(-log_beta));
-  EF(P, R, P.f_gate, e) = alpha * (1.0 - g) - beta * (g - P.gate_min);
+  const double grate = alpha * (1.0 - g) - beta * (g - P.gate_min);
+  if constexpr (AHEAD) {
+    // the junction heat and the gate take no stage-solve correction: (y + 1.5 dt k1) + 0.5 dt
+    // (F1 - 2 k1) straight into the step's result, as the final combination would form them
+    const double* yc = current(A, B, sw);
+    double* o = next(const_cast<double*>(A), const_cast<double*>(B), sw);
+    const double h1 = 1.5 * gdt, h2 = 0.5 * gdt;
+    const double kd = EFc(P, K1g, P.f_jdiss, e), kg = EFc(P, K1g, P.f_gate, e);
+    EF(P, o, P.f_jdiss, e) = (EFc(P, yc, P.f_jdiss, e) + h1 * kd) + h2 * (1e-3 * diss + -2.0 * kd);
+    EF(P, o, P.f_gate, e) = (EFc(P, yc, P.f_gate, e) + h1 * kg) + h2 * (grate + -2.0 * kg);
+  } else {
+    EF(P, R, P.f_jdiss, e) = 1e-3 * diss;
+    EF(P, R, P.f_gate, e) = grate;
+  }
   ok = ok && isfinite(dv) && fabs(dv) <= kGateMaxVoltage && isfinite(g) && g >= P.gate_min &&
        g <= 1.0;
   if (!ok) valid[w] = 0;
@@ -436,14 +449,15 @@ __global__ __launch_bounds__(kBlock) void k_edges_ahead(Params P, const double* 
 // With coef > 0 also the stage matrix's diagonal and its Jacobi scale
 // (numerics/surface_imex.py:228-259 on C + coef G).
 
-template <bool SLOPES>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ? 4 : 5))) void k_cells(Params P, const double* A, const double* B,
-                                                  const int* sel, const double* src_override,
-                                                  int vol_next, int c0, int c1, Ctl c, double* R,
-                                                  double* contrib, double* gm, const double* Gj,
-                                                  double coef, double* mdiag, double* msc,
-                                                  double rate_dt, double* yrhs, const double* K1,
-                                                  int* valid) {
+// LEDGERS (the ROS2 second stage, K1 set): the fields no stage solve corrects (membrane heat, the
+// pumps' cycles, heat and work) go straight to the step's result, (y + 1.5 dt k1) + 0.5 dt (F1 - 2
+// k1) with ldt = dt, as the final combination would form them, instead of into R.
+template <bool SLOPES, bool LEDGERS>
+__device__ __forceinline__ void cells_body(Params P, const double* A, const double* B, const int* sel,
+                                           const double* src_override, int vol_next, int c0, int c1, Ctl c,
+                                           double* R, double* contrib, double* gm, const double* Gj, double coef,
+                                           double* mdiag, double* msc, double rate_dt, double* yrhs,
+                                           const double* K1, int* valid, double ldt) {
   int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c1) return;
   int w = P.cell_world[i];
@@ -547,17 +561,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     }
   }
   double mdiss = 1e-3 * diss;
-  CF(P, R, P.f_mdiss, i) = mdiss;
-  CF(P, R, P.f_pcyc, i) = cyc;
-  CF(P, R, P.f_pdiss, i) = -1e-3 * cyc * energy;
-  CF(P, R, P.f_pwork, i) = -1e-3 * cyc * P.nak_drive;
-  ok = ok && isfinite(mdiss) && isfinite(cyc) && isfinite(-1e-3 * cyc * energy);
-  if (P.calcium) {
-    CF(P, R, P.f_ccyc, i) = ccyc;
-    CF(P, R, P.f_cdiss, i) = -1e-3 * ccyc * cenergy;
-    CF(P, R, P.f_cwork, i) = -1e-3 * ccyc * P.ca_drive;
-    ok = ok && isfinite(ccyc) && isfinite(-1e-3 * ccyc * cenergy);
+  if constexpr (LEDGERS) {
+    const double* yc = current(A, B, sw);
+    double* o = next(const_cast<double*>(A), const_cast<double*>(B), sw);
+    const double h1 = 1.5 * ldt, h2 = 0.5 * ldt;
+    const int nf = P.calcium ? 7 : 4;
+    const double r[7] = {mdiss, cyc, -1e-3 * cyc * energy, -1e-3 * cyc * P.nak_drive, ccyc, -1e-3 * ccyc * cenergy,
+                         -1e-3 * ccyc * P.ca_drive};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      if (k >= nf) break;
+      const int f = k < 4 ? P.f_mdiss + k : P.f_ccyc + (k - 4);
+      const double k1 = CFc(P, K1, f, i);
+      const double k2 = r[k] + -2.0 * k1;
+      CF(P, o, f, i) = (CFc(P, yc, f, i) + h1 * k1) + h2 * k2;
+    }
+  } else {
+    CF(P, R, P.f_mdiss, i) = mdiss;
+    CF(P, R, P.f_pcyc, i) = cyc;
+    CF(P, R, P.f_pdiss, i) = -1e-3 * cyc * energy;
+    CF(P, R, P.f_pwork, i) = -1e-3 * cyc * P.nak_drive;
+    if (P.calcium) {
+      CF(P, R, P.f_ccyc, i) = ccyc;
+      CF(P, R, P.f_cdiss, i) = -1e-3 * ccyc * cenergy;
+      CF(P, R, P.f_cwork, i) = -1e-3 * ccyc * P.ca_drive;
+    }
   }
+  ok = ok && isfinite(mdiss) && isfinite(cyc) && isfinite(-1e-3 * cyc * energy);
+  if (P.calcium) ok = ok && isfinite(ccyc) && isfinite(-1e-3 * ccyc * cenergy);
   if (P.water) {
     CF(P, R, P.f_dvol, i) = 0.0;
     CF(P, R, P.f_wtr, i) = 0.0;
@@ -595,6 +626,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ?
     }
   }
   if (!ok) valid[w] = 0;
+}
+
+template <bool SLOPES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SLOPES ? 4 : 5))) void k_cells(Params P, const double* A, const double* B,
+                                                  const int* sel, const double* src_override,
+                                                  int vol_next, int c0, int c1, Ctl c, double* R,
+                                                  double* contrib, double* gm, const double* Gj,
+                                                  double coef, double* mdiag, double* msc,
+                                                  double rate_dt, double* yrhs, const double* K1,
+                                                  int* valid) {
+  cells_body<SLOPES, false>(P, A, B, sel, src_override, vol_next, c0, c1, c, R, contrib, gm, Gj, coef, mdiag, msc,
+                            rate_dt, yrhs, K1, valid, 0.0);
+}
+
+// The ROS2 second stage's cell rates at Y1 (K1 set), their uncorrected ledger fields finished into
+// the step's result (dt the step).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_cells_ahead(
+    Params P, const double* A, const double* B, const int* sel, const double* Y1, int c0, int c1, Ctl c, double* R,
+    double* contrib, const double* msc, double* yrhs, const double* K1, double dt, int* valid) {
+  cells_body<false, true>(P, A, B, sel, Y1, 0, c0, c1, c, R, contrib, nullptr, nullptr, 0.0, nullptr,
+                          const_cast<double*>(msc), 0.0, yrhs, K1, valid, dt);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2117,8 +2169,8 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
     double dn_c[4], dm_c[4], mx, mag;
     cell_correction<true>(P, i, coef, gm, gj, Gj, dphi, dn_c, dm_c, &mx, &mag);
     double qt = 0.0;  // the charge of F1 - 2 k1, summed as the stage-2 solve sums it
-    for (int f = 0; f < P.cf; ++f) {
-      if (is_water_field(P, f)) continue;
+    // the amounts and membrane transfers (the stage-2 rate kernels finished the ledger fields)
+    for (int f = 0; f < P.f_mtr + P.S; ++f) {
       double k1 = CFc(P, K1, f, i);
       double k2 = CFc(P, K2, f, i) + -2.0 * k1;
       if (f >= P.f_dn && f < P.f_dn + P.S) {
@@ -2148,7 +2200,7 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
     double* o = next(A, B, sel[w]);
     double d = dphi[P.snd[e]] - dphi[P.rcv[e]];
     const double d1 = dphi1 ? dphi1[P.snd[e]] - dphi1[P.rcv[e]] : 0.0;
-    for (int f = 0; f < P.ef; ++f) {
+    for (int f = 0; f < P.f_jtr + P.S; ++f) {  // the transfers (k_edges_ahead finished the rest)
       double k1 = EFc(P, K1, f, e);
       if (dphi1 && f >= P.f_jtr && f < P.f_jtr + P.S) k1 += coef * (gj[(long)(f - P.f_jtr) * P.nedge + e] * d1);
       double k2 = EFc(P, K2, f, e) + -2.0 * k1;
@@ -4173,8 +4225,8 @@ int launch_step(cells_ensemble* e, hipStream_t st, double dt, const Ctl& c, int 
     }
     // stage 2 reads the stage-1 Jacobi scales (msc) for its right-hand side; it writes neither msc
     // nor mdiag (no slopes)
-    hipLaunchKernelGGL(k_cells<false>, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, e->Y1, 0, c0, c1,
-                       c, e->K2, e->contrib2, nullptr, nullptr, 0.0, nullptr, e->msc, 0.0, yrhs, e->K1, e->valid);
+    hipLaunchKernelGGL(k_cells_ahead, dim3(blocks(nc)), dim3(kBlock), 0, st, P, A, B, e->sel, e->Y1, c0, c1, c,
+                       e->K2, e->contrib2, e->msc, yrhs, e->K1, dt, e->valid);
     mark("k_cells");
     solve_stage(e, st, nw, e->fvals, [&](int part) {
     hipLaunchKernelGGL((e->pairwise ? k_solve_stage<false> : k_solve_stage<true>), dim3(nw), solve_block(e), solve_lds_bytes(e), st, P, e->shapes, e->shape_of,
