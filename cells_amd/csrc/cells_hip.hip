@@ -2083,6 +2083,9 @@ __device__ __forceinline__ bool stage_residual_ok(double q, double mx, double ma
 
 // Adds coef (df/dphi) dphi to K's transfers and amounts (and the cell's supply contribution);
 // returns whether the stage solve's residual at the cell holds (q: the solve's right-hand side).
+// MEMBRANE = false: the amounts alone (the membrane transfers and the supply contribution are
+// corrected where they are next read).
+template <bool MEMBRANE>
 __device__ bool correct_cell(const Params& P, int i, double coef, const double* gm, const double* gj,
                              const double* Gj, const double* dphi, double q, double atol, double rtol,
                              double* K, double* contrib) {
@@ -2092,8 +2095,10 @@ __device__ bool correct_cell(const Params& P, int i, double coef, const double* 
   for (int s = 0; s < 4; ++s) {
     if (s >= P.S) break;
     CF(P, K, P.f_dn + s, i) += dn_c[s];
-    CF(P, K, P.f_mtr + s, i) += dm_c[s];
-    contrib[(long)s * P.ncell + i] = contrib[(long)s * P.ncell + i] - dm_c[s];
+    if constexpr (MEMBRANE) {
+      CF(P, K, P.f_mtr + s, i) += dm_c[s];
+      contrib[(long)s * P.ncell + i] = contrib[(long)s * P.ncell + i] - dm_c[s];
+    }
   }
   return stage_residual_ok(q, mx, mag, atol, rtol);
 }
@@ -2109,8 +2114,9 @@ __device__ void correct_edge(const Params& P, int e, double coef, const double* 
 // Stage 1: the stage solve's residual (its right-hand side is K1's charge rate before the
 // correction); K1 += correction(dphi1); Y1 = y + dt K1 on the fields the rates read (amounts,
 // gates, and the water volume, which the stage carries from the water half-step).
-// EDGES = false leaves the junctions to the kernels after it: the second stage's junction rates form
-// Y1's gates (k_edges_ahead) and the final combination adds K1's junction correction itself.
+// EDGES = false leaves all but the amounts to the kernels after it: the second stage's junction
+// rates form Y1's gates (k_edges_ahead), and the final combination adds K1's junction and membrane
+// transfer corrections and the supply contribution's itself.
 template <bool EDGES>
 __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* A, const double* B,
                                                         const int* sel, int c0, int nc, int e0,
@@ -2126,7 +2132,7 @@ __global__ __launch_bounds__(kBlock) void k_ros2_stage1(Params P, const double* 
     int sw = sel[w];
     const double* y = current(A, B, sw);
     const double q = charge_increment(P, K1, i);
-    if (!correct_cell(P, i, coef, gm, gj, Gj, dphi, q, atol, rtol, K1, contrib1)) {
+    if (!correct_cell<EDGES>(P, i, coef, gm, gj, Gj, dphi, q, atol, rtol, K1, contrib1)) {
       valid[w] = 0;
       atomicOr(P.refuse_bits + w, kRefuseResidual);
     }
@@ -2157,7 +2163,7 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
                                                        const double* gm, const double* gj,
                                                        const double* Gj, const double* dphi,
                                                        double atol, double rtol, const double* K1,
-                                                       const double* K2, const double* contrib1,
+                                                       const double* K2, double* contrib1,
                                                        double* contrib2, const double* dphi1, int* valid) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   double h1 = 1.5 * dt, h2 = 0.5 * dt;
@@ -2168,10 +2174,19 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
     double* o = next(A, B, sel[w]);
     double dn_c[4], dm_c[4], mx, mag;
     cell_correction<true>(P, i, coef, gm, gj, Gj, dphi, dn_c, dm_c, &mx, &mag);
+    // with dphi1, k1's membrane-transfer correction (cell_correction's, from the cell's own dphi1)
+    double dm1[4] = {0.0, 0.0, 0.0, 0.0};
+    if (dphi1) {
+      const double dp1 = dphi1[i];
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (s < P.S) dm1[s] = coef * (gm[(long)s * P.ncell + i] * dp1);
+    }
     double qt = 0.0;  // the charge of F1 - 2 k1, summed as the stage-2 solve sums it
     // the amounts and membrane transfers (the stage-2 rate kernels finished the ledger fields)
     for (int f = 0; f < P.f_mtr + P.S; ++f) {
       double k1 = CFc(P, K1, f, i);
+      if (dphi1 && f >= P.f_mtr) k1 += dm1[f - P.f_mtr];
       double k2 = CFc(P, K2, f, i) + -2.0 * k1;
       if (f >= P.f_dn && f < P.f_dn + P.S) {
         const int sp = f - P.f_dn;
@@ -2185,7 +2200,12 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
     for (int s = 0; s < 4; ++s) {
       if (s >= P.S) break;
       long q = (long)s * P.ncell + i;
-      contrib2[q] = (contrib2[q] + -2.0 * contrib1[q]) - dm_c[s];
+      double c1 = contrib1[q];
+      if (dphi1) {
+        c1 = c1 - dm1[s];
+        contrib1[q] = c1;
+      }
+      contrib2[q] = (contrib2[q] + -2.0 * c1) - dm_c[s];
     }
     bool ok = cell_domain(P, o, o, i);
     if (!stage_residual_ok(kChargePerFmol * qt, mx, mag, atol, rtol)) {
