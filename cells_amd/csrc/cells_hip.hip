@@ -2020,7 +2020,10 @@ __global__ __launch_bounds__(kMfThreads) void k_factor_only_mf(const ShapeDev* s
 // the membrane transfers (the bath supply takes -dm_c). With Gj (RES) also the stage matrix's
 // row times dphi, (C + coef G) dphi at the cell (its diagonal summed as k_cells sums it), and the
 // row's magnitude (the sum of the absolute terms of that product).
-template <bool RES>
+// XB > 1: the incident edges are gathered XB at a time, every load of a batch issued before any is
+// used, so a wavefront's requests for the lines its cells share arrive together (the lattice numbers
+// a cell's out-edges and its left neighbour's next to each other); the sums keep their order.
+template <bool RES, int XB = 1>
 __device__ __forceinline__ void cell_correction(const Params& P, int i, double coef, const double* gm,
                                                 const double* gj, const double* Gj, const double* dphi,
                                                 double* dn_c, double* dm_c, double* mx = nullptr,
@@ -2030,7 +2033,48 @@ __device__ __forceinline__ void cell_correction(const Params& P, int i, double c
   // species still sums its outgoing, then incoming, edges in incidence order
   double out[4] = {0.0, 0.0, 0.0, 0.0}, in[4] = {0.0, 0.0, 0.0, 0.0};
   double gsum = 0.0, off = 0.0, offmag = 0.0;
-  for (int p = p0; p < p1; ++p) {
+  if constexpr (XB > 1) {
+    for (int pb = p0; pb < p1; pb += XB) {
+      int e[XB];
+#pragma unroll
+      for (int u = 0; u < XB; ++u) e[u] = pb + u < p1 ? P.inc_edge[pb + u] : -1;
+      int sn[XB], rc[XB];
+#pragma unroll
+      for (int u = 0; u < XB; ++u) {
+        sn[u] = e[u] >= 0 ? P.snd[e[u]] : i;
+        rc[u] = e[u] >= 0 ? P.rcv[e[u]] : i;
+      }
+      double ds[XB], dr[XB], ge[XB], gv[XB][4];
+#pragma unroll
+      for (int u = 0; u < XB; ++u) {
+        ds[u] = dphi[sn[u]];
+        dr[u] = dphi[rc[u]];
+        ge[u] = (RES && e[u] >= 0) ? Gj[e[u]] : 0.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) gv[u][s] = (s < P.S && e[u] >= 0) ? gj[(long)s * P.nedge + e[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < XB; ++u) {
+        if (e[u] < 0) break;
+        const double d = ds[u] - dr[u];
+        const bool outgoing = pb + u < pm;
+        if constexpr (RES) {
+          const double t = ge[u] * (outgoing ? dr[u] : ds[u]);
+          gsum += ge[u];
+          off += t;
+          offmag += fabs(t);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          if (s >= P.S) break;
+          const double t = coef * (gv[u][s] * d);
+          if (outgoing) out[s] += -t;
+          else in[s] += t;
+        }
+      }
+    }
+  }
+  for (int p = XB > 1 ? p1 : p0; p < p1; ++p) {
     const int e = P.inc_edge[p];
     const double ds = dphi[P.snd[e]], dr = dphi[P.rcv[e]];
     const double d = ds - dr;
@@ -2090,7 +2134,7 @@ __device__ bool correct_cell(const Params& P, int i, double coef, const double* 
                              const double* Gj, const double* dphi, double q, double atol, double rtol,
                              double* K, double* contrib) {
   double dn_c[4], dm_c[4], mx, mag;
-  cell_correction<true>(P, i, coef, gm, gj, Gj, dphi, dn_c, dm_c, &mx, &mag);
+  cell_correction<true, 4>(P, i, coef, gm, gj, Gj, dphi, dn_c, dm_c, &mx, &mag);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (s >= P.S) break;
