@@ -340,8 +340,9 @@ class FieldEnsemble:
         return phi, it, ok
 
     def energy_rates(self, y, dt, ctl=None):
-        """The field solve's energy terms at y (cells_field_energy_rates): [W, 11] on the device,
-        columns as lib.FIELD_ENERGY_RATES (numerics/surface_field.py:792-835)."""
+        """The field solve's energy terms at y (cells_field_energy_rates): ([W, 11] on the device,
+        columns as lib.FIELD_ENERGY_RATES (numerics/surface_field.py:792-835), ok [W]: the field
+        solve met its target)."""
         import ctypes
 
         from cells_amd import lib as _lib
@@ -352,11 +353,12 @@ class FieldEnsemble:
         if y.numel() != self.state_size:
             raise ValueError("state size disagrees with the worlds")
         out = t.empty((self.packed.W, len(_lib.FIELD_ENERGY_RATES)), dtype=t.float64, device=self.device)
+        ok = t.empty(self.packed.W, dtype=t.int32, device=self.device)
         ctl = ctl or control()
         _lib.check(_lib.load().cells_field_energy_rates(
             self._h, self._stream(), ctypes.c_void_p(y.data_ptr()), float(dt), ctypes.byref(ctl),
-            ctypes.c_void_p(out.data_ptr())))
-        return out
+            ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ok.data_ptr())))
+        return out, ok
 
     def plan_stats(self):
         """dict(nnz, flops, fronts, max_front, levels) of the operator's factor plan (per world)."""

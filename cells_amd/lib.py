@@ -208,7 +208,7 @@ def load():
     lib.cells_field_refresh.argtypes = [vp]
     lib.cells_field_set_start.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
     lib.cells_field_potentials.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct), vp, vp, vp]
-    lib.cells_field_energy_rates.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct), vp]
+    lib.cells_field_energy_rates.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct), vp, vp]
     lib.cells_field_plan_stats.argtypes = [vp, _i64p, _f64p, _i32p, _i32p, _i32p]
     lib.cells_field_kernel_times.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(ControlStruct),
                                              ctypes.c_int32, ctypes.c_int32, _f64p, ctypes.c_char_p, _i32p]

@@ -447,9 +447,10 @@ int cells_field_potentials(cells_field* f, void* stream, const double* y, double
  * (numerics/surface_field.py:792-835 energy terms). out [W][11], per world: chemical, capacitor,
  * dissipation, reservoir power, electrode power, energy balance, electrode current, capacitor
  * energy, membrane heat, junction heat, face heat. The balance (chemical + capacitor +
- * dissipation - reservoir - electrode) vanishes to rounding at the field solve's potentials. */
+ * dissipation - reservoir - electrode) vanishes to rounding at the field solve's potentials.
+ * ok_out [W] (may be NULL): 1 where the field solve met its target and the domain held. */
 int cells_field_energy_rates(cells_field* f, void* stream, const double* y, double dt,
-                             const cells_control* control, double* out);
+                             const cells_control* control, double* out, int32_t* ok_out);
 /* The operator plan: factor entries per world and flops per factor (sum of column counts squared)
  * of the bound pattern, and the supernodes, largest front and supernodal tree levels of the plan
  * (relaxed supernodes: the chains joined with their explicit zeros). */

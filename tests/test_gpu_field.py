@@ -152,7 +152,9 @@ def test_field_energy_rates_match_the_oracle(torch, cfg, steps):
     assert ok_ref.all()
     op = oracle.FieldOperator(fp)
     e = FieldEnsemble(fp, refresh_every=3)
-    got = e.energy_rates(y, 1e-3).cpu().numpy()
+    got, ok_e = e.energy_rates(y, 1e-3)
+    assert bool(ok_e.all())
+    got = got.cpu().numpy()
     phi, _, ok = e.potentials(y, 1e-3)
     assert bool(ok.all())
     phi = phi.cpu().numpy()
