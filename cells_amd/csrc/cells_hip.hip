@@ -2725,6 +2725,31 @@ __device__ __forceinline__ double bernoulli_second(double x) {
   return ex * (x * (ex + 1.0) - 2.0 * d) / (d * d * d);
 }
 
+// The Bernoulli values the reverse route kernels take at x = (z F / RT) dv and -x: B, B' and (with
+// SECOND) B''. As RouteBasis, x is odd in z operation by operation, so species of equal charge share
+// them and species of opposite charge take them swapped.
+struct VjpBasis {
+  double Bm, Bp, Sm, Sp, Qm, Qp;
+};
+
+__device__ __forceinline__ VjpBasis vjp_basis(double x, bool second) {
+  VjpBasis b;
+  b.Bm = bernoulli(-x);
+  b.Bp = bernoulli(x);
+  b.Sm = bernoulli_slope(-x);
+  b.Sp = bernoulli_slope(x);
+  b.Qm = second ? bernoulli_second(-x) : 0.0;
+  b.Qp = second ? bernoulli_second(x) : 0.0;
+  return b;
+}
+
+__device__ __forceinline__ VjpBasis vjp_species_basis(const double* z, int s, const VjpBasis& b0, double x,
+                                                      bool second) {
+  if (s == 0 || z[s] == z[0]) return b0;
+  if (z[s] == -z[0]) return VjpBasis{b0.Bp, b0.Bm, b0.Sp, b0.Sm, b0.Qp, b0.Qm};
+  return vjp_basis(x, second);
+}
+
 // Per edge: pull the amount-rate cotangent fN [S][n] (J leaves the sender, enters the receiver),
 // the gate-rate cotangent fg [E] and the slope cotangent gjbar [S][E] back to the edge's two
 // concentrations (ca_bar, cb_bar [S][E]), its voltage (ev_bar [E]), its gate (added to gate_out
@@ -2745,13 +2770,15 @@ __global__ __launch_bounds__(kBlock) void k_edges_vjp(Params P, const double* y,
   double g = EFc(P, y, P.f_gate, e);
   double unit = c.junction / kMolPerFmol;
   double vsum = 0.0, gbar = 0.0;
+  const VjpBasis b0 = vjp_basis(P.z[0] * kF / P.RT * dv, gjbar != nullptr);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (s >= P.S) break;
     double z = P.z[s], kk = z * kF / P.RT, x = kk * dv;
     double pa = P.paj[(long)s * P.nedge + e];
-    double Bm = bernoulli(-x), Bp = bernoulli(x);
-    double Sm = bernoulli_slope(-x), Sp = bernoulli_slope(x);
+    const VjpBasis bs = vjp_species_basis(P.z, s, b0, x, gjbar != nullptr);
+    double Bm = bs.Bm, Bp = bs.Bp;
+    double Sm = bs.Sm, Sp = bs.Sp;
     double A = pa * g * unit;
     double core = ca[s] * Bm - cb[s] * Bp;
     double score = -ca[s] * Sm - cb[s] * Sp;
@@ -2760,7 +2787,7 @@ __global__ __launch_bounds__(kBlock) void k_edges_vjp(Params P, const double* y,
     double Gbar = gjbar ? gjbar[(long)s * P.nedge + e] : 0.0;
     double cab = Jbar * A * Bm, cbb = Jbar * (-A * Bp), dvb = Jbar * gjv;
     if (gjbar) {
-      double Qm = bernoulli_second(-x), Qp = bernoulli_second(x);
+      double Qm = bs.Qm, Qp = bs.Qp;
       cab += Gbar * (-A * kk * Sm);
       cbb += Gbar * (-A * kk * Sp);
       dvb += Gbar * (A * kk * kk * (ca[s] * Qm - cb[s] * Qp));
@@ -2803,14 +2830,16 @@ __global__ __launch_bounds__(kBlock) void k_cells_vjp(Params P, const double* y,
   double unit = c.membrane / kMolPerFmol;
   double cbar[4] = {0, 0, 0, 0};
   double vbar = 0.0;
+  const VjpBasis b0 = vjp_basis(P.z[0] * kF / P.RT * dv, gmbar != nullptr);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (s >= P.S) break;
     double z = P.z[s], kk = z * kF / P.RT, x = kk * dv, co = P.cb[s];
     double pa = P.pam[(long)s * P.ncell + i];
     double A = pa * unit;
-    double Bm = bernoulli(-x), Bp = bernoulli(x);
-    double Sm = bernoulli_slope(-x), Sp = bernoulli_slope(x);
+    const VjpBasis bs = vjp_species_basis(P.z, s, b0, x, gmbar != nullptr);
+    double Bm = bs.Bm, Bp = bs.Bp;
+    double Sm = bs.Sm, Sp = bs.Sp;
     double score = -ci[s] * Sm - co * Sp;
     double gmv = A * kk * score;
     double Jbar = fN ? -fN[(long)s * P.ncell + i] : 0.0;
@@ -2818,7 +2847,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_vjp(Params P, const double* y,
     cbar[s] += Jbar * A * Bm;
     vbar += Jbar * gmv;
     if (gmbar) {
-      double Qm = bernoulli_second(-x), Qp = bernoulli_second(x);
+      double Qm = bs.Qm, Qp = bs.Qp;
       cbar[s] += Gbar * (-A * kk * Sm);
       vbar += Gbar * (A * kk * kk * (ci[s] * Qm - co * Qp));
     }
