@@ -136,7 +136,8 @@ def test_field_solve_matches_the_oracle(torch, cfg, steps):
 @pytest.mark.parametrize("cfg,steps", [
     (small_field(worlds=2, seed=21), 8),
     (FieldConfig(worlds=2, cells=36, row_width=6, side=40, spacing=5, seed=22), 6),
-], ids=["14-grid", "40-grid"])
+    (FieldConfig(worlds=2, cells=100, row_width=10, side=130, spacing=11, seed=23), 3),
+], ids=["14-grid", "40-grid", "130-grid"])
 def test_field_energy_rates_match_the_oracle(torch, cfg, steps):
     """The field solve's energy terms (cells_field_energy_rates; numerics/surface_field.py:792-835)
     against the oracle's at the device's own potentials: every rate within 1e-10 of the world's
