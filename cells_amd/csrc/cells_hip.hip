@@ -531,16 +531,34 @@ __device__ __forceinline__ void cells_body(Params P, const double* A, const doub
   // incoming, junction transfers in incidence order
   double jout[4] = {0.0, 0.0, 0.0, 0.0}, jin[4] = {0.0, 0.0, 0.0, 0.0};
   double g = kChargePerFmol * Gm;  // with slopes: the diagonal's conductances, in incidence order
-  for (int p = p0; p < p1; ++p) {
-    const int e = P.inc_edge[p];
-    if (slopes) g += Gj[e];
-    const bool outgoing = p < pm;
+  // two incident edges at a time, both edges' loads issued before either is used (a wavefront's
+  // requests for the lines its cells share arrive closer together); the sums keep their order
+  for (int pb = p0; pb < p1; pb += 2) {
+    const int e0 = P.inc_edge[pb], e1 = pb + 1 < p1 ? P.inc_edge[pb + 1] : -1;
+    double g0 = 0.0, g1 = 0.0, t0[4], t1[4];
+    if (slopes) {
+      g0 = Gj[e0];
+      g1 = e1 >= 0 ? Gj[e1] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      t0[s] = s < P.S ? EFc(P, R, P.f_jtr + s, e0) : 0.0;
+      t1[s] = (s < P.S && e1 >= 0) ? EFc(P, R, P.f_jtr + s, e1) : 0.0;
+    }
+    if (slopes) g += g0;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       if (s >= P.S) break;
-      const double t = EFc(P, R, P.f_jtr + s, e);
-      if (outgoing) jout[s] += -t;
-      else jin[s] += t;
+      if (pb < pm) jout[s] += -t0[s];
+      else jin[s] += t0[s];
+    }
+    if (e1 < 0) break;
+    if (slopes) g += g1;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s >= P.S) break;
+      if (pb + 1 < pm) jout[s] += -t1[s];
+      else jin[s] += t1[s];
     }
   }
   double qrate = 0.0;  // the charge rate of the amounts' rates (less 2 K1's with K1: stage 2)
