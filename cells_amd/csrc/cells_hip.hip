@@ -2452,16 +2452,28 @@ __device__ int ledger_checks(const Params& P, const double* prev, const double* 
     // the junction net per species, edges outer (each index loaded once), each species summed in
     // incidence order
     double nets[4] = {0.0, 0.0, 0.0, 0.0}, netms[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int p = p0; p < p1; ++p) {
-      const int e = P.inc_edge[p];
-      const bool outgoing = p < pm;
+    for (int pb = p0; pb < p1; pb += 2) {  // two edges' loads issued first; the sums keep their order
+      const int e0 = P.inc_edge[pb], e1 = pb + 1 < p1 ? P.inc_edge[pb + 1] : -1;
+      double j0[4], j1[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        j0[s] = s < P.S ? EFc(P, y, P.f_jtr + s, e0) : 0.0;
+        j1[s] = (s < P.S && e1 >= 0) ? EFc(P, y, P.f_jtr + s, e1) : 0.0;
+      }
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         if (s >= P.S) break;
-        const double j = EFc(P, y, P.f_jtr + s, e);
-        if (outgoing) nets[s] -= j;
-        else nets[s] += j;
-        netms[s] += fabs(j);
+        if (pb < pm) nets[s] -= j0[s];
+        else nets[s] += j0[s];
+        netms[s] += fabs(j0[s]);
+      }
+      if (e1 < 0) break;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        if (s >= P.S) break;
+        if (pb + 1 < pm) nets[s] -= j1[s];
+        else nets[s] += j1[s];
+        netms[s] += fabs(j1[s]);
       }
     }
     double zq = 0.0, zm = 0.0, chem = 0.0;
