@@ -1,7 +1,8 @@
 """Oracle (test infrastructure only): ctypes access to oracle/_build/libcells_ref.so.
 
-Only tests, the smoke check and the benchmark's CPU-baseline leg import this package. It is the
-checker, never the thing measured or shipped: the product (cells_amd) must not import it.
+Only tests, the smoke check and the benchmark's checker legs import this package: its CPU
+baseline and its parity checks, each outside every timed region. It is the checker, never the
+thing measured or shipped: the product (cells_amd) must not import it.
 """
 
 import ctypes

@@ -1,6 +1,7 @@
 /* Oracle (test infrastructure only): FP64 CPU restatement of the reference's tissue step for
- * lattices of cells on one held bath. Only tests, the smoke check and the benchmark's CPU-baseline
- * leg load it; the product never does.
+ * lattices of cells on one held bath. Only tests, the smoke check and the benchmark's checker legs
+ * (its CPU baseline and its parity checks, outside every timed region) load it; the product never
+ * does.
  *
  * Reference files followed (paths under src/cells/ of the reference):
  *   laws/electrochemistry.py:11-48            F, R, bernoulli, electrochemical potential

@@ -1,6 +1,7 @@
 /* Oracle (test infrastructure only): FP64 CPU restatement of the reference's batched tissue step.
  *
- * Only tests, the smoke check and the CPU-baseline leg of the benchmark may load this library.
+ * Only tests, the smoke check and the benchmark's checker legs (its CPU baseline and its parity
+ * checks, outside every timed region) may load this library.
  * The product (cells_amd/csrc, include/cells_hip.h) never links it.
  *
  * The world description below is the packed ensemble layout the product also takes

@@ -1,7 +1,7 @@
 """Oracle (test infrastructure only): the sparse Cholesky's host symbolic analysis, restated.
 
-Only tests, the smoke check and the CPU-baseline leg of the benchmark may import this module; the
-product's own symbolic analysis is the C++ in cells_amd/csrc/symbolic.cpp, which these functions
+Only tests, the smoke check and the benchmark's checker legs (its CPU baseline and its parity checks,
+outside every timed region) may import this module; the product's own symbolic analysis is the C++ in cells_amd/csrc/symbolic.cpp, which these functions
 check bit for bit.
 
 Restates src/cells/numerics/sparse_cholesky.py of the reference:
