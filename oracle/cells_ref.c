@@ -1827,7 +1827,7 @@ static void solve_vjp(const cells_world* w, const cells_symbolic* s, int32_t k, 
                       const double* sc, double coef, const double* dbar, const double* delta,
                       double* rate_bar, double* gmbar, double* gjbar) {
     int64_t c0 = w->cell_off[k], n = w->cell_off[k + 1] - c0;
-    double* rhs = (double*)malloc(sizeof(double) * (size_t)n);
+    double* rhs = (double*)calloc((size_t)(n ? n : 1), sizeof(double));
     double* x = (double*)malloc(sizeof(double) * (size_t)n);
     double* wv = (double*)calloc((size_t)w->ncell, sizeof(double));
     for (int64_t i = 0; i < n; ++i) rhs[i] = dbar[c0 + i] * sc[i];
