@@ -7,8 +7,9 @@ captured HIP graph per step), with the state resident in HBM before it starts.
 
 Beside the headline: the same step one world per launch ("plain", timed on the ensemble's first
 --plain-worlds worlds), the batched/plain ratio, the dominant kernel's roofline figures from HIP
-events, a parity check of the first worlds against the FP64 oracle, and the oracle's own
-throughput on this host (CPU baseline).
+events (`roofline`; the other sparse kernel, factor or stage solve, on the same terms in
+`roofline_others`), a parity check of the first worlds against the FP64 oracle, the oracle's own
+throughput on this host (CPU baseline), and the N2, stationary, spatial and adjoint legs.
 """
 
 import argparse
