@@ -416,8 +416,12 @@ __device__ __forceinline__ void edges_body(const Params& P, const double* A, con
     double* o = next(const_cast<double*>(A), const_cast<double*>(B), sw);
     const double h1 = 1.5 * gdt, h2 = 0.5 * gdt;
     const double kd = EFc(P, K1g, P.f_jdiss, e), kg = EFc(P, K1g, P.f_gate, e);
-    EF(P, o, P.f_jdiss, e) = (EFc(P, yc, P.f_jdiss, e) + h1 * kd) + h2 * (1e-3 * diss + -2.0 * kd);
-    EF(P, o, P.f_gate, e) = (EFc(P, yc, P.f_gate, e) + h1 * kg) + h2 * (grate + -2.0 * kg);
+    const double od = (EFc(P, yc, P.f_jdiss, e) + h1 * kd) + h2 * (1e-3 * diss + -2.0 * kd);
+    const double og = (EFc(P, yc, P.f_gate, e) + h1 * kg) + h2 * (grate + -2.0 * kg);
+    EF(P, o, P.f_jdiss, e) = od;
+    EF(P, o, P.f_gate, e) = og;
+    // the new state's domain on these fields (the final combination checks the transfers)
+    ok = ok && isfinite(od) && isfinite(og) && og >= P.gate_min && og <= 1.0;
   } else {
     EF(P, R, P.f_jdiss, e) = 1e-3 * diss;
     EF(P, R, P.f_gate, e) = grate;
@@ -592,7 +596,9 @@ __device__ __forceinline__ void cells_body(Params P, const double* A, const doub
       const int f = k < 4 ? P.f_mdiss + k : P.f_ccyc + (k - 4);
       const double k1 = CFc(P, K1, f, i);
       const double k2 = r[k] + -2.0 * k1;
-      CF(P, o, f, i) = (CFc(P, yc, f, i) + h1 * k1) + h2 * k2;
+      const double v = (CFc(P, yc, f, i) + h1 * k1) + h2 * k2;
+      CF(P, o, f, i) = v;
+      ok = ok && isfinite(v);  // the new state's domain on these fields (the final combination has the rest)
     }
   } else {
     CF(P, R, P.f_mdiss, i) = mdiss;
@@ -725,6 +731,25 @@ __device__ bool edge_domain(const Params& P, const double* y, int e) {
 
 __device__ __forceinline__ bool is_water_field(const Params& P, int f) {
   return P.water && (f == P.f_dvol || f == P.f_wtr || f == P.f_wdiss);
+}
+
+// cell_domain without the fields the ROS2 second stage's cell kernel finishes and checks itself
+// (membrane heat, the pumps' cycles, heat and work): the amounts, transfers and water fields.
+__device__ bool cell_domain_final(const Params& P, const double* y, const double* yv, int i) {
+  bool ok = true;
+  for (int f = 0; f < P.cf; ++f)
+    if (f < P.f_mtr + P.S || is_water_field(P, f)) ok = ok && isfinite(CFc(P, y, f, i));
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= P.S) break;
+    double n = P.n0[(long)s * P.ncell + i] + CFc(P, y, P.f_dn + s, i);
+    ok = ok && isfinite(n) && n > 0;
+  }
+  if (P.water) {
+    double v = P.vol0[i] + CFc(P, yv, P.f_dvol, i);
+    ok = ok && v > P.vmin * P.vol0[i] && v < P.vmax * P.vol0[i];
+  }
+  return ok;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2269,7 +2294,7 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
       }
       contrib2[q] = (contrib2[q] + -2.0 * c1) - dm_c[s];
     }
-    bool ok = cell_domain(P, o, o, i);
+    bool ok = cell_domain_final(P, o, o, i);
     if (!stage_residual_ok(kChargePerFmol * qt, mx, mag, atol, rtol)) {
       ok = false;
       atomicOr(P.refuse_bits + w, kRefuseResidual);
@@ -2289,7 +2314,10 @@ __global__ __launch_bounds__(kBlock) void k_ros2_final(Params P, double* A, doub
       if (f >= P.f_jtr && f < P.f_jtr + P.S) k2 += coef * (gj[(long)(f - P.f_jtr) * P.nedge + e] * d);
       EF(P, o, f, e) = (EFc(P, y, f, e) + h1 * k1) + h2 * k2;
     }
-    if (!edge_domain(P, o, e)) valid[w] = 0;
+    // the transfers' domain (k_edges_ahead checked the junction heat and the gate)
+    bool eok = true;
+    for (int f = P.f_jtr; f < P.f_jtr + P.S; ++f) eok = eok && isfinite(EFc(P, o, f, e));
+    if (!eok) valid[w] = 0;
   }
 }
 
